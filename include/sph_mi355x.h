@@ -1,0 +1,326 @@
+/*
+ * sph_mi355x.h -- C-ABI of the MI355X (gfx950) per-step particle hot path of
+ * a 2D SPH solver: hashed-grid neighbour search into CSR pair lists, Wendland
+ * kernel-sum gathers (delta-SPH density / continuity / diffusion / momentum),
+ * the DFSPH relaxed-Jacobi density solve with an FP64 residual, and the
+ * ghost-particle (mDBC) and Akinci boundary gathers.
+ *
+ * Conventions
+ *  - Every entry point is batched-first: particles of many independent scenes
+ *    live in one flat index space; `pscene[p]` names the scene of point p and
+ *    per-scene parameters come from device arrays of SphGridDesc / SphPhysics.
+ *    A single scene is nscenes == 1.
+ *  - All pointers are device pointers unless the parameter says "host".
+ *    Positions / velocities / accelerations are float[n][2] (x, y interleaved).
+ *  - Nothing here allocates device memory.  Callers pass buffers together with
+ *    their capacities; scratch needs are reported by *_workspace_bytes().
+ *  - Every function launches on the caller's stream and never synchronises,
+ *    so whole steps can be captured into a hipGraph.
+ *  - Return value: SPH_OK or an SPH_E_* code; sph_last_error() explains the
+ *    last failure (thread-local).  Data-dependent failures that can only be
+ *    seen on the device (CSR overflow, particles outside the grid, NaN
+ *    pressure, iteration cap) are OR-ed into a per-scene device status word.
+ *
+ * Reference interfaces replaced (file:line in wi-re/pytorchSPH):
+ *   sph_grid_build            <- sortPositions + constructHashMap
+ *                                (src/modules/neighborSearch.py:30-88)
+ *   sph_neighbor_count/fill   <- buildNeighborList / buildNeighborListCUDA /
+ *                                buildNeighborListUnsortedPerParticle
+ *                                (cppSrc/neighSearch.cpp:143-331,
+ *                                 cppSrc/neighSearch_cuda.cu:101-307)
+ *   sph_csr_sort_rows + sph_pair_geometry
+ *                             <- finalize sort + pair geometry
+ *                                (src/modules/neighborSearch.py:153-157,216-250)
+ *   sph_fluid_density         <- computeDensity (src/modules/density.py:26-53)
+ *                                + akinci evalBoundaryDensity
+ *                                (src/modules/akinciBoundary.py:395-420)
+ *   sph_dsph_gather           <- computeDivergenceTerm (momentum.py:31-38),
+ *                                computeDensityDiffusionMOG/DeltaSPH
+ *                                (densityDiffusion.py:63-82),
+ *                                computePressureAccel (pressure.py:31-38),
+ *                                computeVelocityDiffusion (diffusion.py:30-44),
+ *                                computeLaminarViscosity (laminar.py:30-46),
+ *                                gravity (gravity.py:46-57) and the solid
+ *                                boundary terms (solidBoundary.py:746-895)
+ *   sph_dsph_normalization / sph_dsph_density_gradient
+ *                             <- computeNormalizationMatrix + pinv2x2 +
+ *                                computeRenormalizedDensityGradient
+ *                                (densityDiffusion.py:30-61, util.py:386-433)
+ *   sph_mdbc_density          <- computeBoundaryDensitymDBC
+ *                                (solidBoundary.py:561-616)
+ *   sph_dfsph_*               <- DFSPH / densitySolve and the akinci PBSPH
+ *                                hooks (modules/dfsph.py:25-311,
+ *                                akinciBoundary.py:166-389)
+ *   sph_integrate_*           <- integrateValues / integrate
+ *                                (simulationBase.py:564-569,632-705)
+ */
+#ifndef SPH_MI355X_H
+#define SPH_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_OK 0
+#define SPH_E_ARG 1      /* bad argument (null pointer, negative size, ...) */
+#define SPH_E_HIP 2      /* a HIP runtime call failed                       */
+#define SPH_E_SPACE 3    /* a caller buffer is smaller than required        */
+
+/* per-scene device status bits */
+#define SPH_ST_OUTSIDE   1u  /* a point's cell key fell outside the cell table */
+#define SPH_ST_OVERFLOW  2u  /* CSR capacity exceeded: pair list truncated     */
+#define SPH_ST_NAN       4u  /* DFSPH pressure became NaN / Inf                */
+#define SPH_ST_ITERCAP   8u  /* DFSPH solve stopped at the iteration cap       */
+
+/* Uniform grid of one scene.  Cells are hmax wide; a point's cell is
+ * (ceil((x - qmin_x) / hmax), ceil((y - qmin_y) / hmax)) in fp32 with
+ * correctly rounded subtraction and division, its key is
+ * ix + cells_x * iy (neighborSearch.py:44-47). */
+typedef struct SphGridDesc {
+  float qmin_x, qmin_y;   /* minDomain - hmax                              */
+  float hmax;             /* max support = cell size                      */
+  int cells_x, cells_y;   /* ceil((qmax - qmin) / hmax), qmax = max + 2 hmax */
+  int cell_begin;         /* first slot of this scene in the cell table    */
+  int p_begin, p_count;   /* this scene's point range                      */
+} SphGridDesc;
+
+/* Grid over a point set (many scenes).  All members are device pointers
+ * except the sizes. */
+typedef struct SphGrid {
+  const SphGridDesc* scenes; /* [nscenes]                                  */
+  const int* pscene;         /* [np] scene of each point                   */
+  const float* x;            /* [np][2] point positions                    */
+  int np, nscenes, ncells;   /* ncells = sum of cells_x*cells_y            */
+  int* cell_count;           /* [ncells] scratch                           */
+  int* cell_start;           /* [ncells+1] first sorted slot of each cell  */
+  int* key;                  /* [np] cell key of each point (-1: outside)  */
+  int* slot;                 /* [np] scratch                               */
+  int* sorted_idx;           /* [np] point index in (scene, key, index) order */
+  float* sorted_x;           /* [np][2] positions in sorted order          */
+  int* lost;                 /* [1] scratch counter                        */
+  unsigned* status;          /* [nscenes] status bits                      */
+} SphGrid;
+
+/* A set of query points searched against a grid. */
+typedef struct SphQuery {
+  const float* x;     /* [nq][2] query positions                           */
+  const float* h;     /* [nq] query supports: pair iff |xq - xj| < h[q]    */
+  const int* qscene;  /* [nq] scene of each query                          */
+  int nq;
+  int radius;         /* stencil half width in cells (searchRadius)        */
+  int self_sorted;    /* 1: queries are the grid's own points; rows are
+                         visited in the grid's sorted order for locality  */
+} SphQuery;
+
+/* Physical constants of one scene (simulationBase.py:22-111 defaults). */
+typedef struct SphPhysics {
+  float h;          /* particle support (config particle.support)          */
+  float rho0;       /* rest density                                        */
+  float c0;         /* speed of sound                                      */
+  float gamma;      /* Tait exponent                                       */
+  float kappa;      /* compressible EOS stiffness                          */
+  float alpha;      /* delta-SPH artificial viscosity                      */
+  float delta;      /* density diffusion coefficient                       */
+  float nu;         /* kinematic viscosity (laminar)                       */
+  float eps;        /* 0.1 h^2                                             */
+  float gx, gy;     /* gravity acceleration                                */
+  float dt;         /* time step                                           */
+  float beta;       /* akinci beta                                         */
+  float omega;      /* boundary relaxed-Jacobi factor                      */
+  float tau;        /* DFSPH density threshold                             */
+  float fluid_omega;/* fluid relaxed-Jacobi factor (0.3 in the reference)  */
+  int min_iters, max_iters;
+  int eos;          /* 0 Tait, 1 deltaSPH (linear), 2 compressible         */
+  int pad;
+} SphPhysics;
+
+/* delta-SPH gather terms (bitmask for sph_dsph_gather) */
+#define SPH_T_CONT     (1u << 0)  /* continuity, fluid neighbours         */
+#define SPH_T_CONT_B   (1u << 1)  /* continuity, boundary neighbours      */
+#define SPH_T_MOG      (1u << 2)  /* MOG density diffusion                */
+#define SPH_T_DDSPH    (1u << 3)  /* renormalised delta-SPH diffusion     */
+#define SPH_T_PRESS    (1u << 4)  /* pressure acceleration, fluid         */
+#define SPH_T_PRESS_B  (1u << 5)  /* pressure acceleration, boundary      */
+#define SPH_T_VISC     (1u << 6)  /* artificial viscosity, fluid          */
+#define SPH_T_VISC_B   (1u << 7)  /* artificial viscosity, boundary       */
+#define SPH_T_LAM      (1u << 8)  /* laminar viscosity                    */
+#define SPH_T_GRAV     (1u << 9)  /* gravity                              */
+#define SPH_T_ACCUM    (1u << 16) /* add into dpdt/acc instead of overwrite */
+
+/* Fluid state (device pointers, [np] or [np][2]). */
+typedef struct SphFluid {
+  const float* x; const float* v; const float* rho; /* rho = normalised density */
+  const float* area; const float* rest;             /* area A, rest density   */
+  const float* vol;   /* optional volume override; NULL means A / rho       */
+  const int* pscene;
+  int np;
+} SphFluid;
+
+/* Boundary particle state.  vol is the boundary volume (boundaryVolume);
+ * rho is the normalised boundary density; p the boundary pressure. */
+typedef struct SphBoundary {
+  const float* x; const float* v; const float* rho; const float* vol;
+  const float* p; const float* rest; const int* bscene;
+  int nb;
+} SphBoundary;
+
+/* A CSR pair list: row r holds col[rowptr[r] .. rowptr[r+1]). */
+typedef struct SphCSR {
+  const int* rowptr; const int* col; int nrows;
+} SphCSR;
+
+/* ---------------------------------------------------------------- misc */
+const char* sph_last_error(void);
+int sph_version(void);
+size_t sph_scan_workspace_bytes(int n);
+/* out[0] = 0, out[k+1] = in[0] + ... + in[k]; out has n+1 entries. */
+int sph_scan_exclusive(const int* in, int n, int* out, void* ws, size_t ws_bytes,
+                       hipStream_t s);
+int sph_fill_i32(int* p, int n, int value, hipStream_t s);
+int sph_fill_f32(float* p, int n, float value, hipStream_t s);
+
+/* -------------------------------------------------------- neighbour search */
+/* Counting sort of the grid's points by (scene, cell key, index) and the
+ * dense cell-start table.  Replaces the reference's hash map; the pair set
+ * found through it is identical because a stencil cell (qx, qy) with
+ * qx, qy >= 0 matches exactly the points whose key is qx + cells_x * qy. */
+size_t sph_grid_workspace_bytes(int ncells);
+int sph_grid_build(const SphGrid* g, void* ws, size_t ws_bytes, hipStream_t s);
+/* Number of pairs of each query (rows indexed by query number). */
+int sph_neighbor_count(const SphGrid* g, const SphQuery* q, int* counts, hipStream_t s);
+/* Writes col[rowptr[r] + k] = grid point index; cap = capacity of col.
+ * Pairs past the capacity are dropped, the scene is flagged
+ * SPH_ST_OVERFLOW and rowptr is clamped to cap so that every consumer of
+ * the truncated list stays inside col. */
+int sph_neighbor_fill(const SphGrid* g, const SphQuery* q, int* rowptr, int* col,
+                      int64_t cap, hipStream_t s);
+/* Sorts the columns of every row ascending (lexicographic (i, j) order). */
+int sph_csr_sort_rows(const SphCSR* a, int* col_mut, hipStream_t s);
+/* Transposes an nrows x ncols_t CSR.  t_counts is [ncols_t] scratch,
+ * t_slot is [nnz] scratch, t_rowptr is [ncols_t+1], t_col [nnz].  Rows of
+ * the transpose are sorted ascending. */
+int sph_csr_transpose(const SphCSR* a, int ncols_t, int* t_counts, int* t_slot,
+                      int* t_rowptr, int* t_col, void* ws, size_t ws_bytes,
+                      hipStream_t s);
+/* Reference pair geometry: rows64/cols64 (int64 COO), rhat = (xq_i - x_j)/|.|
+ * (0 below thresh), q = |xq_i - x_j| / hdiv, with hdiv = support (fluid-fluid)
+ * or the query support when hq != NULL. */
+int sph_pair_geometry(const SphCSR* a, const float* xq, const float* xg, const float* hq,
+                      float support, float thresh, int64_t* rows64, int64_t* cols64,
+                      float* rhat, float* q, hipStream_t s);
+
+/* ------------------------------------------------------ kernel-sum gathers */
+/* rho_out[p] = sum_j A_j W_pj (+ sum_b V_b W_pb when fb != NULL, the Akinci
+ * boundary density contribution); vol_out[p] = A_p / (fluid-only sum). */
+int sph_fluid_density(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                      const SphBoundary* b, const SphCSR* fb, const int* order,
+                      float* rho_out, float* vol_out, hipStream_t s);
+/* delta-SPH right-hand side of every fluid particle over its fluid (ff) and
+ * boundary (fb) neighbours; terms = SPH_T_* mask. gradrho [np][2] is needed
+ * for SPH_T_DDSPH.  Writes (or adds into, SPH_T_ACCUM) dpdt and acc. */
+int sph_dsph_gather(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                    const SphBoundary* b, const SphCSR* fb, const float* gradrho,
+                    unsigned terms, const int* order, float* dpdt, float* acc,
+                    hipStream_t s);
+/* Renormalisation matrix M_i (fluid + boundary neighbours) and L_i =
+ * pinv2x2(M_i); M, L are [np][2][2]. */
+int sph_dsph_normalization(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                           const SphBoundary* b, const SphCSR* fb, const int* order,
+                           float* M, float* L, float* eig, hipStream_t s);
+/* Renormalised density gradient <grad rho>_i, [np][2]. */
+int sph_dsph_density_gradient(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                              const SphBoundary* b, const SphCSR* fb, const float* L,
+                              const int* order, float* gradrho, hipStream_t s);
+/* Batched closed-form 2x2 pseudo-inverse (util.py:386-433). */
+int sph_pinv2x2(const float* M, int n, float* L, float* eig, hipStream_t s);
+/* mDBC boundary density from ghost->fluid pairs (gf rows = boundary index).
+ * Writes rho_b, p_b (EOS) and neighbour counts. */
+int sph_mdbc_density(const SphFluid* f, const SphPhysics* phys, const float* bx,
+                     const float* gx, const int* bscene, int nb, const SphCSR* gf,
+                     float* rho_b, float* p_b, int* counts, hipStream_t s);
+/* EOS for n particles: p = EOS(rho). */
+int sph_eos(const float* rho, const int* pscene, const SphPhysics* phys, int n, float* p,
+            hipStream_t s);
+/* Akinci boundary density rho_b = term_b + sum_f A_f W_bf + beta (bf rows =
+ * boundary index). */
+int sph_akinci_boundary_density(const SphFluid* f, const SphPhysics* phys, const float* bx,
+                                const int* bscene, int nb, const float* term,
+                                const SphCSR* bf, float* rho_b, hipStream_t s);
+
+/* ------------------------------------------------------------------ DFSPH */
+/* Solver state of all scenes.  Per-particle arrays are [np] / [np][2]
+ * (fluid) or [nb] (boundary). */
+typedef struct SphDfsph {
+  float* vpred;  /* [np][2] predicted velocity                        */
+  float* aact;   /* [np] actual area A / rho                          */
+  float* alpha;  /* [np]                                              */
+  float* src;    /* [np] source term                                  */
+  float* p;      /* [np] pressure (previous iterate)                  */
+  float* p2;     /* [np] pressure (current iterate)                   */
+  float* apred;  /* [np][2] pressure acceleration                     */
+  float* resid;  /* [np] residual                                     */
+  float* b_aact; /* [nb]                                              */
+  float* b_alpha;/* [nb]                                              */
+  float* b_src;  /* [nb]                                              */
+  float* b_p;    /* [nb] boundary pressure                            */
+  float* b_resid;/* [nb]                                              */
+  double* err_acc;  /* [nscenes] FP64 sum of clamped residuals        */
+  double* err_hist; /* [nscenes][max_hist] per-iteration errors        */
+  int* iters;       /* [nscenes] iterations done                       */
+  int* active;      /* [nscenes] 1 while the scene still iterates      */
+  int* any_active;  /* [1]                                            */
+  unsigned* status; /* [nscenes]                                       */
+  const int* nfluid;    /* [nscenes] fluid count per scene             */
+  const int* nbdy;      /* [nscenes] boundary count per scene          */
+  int nscenes, max_hist;
+} SphDfsph;
+
+/* v* = v + dt a, A^act = A / rho; boundary A^act_b = V_b / rho_b, p_b = 0;
+ * pressures zeroed; iteration counters reset. */
+int sph_dfsph_prepare(const SphFluid* f, const float* acc, const SphPhysics* phys,
+                      const SphBoundary* b, SphDfsph* d, hipStream_t s);
+/* alpha and source term of fluid and boundary particles. */
+int sph_dfsph_alpha_source(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                           const SphBoundary* b, const SphCSR* fb, const SphCSR* bf,
+                           const int* order, SphDfsph* d, hipStream_t s);
+/* One relaxed-Jacobi iteration of every still-active scene: pressure
+ * acceleration from p2, then the pressure / residual update and the FP64
+ * residual reduction, then the per-scene convergence decision. */
+int sph_dfsph_iterate(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                      const SphBoundary* b, const SphCSR* fb, const SphCSR* bf,
+                      const int* order, SphDfsph* d, hipStream_t s);
+/* Final pressure acceleration, v* += dt a*, acc += a*. */
+int sph_dfsph_finish(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                     const SphBoundary* b, const SphCSR* fb, const int* order,
+                     SphDfsph* d, float* acc, hipStream_t s);
+
+/* ------------------------------------------------------------- integrators */
+/* RK4 stage s in 1..4 (simulationBase.py:679-705): accumulates
+ * k = (acc, v_stage, dpdt) into kacc[np][5] and writes the next stage state
+ * (x_out, v_out, rho_out) = perennial + c_s dt k; at s = 4 the final state
+ * perennial + dt/6 sum(w k) and acc_out = sum(w k). */
+int sph_integrate_rk4(int stage, const int* pscene, const SphPhysics* phys, int np,
+                      const float* x0, const float* v0, const float* rho0,
+                      const float* v_stage, const float* acc, const float* dpdt,
+                      float* kacc, float* x_out, float* v_out, float* rho_out,
+                      float* acc_out, hipStream_t s);
+/* semi-implicit Euler: v = v0 + dt a; x = x0 + dt (v_stage + dt a);
+ * rho = rho0 + dt dpdt / rho0 when dpdt != NULL. */
+int sph_integrate_semi_implicit(const int* pscene, const SphPhysics* phys, int np,
+                                const float* x0, const float* v0, const float* rho0,
+                                const float* v_stage, const float* acc, const float* dpdt,
+                                float* x_out, float* v_out, float* rho_out,
+                                hipStream_t s);
+/* Gathers float arrays by a permutation: out[k] = in[perm[k]] for
+ * `width` floats per element. */
+int sph_permute_f32(const float* in, const int* perm, int n, int width, float* out,
+                    hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

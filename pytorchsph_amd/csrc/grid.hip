@@ -1,0 +1,329 @@
+// Neighbour search on a dense per-scene cell table.
+//
+// The reference sorts particles by a linear cell key and looks cells up
+// through a hash table keyed (3 ix + 5 iy) mod P (neighborSearch.py:58-88,
+// neighSearch.cpp:93-114).  On a bounded 2D domain the table can simply be
+// dense: a counting sort by (scene, key) produces the cell-start array in
+// one scan, and every stencil row of 2R+1 cells becomes one contiguous range
+// of sorted points.  Pair predicate and cell formula are the reference's,
+// so the pair set is the same.
+#include "sph_common.h"
+
+namespace sph {
+namespace {
+
+constexpr int kBS = 256;
+
+__global__ __launch_bounds__(kBS) void k_grid_keys(SphGrid g) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  const int s = g.pscene[p];
+  const SphGridDesc d = g.scenes[s];
+  const float x = g.x[2 * p], y = g.x[2 * p + 1];
+  const int ix = cell_coord(x, d.qmin_x, d.hmax);
+  const int iy = cell_coord(y, d.qmin_y, d.hmax);
+  // reference key: ix + cellsX * iy in int32 (neighborSearch.py:47)
+  const int64_t key = (int64_t)ix + (int64_t)d.cells_x * (int64_t)iy;
+  const int64_t ncell = (int64_t)d.cells_x * d.cells_y;
+  if (key >= 0 && key < ncell) {
+    g.key[p] = (int)key;
+    g.slot[p] = atomicAdd(&g.cell_count[d.cell_begin + key], 1);
+  } else {
+    g.key[p] = -1;
+    g.slot[p] = atomicAdd(g.lost, 1);
+    atomicOr(&g.status[s], SPH_ST_OUTSIDE);
+  }
+}
+
+__global__ __launch_bounds__(kBS) void k_grid_scatter(SphGrid g) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  const int key = g.key[p];
+  int pos;
+  if (key >= 0) {
+    const int s = g.pscene[p];
+    pos = g.cell_start[g.scenes[s].cell_begin + key] + g.slot[p];
+  } else {
+    pos = g.cell_start[g.ncells] + g.slot[p];
+  }
+  g.sorted_idx[pos] = p;
+}
+
+// One thread per cell: order the cell's points by index (atomics gave them
+// in arbitrary order) and copy their positions into sorted order.  The last
+// thread handles the points that fell outside every table.
+__global__ __launch_bounds__(kBS) void k_grid_cell_sort(SphGrid g) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > g.ncells) return;
+  const int a = g.cell_start[c];
+  const int b = (c == g.ncells) ? g.np : g.cell_start[c + 1];
+  int* idx = g.sorted_idx;
+  for (int i = a + 1; i < b; ++i) {
+    const int v = idx[i];
+    int j = i - 1;
+    while (j >= a && idx[j] > v) {
+      idx[j + 1] = idx[j];
+      --j;
+    }
+    idx[j + 1] = v;
+  }
+  const float2* x = reinterpret_cast<const float2*>(g.x);
+  float2* sx = reinterpret_cast<float2*>(g.sorted_x);
+  for (int i = a; i < b; ++i) sx[i] = x[idx[i]];
+}
+
+// Count (FILL = false) or emit (FILL = true) the pairs of every query.
+// Stencil rows are walked bottom to top and each row is one contiguous range
+// [cell_start[k0], cell_start[k1 + 1]) of sorted points, so the columns of a
+// row come out in sorted-point order.  Cells with qx < 0 or qy < 0 are
+// skipped as in queryHashMap (neighSearch.cpp:95); keys past the scene's
+// table are clipped.
+template <bool FILL>
+__global__ __launch_bounds__(kBS) void k_neighbors(SphGrid g, SphQuery q,
+                                                   const int* __restrict__ rowptr,
+                                                   int* __restrict__ out, int64_t cap) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= q.nq) return;
+  const int qi = q.self_sorted ? g.sorted_idx[t] : t;
+  const int s = q.qscene[qi];
+  const SphGridDesc d = g.scenes[s];
+  const float xq = q.x[2 * qi], yq = q.x[2 * qi + 1];
+  const float thr = sq_threshold(q.h[qi]);
+  const int ix = cell_coord(xq, d.qmin_x, d.hmax);
+  const int iy = cell_coord(yq, d.qmin_y, d.hmax);
+  const int64_t ncell = (int64_t)d.cells_x * d.cells_y;
+  const int R = q.radius;
+  const float2* __restrict__ sx = reinterpret_cast<const float2*>(g.sorted_x);
+  const int* __restrict__ cs = g.cell_start + d.cell_begin;
+  int64_t base = 0;
+  if (FILL) base = rowptr[qi];
+  int cnt = 0;
+  for (int dy = -R; dy <= R; ++dy) {
+    const int qy = iy + dy;
+    if (qy < 0) continue;
+    const int qx0 = ix - R < 0 ? 0 : ix - R;
+    const int qx1 = ix + R;
+    if (qx1 < qx0) continue;
+    const int64_t k0 = qx0 + (int64_t)d.cells_x * qy;
+    int64_t k1 = qx1 + (int64_t)d.cells_x * qy;
+    if (k0 >= ncell) break;
+    if (k1 >= ncell) k1 = ncell - 1;
+    const int a = cs[k0], b = cs[k1 + 1];
+    for (int j = a; j < b; ++j) {
+      const float2 xj = sx[j];
+      if (dist2_nofma(xq, yq, xj.x, xj.y) <= thr) {
+        if (FILL) {
+          const int64_t pos = base + cnt;
+          if (pos < cap) out[pos] = g.sorted_idx[j];
+        }
+        ++cnt;
+      }
+    }
+  }
+  if (!FILL) {
+    out[qi] = cnt;
+  } else if (base + cnt > cap) {
+    atomicOr(&g.status[s], SPH_ST_OVERFLOW);
+  }
+}
+
+// Clamps row pointers to the column capacity so that every consumer of a
+// truncated list stays inside the column buffer.
+__global__ __launch_bounds__(kBS) void k_clamp_rowptr(int* __restrict__ rowptr, int n, int64_t cap) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if ((int64_t)rowptr[i] > cap) rowptr[i] = (int)cap;
+}
+
+__global__ __launch_bounds__(kBS) void k_sort_rows(const int* __restrict__ rowptr, int nrows,
+                                                   int* __restrict__ col) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const int a = rowptr[r], b = rowptr[r + 1];
+  for (int i = a + 1; i < b; ++i) {
+    const int v = col[i];
+    int j = i - 1;
+    while (j >= a && col[j] > v) {
+      col[j + 1] = col[j];
+      --j;
+    }
+    col[j + 1] = v;
+  }
+}
+
+__global__ __launch_bounds__(kBS) void k_tr_count(const int* __restrict__ rowptr,
+                                                  const int* __restrict__ col, int nrows,
+                                                  int* __restrict__ tcount,
+                                                  int* __restrict__ slot) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) slot[k] = atomicAdd(&tcount[col[k]], 1);
+}
+
+__global__ __launch_bounds__(kBS) void k_tr_scatter(const int* __restrict__ rowptr,
+                                                    const int* __restrict__ col, int nrows,
+                                                    const int* __restrict__ slot,
+                                                    const int* __restrict__ trowptr,
+                                                    int* __restrict__ tcol) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) tcol[trowptr[col[k]] + slot[k]] = r;
+}
+
+__global__ __launch_bounds__(kBS) void k_pair_geometry(const int* __restrict__ rowptr,
+                                                       const int* __restrict__ col, int nrows,
+                                                       const float* __restrict__ xq,
+                                                       const float* __restrict__ xg,
+                                                       const float* __restrict__ hq,
+                                                       float support, float thresh,
+                                                       int64_t* __restrict__ rows64,
+                                                       int64_t* __restrict__ cols64,
+                                                       float* __restrict__ rhat,
+                                                       float* __restrict__ qout) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const float xi = xq[2 * r], yi = xq[2 * r + 1];
+  const float hdiv = hq ? hq[r] : support;
+  for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+    const int j = col[k];
+    const float rx = xi - xg[2 * j], ry = yi - xg[2 * j + 1];
+    const float d = sqrtf(rx * rx + ry * ry);
+    float hx = 0.f, hy = 0.f;
+    if (d >= thresh) {
+      hx = rx / d;
+      hy = ry / d;
+    }
+    if (rows64) rows64[k] = r;
+    if (cols64) cols64[k] = j;
+    if (rhat) {
+      rhat[2 * (int64_t)k] = hx;
+      rhat[2 * (int64_t)k + 1] = hy;
+    }
+    if (qout) qout[k] = d / hdiv;
+  }
+}
+
+}  // namespace
+
+int grid_build(const SphGrid* gp, void* ws, size_t ws_bytes, hipStream_t s) {
+  SPH_REQUIRE(gp, "sph_grid_build: null grid");
+  const SphGrid& g = *gp;
+  SPH_REQUIRE(g.np >= 0 && g.ncells > 0 && g.nscenes > 0, "sph_grid_build: bad sizes");
+  SPH_REQUIRE(g.scenes && g.pscene && g.x && g.cell_count && g.cell_start && g.key && g.slot &&
+                  g.sorted_idx && g.sorted_x && g.lost && g.status,
+              "sph_grid_build: null buffer");
+  int rc;
+  if ((rc = fill_i32(g.cell_count, g.ncells, 0, s))) return rc;
+  if ((rc = fill_i32(g.lost, 1, 0, s))) return rc;
+  if (g.np > 0)
+    hipLaunchKernelGGL(k_grid_keys, dim3(blocks_for(g.np, kBS)), dim3(kBS), 0, s, g);
+  if ((rc = launch_check("sph_grid_build(keys)"))) return rc;
+  if ((rc = scan_exclusive(g.cell_count, g.ncells, g.cell_start, ws, ws_bytes, s))) return rc;
+  if (g.np > 0)
+    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks_for(g.np, kBS)), dim3(kBS), 0, s, g);
+  hipLaunchKernelGGL(k_grid_cell_sort, dim3(blocks_for(g.ncells + 1, kBS)), dim3(kBS), 0, s, g);
+  return launch_check("sph_grid_build");
+}
+
+static int check_query(const SphGrid* g, const SphQuery* q, const char* who) {
+  if (!g || !q || !q->x || !q->h || !q->qscene || q->nq < 0 || q->radius < 0) {
+    set_error(std::string(who) + ": bad query");
+    return SPH_E_ARG;
+  }
+  if (q->self_sorted && q->nq != g->np) {
+    set_error(std::string(who) + ": self_sorted query must cover the grid's points");
+    return SPH_E_ARG;
+  }
+  return SPH_OK;
+}
+
+int neighbor_count(const SphGrid* g, const SphQuery* q, int* counts, hipStream_t s) {
+  int rc = check_query(g, q, "sph_neighbor_count");
+  if (rc) return rc;
+  SPH_REQUIRE(counts, "sph_neighbor_count: null counts");
+  if (q->nq == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_neighbors<false>, dim3(blocks_for(q->nq, kBS)), dim3(kBS), 0, s, *g, *q,
+                     (const int*)nullptr, counts, (int64_t)0);
+  return launch_check("sph_neighbor_count");
+}
+
+int neighbor_fill(const SphGrid* g, const SphQuery* q, int* rowptr, int* col, int64_t cap,
+                  hipStream_t s) {
+  int rc = check_query(g, q, "sph_neighbor_fill");
+  if (rc) return rc;
+  SPH_REQUIRE(rowptr && (col || cap == 0) && cap >= 0, "sph_neighbor_fill: bad output");
+  if (q->nq == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_neighbors<true>, dim3(blocks_for(q->nq, kBS)), dim3(kBS), 0, s, *g, *q,
+                     rowptr, col, cap);
+  hipLaunchKernelGGL(k_clamp_rowptr, dim3(blocks_for(q->nq + 1, kBS)), dim3(kBS), 0, s, rowptr,
+                     q->nq + 1, cap);
+  return launch_check("sph_neighbor_fill");
+}
+
+int csr_sort_rows(const SphCSR* a, int* col, hipStream_t s) {
+  SPH_REQUIRE(a && a->rowptr && a->nrows >= 0, "sph_csr_sort_rows: bad csr");
+  if (a->nrows == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_sort_rows, dim3(blocks_for(a->nrows, kBS)), dim3(kBS), 0, s, a->rowptr,
+                     a->nrows, col);
+  return launch_check("sph_csr_sort_rows");
+}
+
+int csr_transpose(const SphCSR* a, int ncols_t, int* t_counts, int* t_slot, int* t_rowptr,
+                  int* t_col, void* ws, size_t ws_bytes, hipStream_t s) {
+  SPH_REQUIRE(a && a->rowptr && a->nrows >= 0 && ncols_t >= 0 && t_counts && t_rowptr,
+              "sph_csr_transpose: bad arguments");
+  int rc;
+  if ((rc = fill_i32(t_counts, ncols_t, 0, s))) return rc;
+  if (a->nrows > 0)
+    hipLaunchKernelGGL(k_tr_count, dim3(blocks_for(a->nrows, kBS)), dim3(kBS), 0, s, a->rowptr,
+                       a->col, a->nrows, t_counts, t_slot);
+  if ((rc = launch_check("sph_csr_transpose(count)"))) return rc;
+  if ((rc = scan_exclusive(t_counts, ncols_t, t_rowptr, ws, ws_bytes, s))) return rc;
+  if (a->nrows > 0)
+    hipLaunchKernelGGL(k_tr_scatter, dim3(blocks_for(a->nrows, kBS)), dim3(kBS), 0, s,
+                       a->rowptr, a->col, a->nrows, t_slot, t_rowptr, t_col);
+  if ((rc = launch_check("sph_csr_transpose(scatter)"))) return rc;
+  if (ncols_t > 0)
+    hipLaunchKernelGGL(k_sort_rows, dim3(blocks_for(ncols_t, kBS)), dim3(kBS), 0, s, t_rowptr,
+                       ncols_t, t_col);
+  return launch_check("sph_csr_transpose");
+}
+
+int pair_geometry(const SphCSR* a, const float* xq, const float* xg, const float* hq,
+                  float support, float thresh, int64_t* rows64, int64_t* cols64, float* rhat,
+                  float* q, hipStream_t s) {
+  SPH_REQUIRE(a && a->rowptr && xq && xg && a->nrows >= 0, "sph_pair_geometry: bad arguments");
+  if (a->nrows == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_pair_geometry, dim3(blocks_for(a->nrows, kBS)), dim3(kBS), 0, s,
+                     a->rowptr, a->col, a->nrows, xq, xg, hq, support, thresh, rows64, cols64,
+                     rhat, q);
+  return launch_check("sph_pair_geometry");
+}
+
+}  // namespace sph
+
+extern "C" {
+size_t sph_grid_workspace_bytes(int ncells) { return sph::scan_ws_bytes(ncells); }
+int sph_grid_build(const SphGrid* g, void* ws, size_t ws_bytes, hipStream_t s) {
+  return sph::grid_build(g, ws, ws_bytes, s);
+}
+int sph_neighbor_count(const SphGrid* g, const SphQuery* q, int* counts, hipStream_t s) {
+  return sph::neighbor_count(g, q, counts, s);
+}
+int sph_neighbor_fill(const SphGrid* g, const SphQuery* q, int* rowptr, int* col,
+                      int64_t cap, hipStream_t s) {
+  return sph::neighbor_fill(g, q, rowptr, col, cap, s);
+}
+int sph_csr_sort_rows(const SphCSR* a, int* col_mut, hipStream_t s) {
+  return sph::csr_sort_rows(a, col_mut, s);
+}
+int sph_csr_transpose(const SphCSR* a, int ncols_t, int* t_counts, int* t_slot, int* t_rowptr,
+                      int* t_col, void* ws, size_t ws_bytes, hipStream_t s) {
+  return sph::csr_transpose(a, ncols_t, t_counts, t_slot, t_rowptr, t_col, ws, ws_bytes, s);
+}
+int sph_pair_geometry(const SphCSR* a, const float* xq, const float* xg, const float* hq,
+                      float support, float thresh, int64_t* rows64, int64_t* cols64,
+                      float* rhat, float* q, hipStream_t s) {
+  return sph::pair_geometry(a, xq, xg, hq, support, thresh, rows64, cols64, rhat, q, s);
+}
+}

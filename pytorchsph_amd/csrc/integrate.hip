@@ -1,0 +1,149 @@
+// Time integrators (simulationBase.py:564-569, 647-652, 679-705) and a
+// permutation gather.  All elementwise, one thread per particle.
+#include "sph_common.h"
+
+namespace sph {
+namespace {
+
+constexpr int kBS = 256;
+
+__global__ __launch_bounds__(kBS) void k_rk4(int stage, const int* __restrict__ pscene,
+                                             const SphPhysics* __restrict__ phys, int np,
+                                             const float* x0,
+                                             const float* v0,
+                                             const float* r0,
+                                             const float* __restrict__ vs,
+                                             const float* __restrict__ acc,
+                                             const float* __restrict__ dpdt,
+                                             float* __restrict__ K, float* xo,
+                                             float* vo, float* ro,
+                                             float* __restrict__ ao) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const SphPhysics P = phys[pscene[i]];
+  const float kax = acc[2 * i], kay = acc[2 * i + 1];
+  const float kvx = vs[2 * i], kvy = vs[2 * i + 1];
+  const float kr = dpdt ? dpdt[i] : 0.f;
+  float* k = K + 5 * (size_t)i;
+  const float w = (stage == 1 || stage == 4) ? 1.f : 2.f;
+  float s0, s1, s2, s3, s4;
+  if (stage == 1) {
+    s0 = kax; s1 = kay; s2 = kvx; s3 = kvy; s4 = kr;
+  } else {
+    s0 = k[0] + w * kax; s1 = k[1] + w * kay; s2 = k[2] + w * kvx; s3 = k[3] + w * kvy;
+    s4 = k[4] + w * kr;
+  }
+  if (stage < 4) {
+    k[0] = s0; k[1] = s1; k[2] = s2; k[3] = s3; k[4] = s4;
+    const float c = (stage == 3 ? 1.f : 0.5f) * P.dt;
+    vo[2 * i] = v0[2 * i] + c * kax;
+    vo[2 * i + 1] = v0[2 * i + 1] + c * kay;
+    xo[2 * i] = x0[2 * i] + c * kvx;
+    xo[2 * i + 1] = x0[2 * i + 1] + c * kvy;
+    if (dpdt) ro[i] = r0[i] + c * kr / P.rho0;
+    else if (ro != r0) ro[i] = r0[i];
+  } else {
+    const float c = (1.f / 6.f) * P.dt;
+    vo[2 * i] = v0[2 * i] + c * s0;
+    vo[2 * i + 1] = v0[2 * i + 1] + c * s1;
+    xo[2 * i] = x0[2 * i] + c * s2;
+    xo[2 * i + 1] = x0[2 * i + 1] + c * s3;
+    if (dpdt) ro[i] = r0[i] + c * s4 / P.rho0;
+    else if (ro != r0) ro[i] = r0[i];
+    if (ao) {
+      ao[2 * i] = s0;
+      ao[2 * i + 1] = s1;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBS) void k_semi(const int* __restrict__ pscene,
+                                              const SphPhysics* __restrict__ phys, int np,
+                                              const float* x0,
+                                              const float* v0,
+                                              const float* r0,
+                                              const float* vs,
+                                              const float* acc,
+                                              const float* dpdt,
+                                              float* xo, float* vo,
+                                              float* ro) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const SphPhysics P = phys[pscene[i]];
+  const float dt = P.dt;
+  const float ax = acc[2 * i], ay = acc[2 * i + 1];
+  const float dxx = vs[2 * i] + dt * ax, dxy = vs[2 * i + 1] + dt * ay;
+  vo[2 * i] = v0[2 * i] + dt * ax;
+  vo[2 * i + 1] = v0[2 * i + 1] + dt * ay;
+  xo[2 * i] = x0[2 * i] + dt * dxx;
+  xo[2 * i + 1] = x0[2 * i + 1] + dt * dxy;
+  if (ro) ro[i] = dpdt ? r0[i] + dt * dpdt[i] / P.rho0 : r0[i];
+}
+
+__global__ __launch_bounds__(kBS) void k_permute(const float* __restrict__ in,
+                                                 const int* __restrict__ perm, int n, int width,
+                                                 float* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * width) return;
+  const int k = (int)(t / width), c = (int)(t % width);
+  out[t] = in[(int64_t)perm[k] * width + c];
+}
+
+}  // namespace
+
+int integrate_rk4(int stage, const int* pscene, const SphPhysics* phys, int np, const float* x0,
+                  const float* v0, const float* rho0, const float* vs, const float* acc,
+                  const float* dpdt, float* K, float* xo, float* vo, float* ro, float* ao,
+                  hipStream_t s) {
+  SPH_REQUIRE(stage >= 1 && stage <= 4, "sph_integrate_rk4: stage must be 1..4");
+  SPH_REQUIRE(pscene && phys && x0 && v0 && vs && acc && K && xo && vo && np >= 0,
+              "sph_integrate_rk4: bad arguments");
+  SPH_REQUIRE(!dpdt || (rho0 && ro), "sph_integrate_rk4: density integration needs rho buffers");
+  if (np == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_rk4, dim3(blocks_for(np, kBS)), dim3(kBS), 0, s, stage, pscene, phys, np,
+                     x0, v0, rho0, vs, acc, dpdt, K, xo, vo, ro, ao);
+  return launch_check("sph_integrate_rk4");
+}
+
+int integrate_semi(const int* pscene, const SphPhysics* phys, int np, const float* x0,
+                   const float* v0, const float* rho0, const float* vs, const float* acc,
+                   const float* dpdt, float* xo, float* vo, float* ro, hipStream_t s) {
+  SPH_REQUIRE(pscene && phys && x0 && v0 && vs && acc && xo && vo && np >= 0,
+              "sph_integrate_semi_implicit: bad arguments");
+  SPH_REQUIRE(!ro || rho0, "sph_integrate_semi_implicit: rho_out needs rho0");
+  if (np == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_semi, dim3(blocks_for(np, kBS)), dim3(kBS), 0, s, pscene, phys, np, x0, v0,
+                     rho0, vs, acc, dpdt, xo, vo, ro);
+  return launch_check("sph_integrate_semi_implicit");
+}
+
+int permute_f32(const float* in, const int* perm, int n, int width, float* out, hipStream_t s) {
+  SPH_REQUIRE(in && perm && out && n >= 0 && width > 0, "sph_permute_f32: bad arguments");
+  if (n == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_permute, dim3(blocks_for((int64_t)n * width, kBS)), dim3(kBS), 0, s, in,
+                     perm, n, width, out);
+  return launch_check("sph_permute_f32");
+}
+
+}  // namespace sph
+
+extern "C" {
+int sph_integrate_rk4(int stage, const int* pscene, const SphPhysics* phys, int np,
+                      const float* x0, const float* v0, const float* rho0, const float* v_stage,
+                      const float* acc, const float* dpdt, float* kacc, float* x_out,
+                      float* v_out, float* rho_out, float* acc_out, hipStream_t s) {
+  return sph::integrate_rk4(stage, pscene, phys, np, x0, v0, rho0, v_stage, acc, dpdt, kacc,
+                            x_out, v_out, rho_out, acc_out, s);
+}
+int sph_integrate_semi_implicit(const int* pscene, const SphPhysics* phys, int np,
+                                const float* x0, const float* v0, const float* rho0,
+                                const float* v_stage, const float* acc, const float* dpdt,
+                                float* x_out, float* v_out, float* rho_out, hipStream_t s) {
+  return sph::integrate_semi(pscene, phys, np, x0, v0, rho0, v_stage, acc, dpdt, x_out, v_out,
+                             rho_out, s);
+}
+int sph_permute_f32(const float* in, const int* perm, int n, int width, float* out,
+                    hipStream_t s) {
+  return sph::permute_f32(in, perm, n, width, out, s);
+}
+}

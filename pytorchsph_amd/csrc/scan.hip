@@ -1,0 +1,142 @@
+// Exclusive prefix sums and fills.  Three launches: per-tile sums, a
+// single-workgroup scan of the tile sums, and the tile-local scans.
+#include "sph_common.h"
+
+#include <stdio.h>
+
+namespace sph {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int hip_fail(const char* where) {
+  hipError_t e = hipGetLastError();
+  char buf[256];
+  snprintf(buf, sizeof(buf), "%s: %s", where, hipGetErrorString(e));
+  g_last_error = buf;
+  return SPH_E_HIP;
+}
+
+int launch_check(const char* where) {
+  hipError_t e = hipPeekAtLastError();
+  if (e != hipSuccess) return hip_fail(where);
+  return SPH_OK;
+}
+
+namespace {
+constexpr int kScanBS = 256;
+constexpr int kScanItems = 8;
+constexpr int kTile = kScanBS * kScanItems;
+
+__global__ __launch_bounds__(kScanBS) void k_scan_reduce(const int* __restrict__ in, int n,
+                                                         int* __restrict__ partial) {
+  const int base = blockIdx.x * kTile + threadIdx.x * kScanItems;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int i = base + k;
+    if (i < n) s += in[i];
+  }
+  s = wave_sum(s);
+  __shared__ int lds[kScanBS / kWave];
+  if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < kScanBS / kWave; ++w) t += lds[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_partials(int* __restrict__ partial, int nb,
+                                                        int* __restrict__ out, int n) {
+  __shared__ int lds[1024 / kWave + 1];
+  int carry = 0;
+  for (int base = 0; base < nb; base += blockDim.x) {
+    int i = base + threadIdx.x;
+    int v = i < nb ? partial[i] : 0;
+    int tot;
+    int ex = block_exclusive_scan(v, lds, &tot);
+    if (i < nb) partial[i] = ex + carry;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ __launch_bounds__(kScanBS) void k_scan_apply(const int* __restrict__ in, int n,
+                                                        const int* __restrict__ partial,
+                                                        int* __restrict__ out) {
+  __shared__ int lds[kScanBS / kWave + 1];
+  const int base = blockIdx.x * kTile + threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int i = base + k;
+    v[k] = i < n ? in[i] : 0;
+    s += v[k];
+  }
+  int tot;
+  int ex = block_exclusive_scan(s, lds, &tot) + partial[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int i = base + k;
+    if (i < n) out[i] = ex;
+    ex += v[k];
+  }
+}
+
+template <typename T>
+__global__ void k_fill(T* p, int n, T v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+}  // namespace
+
+int scan_exclusive(const int* in, int n, int* out, void* ws, size_t ws_bytes, hipStream_t s) {
+  SPH_REQUIRE(n >= 0 && out, "sph_scan_exclusive: bad size or output");
+  if (n == 0) {
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, out, 1, 0);
+    return launch_check("sph_scan_exclusive(fill)");
+  }
+  const int nb = blocks_for(n, kTile);
+  if (ws_bytes < sizeof(int) * (size_t)(nb + 1) || !ws) {
+    set_error("sph_scan_exclusive: workspace too small");
+    return SPH_E_SPACE;
+  }
+  int* partial = (int*)ws;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBS), 0, s, in, n, partial);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, partial, nb, out, n);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBS), 0, s, in, n, partial, out);
+  return launch_check("sph_scan_exclusive");
+}
+
+size_t scan_ws_bytes(int n) { return sizeof(int) * (size_t)(blocks_for(n < 1 ? 1 : n, kTile) + 1); }
+
+int fill_i32(int* p, int n, int v, hipStream_t s) {
+  if (n <= 0) return SPH_OK;
+  hipLaunchKernelGGL(k_fill<int>, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
+  return launch_check("sph_fill_i32");
+}
+int fill_f32(float* p, int n, float v, hipStream_t s) {
+  if (n <= 0) return SPH_OK;
+  hipLaunchKernelGGL(k_fill<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
+  return launch_check("sph_fill_f32");
+}
+
+}  // namespace sph
+
+extern "C" {
+const char* sph_last_error(void) { return sph::g_last_error.c_str(); }
+int sph_version(void) { return 1; }
+size_t sph_scan_workspace_bytes(int n) { return sph::scan_ws_bytes(n); }
+int sph_scan_exclusive(const int* in, int n, int* out, void* ws, size_t ws_bytes,
+                       hipStream_t s) {
+  return sph::scan_exclusive(in, n, out, ws, ws_bytes, s);
+}
+int sph_fill_i32(int* p, int n, int value, hipStream_t s) { return sph::fill_i32(p, n, value, s); }
+int sph_fill_f32(float* p, int n, float value, hipStream_t s) {
+  return sph::fill_f32(p, n, value, s);
+}
+}

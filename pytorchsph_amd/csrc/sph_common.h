@@ -1,0 +1,172 @@
+// Shared device helpers of the gfx950 SPH hot path: error plumbing, wave64
+// scans, the canonical fp32 neighbour predicate and the SPH kernel functions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include "sph_mi355x.h"
+
+namespace sph {
+
+void set_error(const std::string& msg);
+int hip_fail(const char* where);
+// Checks the launch that was just issued.
+int launch_check(const char* where);
+
+constexpr int kWave = 64;
+
+inline int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+int scan_exclusive(const int* in, int n, int* out, void* ws, size_t ws_bytes, hipStream_t s);
+size_t scan_ws_bytes(int n);
+int fill_i32(int* p, int n, int v, hipStream_t s);
+int fill_f32(float* p, int n, float v, hipStream_t s);
+
+#define SPH_REQUIRE(cond, msg)            \
+  do {                                    \
+    if (!(cond)) {                        \
+      ::sph::set_error(msg);              \
+      return SPH_E_ARG;                   \
+    }                                     \
+  } while (0)
+
+// ---------------------------------------------------------------- predicate
+// Cell coordinate of a point: ceil(fl(fl(x - qmin) / hmax)) converted to
+// int32, exactly the reference formula (neighborSearch.py:46,
+// neighSearch_cuda.cu:122-123).  fp32 subtraction and division are IEEE
+// correctly rounded under hipcc's defaults (no fast-math).
+__device__ __forceinline__ int cell_coord(float x, float qmin, float hmax) {
+  return (int)ceilf((x - qmin) / hmax);
+}
+
+// Squared distance fl(fl(dx*dx) + fl(dy*dy)) with no FMA contraction.
+__device__ __forceinline__ float dist2_nofma(float xq, float yq, float xj, float yj) {
+#pragma clang fp contract(off)
+  const float dx = xq - xj;
+  const float dy = yq - yj;
+  return dx * dx + dy * dy;
+}
+
+// Canonical neighbour predicate of the reference, fl(sqrt(s)) < h with a
+// correctly rounded sqrt (neighSearch.cpp:222-223), evaluated without any
+// sqrt: for every float s >= 0,  fl(sqrt(s)) < h  <=>  s <= sq_threshold(h).
+// Rounding is monotone, so fl(sqrt(s)) < h iff sqrt(s) lies below the
+// midpoint m of h and its predecessor hp (or on it, when round-half-even
+// picks hp).  m has at most 25 significant bits, so m*m is exact in FP64 and
+// the comparison against the float s is exact.
+__device__ __forceinline__ float sq_threshold(float h) {
+  if (!(h > 0.f)) return -1.f;
+  if (isinf(h)) return INFINITY;
+  const float hp = __uint_as_float(__float_as_uint(h) - 1u);
+  const double m = ((double)hp + (double)h) * 0.5;
+  const double m2 = m * m;
+  float f = (float)m2;
+  if ((double)f > m2) f = __uint_as_float(__float_as_uint(f) - 1u);
+  // s == m*m: sqrt(s) is the midpoint; it rounds to hp (a pair) only when
+  // hp's last significand bit is even.
+  if ((double)f == m2 && (__float_as_uint(hp) & 1u)) f = __uint_as_float(__float_as_uint(f) - 1u);
+  return f;
+}
+
+// ----------------------------------------------------------- SPH kernels
+// Wendland C2 in 2D (kernels.py:21-29): W = 7/pi (1-q)^4 (1+4q) / h^2,
+// grad W = -rhat 7/pi / h^3 20 q (1-q)^3.  Spiky gradient (kernels.py:32-33):
+// -rhat 30/pi / h^3 (1-q)^2.
+constexpr float kC7pi = 2.228169203286535f;   // 7 / pi
+constexpr float kC30pi = 9.549296585513721f;  // 30 / pi
+
+__device__ __forceinline__ float wendland(float q, float h) {
+  float a = 1.f - q;
+  float a2 = a * a;
+  return a2 * a2 * (1.f + 4.f * q) * kC7pi / (h * h);
+}
+// magnitude factor g with grad W = -g * rhat
+__device__ __forceinline__ float wendland_grad(float q, float h) {
+  float a = 1.f - q;
+  return kC7pi / (h * h * h) * (20.f * q * a * a * a);
+}
+__device__ __forceinline__ float spiky_grad(float q, float h) {
+  float a = 1.f - q;
+  return kC30pi / (h * h * h) * (a * a);
+}
+
+// Pair geometry as the reference builds it (neighborSearch.py:218-223):
+// r = x_i - x_j, d = |r|, rhat = r / d (zero when d < 1e-7), q = d / hdiv.
+struct PairGeo {
+  float rx, ry;    // x_i - x_j
+  float d;         // |x_i - x_j|
+  float hx, hy;    // rhat
+  float q;
+};
+__device__ __forceinline__ PairGeo pair_geo(float xi, float yi, float xj, float yj, float hdiv) {
+  PairGeo g;
+  g.rx = xi - xj;
+  g.ry = yi - yj;
+  g.d = sqrtf(g.rx * g.rx + g.ry * g.ry);
+  if (g.d >= 1e-7f) {
+    float inv = 1.f / g.d;
+    g.hx = g.rx * inv;
+    g.hy = g.ry * inv;
+  } else {
+    g.hx = 0.f;
+    g.hy = 0.f;
+  }
+  g.q = g.d / hdiv;
+  return g;
+}
+
+// Equation of state (pressure.py:99-111).
+__device__ __forceinline__ float eos(const SphPhysics& P, float rhohat) {
+  if (P.eos == 0) {
+    float r2 = rhohat * rhohat;
+    float r4 = r2 * r2;
+    float r7 = r4 * r2 * rhohat;
+    // gamma is 7 in every shipped configuration; other exponents use powf.
+    float rg = (P.gamma == 7.f) ? r7 : powf(rhohat, P.gamma);
+    return P.rho0 * P.c0 * P.c0 / P.gamma * (rg - 1.f);
+  }
+  if (P.eos == 1) return P.c0 * P.c0 * (rhohat * P.rho0 - P.rho0);
+  return P.kappa * (rhohat * P.rho0 - P.rho0);
+}
+
+// --------------------------------------------------------------- wave scans
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    int t = __shfl_up(v, o, kWave);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Block-wide exclusive scan (blockDim.x <= 1024, multiple of 64).  `lds`
+// needs blockDim.x / 64 + 1 ints.  Returns the exclusive prefix, total in *tot.
+__device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int* tot) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  const int nw = blockDim.x / kWave;
+  int inc = wave_inclusive_scan(v);
+  if (lane == kWave - 1) lds[wid] = inc;
+  __syncthreads();
+  if (wid == 0) {
+    int w = lane < nw ? lds[lane] : 0;
+    int wi = wave_inclusive_scan(w);
+    if (lane < nw) lds[lane] = wi - w;
+    if (lane == nw - 1) lds[nw] = wi;
+  }
+  __syncthreads();
+  int res = inc - v + lds[wid];
+  *tot = lds[nw];
+  __syncthreads();
+  return res;
+}
+
+}  // namespace sph

@@ -1,0 +1,371 @@
+"""Device-resident batched SPH engine on the C-ABI library.
+
+`SceneBatch` holds many independent scenes of one scheme in flat device
+arrays (scene s owns fluid particles [p_begin, p_begin + n_s) and boundary
+particles [b_begin, b_begin + nb_s)).  One step advances every scene:
+  delta-SPH  RK4, four right-hand sides (simulationBase.py:679-705), each
+             = grid + fluid pairs + ghost/boundary pairs, mDBC boundary
+             density, one fused fluid gather (deltaSPH.py:146-200);
+  DFSPH      semi-implicit Euler (simulationBase.py:647-652) around one
+             right-hand side: summation density with the Akinci boundary,
+             the relaxed-Jacobi density solve, viscosity terms
+             (dfsph.py:134-202).
+All buffers are allocated up front with fixed pair capacities, so a step
+launches kernels only and can be captured into a hipGraph.  The single
+plain scene is nscenes == 1.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .scenes import Scene
+from .search import CSR, NeighborGrid, scan_ws_bytes
+
+DSPH_TERMS_MOG = (L.T_CONT | L.T_CONT_B | L.T_MOG | L.T_PRESS | L.T_PRESS_B | L.T_VISC |
+                  L.T_VISC_B | L.T_LAM | L.T_GRAV)
+DSPH_TERMS_DD = (DSPH_TERMS_MOG & ~L.T_MOG) | L.T_DDSPH
+
+
+def phys_records(scenes) -> np.ndarray:
+    rec = np.zeros(len(scenes), dtype=L.PHYS_DTYPE)
+    for s, sc in enumerate(scenes):
+        p = sc.phys
+        for k in L.PHYS_DTYPE.names:
+            if k == "pad":
+                continue
+            if k in p:
+                rec[s][k] = p[k]
+    return rec
+
+
+class _PairBuf:
+    """Fixed-capacity CSR for nrows query rows."""
+
+    def __init__(self, nrows: int, cap: int, dev):
+        self.nrows = nrows
+        self.counts = torch.zeros(max(nrows, 1), dtype=torch.int32, device=dev)
+        self.rowptr = torch.zeros(nrows + 1, dtype=torch.int32, device=dev)
+        self.col = torch.zeros(max(cap, 1), dtype=torch.int32, device=dev)
+        self.cap = max(cap, 1)
+
+    def csr(self) -> CSR:
+        return CSR(self.rowptr, self.col, self.nrows)
+
+
+class SceneBatch:
+    """Many scenes of one scheme ("dsph" or "dfsph"), resident on one GPU."""
+
+    def __init__(self, scenes: list[Scene], device="cuda", pair_cap: float = 40.0,
+                 bdy_pair_cap: float = 40.0, dfsph_chunk: int = 8):
+        assert scenes, "empty batch"
+        scheme = scenes[0].scheme
+        assert all(s.scheme == scheme for s in scenes), "one scheme per batch"
+        self.scheme = scheme
+        self.scenes = scenes
+        self.dev = dev = torch.device(device)
+        self.nscenes = len(scenes)
+        self.n = np.array([s.n for s in scenes])
+        self.nb = np.array([s.nb for s in scenes])
+        self.p_begin = np.concatenate([[0], np.cumsum(self.n)])[:-1]
+        self.b_begin = np.concatenate([[0], np.cumsum(self.nb)])[:-1]
+        NP, NB = int(self.n.sum()), int(self.nb.sum())
+        self.NP, self.NB = NP, NB
+        self.solid = scenes[0].boundary == "solid"
+        self.dfsph_chunk = dfsph_chunk
+        f32 = torch.float32
+
+        def cat(arrs, shape_tail=()):
+            if not arrs:
+                return torch.zeros((1,) + shape_tail, dtype=f32, device=dev)
+            return torch.from_numpy(np.ascontiguousarray(np.concatenate(arrs))).to(dev)
+
+        # ---- fluid state (perennial)
+        self.x = cat([s.x for s in scenes]).float().contiguous()
+        self.v = cat([s.v for s in scenes]).float().contiguous()
+        self.rho = torch.ones(NP, dtype=f32, device=dev)
+        self.area = cat([np.full(s.n, s.area, np.float32) for s in scenes])
+        self.rest = cat([np.full(s.n, s.rho0, np.float32) for s in scenes])
+        self.h = cat([np.full(s.n, s.support, np.float32) for s in scenes])
+        self.pscene = torch.from_numpy(np.repeat(np.arange(self.nscenes, dtype=np.int32),
+                                                 self.n)).to(dev)
+        self.uid = torch.arange(NP, dtype=torch.int64, device=dev)
+        # ---- boundary state
+        self.bx = cat([s.bx for s in scenes]).float().contiguous()
+        self.bgx = cat([s.bgx for s in scenes]).float().contiguous() if self.solid else None
+        self.bvol = cat([s.bvol for s in scenes])
+        self.brest = cat([np.full(s.nb, s.rho0, np.float32) for s in scenes])
+        self.bh = cat([np.full(s.nb, s.support, np.float32) for s in scenes])
+        self.bterm = None if self.solid else cat([s.bterm for s in scenes])
+        self.bscene = torch.from_numpy(np.repeat(np.arange(self.nscenes, dtype=np.int32),
+                                                 self.nb)).to(dev)
+        self.brho = torch.ones(max(NB, 1), dtype=f32, device=dev)
+        self.bp = torch.zeros(max(NB, 1), dtype=f32, device=dev)
+        self.bcount = torch.zeros(max(NB, 1), dtype=torch.int32, device=dev)
+        # ---- per-scene constants
+        self.phys_host = phys_records(scenes)
+        self.phys = L.records_to_tensor(self.phys_host, dev)
+        d0 = scenes[0].phys
+        if scheme == "dsph":
+            self.terms = DSPH_TERMS_DD if d0.get("diffusion") == "deltaSPH" else DSPH_TERMS_MOG
+            if not d0.get("boundary_diffusion", True):
+                self.terms &= ~L.T_VISC_B
+        # ---- neighbour search
+        grid_scenes = [dict(vmin=s.vmin, vmax=s.vmax, hmax=s.support, p_begin=int(pb),
+                            p_count=int(s.n)) for s, pb in zip(scenes, self.p_begin)]
+        self.grid = NeighborGrid(grid_scenes, NP, dev)
+        self.ff = _PairBuf(NP, int(math.ceil(pair_cap * NP)), dev)
+        bcap = int(math.ceil(bdy_pair_cap * max(NB, 1)))
+        self.bf = _PairBuf(NB, bcap, dev)
+        self.gf = _PairBuf(NB, bcap, dev) if self.solid else None
+        self.fb = _PairBuf(NP, bcap, dev)
+        self.t_slot = torch.zeros(bcap, dtype=torch.int32, device=dev)
+        self.scan_ws = torch.zeros(scan_ws_bytes(max(NP, NB, 1)), dtype=torch.uint8, device=dev)
+        # ---- step buffers
+        self.acc = torch.zeros((NP, 2), dtype=f32, device=dev)
+        self.dpdt = torch.zeros(NP, dtype=f32, device=dev)
+        if scheme == "dsph":
+            self.S = [(torch.empty_like(self.x), torch.empty_like(self.v), torch.empty_like(self.rho))
+                      for _ in range(2)]
+            self.K = torch.zeros((NP, 5), dtype=f32, device=dev)
+            self.gradrho = torch.zeros((NP, 2), dtype=f32, device=dev)
+            self.Lmat = torch.zeros((NP, 4), dtype=f32, device=dev)
+        else:
+            self.vol = torch.zeros(NP, dtype=f32, device=dev)
+            self._alloc_dfsph()
+        self.steps_done = 0
+        self.graph = None
+
+    # ----------------------------------------------------------- helpers
+    def _alloc_dfsph(self):
+        dev, NP, NB, S = self.dev, self.NP, self.NB, self.nscenes
+        z = lambda *s, dt=torch.float32: torch.zeros(*s, dtype=dt, device=dev)  # noqa: E731
+        self.max_hist = 260
+        self.df = dict(
+            vpred=z(NP, 2), aact=z(NP), alpha=z(NP), src=z(NP), p=z(NP), p2=z(NP), apred=z(NP, 2),
+            resid=z(NP), b_aact=z(max(NB, 1)), b_alpha=z(max(NB, 1)), b_src=z(max(NB, 1)),
+            b_p=z(max(NB, 1)), b_resid=z(max(NB, 1)), err_acc=z(S, dt=torch.float64),
+            err_hist=z(S, self.max_hist, dt=torch.float64), iters=z(S, dt=torch.int32),
+            active=z(S, dt=torch.int32), any_active=z(1, dt=torch.int32),
+            status=z(S, dt=torch.int32),
+            nfluid=torch.from_numpy(self.n.astype(np.int32)).to(dev),
+            nbdy=torch.from_numpy(self.nb.astype(np.int32)).to(dev))
+        self.any_active_host = torch.zeros(1, dtype=torch.int32, pin_memory=True)
+
+    def _dfsph_struct(self) -> L.SphDfsph:
+        d = self.df
+        names = ("vpred", "aact", "alpha", "src", "p", "p2", "apred", "resid", "b_aact",
+                 "b_alpha", "b_src", "b_p", "b_resid", "err_acc", "err_hist", "iters", "active",
+                 "any_active", "status", "nfluid", "nbdy")
+        return L.SphDfsph(*[L.ptr(d[k]) for k in names], self.nscenes, self.max_hist)
+
+    def fluid(self, x, v, rho, vol=None) -> L.SphFluid:
+        return L.SphFluid(L.ptr(x), L.ptr(v), L.ptr(rho), L.ptr(self.area), L.ptr(self.rest),
+                          L.ptr(vol), L.ptr(self.pscene), self.NP)
+
+    def boundary(self) -> L.SphBoundary:
+        return L.SphBoundary(L.ptr(self.bx), None, L.ptr(self.brho), L.ptr(self.bvol),
+                             L.ptr(self.bp), L.ptr(self.brest), L.ptr(self.bscene), self.NB)
+
+    def _pairs(self, buf: _PairBuf, q: L.SphQuery, s):
+        g = self.grid
+        g.count(q, buf.counts, s)
+        L.call("sph_scan_exclusive", L.ptr(buf.counts), buf.nrows, L.ptr(buf.rowptr),
+               L.ptr(self.scan_ws), int(self.scan_ws.numel()), L.stream_handle(s))
+        g.fill(q, buf.rowptr, buf.col, s)
+
+    def search(self, x, s=None):
+        """Grid + every pair list of the current positions."""
+        g = self.grid
+        g.build(x, self.pscene, s)
+        self._pairs(self.ff, g.query_struct(x, self.h, self.pscene, 1, True), s)
+        if self.NB:
+            if self.solid:
+                self._pairs(self.gf, g.query_struct(self.bgx, self.bh, self.bscene, 1, False), s)
+            self._pairs(self.bf, g.query_struct(self.bx, self.bh, self.bscene, 1, False), s)
+            c = self.bf.csr().c()
+            L.call("sph_csr_transpose", C.byref(c), self.NP, L.ptr(self.fb.counts),
+                   L.ptr(self.t_slot), L.ptr(self.fb.rowptr), L.ptr(self.fb.col),
+                   L.ptr(self.scan_ws), int(self.scan_ws.numel()), L.stream_handle(s))
+
+    def _order(self):
+        return L.ptr(self.grid.sorted_idx)
+
+    # ---------------------------------------------------------- delta-SPH
+    def dsph_rhs(self, x, v, rho, s=None):
+        """dpdt / acc of the state (x, v, rho) into self.dpdt / self.acc."""
+        self.search(x, s)
+        F = self.fluid(x, v, rho)
+        sh = L.stream_handle(s)
+        ff = self.ff.csr().c()
+        fb = self.fb.csr().c() if self.NB else None
+        if self.NB:
+            gf = self.gf.csr().c()
+            L.call("sph_mdbc_density", C.byref(F), L.ptr(self.phys), L.ptr(self.bx),
+                   L.ptr(self.bgx), L.ptr(self.bscene), self.NB, C.byref(gf), L.ptr(self.brho),
+                   L.ptr(self.bp), L.ptr(self.bcount), sh)
+        B = self.boundary()
+        fbp = C.byref(fb) if fb is not None else None
+        if self.terms & L.T_DDSPH:
+            L.call("sph_dsph_normalization", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B),
+                   fbp, self._order(), None, L.ptr(self.Lmat), None, sh)
+            L.call("sph_dsph_density_gradient", C.byref(F), L.ptr(self.phys), C.byref(ff),
+                   C.byref(B), fbp, L.ptr(self.Lmat), self._order(), L.ptr(self.gradrho), sh)
+        L.call("sph_dsph_gather", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B), fbp,
+               L.ptr(self.gradrho) if self.terms & L.T_DDSPH else None, self.terms, self._order(),
+               L.ptr(self.dpdt), L.ptr(self.acc), sh)
+
+    def _dsph_step(self, s=None):
+        P = (self.x, self.v, self.rho)
+        S0, S1 = self.S
+        stages = [(P, S0), (S0, S1), (S1, S0), (S0, P)]
+        sh = L.stream_handle(s)
+        for k, (cur, out) in enumerate(stages, start=1):
+            self.dsph_rhs(*cur, s=s)
+            L.call("sph_integrate_rk4", k, L.ptr(self.pscene), L.ptr(self.phys), self.NP,
+                   L.ptr(self.x), L.ptr(self.v), L.ptr(self.rho), L.ptr(cur[1]), L.ptr(self.acc),
+                   L.ptr(self.dpdt), L.ptr(self.K), L.ptr(out[0]), L.ptr(out[1]), L.ptr(out[2]),
+                   L.ptr(self.acc) if k == 4 else None, sh)
+
+    # -------------------------------------------------------------- DFSPH
+    def _dfsph_pre(self, s=None):
+        sh = L.stream_handle(s)
+        self.search(self.x, s)
+        F = self.fluid(self.x, self.v, self.rho)
+        ff = self.ff.csr().c()
+        fb = self.fb.csr().c() if self.NB else None
+        B = self.boundary()
+        L.call("sph_fluid_density", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B),
+               C.byref(fb) if fb is not None else None, self._order(), L.ptr(self.rho),
+               L.ptr(self.vol), sh)
+        if self.NB:
+            bf = self.bf.csr().c()
+            L.call("sph_akinci_boundary_density", C.byref(F), L.ptr(self.phys), L.ptr(self.bx),
+                   L.ptr(self.bscene), self.NB, L.ptr(self.bterm), C.byref(bf), L.ptr(self.brho), sh)
+        L.call("sph_dsph_gather", C.byref(F), L.ptr(self.phys), C.byref(ff), None, None, None,
+               L.T_GRAV, None, None, L.ptr(self.acc), sh)
+        D = self._dfsph_struct()
+        L.call("sph_dfsph_prepare", C.byref(F), L.ptr(self.acc), L.ptr(self.phys), C.byref(B),
+               C.byref(D), sh)
+        bfc = self.bf.csr().c()
+        L.call("sph_dfsph_alpha_source", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B),
+               C.byref(fb) if fb is not None else None, C.byref(bfc), self._order(), C.byref(D), sh)
+
+    def _dfsph_iters(self, k, s=None):
+        sh = L.stream_handle(s)
+        F = self.fluid(self.x, self.v, self.rho)
+        ff = self.ff.csr().c()
+        fb = self.fb.csr().c() if self.NB else None
+        bfc = self.bf.csr().c()
+        B = self.boundary()
+        D = self._dfsph_struct()
+        for _ in range(k):
+            L.call("sph_dfsph_iterate", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B),
+                   C.byref(fb) if fb is not None else None, C.byref(bfc), self._order(),
+                   C.byref(D), sh)
+
+    def _dfsph_post(self, s=None):
+        sh = L.stream_handle(s)
+        ff = self.ff.csr().c()
+        fb = self.fb.csr().c() if self.NB else None
+        B = self.boundary()
+        F = self.fluid(self.x, self.v, self.rho)
+        D = self._dfsph_struct()
+        L.call("sph_dfsph_finish", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B),
+               C.byref(fb) if fb is not None else None, self._order(), C.byref(D),
+               L.ptr(self.acc), sh)
+        # velocity diffusion + laminar with V = A / fluid-only density (density.py:53)
+        Fv = self.fluid(self.x, self.v, self.rho, self.vol)
+        L.call("sph_dsph_gather", C.byref(Fv), L.ptr(self.phys), C.byref(ff), None, None, None,
+               L.T_VISC | L.T_LAM | L.T_ACCUM, self._order(), None, L.ptr(self.acc), sh)
+        L.call("sph_integrate_semi_implicit", L.ptr(self.pscene), L.ptr(self.phys), self.NP,
+               L.ptr(self.x), L.ptr(self.v), None, L.ptr(self.v), L.ptr(self.acc), None,
+               L.ptr(self.x), L.ptr(self.v), None, sh)
+
+    def _dfsph_loop(self, s=None, graphs=None):
+        """Iterates in chunks until every scene has converged."""
+        stream = s if s is not None else torch.cuda.current_stream()
+        while True:
+            if graphs is not None:
+                graphs.replay()
+            else:
+                self._dfsph_iters(self.dfsph_chunk, s)
+            self.any_active_host.copy_(self.df["any_active"], non_blocking=True)
+            stream.synchronize()
+            if int(self.any_active_host[0]) == 0:
+                break
+
+    # ----------------------------------------------------------- stepping
+    def step(self, s=None):
+        """One time step of every scene (eager launches)."""
+        if self.scheme == "dsph":
+            self._dsph_step(s)
+        else:
+            self._dfsph_pre(s)
+            self._dfsph_loop(s)
+            self._dfsph_post(s)
+        self.steps_done += 1
+
+    def capture(self):
+        """Captures the step into hipGraphs (delta-SPH: one graph per step;
+        DFSPH: prologue, a chunk of iterations, epilogue)."""
+        torch.cuda.synchronize()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            # warm-up launch outside capture
+            self.step(s)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        if self.scheme == "dsph":
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._dsph_step()
+            self.graph = (g,)
+        else:
+            g1, g2, g3 = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g1):
+                self._dfsph_pre()
+            with torch.cuda.graph(g2):
+                self._dfsph_iters(self.dfsph_chunk)
+            with torch.cuda.graph(g3):
+                self._dfsph_post()
+            self.graph = (g1, g2, g3)
+        torch.cuda.synchronize()
+
+    def step_graph(self):
+        """One step by graph replay (capture() first)."""
+        if self.scheme == "dsph":
+            self.graph[0].replay()
+        else:
+            g1, g2, g3 = self.graph
+            g1.replay()
+            self._dfsph_loop(graphs=g2)
+            g3.replay()
+        self.steps_done += 1
+
+    # ------------------------------------------------------------ queries
+    def status(self) -> np.ndarray:
+        st = self.grid.status.cpu().numpy()
+        if self.scheme == "dfsph":
+            st = st | self.df["status"].cpu().numpy()
+        return st
+
+    def check(self):
+        st = self.status()
+        if (st & L.ST_OVERFLOW).any():
+            raise RuntimeError("pair-list capacity exceeded; rebuild the batch with a larger cap")
+        nnz = int(self.ff.rowptr[self.NP].item())
+        return st, nnz
+
+    def scene_state(self, k: int) -> dict:
+        a, b = int(self.p_begin[k]), int(self.p_begin[k] + self.n[k])
+        return dict(x=self.x[a:b].cpu().numpy(), v=self.v[a:b].cpu().numpy(),
+                    rho=self.rho[a:b].cpu().numpy())
+
+    def dfsph_iterations(self) -> np.ndarray:
+        return self.df["iters"].cpu().numpy()

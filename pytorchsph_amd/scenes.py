@@ -1,0 +1,236 @@
+"""Synthetic scenes for the hot path (dam breaks, colliding drops, point clouds).
+
+Scene initialisation is outside the hot path; these generators reproduce the
+reference's geometric contract rather than its init code:
+  - fluid on a lattice x = (i + 1/2) dx with area A = dx^2, radius
+    r = sqrt(A / pi) and support h = sqrt(A / pi * targetNeighbors) in fp32
+    (simulationBase.py:854-855);
+  - solid (mDBC) walls: `layers` rows of boundary particles outside each wall
+    at (k + 1/2) dx with ghosts mirrored across the wall
+    (solidBoundary.py:139-161, ghostParticles.py:128-133), boundary volume
+    pi r^2 (solidBoundary.py:196-197);
+  - Akinci walls: one row at dx / 2 outside, volume gamma / sum_b' W_bb' and
+    density term sum_b' W_bb' V_b' (akinciBoundary.py:137-157), computed with
+    this package's own predicate |x_b - x_b'| < h instead of torch_cluster;
+  - search domain = tank +/- 2 h (simulationBase.py:968-969).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+G = 9.81
+
+
+@dataclass
+class Scene:
+    name: str
+    scheme: str                 # "dsph" (delta-SPH, RK4) or "dfsph" (semi-implicit)
+    boundary: str               # "solid" (mDBC) or "akinci"
+    x: np.ndarray               # [n, 2] fp32
+    v: np.ndarray               # [n, 2] fp32
+    area: float
+    support: float              # h (fp32 value)
+    rho0: float
+    bx: np.ndarray              # [nb, 2] fp32 boundary positions
+    bgx: np.ndarray | None      # [nb, 2] ghost positions (solid)
+    bvol: np.ndarray            # [nb] boundary volume
+    bterm: np.ndarray | None    # [nb] akinci density term
+    vmin: tuple                 # search domain
+    vmax: tuple
+    phys: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return len(self.x)
+
+    @property
+    def nb(self) -> int:
+        return len(self.bx)
+
+
+def support_for(dx: float, target_neighbors: int = 20) -> np.float32:
+    area = dx * dx
+    return np.float32(math.sqrt(area / math.pi * target_neighbors))
+
+
+def default_phys(scheme: str, h: float, rho0: float = 1000.0) -> dict:
+    """Solver constants of the shipped dam-break configs
+    (configs/dambreak_deltasph.toml, configs/dambreak_dfsph.toml)."""
+    if scheme == "dsph":
+        c0 = 10.0 * math.sqrt(2.0 * G * 0.3)   # momentum.py:62-66
+        dt = 0.4 * h / c0                       # CFL 0.4, fixed
+        return dict(h=h, rho0=rho0, c0=c0, gamma=7.0, kappa=1.3, alpha=0.01, delta=0.1,
+                    nu=0.001, eps=0.1 * h * h, gx=0.0, gy=-G, dt=dt, beta=0.125, omega=0.5,
+                    tau=1e-3, fluid_omega=0.3, min_iters=2, max_iters=256, eos=0,
+                    diffusion="MOG", boundary_diffusion=True)
+    c0 = 100.0
+    return dict(h=h, rho0=rho0, c0=c0, gamma=7.0, kappa=1.3, alpha=0.01, delta=0.1, nu=0.01,
+                eps=0.1 * h * h, gx=0.0, gy=-G, dt=0.04 * h, beta=0.125, omega=0.5, tau=1e-3,
+                fluid_omega=0.3, min_iters=2, max_iters=256, eos=0, diffusion="none",
+                boundary_diffusion=False)
+
+
+def _jitter(rng, pts, dx, amp=0.01):
+    return pts + rng.uniform(-amp * dx, amp * dx, size=pts.shape)
+
+
+def _walls_solid(W, H, dx, layers=3):
+    nx, ny = int(round(W / dx)), int(round(H / dx))
+    i = np.arange(-layers, nx + layers)
+    j = np.arange(-layers, ny + layers)
+    I, J = np.meshgrid(i, j, indexing="xy")
+    I, J = I.ravel(), J.ravel()
+    outside = (I < 0) | (I >= nx) | (J < 0) | (J >= ny)
+    bx = np.stack([(I[outside] + 0.5) * dx, (J[outside] + 0.5) * dx], axis=1)
+    gx = bx.copy()
+    lo = bx[:, 0] < 0
+    hi = bx[:, 0] > W
+    gx[lo, 0] = -bx[lo, 0]
+    gx[hi, 0] = 2 * W - bx[hi, 0]
+    lo = bx[:, 1] < 0
+    hi = bx[:, 1] > H
+    gx[lo, 1] = -bx[lo, 1]
+    gx[hi, 1] = 2 * H - bx[hi, 1]
+    return bx, gx
+
+
+def _walls_akinci(W, H, dx):
+    nx, ny = int(round(W / dx)), int(round(H / dx))
+    off = 0.5 * dx
+    xs = (np.arange(-1, nx + 1) + 0.5) * dx
+    ys = (np.arange(0, ny) + 0.5) * dx
+    bottom = np.stack([xs, np.full_like(xs, -off)], 1)
+    top = np.stack([xs, np.full_like(xs, H + off)], 1)
+    left = np.stack([np.full_like(ys, -off), ys], 1)
+    right = np.stack([np.full_like(ys, W + off), ys], 1)
+    return np.concatenate([bottom, top, left, right])
+
+
+def _wendland_np(q, h):
+    a = 1.0 - q
+    return a ** 4 * (1.0 + 4.0 * q) * (7.0 / math.pi) / (h * h)
+
+
+def akinci_volumes(bx: np.ndarray, h: float, gamma: float = 0.7):
+    """V_b = gamma / sum_b' W_bb' and term_b = sum_b' W_bb' V_b'
+    (akinciBoundary.py:152-157), boundary pairs |x_b - x_b'| < h."""
+    from scipy.spatial import cKDTree
+
+    tree = cKDTree(bx.astype(np.float64))
+    pairs = tree.query_pairs(r=float(h) * 1.0001, output_type="ndarray")
+    i = np.concatenate([pairs[:, 0], pairs[:, 1], np.arange(len(bx))])
+    j = np.concatenate([pairs[:, 1], pairs[:, 0], np.arange(len(bx))])
+    bxf = bx.astype(np.float32)
+    d = np.sqrt(((bxf[i] - bxf[j]) ** 2).sum(1).astype(np.float32))
+    keep = d < np.float32(h)
+    i, j, d = i[keep], j[keep], d[keep].astype(np.float64)
+    w = _wendland_np(d / h, h)
+    wsum = np.bincount(i, weights=w, minlength=len(bx))
+    vol = gamma / wsum
+    term = np.bincount(i, weights=w * vol[j], minlength=len(bx))
+    return vol.astype(np.float32), term.astype(np.float32)
+
+
+def _finish(name, scheme, boundary, fluid, vel, dx, W, H, seed, h=None):
+    h = support_for(dx) if h is None else np.float32(h)
+    area = np.float32(dx * dx)
+    if boundary == "solid":
+        bx, bgx = _walls_solid(W, H, dx)
+        bvol = np.full(len(bx), area, dtype=np.float32)   # pi r^2 = A
+        bterm = None
+    else:
+        bx = _walls_akinci(W, H, dx)
+        bgx = None
+        bvol, bterm = akinci_volumes(bx, h)
+    phys = default_phys(scheme, float(h))
+    buf = 2.0 * float(h)
+    return Scene(name=name, scheme=scheme, boundary=boundary, x=fluid.astype(np.float32),
+                 v=vel.astype(np.float32), area=float(area), support=float(h),
+                 rho0=1000.0, bx=bx.astype(np.float32),
+                 bgx=None if bgx is None else bgx.astype(np.float32), bvol=bvol, bterm=bterm,
+                 vmin=(-buf, -buf), vmax=(W + buf, H + buf), phys=phys)
+
+
+def dam_break(dx=0.01, block=(1.0, 0.5), tank=(4.0, 2.0), scheme="dsph", boundary=None,
+              seed=0) -> Scene:
+    """Fluid block [0, bw] x [0, bh] in a closed tank; jitter U(+-0.01 dx)."""
+    boundary = boundary or ("solid" if scheme == "dsph" else "akinci")
+    rng = np.random.default_rng(seed)
+    nx, ny = int(round(block[0] / dx)), int(round(block[1] / dx))
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    pts = np.stack([(I.ravel() + 0.5) * dx, (J.ravel() + 0.5) * dx], axis=1)
+    pts = _jitter(rng, pts, dx)
+    vel = np.zeros_like(pts)
+    return _finish(f"dambreak_{scheme}_{len(pts)}", scheme, boundary, pts, vel, dx, *tank, seed)
+
+
+def colliding_drops(dx=0.01, radius=0.25, speed=2.0, tank=(4.0, 2.0), scheme="dsph",
+                    boundary=None, seed=0) -> Scene:
+    """Two discs approaching each other at +-speed (configs/collision_*.toml)."""
+    boundary = boundary or ("solid" if scheme == "dsph" else "akinci")
+    rng = np.random.default_rng(seed)
+    W, H = tank
+    pts, vel = [], []
+    for cx, u in ((W / 2 - 2 * radius, speed), (W / 2 + 2 * radius, -speed)):
+        n = int(math.ceil(radius / dx)) + 1
+        I, J = np.meshgrid(np.arange(-n, n), np.arange(-n, n), indexing="xy")
+        p = np.stack([(I.ravel() + 0.5) * dx, (J.ravel() + 0.5) * dx], axis=1)
+        p = p[(p ** 2).sum(1) <= radius * radius]
+        p = p + np.array([cx, H / 2])
+        pts.append(p)
+        vel.append(np.tile([u, 0.0], (len(p), 1)))
+    pts = _jitter(rng, np.concatenate(pts), dx)
+    vel = np.concatenate(vel)
+    return _finish(f"drops_{scheme}_{len(pts)}", scheme, boundary, pts, vel, dx, W, H, seed)
+
+
+def scene_for_count(kind: str, scheme: str, n_target: int, seed: int) -> Scene:
+    """Dam break or colliding drops with dx chosen to give about n_target
+    fluid particles."""
+    if kind == "dambreak":
+        dx = math.sqrt(1.0 * 0.5 / n_target)
+        return dam_break(dx=dx, scheme=scheme, seed=seed)
+    area = 2 * math.pi * 0.25 ** 2
+    dx = math.sqrt(area / n_target)
+    return colliding_drops(dx=dx, scheme=scheme, seed=seed)
+
+
+def point_cloud(n: int, seed: int = 1, clusters: int = 8, sigma: float = 0.05) -> np.ndarray:
+    """Half uniform in [0,1)^2, half from Gaussian clusters (sigma 0.05,
+    centres U[0.15, 0.85]^2), resampling points that leave the box."""
+    rng = np.random.default_rng(seed)
+    nu = n // 2
+    pts = [rng.uniform(0.0, 1.0, size=(nu, 2))]
+    centres = rng.uniform(0.15, 0.85, size=(clusters, 2))
+    per = [(n - nu) // clusters + (1 if k < (n - nu) % clusters else 0) for k in range(clusters)]
+    for c, m in zip(centres, per):
+        out = np.empty((0, 2))
+        while len(out) < m:
+            p = rng.normal(c, sigma, size=(2 * (m - len(out)) + 16, 2))
+            p = p[(p >= 0).all(1) & (p < 1).all(1)]
+            out = np.concatenate([out, p])
+        pts.append(out[:m])
+    return np.concatenate(pts).astype(np.float32)
+
+
+def support_for_mean_neighbors(pts: np.ndarray, target: float = 20.0, sample: int = 20000,
+                               seed: int = 0) -> np.float32:
+    """Support radius giving `target` mean neighbours (self included),
+    bisected on a sample of query points against all points."""
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(seed)
+    tree = cKDTree(pts.astype(np.float64))
+    qs = pts[rng.choice(len(pts), size=min(sample, len(pts)), replace=False)].astype(np.float64)
+    lo, hi = 1e-6, 0.1
+    for _ in range(40):
+        mid = 0.5 * (lo + hi)
+        cnt = np.mean([len(c) for c in tree.query_ball_point(qs, mid)])
+        if cnt < target:
+            lo = mid
+        else:
+            hi = mid
+    return np.float32(0.5 * (lo + hi))

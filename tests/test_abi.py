@@ -1,0 +1,56 @@
+"""C-ABI boundary checks that need no GPU: the library loads and exports
+every entry point include/sph_mi355x.h declares; record layouts match."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+from pytorchsph_amd import _lib as L
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names |= set(re.findall(r"\b(sph_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.load()
+    missing = [n for n in sorted(declared_symbols()) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_declarations():
+    assert declared_symbols() == set(L.exported_symbols())
+
+
+def test_version_and_error_string():
+    lib = L.load()
+    assert lib.sph_version() >= 1
+    assert isinstance(lib.sph_last_error(), bytes)
+
+
+def test_argument_errors_are_reported_without_gpu():
+    # a null grid is rejected on the host before any launch
+    lib = L.load()
+    rc = lib.sph_grid_build(None, None, 0, None)
+    assert rc == 1
+    assert b"null grid" in lib.sph_last_error()
+
+
+def test_record_layouts():
+    assert L.GRID_DESC_DTYPE.itemsize == 32
+    assert L.PHYS_DTYPE.itemsize == 80
+    assert C.sizeof(L.SphGrid) == 8 * 3 + 4 * 3 + 4 + 8 * 8
+    assert C.sizeof(L.SphQuery) == 8 * 3 + 4 * 3 + 4
+
+
+def test_scan_workspace_sizes():
+    lib = L.load()
+    assert lib.sph_scan_workspace_bytes(1) >= 8
+    assert lib.sph_scan_workspace_bytes(10_000_000) >= 4 * (10_000_000 // 2048)

@@ -1,0 +1,202 @@
+"""GPU neighbour search: pair sets bit-exact against the oracle's restatement
+of the reference grid search (neighSearch.cpp:172-331)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from pytorchsph_amd import scenes as S
+from pytorchsph_amd import _lib as L
+from pytorchsph_amd.search import NeighborGrid, grid_params, pair_geometry, transpose
+from tests.gpu_helpers import assert_same_pairs, gpu_search, sorted_rows
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_self(x, h, vmin, vmax, hmax):
+    qx, qy, cx, _, _ = grid_params(vmin, vmax, hmax)
+    return O.search(x, h, x, (qx, qy), hmax, cx, 1, self_mode=True)
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (7, 1), (2000, 2), (50000, 3)])
+def test_uniform_cloud(cuda, n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, 1, (n, 2)).astype(np.float32)
+    hmax = np.float32(np.sqrt(20 / (np.pi * max(n, 1))))
+    h = np.full(n, hmax, np.float32)
+    rp, c, st, _ = gpu_search(x, h, (0, 0), (1, 1), hmax)
+    assert st == 0
+    assert_same_pairs(rp, c, oracle_self(x, h, (0, 0), (1, 1), hmax))
+
+
+def test_clustered_cloud(cuda):
+    x = S.point_cloud(200_000, seed=1)
+    hmax = S.support_for_mean_neighbors(x, 20.0, sample=5000)
+    h = np.full(len(x), hmax, np.float32)
+    rp, c, st, _ = gpu_search(x, h, (0, 0), (1, 1), hmax)
+    assert st == 0
+    assert_same_pairs(rp, c, oracle_self(x, h, (0, 0), (1, 1), hmax))
+    mean = (rp[-1]) / len(x)
+    assert 15 < mean < 25
+
+
+def test_variable_support(cuda):
+    rng = np.random.default_rng(9)
+    x = rng.uniform(0, 1, (20000, 2)).astype(np.float32)
+    hmax = np.float32(0.02)
+    h = (rng.uniform(0.3, 1.0, len(x)) * hmax).astype(np.float32)
+    h[0] = hmax
+    rp, c, _, _ = gpu_search(x, h, (0, 0), (1, 1), hmax)
+    assert_same_pairs(rp, c, oracle_self(x, h, (0, 0), (1, 1), hmax))
+
+
+def test_adversarial_points(cuda):
+    """Near-tie distances (+-3 ulp of h), points exactly on cell faces,
+    duplicates, a dense single cell and points at the domain corner."""
+    hmax = np.float32(0.0625)
+    qx, qy, cx, cy, _ = grid_params((0, 0), (1, 1), hmax)
+    pts = [[0.5, 0.5]]
+    for k in range(-3, 4):
+        d = hmax
+        for _ in range(abs(k)):
+            d = np.nextafter(d, np.float32(1 if k > 0 else 0))
+        pts += [[0.5 + d, 0.5], [0.5, 0.5 - d], [0.5 - d, 0.5]]
+    faces = qx + hmax * np.arange(2, 12, dtype=np.float32)
+    pts += [[f, 0.3] for f in faces] + [[0.3, f] for f in faces]
+    pts += [[0.25, 0.25]] * 5                                   # duplicates
+    rng = np.random.default_rng(1)
+    pts += list(0.8 + rng.uniform(0, 0.01, (60, 2)))            # one dense cell
+    pts += [[0.0, 0.0], [1.0, 1.0], [0.0, 1.0], [1.0, 0.0]]
+    x = np.array(pts, np.float32)
+    h = np.full(len(x), hmax, np.float32)
+    rp, c, st, _ = gpu_search(x, h, (0, 0), (1, 1), hmax)
+    assert st == 0
+    assert_same_pairs(rp, c, oracle_self(x, h, (0, 0), (1, 1), hmax))
+
+
+def test_foreign_queries_and_transpose(cuda):
+    sc = S.dam_break(dx=0.01, scheme="dsph", seed=0)
+    hmax = np.float32(sc.support)
+    qx, qy, cx, _, _ = grid_params(sc.vmin, sc.vmax, hmax)
+    bh = np.full(sc.nb, hmax, np.float32)
+    for q in (sc.bx, sc.bgx):
+        rp, c, st, g = gpu_search(sc.x, np.full(sc.n, hmax, np.float32), sc.vmin, sc.vmax, hmax,
+                                  queries=(q, bh))
+        orc = O.search(q, bh, sc.x, (qx, qy), hmax, cx, 1)
+        assert_same_pairs(rp, c, orc)
+    # device transpose of boundary->fluid equals the oracle transpose
+    dev = torch.device("cuda")
+    from pytorchsph_amd.search import CSR
+    a = CSR(torch.from_numpy(rp.astype(np.int32)).to(dev),
+            torch.from_numpy(c.astype(np.int32)).to(dev), len(rp) - 1, int(rp[-1]))
+    t = transpose(a, sc.n)
+    trp = t.rowptr.cpu().numpy().astype(np.int64)
+    tcol = t.col[: int(trp[-1])].cpu().numpy().astype(np.int64)
+    ot = O.transpose(orc, sc.n)
+    assert np.array_equal(trp, ot.rowptr) and np.array_equal(tcol, ot.col)
+
+
+def test_multi_scene_grid(cuda):
+    """Three scenes with different domains and supports in one grid; each
+    scene's pairs equal its own single-scene oracle result."""
+    rng = np.random.default_rng(4)
+    specs = [((0, 0), (1, 1), 0.03, 3000), ((-2, -1), (2, 1), 0.05, 5000),
+             ((10, 10), (10.5, 12), 0.02, 4000)]
+    xs, scenes, off = [], [], 0
+    for vmin, vmax, h, n in specs:
+        x = rng.uniform(vmin, vmax, (n, 2)).astype(np.float32)
+        xs.append(x)
+        scenes.append(dict(vmin=vmin, vmax=vmax, hmax=np.float32(h), p_begin=off, p_count=n))
+        off += n
+    X = np.concatenate(xs)
+    H = np.concatenate([np.full(n, np.float32(h), np.float32) for _, _, h, n in specs])
+    P = np.concatenate([np.full(n, s, np.int32) for s, (_, _, _, n) in enumerate(specs)])
+    dev = torch.device("cuda")
+    g = NeighborGrid(scenes, len(X), dev)
+    xt, ht, pt = (torch.from_numpy(a).to(dev) for a in (X, H, P))
+    g.build(xt, pt)
+    csr = g.search(xt, ht, pt, 1, self_sorted=True, sort_rows=True)
+    rp = csr.rowptr.cpu().numpy().astype(np.int64)
+    col = csr.col[: int(rp[-1])].cpu().numpy().astype(np.int64)
+    off = 0
+    for (vmin, vmax, h, n), x in zip(specs, xs):
+        orc = oracle_self(x, np.full(n, np.float32(h), np.float32), vmin, vmax, np.float32(h))
+        seg_rp = rp[off:off + n + 1] - rp[off]
+        seg_col = col[rp[off]:rp[off + n]] - off
+        assert np.array_equal(seg_rp, orc.rowptr)
+        assert np.array_equal(seg_col, orc.col)
+        off += n
+
+
+def test_pair_geometry_matches_reference_layout(cuda):
+    rng = np.random.default_rng(6)
+    x = rng.uniform(0, 1, (3000, 2)).astype(np.float32)
+    hmax = np.float32(0.05)
+    h = np.full(len(x), hmax, np.float32)
+    dev = torch.device("cuda")
+    g = NeighborGrid([dict(vmin=(0, 0), vmax=(1, 1), hmax=hmax, p_begin=0, p_count=len(x))],
+                     len(x), dev)
+    xt, ht = torch.from_numpy(x).to(dev), torch.from_numpy(h).to(dev)
+    ps = torch.zeros(len(x), dtype=torch.int32, device=dev)
+    g.build(xt, ps)
+    csr = g.search(xt, ht, ps, 1, self_sorted=True, sort_rows=True)
+    rows, cols, rhat, q = pair_geometry(csr, xt, xt, None, float(hmax))
+    rows, cols = rows.cpu().numpy(), cols.cpu().numpy()
+    orc = oracle_self(x, h, (0, 0), (1, 1), hmax)
+    r_o, c_o = orc.pairs()
+    assert np.array_equal(rows, r_o) and np.array_equal(cols, c_o)   # lexicographic (i, j)
+    d = x[rows] - x[cols]
+    dist = np.linalg.norm(d.astype(np.float64), axis=1)
+    want = np.where(dist[:, None] >= 1e-7, d / np.maximum(dist, 1e-30)[:, None], 0.0)
+    assert np.allclose(rhat.cpu().numpy(), want, atol=2e-6)
+    assert np.allclose(q.cpu().numpy(), dist / hmax, atol=2e-6)
+    assert np.all(rhat.cpu().numpy()[rows == cols] == 0.0)
+
+
+def test_overflow_is_flagged_and_bounded(cuda):
+    rng = np.random.default_rng(8)
+    x = rng.uniform(0, 1, (4000, 2)).astype(np.float32)
+    hmax = np.float32(0.05)
+    h = np.full(len(x), hmax, np.float32)
+    dev = torch.device("cuda")
+    g = NeighborGrid([dict(vmin=(0, 0), vmax=(1, 1), hmax=hmax, p_begin=0, p_count=len(x))],
+                     len(x), dev)
+    xt, ht = torch.from_numpy(x).to(dev), torch.from_numpy(h).to(dev)
+    ps = torch.zeros(len(x), dtype=torch.int32, device=dev)
+    g.build(xt, ps)
+    q = g.query_struct(xt, ht, ps, 1, True)
+    counts = torch.zeros(len(x), dtype=torch.int32, device=dev)
+    rowptr = torch.zeros(len(x) + 1, dtype=torch.int32, device=dev)
+    g.count(q, counts)
+    from pytorchsph_amd.search import scan
+    scan(counts, len(x), rowptr)
+    cap = int(rowptr[-1].item()) // 2
+    col = torch.full((cap,), -1, dtype=torch.int32, device=dev)
+    g.fill(q, rowptr, col)
+    torch.cuda.synchronize()
+    assert int(g.status[0].item()) & L.ST_OVERFLOW
+    rp = rowptr.cpu().numpy()
+    assert rp.max() <= cap and np.all(np.diff(rp) >= 0)
+    assert int((col < 0).sum()) == 0
+
+
+def test_points_outside_table_are_flagged(cuda):
+    x = np.array([[0.5, 0.5], [0.51, 0.5], [50.0, 50.0]], np.float32)
+    hmax = np.float32(0.05)
+    rp, c, st, _ = gpu_search(x, np.full(3, hmax, np.float32), (0, 0), (1, 1), hmax)
+    assert st & L.ST_OUTSIDE
+    orc = oracle_self(x[:2], np.full(2, hmax, np.float32), (0, 0), (1, 1), hmax)
+    assert np.array_equal(sorted_rows(rp[:3], c[: rp[2]]), orc.col.astype(np.int64))
+
+
+@pytest.mark.slow
+def test_c2_four_million_points(cuda):
+    """Neighbour search alone at 4M points (half uniform, half 8 Gaussian
+    clusters, ~20 mean neighbours), bit-exact against the oracle."""
+    x = S.point_cloud(4_000_000, seed=1)
+    hmax = S.support_for_mean_neighbors(x, 20.0)
+    h = np.full(len(x), hmax, np.float32)
+    rp, c, st, _ = gpu_search(x, h, (0, 0), (1, 1), hmax)
+    assert st == 0
+    orc = oracle_self(x, h, (0, 0), (1, 1), hmax)
+    assert_same_pairs(rp, c, orc)
