@@ -369,3 +369,51 @@ class SceneBatch:
 
     def dfsph_iterations(self) -> np.ndarray:
         return self.df["iters"].cpu().numpy()
+
+
+class MixedBatch:
+    """delta-SPH and DFSPH scenes advanced together: each scheme is one
+    SceneBatch; the two run on their own streams so their kernels overlap
+    on the device (default stream priority for both)."""
+
+    def __init__(self, scenes: list[Scene], device="cuda", **kw):
+        self.scenes = scenes
+        self.index = {}   # scene number -> (sub-batch, position)
+        self.parts = []
+        for scheme in ("dsph", "dfsph"):
+            ids = [k for k, s in enumerate(scenes) if s.scheme == scheme]
+            if ids:
+                b = SceneBatch([scenes[k] for k in ids], device, **kw)
+                for pos, k in enumerate(ids):
+                    self.index[k] = (b, pos)
+                self.parts.append(b)
+        self.streams = [torch.cuda.Stream() for _ in self.parts]
+        self.n_particles = int(sum(s.n for s in scenes))
+        self.captured = False
+
+    def capture(self):
+        for b in self.parts:
+            b.capture()
+        self.captured = True
+
+    def step(self):
+        cur = torch.cuda.current_stream()
+        for s in self.streams:
+            s.wait_stream(cur)
+        # the DFSPH part polls the host between iteration chunks, so the
+        # delta-SPH graph is queued first and runs underneath those polls
+        for b, s in sorted(zip(self.parts, self.streams), key=lambda t: t[0].scheme != "dsph"):
+            with torch.cuda.stream(s):
+                if self.captured:
+                    b.step_graph()
+                else:
+                    b.step(s)
+        for s in self.streams:
+            cur.wait_stream(s)
+
+    def scene_state(self, k):
+        b, pos = self.index[k]
+        return b.scene_state(pos)
+
+    def status(self):
+        return np.concatenate([b.status() for b in self.parts])
