@@ -34,6 +34,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serial", action="store_true",
+                    help="run the delta-SPH and DFSPH parts one after the other (profiling)")
     return ap.parse_args()
 
 
@@ -132,14 +134,14 @@ def main():
     torch.cuda.synchronize()
     log("captured")
     for _ in range(args.warmup):
-        batch.step()
+        batch.step(serial=args.serial)
     torch.cuda.synchronize()
     lat = []
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for k in range(args.steps):
         t0 = time.perf_counter()
-        batch.step()
+        batch.step(serial=args.serial)
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t0)
         log(f"step {k}: {lat[-1] * 1e3:.1f} ms")

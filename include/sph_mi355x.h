@@ -84,7 +84,9 @@ typedef struct SphGridDesc {
   float qmin_x, qmin_y;   /* minDomain - hmax                              */
   float hmax;             /* max support = cell size                      */
   int cells_x, cells_y;   /* ceil((qmax - qmin) / hmax), qmax = max + 2 hmax */
-  int cell_begin;         /* first slot of this scene in the cell table    */
+  int cell_begin;         /* first slot of this scene in the cell table; a
+                             scene owns cells_x * cells_y + 1 slots, the last
+                             one collecting points outside its table        */
   int p_begin, p_count;   /* this scene's point range                      */
 } SphGridDesc;
 
@@ -94,14 +96,13 @@ typedef struct SphGrid {
   const SphGridDesc* scenes; /* [nscenes]                                  */
   const int* pscene;         /* [np] scene of each point                   */
   const float* x;            /* [np][2] point positions                    */
-  int np, nscenes, ncells;   /* ncells = sum of cells_x*cells_y            */
+  int np, nscenes, ncells;   /* ncells = sum of (cells_x*cells_y + 1)      */
   int* cell_count;           /* [ncells] scratch                           */
   int* cell_start;           /* [ncells+1] first sorted slot of each cell  */
-  int* key;                  /* [np] cell key of each point (-1: outside)  */
+  int* key;                  /* [np] cell key of each point                */
   int* slot;                 /* [np] scratch                               */
   int* sorted_idx;           /* [np] point index in (scene, key, index) order */
   float* sorted_x;           /* [np][2] positions in sorted order          */
-  int* lost;                 /* [1] scratch counter                        */
   unsigned* status;          /* [nscenes] status bits                      */
 } SphGrid;
 
@@ -226,6 +227,17 @@ int sph_dsph_gather(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
                     const SphBoundary* b, const SphCSR* fb, const float* gradrho,
                     unsigned terms, const int* order, float* dpdt, float* acc,
                     hipStream_t s);
+/* Packs one 32-byte record per particle for sph_dsph_gather_packed:
+ * frec[np][8] = {x, y, vx, vy, rho_hat rho0, V, p, rho_hat} with
+ * V = A / rho_hat (or f->vol) and p = EOS(rho_hat); brec[nb][8] likewise for
+ * the boundary with V = V_b / rho_b and p = b->p. */
+int sph_dsph_pack(const SphFluid* f, const SphPhysics* phys, const SphBoundary* b,
+                  float* frec, float* brec, hipStream_t s);
+/* sph_dsph_gather on packed records (same terms, same results). */
+int sph_dsph_gather_packed(const float* frec, const float* brec, const int* pscene, int np,
+                           const SphPhysics* phys, const SphCSR* ff, const SphCSR* fb,
+                           const float* gradrho, unsigned terms, const int* order,
+                           float* dpdt, float* acc, hipStream_t s);
 /* Renormalisation matrix M_i (fluid + boundary neighbours) and L_i =
  * pinv2x2(M_i); M, L are [np][2][2]. */
 int sph_dsph_normalization(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
@@ -261,13 +273,16 @@ typedef struct SphDfsph {
   float* src;    /* [np] source term                                  */
   float* p;      /* [np] pressure (previous iterate)                  */
   float* p2;     /* [np] pressure (current iterate)                   */
-  float* apred;  /* [np][2] pressure acceleration                     */
+  float* apred;  /* [np][2] pressure acceleration (after finish)      */
   float* resid;  /* [np] residual                                     */
+  float* xrec;   /* [np][4] {x, y, p2 / (rho rho0)^2, A rho0}         */
+  float* arec;   /* [np][4] {a*_x, a*_y, dt^2 A^act, 0}               */
   float* b_aact; /* [nb]                                              */
   float* b_alpha;/* [nb]                                              */
   float* b_src;  /* [nb]                                              */
   float* b_p;    /* [nb] boundary pressure                            */
   float* b_resid;/* [nb]                                              */
+  float* bxrec;  /* [nb][4] {x, y, p_b / (rho_b rho0_b)^2, V_b rho0_b} */
   double* err_acc;  /* [nscenes] FP64 sum of clamped residuals        */
   double* err_hist; /* [nscenes][max_hist] per-iteration errors        */
   int* iters;       /* [nscenes] iterations done                       */
@@ -276,27 +291,36 @@ typedef struct SphDfsph {
   unsigned* status; /* [nscenes]                                       */
   const int* nfluid;    /* [nscenes] fluid count per scene             */
   const int* nbdy;      /* [nscenes] boundary count per scene          */
+  const int* fbegin;    /* [nscenes] first fluid index of each scene   */
+  const int* bbegin;    /* [nscenes] first boundary index of each scene */
+  int* act_scene;   /* [nscenes] scenes still iterating                */
+  int* act_pre;     /* [2 (nscenes + 1)] prefix sums of their fluid,
+                       then boundary, counts                            */
+  int* act_n;       /* [3] active fluid items, boundary items, scenes  */
   int nscenes, max_hist;
+  int grid_blocks;  /* grid of the iteration kernels (work is strided) */
 } SphDfsph;
 
 /* v* = v + dt a, A^act = A / rho; boundary A^act_b = V_b / rho_b, p_b = 0;
- * pressures zeroed; iteration counters reset. */
+ * pressures zeroed; iteration counters reset; every scene active.  The
+ * fluid of scene s must be [fbegin[s], fbegin[s] + nfluid[s]) and its
+ * boundary [bbegin[s], bbegin[s] + nbdy[s]). */
 int sph_dfsph_prepare(const SphFluid* f, const float* acc, const SphPhysics* phys,
                       const SphBoundary* b, SphDfsph* d, hipStream_t s);
 /* alpha and source term of fluid and boundary particles. */
 int sph_dfsph_alpha_source(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
                            const SphBoundary* b, const SphCSR* fb, const SphCSR* bf,
-                           const int* order, SphDfsph* d, hipStream_t s);
+                           SphDfsph* d, hipStream_t s);
 /* One relaxed-Jacobi iteration of every still-active scene: pressure
  * acceleration from p2, then the pressure / residual update and the FP64
- * residual reduction, then the per-scene convergence decision. */
+ * residual reduction, then the per-scene convergence decision and the new
+ * active list.  Scenes that stopped are not touched. */
 int sph_dfsph_iterate(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
                       const SphBoundary* b, const SphCSR* fb, const SphCSR* bf,
-                      const int* order, SphDfsph* d, hipStream_t s);
-/* Final pressure acceleration, v* += dt a*, acc += a*. */
+                      SphDfsph* d, hipStream_t s);
+/* Final pressure acceleration, v* += dt a*, acc += a*, p = p2. */
 int sph_dfsph_finish(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
-                     const SphBoundary* b, const SphCSR* fb, const int* order,
-                     SphDfsph* d, float* acc, hipStream_t s);
+                     const SphCSR* fb, SphDfsph* d, float* acc, hipStream_t s);
 
 /* ------------------------------------------------------------- integrators */
 /* RK4 stage s in 1..4 (simulationBase.py:679-705): accumulates
@@ -315,8 +339,9 @@ int sph_integrate_semi_implicit(const int* pscene, const SphPhysics* phys, int n
                                 const float* v_stage, const float* acc, const float* dpdt,
                                 float* x_out, float* v_out, float* rho_out,
                                 hipStream_t s);
-/* Gathers float arrays by a permutation: out[k] = in[perm[k]] for
- * `width` floats per element. */
+/* Gathers arrays of 32-bit words by a permutation: out[k] = in[perm[k]]
+ * for `width` words per element (perm == NULL copies).  Words are moved
+ * bit for bit, so int32 / int64 arrays permute as well. */
 int sph_permute_f32(const float* in, const int* perm, int n, int width, float* out,
                     hipStream_t s);
 

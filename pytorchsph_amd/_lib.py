@@ -25,7 +25,7 @@ class SphGrid(C.Structure):
         ("scenes", c_vp), ("pscene", c_vp), ("x", c_vp),
         ("np", c_int), ("nscenes", c_int), ("ncells", c_int),
         ("cell_count", c_vp), ("cell_start", c_vp), ("key", c_vp), ("slot", c_vp),
-        ("sorted_idx", c_vp), ("sorted_x", c_vp), ("lost", c_vp), ("status", c_vp),
+        ("sorted_idx", c_vp), ("sorted_x", c_vp), ("status", c_vp),
     ]
 
 
@@ -54,15 +54,15 @@ class SphCSR(C.Structure):
     _fields_ = [("rowptr", c_vp), ("col", c_vp), ("nrows", c_int)]
 
 
+DFSPH_FIELDS = ("vpred", "aact", "alpha", "src", "p", "p2", "apred", "resid", "xrec", "arec",
+                "b_aact", "b_alpha", "b_src", "b_p", "b_resid", "bxrec", "err_acc", "err_hist",
+                "iters", "active", "any_active", "status", "nfluid", "nbdy", "fbegin", "bbegin",
+                "act_scene", "act_pre", "act_n")
+
+
 class SphDfsph(C.Structure):
-    _fields_ = [
-        ("vpred", c_vp), ("aact", c_vp), ("alpha", c_vp), ("src", c_vp), ("p", c_vp),
-        ("p2", c_vp), ("apred", c_vp), ("resid", c_vp), ("b_aact", c_vp), ("b_alpha", c_vp),
-        ("b_src", c_vp), ("b_p", c_vp), ("b_resid", c_vp), ("err_acc", c_vp),
-        ("err_hist", c_vp), ("iters", c_vp), ("active", c_vp), ("any_active", c_vp),
-        ("status", c_vp), ("nfluid", c_vp), ("nbdy", c_vp), ("nscenes", c_int),
-        ("max_hist", c_int),
-    ]
+    _fields_ = [(n, c_vp) for n in DFSPH_FIELDS] + [
+        ("nscenes", c_int), ("max_hist", c_int), ("grid_blocks", c_int)]
 
 
 # device-array record types (numpy layouts identical to the C structs)
@@ -119,11 +119,14 @@ _SIGS = {
                                      c_vp, c_vp], c_int),
     "sph_dfsph_prepare": ([_P(SphFluid), c_vp, c_vp, _P(SphBoundary), _P(SphDfsph), c_vp], c_int),
     "sph_dfsph_alpha_source": ([_P(SphFluid), c_vp, _P(SphCSR), _P(SphBoundary), _P(SphCSR),
-                                _P(SphCSR), c_vp, _P(SphDfsph), c_vp], c_int),
+                                _P(SphCSR), _P(SphDfsph), c_vp], c_int),
     "sph_dfsph_iterate": ([_P(SphFluid), c_vp, _P(SphCSR), _P(SphBoundary), _P(SphCSR),
-                           _P(SphCSR), c_vp, _P(SphDfsph), c_vp], c_int),
-    "sph_dfsph_finish": ([_P(SphFluid), c_vp, _P(SphCSR), _P(SphBoundary), _P(SphCSR), c_vp,
-                          _P(SphDfsph), c_vp, c_vp], c_int),
+                           _P(SphCSR), _P(SphDfsph), c_vp], c_int),
+    "sph_dfsph_finish": ([_P(SphFluid), c_vp, _P(SphCSR), _P(SphCSR), _P(SphDfsph), c_vp, c_vp],
+                         c_int),
+    "sph_dsph_pack": ([_P(SphFluid), c_vp, _P(SphBoundary), c_vp, c_vp, c_vp], c_int),
+    "sph_dsph_gather_packed": ([c_vp, c_vp, c_vp, c_int, c_vp, _P(SphCSR), _P(SphCSR), c_vp,
+                                C.c_uint, c_vp, c_vp, c_vp, c_vp], c_int),
     "sph_integrate_rk4": ([c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                            c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "sph_integrate_semi_implicit": ([c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,

@@ -1,14 +1,22 @@
 // DFSPH relaxed-Jacobi density solve (modules/dfsph.py:25-311) with the
 // Akinci PBSPH boundary hooks (akinciBoundary.py:166-389), batched over
-// scenes.  The reference syncs to the host once per iteration to read an
-// fp32 mean residual; here the residual is summed in FP64 on the device and
-// a one-workgroup control kernel decides, per scene, whether to continue.
-// Kernels of scenes that have converged return immediately, so the host
-// can issue iterations in chunks and poll `any_active` between chunks.
+// scenes.
+//
+// The reference syncs to the host once per iteration to read an fp32 mean
+// residual.  Here the residual is summed in FP64 on the device, and a
+// one-workgroup control kernel decides per scene whether to continue and
+// rebuilds a compact list of the scenes still iterating.  The two gathers of
+// an iteration stride over that list only, so scenes that have converged cost
+// nothing while slower ones finish; the host issues iterations in chunks and
+// polls `any_active` between chunks.
+//
+// Each iteration reads neighbours through 16-byte records written by the
+// previous kernel: xrec = {x, y, p2 / (rho rho0)^2, A rho0} for the
+// pressure acceleration, arec = {a*, dt^2 A^act} for the pressure update.
 //
 // Spiky gradient in the fluid-row convention: G_ij = -spiky(q) rhat_ij with
 // rhat_ij = (x_i - x_j) / |x_i - x_j|.  Boundary neighbours enter fluid rows
-// exactly like fluid neighbours with their own area (V_b), actual area
+// like fluid neighbours with their own area (V_b), actual area
 // (V_b / rho_b), predicted velocity (0) and pressure acceleration (0).
 #include "sph_common.h"
 
@@ -23,6 +31,30 @@ __device__ __forceinline__ float2 ld2(const float* p, int i) {
 __device__ __forceinline__ void st2(float* p, int i, float2 v) {
   reinterpret_cast<float2*>(p)[i] = v;
 }
+__device__ __forceinline__ float4 ld4(const float* p, int i) {
+  return reinterpret_cast<const float4*>(p)[i];
+}
+__device__ __forceinline__ void st4(float* p, int i, float4 v) {
+  reinterpret_cast<float4*>(p)[i] = v;
+}
+
+__device__ __forceinline__ float2 spiky_G(float xi, float yi, float xj, float yj, const KConst& K) {
+  const PairGeo g = pair_geo(xi, yi, xj, yj, K.inv_h);
+  const float s = -spiky_grad_k(g.q, K);
+  return make_float2(s * g.hx, s * g.hy);
+}
+
+// Maps active work item t to (scene, index): pre[0..nact] are the prefix
+// sums of the active scenes' counts; returns the rank k with
+// pre[k] <= t < pre[k + 1].
+__device__ __forceinline__ int find_rank(const int* __restrict__ pre, int nact, int t) {
+  int lo = 0, hi = nact;  // invariant: pre[lo] <= t < pre[hi]
+  while (hi - lo > 1) {
+    const int m = (lo + hi) >> 1;
+    if (pre[m] <= t) lo = m; else hi = m;
+  }
+  return lo;
+}
 
 __global__ __launch_bounds__(kBS) void k_prepare(SphFluid f, const float* __restrict__ acc,
                                                  const SphPhysics* __restrict__ phys, SphBoundary b,
@@ -33,14 +65,20 @@ __global__ __launch_bounds__(kBS) void k_prepare(SphFluid f, const float* __rest
     const float2 v = ld2(f.v, t);
     const float2 a = acc ? ld2(acc, t) : make_float2(0.f, 0.f);
     st2(d.vpred, t, make_float2(v.x + P.dt * a.x, v.y + P.dt * a.y));
-    d.aact[t] = f.area[t] / f.rho[t];
+    const float aact = f.area[t] / f.rho[t];
+    d.aact[t] = aact;
     d.p[t] = 0.f;
     d.p2[t] = 0.f;
     st2(d.apred, t, make_float2(0.f, 0.f));
+    const float2 x = ld2(f.x, t);
+    st4(d.xrec, t, make_float4(x.x, x.y, 0.f, f.area[t] * f.rest[t]));
+    st4(d.arec, t, make_float4(0.f, 0.f, P.dt * P.dt * aact, 0.f));
   } else if (t < f.np + b.nb) {
     const int k = t - f.np;
     d.b_aact[k] = b.vol[k] / b.rho[k];
     d.b_p[k] = 0.f;
+    const float2 x = ld2(b.x, k);
+    st4(d.bxrec, k, make_float4(x.x, x.y, 0.f, b.vol[k] * b.rest[k]));
   }
   if (t < d.nscenes) {
     d.err_acc[t] = 0.0;
@@ -50,40 +88,74 @@ __global__ __launch_bounds__(kBS) void k_prepare(SphFluid f, const float* __rest
   if (t == 0) *d.any_active = 1;
 }
 
+// Rebuilds the active scene list and its prefix sums (one workgroup).
+__device__ void build_active_lists(SphDfsph& d) {
+  __shared__ int lds[kBS / kWave + 1];
+  int carry_s = 0, carry_f = 0, carry_b = 0;
+  for (int base = 0; base < d.nscenes; base += blockDim.x) {
+    const int s = base + threadIdx.x;
+    const int act = (s < d.nscenes) ? d.active[s] : 0;
+    const int nf = act ? d.nfluid[s] : 0;
+    const int nb = act ? d.nbdy[s] : 0;
+    int tot;
+    const int rank = block_exclusive_scan(act, lds, &tot) + carry_s;
+    const int tot_s = tot;
+    const int pf = block_exclusive_scan(nf, lds, &tot) + carry_f;
+    const int tot_f = tot;
+    const int pb = block_exclusive_scan(nb, lds, &tot) + carry_b;
+    const int tot_b = tot;
+    if (act) {
+      d.act_scene[rank] = s;
+      d.act_pre[rank] = pf;
+      d.act_pre[d.nscenes + 1 + rank] = pb;
+    }
+    carry_s += tot_s;
+    carry_f += tot_f;
+    carry_b += tot_b;
+  }
+  if (threadIdx.x == 0) {
+    d.act_pre[carry_s] = carry_f;
+    d.act_pre[d.nscenes + 1 + carry_s] = carry_b;
+    d.act_n[0] = carry_f;
+    d.act_n[1] = carry_b;
+    d.act_n[2] = carry_s;
+    *d.any_active = carry_s > 0;
+  }
+}
+
+__global__ __launch_bounds__(kBS) void k_active_lists(SphDfsph d) { build_active_lists(d); }
+
 __global__ __launch_bounds__(kBS) void k_alpha_source(SphFluid f, const SphPhysics* __restrict__ phys,
                                                       SphCSR ff, SphBoundary b, SphCSR fb, SphCSR bf,
-                                                      const int* __restrict__ order, SphDfsph d) {
+                                                      SphDfsph d) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < f.np) {
-    const int i = order ? order[t] : t;
+    const int i = t;
     const SphPhysics P = phys[f.pscene[i]];
+    const KConst K = kconst(P.h);
     const float2 xi = ld2(f.x, i), vi = ld2(d.vpred, i);
     float k1x = 0.f, k1y = 0.f, k2 = 0.f, src = 0.f;
     for (int k = ff.rowptr[i]; k < ff.rowptr[i + 1]; ++k) {
       const int j = ff.col[k];
       const float2 xj = ld2(f.x, j), vj = ld2(d.vpred, j);
-      const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, P.h);
-      const float gs = -spiky_grad(g.q, P.h);
-      const float Gx = gs * g.hx, Gy = gs * g.hy;
+      const float2 G = spiky_G(xi.x, xi.y, xj.x, xj.y, K);
       const float aa = d.aact[j];
-      k1x += aa * Gx;
-      k1y += aa * Gy;
-      k2 += aa * aa / (f.area[j] * f.rest[j]) * (Gx * Gx + Gy * Gy);
-      src += -P.dt * aa * ((vi.x - vj.x) * Gx + (vi.y - vj.y) * Gy);
+      k1x += aa * G.x;
+      k1y += aa * G.y;
+      k2 += aa * aa / (f.area[j] * f.rest[j]) * (G.x * G.x + G.y * G.y);
+      src += -P.dt * aa * ((vi.x - vj.x) * G.x + (vi.y - vj.y) * G.y);
     }
     if (fb.rowptr) {
       for (int k = fb.rowptr[i]; k < fb.rowptr[i + 1]; ++k) {
         const int j = fb.col[k];
         const float2 xb = ld2(b.x, j);
-        const PairGeo g = pair_geo(xi.x, xi.y, xb.x, xb.y, P.h);
-        const float gs = -spiky_grad(g.q, P.h);
-        const float Gx = gs * g.hx, Gy = gs * g.hy;
+        const float2 G = spiky_G(xi.x, xi.y, xb.x, xb.y, K);
         const float aa = d.b_aact[j];
-        k1x += aa * Gx;
-        k1y += aa * Gy;
-        k2 += aa * aa / (b.vol[j] * b.rest[j]) * (Gx * Gx + Gy * Gy);
+        k1x += aa * G.x;
+        k1y += aa * G.y;
+        k2 += aa * aa / (b.vol[j] * b.rest[j]) * (G.x * G.x + G.y * G.y);
         // boundary predicted velocity is zero (static walls)
-        src += -P.dt * aa * (vi.x * Gx + vi.y * Gy);
+        src += -P.dt * aa * (vi.x * G.x + vi.y * G.y);
       }
     }
     const float ai = d.aact[i];
@@ -96,70 +168,66 @@ __global__ __launch_bounds__(kBS) void k_alpha_source(SphFluid f, const SphPhysi
   } else if (t < f.np + b.nb) {
     const int k0 = t - f.np;
     const SphPhysics P = phys[b.bscene[k0]];
+    const KConst K = kconst(P.h);
     const float2 xb = ld2(b.x, k0);
     float s2 = 0.f, src = 0.f;
     for (int k = bf.rowptr[k0]; k < bf.rowptr[k0 + 1]; ++k) {
       const int j = bf.col[k];
       const float2 xf = ld2(f.x, j), vf = ld2(d.vpred, j);
-      const PairGeo g = pair_geo(xb.x, xb.y, xf.x, xf.y, P.h);
-      const float gs = -spiky_grad(g.q, P.h);
-      const float Gx = gs * g.hx, Gy = gs * g.hy;
+      const float2 G = spiky_G(xb.x, xb.y, xf.x, xf.y, K);
       const float aa = d.aact[j];
-      s2 += aa * aa / (f.area[j] * f.rest[j]) * (Gx * Gx + Gy * Gy);
-      src += aa * ((0.f - vf.x) * Gx + (0.f - vf.y) * Gy);
+      s2 += aa * aa / (f.area[j] * f.rest[j]) * (G.x * G.x + G.y * G.y);
+      src += aa * ((0.f - vf.x) * G.x + (0.f - vf.y) * G.y);
     }
     d.b_alpha[k0] = -P.dt * P.dt * d.b_aact[k0] * s2;   // unclamped (akinciBoundary.py:340)
     d.b_src[k0] = 1.f - b.rho[k0] - P.dt * src;
   }
 }
 
-// pressure acceleration a*_i from p2 (fluid) and p_b (boundary)
-__device__ __forceinline__ float2 pressure_accel(const SphFluid& f, const SphPhysics& P,
-                                                 const SphCSR& ff, const SphBoundary& b,
-                                                 const SphCSR& fb, const SphDfsph& d, int i) {
-  const float2 xi = ld2(f.x, i);
-  const float ri = f.rho[i] * f.rest[i];
-  const float pi = d.p2[i] / (ri * ri);
+// a*_i = sum_j -(A_j rho0_j)(pt_i + pt_j) G_ij + boundary (dfsph.py:25-42,
+// akinciBoundary.py:229-252), pt = p / (rho rho0)^2
+__device__ __forceinline__ float2 pressure_accel(const SphCSR& ff, const SphCSR& fb,
+                                                 const SphDfsph& d, const KConst& K, int i) {
+  const float4 ri = ld4(d.xrec, i);
   float ax = 0.f, ay = 0.f;
-  for (int k = ff.rowptr[i]; k < ff.rowptr[i + 1]; ++k) {
-    const int j = ff.col[k];
-    const float2 xj = ld2(f.x, j);
-    const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, P.h);
-    const float gs = -spiky_grad(g.q, P.h);
-    const float rj = f.rho[j] * f.rest[j];
-    const float c = -(f.area[j] * f.rest[j]) * (pi + d.p2[j] / (rj * rj));
-    ax += c * gs * g.hx;
-    ay += c * gs * g.hy;
+  const int k1 = ff.rowptr[i + 1];
+  for (int k = ff.rowptr[i]; k < k1; ++k) {
+    const float4 rj = ld4(d.xrec, ff.col[k]);
+    const float2 G = spiky_G(ri.x, ri.y, rj.x, rj.y, K);
+    const float c = -rj.w * (ri.z + rj.z);
+    ax += c * G.x;
+    ay += c * G.y;
   }
   if (fb.rowptr) {
-    for (int k = fb.rowptr[i]; k < fb.rowptr[i + 1]; ++k) {
-      const int j = fb.col[k];
-      const float2 xb = ld2(b.x, j);
-      const PairGeo g = pair_geo(xi.x, xi.y, xb.x, xb.y, P.h);
-      const float gs = -spiky_grad(g.q, P.h);
-      const float rb = b.rho[j] * b.rest[j];
-      const float c = -(b.vol[j] * b.rest[j]) * (pi + d.b_p[j] / (rb * rb));
-      ax += c * gs * g.hx;
-      ay += c * gs * g.hy;
+    const int e1 = fb.rowptr[i + 1];
+    for (int k = fb.rowptr[i]; k < e1; ++k) {
+      const float4 rb = ld4(d.bxrec, fb.col[k]);
+      const float2 G = spiky_G(ri.x, ri.y, rb.x, rb.y, K);
+      const float c = -rb.w * (ri.z + rb.z);
+      ax += c * G.x;
+      ay += c * G.y;
     }
   }
   return make_float2(ax, ay);
 }
 
 __global__ __launch_bounds__(kBS) void k_accel(SphFluid f, const SphPhysics* __restrict__ phys,
-                                               SphCSR ff, SphBoundary b, SphCSR fb,
-                                               const int* __restrict__ order, SphDfsph d) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= f.np) return;
-  const int i = order ? order[t] : t;
-  const int s = f.pscene[i];
-  if (!d.active[s]) return;
-  const SphPhysics P = phys[s];
-  st2(d.apred, i, pressure_accel(f, P, ff, b, fb, d, i));
+                                               SphCSR ff, SphCSR fb, SphDfsph d) {
+  const int nact = d.act_n[2];
+  const int nitems = d.act_n[0];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nitems; t += gridDim.x * blockDim.x) {
+    const int r = find_rank(d.act_pre, nact, t);
+    const int s = d.act_scene[r];
+    const int i = d.fbegin[s] + (t - d.act_pre[r]);
+    const SphPhysics P = phys[s];
+    const KConst K = kconst(P.h);
+    const float2 a = pressure_accel(ff, fb, d, K, i);
+    st4(d.arec, i, make_float4(a.x, a.y, P.dt * P.dt * d.aact[i], 0.f));
+  }
 }
 
-// Adds v (FP64) into err_acc[scene], one atomic per wavefront when the
-// whole wavefront belongs to one scene.
+// Adds v (FP64) into acc[scene], one atomic per wavefront when the whole
+// wavefront belongs to one scene.
 __device__ __forceinline__ void scene_sum(double v, int s, bool valid, double* acc) {
   const int lane = threadIdx.x & (kWave - 1);
   const int s0 = __shfl(valid ? s : -1, 0, kWave);
@@ -175,82 +243,91 @@ __device__ __forceinline__ void scene_sum(double v, int s, bool valid, double* a
 
 __global__ __launch_bounds__(kBS) void k_pressure(SphFluid f, const SphPhysics* __restrict__ phys,
                                                   SphCSR ff, SphBoundary b, SphCSR fb, SphCSR bf,
-                                                  const int* __restrict__ order, SphDfsph d) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  double contrib = 0.0;
-  int s = -1;
-  bool valid = false;
-  if (t < f.np) {
-    const int i = order ? order[t] : t;
-    s = f.pscene[i];
-    if (d.active[s]) {
+                                                  SphDfsph d) {
+  const int nact = d.act_n[2];
+  const int nf = d.act_n[0];
+  const int nitems = nf + d.act_n[1];
+  const int stride = gridDim.x * blockDim.x;
+  // every lane runs the same number of loop trips so the wavefront-wide
+  // reductions inside see all lanes
+  for (int base = blockIdx.x * blockDim.x; base < nitems; base += stride) {
+    const int t = base + threadIdx.x;
+    double contrib = 0.0;
+    int s = -1;
+    bool valid = false;
+    if (t < nf) {
+      const int r = find_rank(d.act_pre, nact, t);
+      s = d.act_scene[r];
+      const int i = d.fbegin[s] + (t - d.act_pre[r]);
       const SphPhysics P = phys[s];
-      const float2 xi = ld2(f.x, i), ai = ld2(d.apred, i);
+      const KConst K = kconst(P.h);
+      const float2 xi = ld2(d.xrec, 2 * i);          // {x, y} of xrec[i]
+      const float4 ai = ld4(d.arec, i);
       const float dt2 = P.dt * P.dt;
       float ks = 0.f;
-      for (int k = ff.rowptr[i]; k < ff.rowptr[i + 1]; ++k) {
+      const int k1 = ff.rowptr[i + 1];
+      for (int k = ff.rowptr[i]; k < k1; ++k) {
         const int j = ff.col[k];
-        const float2 xj = ld2(f.x, j), aj = ld2(d.apred, j);
-        const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, P.h);
-        const float gs = -spiky_grad(g.q, P.h);
-        const float c = dt2 * d.aact[j];
-        ks += c * ((ai.x - aj.x) * gs * g.hx + (ai.y - aj.y) * gs * g.hy);
+        const float2 xj = ld2(d.xrec, 2 * j);
+        const float4 aj = ld4(d.arec, j);
+        const float2 G = spiky_G(xi.x, xi.y, xj.x, xj.y, K);
+        ks += aj.z * ((ai.x - aj.x) * G.x + (ai.y - aj.y) * G.y);
       }
       if (fb.rowptr) {
-        for (int k = fb.rowptr[i]; k < fb.rowptr[i + 1]; ++k) {
+        const int e1 = fb.rowptr[i + 1];
+        for (int k = fb.rowptr[i]; k < e1; ++k) {
           const int j = fb.col[k];
-          const float2 xb = ld2(b.x, j);
-          const PairGeo g = pair_geo(xi.x, xi.y, xb.x, xb.y, P.h);
-          const float gs = -spiky_grad(g.q, P.h);
-          const float c = dt2 * d.b_aact[j];
-          ks += c * (ai.x * gs * g.hx + ai.y * gs * g.hy);
+          const float2 xb = ld2(d.bxrec, 2 * j);
+          const float2 G = spiky_G(xi.x, xi.y, xb.x, xb.y, K);
+          ks += dt2 * d.b_aact[j] * (ai.x * G.x + ai.y * G.y);
         }
       }
-      const float r = ks - d.src[i];
+      const float rr = ks - d.src[i];
       const float pold = d.p2[i];
       d.p[i] = pold;
-      float pn = pold - P.fluid_omega * r / d.alpha[i];
+      float pn = pold - P.fluid_omega * rr / d.alpha[i];
       pn = pn < 0.f ? 0.f : pn;  // clamp(min = 0) keeps NaN
       if (isnan(pn) || isinf(pn)) atomicOr(&d.status[s], SPH_ST_NAN);
       d.p2[i] = pn;
-      d.resid[i] = r;
-      contrib = (double)(r < -P.tau ? -P.tau : r);
+      d.resid[i] = rr;
+      const float rho = f.rho[i] * f.rest[i];
+      d.xrec[4 * i + 2] = pn / (rho * rho);
+      contrib = (double)(rr < -P.tau ? -P.tau : rr);
       valid = true;
-    }
-  } else if (t < f.np + b.nb) {
-    const int k0 = t - f.np;
-    s = b.bscene[k0];
-    if (d.active[s]) {
+    } else if (t < nitems) {
+      const int tb = t - nf;
+      const int* preb = d.act_pre + d.nscenes + 1;
+      const int r = find_rank(preb, nact, tb);
+      s = d.act_scene[r];
+      const int k0 = d.bbegin[s] + (tb - preb[r]);
       const SphPhysics P = phys[s];
-      const float2 xb = ld2(b.x, k0);
-      const float dt2 = P.dt * P.dt;
+      const KConst K = kconst(P.h);
+      const float2 xb = ld2(d.bxrec, 2 * k0);
       float ks = 0.f;
       for (int k = bf.rowptr[k0]; k < bf.rowptr[k0 + 1]; ++k) {
         const int j = bf.col[k];
-        const float2 xf = ld2(f.x, j), af = ld2(d.apred, j);
-        const PairGeo g = pair_geo(xb.x, xb.y, xf.x, xf.y, P.h);
-        const float gs = -spiky_grad(g.q, P.h);
-        const float c = dt2 * d.aact[j];
-        ks += c * (af.x * (-gs * g.hx) + af.y * (-gs * g.hy));
+        const float2 xf = ld2(d.xrec, 2 * j);
+        const float4 af = ld4(d.arec, j);
+        const float2 G = spiky_G(xb.x, xb.y, xf.x, xf.y, K);
+        ks += af.z * (af.x * (-G.x) + af.y * (-G.y));
       }
-      const float r = ks - d.b_src[k0];
-      float pn = d.b_p[k0] - P.omega * r / d.b_alpha[k0];
+      const float rr = ks - d.b_src[k0];
+      float pn = d.b_p[k0] - P.omega * rr / d.b_alpha[k0];
       pn = pn < 0.f ? 0.f : pn;
       d.b_p[k0] = pn;
-      d.b_resid[k0] = r;
-      contrib = (double)(r < -P.tau ? -P.tau : r);
+      d.b_resid[k0] = rr;
+      const float rb = b.rho[k0] * b.rest[k0];
+      d.bxrec[4 * k0 + 2] = pn / (rb * rb);
+      contrib = (double)(rr < -P.tau ? -P.tau : rr);
       valid = true;
     }
+    scene_sum(contrib, s, valid, d.err_acc);
   }
-  scene_sum(contrib, s, valid, d.err_acc);
 }
 
 // Per-scene loop control, densitySolve (dfsph.py:205-207): after iteration
 // i the solve continues while (i < minIters or err > tau) and i <= maxIters.
 __global__ __launch_bounds__(kBS) void k_control(const SphPhysics* __restrict__ phys, SphDfsph d) {
-  __shared__ int any;
-  if (threadIdx.x == 0) any = 0;
-  __syncthreads();
   for (int s = threadIdx.x; s < d.nscenes; s += blockDim.x) {
     if (!d.active[s]) continue;
     const SphPhysics P = phys[s];
@@ -265,23 +342,20 @@ __global__ __launch_bounds__(kBS) void k_control(const SphPhysics* __restrict__ 
     if (!go) {
       d.active[s] = 0;
       if (err > (double)P.tau) atomicOr(&d.status[s], SPH_ST_ITERCAP);
-    } else {
-      atomicOr(&any, 1);
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) *d.any_active = any;
+  build_active_lists(d);
 }
 
 __global__ __launch_bounds__(kBS) void k_finish(SphFluid f, const SphPhysics* __restrict__ phys,
-                                                SphCSR ff, SphBoundary b, SphCSR fb,
-                                                const int* __restrict__ order, SphDfsph d,
+                                                SphCSR ff, SphCSR fb, SphDfsph d,
                                                 float* __restrict__ acc) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= f.np) return;
-  const int i = order ? order[t] : t;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.np) return;
   const SphPhysics P = phys[f.pscene[i]];
-  const float2 a = pressure_accel(f, P, ff, b, fb, d, i);
+  const KConst K = kconst(P.h);
+  const float2 a = pressure_accel(ff, fb, d, K, i);
   st2(d.apred, i, a);
   d.p[i] = d.p2[i];
   const float2 vp = ld2(d.vpred, i);
@@ -296,8 +370,9 @@ __global__ __launch_bounds__(kBS) void k_finish(SphFluid f, const SphPhysics* __
 
 static bool dfsph_ok(const SphDfsph* d) {
   return d && d->vpred && d->aact && d->alpha && d->src && d->p && d->p2 && d->apred &&
-         d->resid && d->err_acc && d->iters && d->active && d->any_active && d->status &&
-         d->nfluid && d->nbdy && d->nscenes > 0;
+         d->resid && d->xrec && d->arec && d->err_acc && d->iters && d->active &&
+         d->any_active && d->status && d->nfluid && d->nbdy && d->fbegin && d->bbegin &&
+         d->act_scene && d->act_pre && d->act_n && d->nscenes > 0 && d->grid_blocks > 0;
 }
 static SphCSR csr_or_null(const SphCSR* c) {
   SphCSR r = {nullptr, nullptr, 0};
@@ -309,58 +384,61 @@ static SphBoundary bdy_or_null(const SphBoundary* b) {
   if (b) r = *b;
   return r;
 }
+static bool bdy_ok(const SphBoundary& bb, const SphDfsph* d) {
+  return bb.nb == 0 || (bb.x && bb.vol && bb.rho && bb.rest && bb.bscene && d->b_aact &&
+                        d->b_alpha && d->b_src && d->b_p && d->b_resid && d->bxrec);
+}
 
 int dfsph_prepare(const SphFluid* f, const float* acc, const SphPhysics* phys, const SphBoundary* b,
                   SphDfsph* d, hipStream_t s) {
-  SPH_REQUIRE(f && f->v && f->area && f->rho && f->pscene && phys && dfsph_ok(d),
+  SPH_REQUIRE(f && f->x && f->v && f->area && f->rho && f->rest && f->pscene && phys &&
+                  dfsph_ok(d),
               "sph_dfsph_prepare: bad arguments");
   const SphBoundary bb = bdy_or_null(b);
-  SPH_REQUIRE(bb.nb == 0 || (bb.vol && bb.rho && d->b_aact && d->b_p),
-              "sph_dfsph_prepare: bad boundary");
+  SPH_REQUIRE(bdy_ok(bb, d), "sph_dfsph_prepare: bad boundary");
   int n = f->np + bb.nb;
   if (n < d->nscenes) n = d->nscenes;
   if (n < 1) n = 1;
   hipLaunchKernelGGL(k_prepare, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, *f, acc, phys, bb, *d);
+  hipLaunchKernelGGL(k_active_lists, dim3(1), dim3(kBS), 0, s, *d);
   return launch_check("sph_dfsph_prepare");
 }
 
 int dfsph_alpha_source(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
-                       const SphBoundary* b, const SphCSR* fb, const SphCSR* bf, const int* order,
-                       SphDfsph* d, hipStream_t s) {
+                       const SphBoundary* b, const SphCSR* fb, const SphCSR* bf, SphDfsph* d,
+                       hipStream_t s) {
   SPH_REQUIRE(f && phys && ff && ff->rowptr && dfsph_ok(d), "sph_dfsph_alpha_source: bad arguments");
   const SphBoundary bb = bdy_or_null(b);
-  SPH_REQUIRE(bb.nb == 0 || (bf && bf->rowptr && d->b_alpha && d->b_src),
+  SPH_REQUIRE(bdy_ok(bb, d) && (bb.nb == 0 || (bf && bf->rowptr)),
               "sph_dfsph_alpha_source: boundary needs its pair list");
   const int n = f->np + bb.nb;
   if (n == 0) return SPH_OK;
   hipLaunchKernelGGL(k_alpha_source, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, *f, phys, *ff, bb,
-                     csr_or_null(fb), csr_or_null(bf), order, *d);
+                     csr_or_null(fb), csr_or_null(bf), *d);
   return launch_check("sph_dfsph_alpha_source");
 }
 
 int dfsph_iterate(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, const SphBoundary* b,
-                  const SphCSR* fb, const SphCSR* bf, const int* order, SphDfsph* d,
-                  hipStream_t s) {
-  SPH_REQUIRE(f && phys && ff && ff->rowptr && dfsph_ok(d), "sph_dfsph_iterate: bad arguments");
+                  const SphCSR* fb, const SphCSR* bf, SphDfsph* d, hipStream_t s) {
+  SPH_REQUIRE(f && f->rho && f->rest && phys && ff && ff->rowptr && dfsph_ok(d),
+              "sph_dfsph_iterate: bad arguments");
   const SphBoundary bb = bdy_or_null(b);
-  SPH_REQUIRE(bb.nb == 0 || (bf && bf->rowptr && d->b_resid), "sph_dfsph_iterate: bad boundary");
-  if (f->np > 0)
-    hipLaunchKernelGGL(k_accel, dim3(blocks_for(f->np, kBS)), dim3(kBS), 0, s, *f, phys, *ff, bb,
-                       csr_or_null(fb), order, *d);
-  const int n = f->np + bb.nb;
-  if (n > 0)
-    hipLaunchKernelGGL(k_pressure, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, *f, phys, *ff, bb,
-                       csr_or_null(fb), csr_or_null(bf), order, *d);
+  SPH_REQUIRE(bdy_ok(bb, d) && (bb.nb == 0 || (bf && bf->rowptr)), "sph_dfsph_iterate: bad boundary");
+  const dim3 grid(d->grid_blocks), block(kBS);
+  hipLaunchKernelGGL(k_accel, grid, block, 0, s, *f, phys, *ff, csr_or_null(fb), *d);
+  hipLaunchKernelGGL(k_pressure, grid, block, 0, s, *f, phys, *ff, bb, csr_or_null(fb),
+                     csr_or_null(bf), *d);
   hipLaunchKernelGGL(k_control, dim3(1), dim3(kBS), 0, s, phys, *d);
   return launch_check("sph_dfsph_iterate");
 }
 
-int dfsph_finish(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, const SphBoundary* b,
-                 const SphCSR* fb, const int* order, SphDfsph* d, float* acc, hipStream_t s) {
-  SPH_REQUIRE(f && phys && ff && ff->rowptr && dfsph_ok(d), "sph_dfsph_finish: bad arguments");
+int dfsph_finish(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, const SphCSR* fb,
+                 SphDfsph* d, float* acc, hipStream_t s) {
+  SPH_REQUIRE(f && f->pscene && phys && ff && ff->rowptr && dfsph_ok(d),
+              "sph_dfsph_finish: bad arguments");
   if (f->np == 0) return SPH_OK;
   hipLaunchKernelGGL(k_finish, dim3(blocks_for(f->np, kBS)), dim3(kBS), 0, s, *f, phys, *ff,
-                     bdy_or_null(b), csr_or_null(fb), order, *d, acc);
+                     csr_or_null(fb), *d, acc);
   return launch_check("sph_dfsph_finish");
 }
 
@@ -372,18 +450,17 @@ int sph_dfsph_prepare(const SphFluid* f, const float* acc, const SphPhysics* phy
   return sph::dfsph_prepare(f, acc, phys, b, d, s);
 }
 int sph_dfsph_alpha_source(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
-                           const SphBoundary* b, const SphCSR* fb, const SphCSR* bf,
-                           const int* order, SphDfsph* d, hipStream_t s) {
-  return sph::dfsph_alpha_source(f, phys, ff, b, fb, bf, order, d, s);
+                           const SphBoundary* b, const SphCSR* fb, const SphCSR* bf, SphDfsph* d,
+                           hipStream_t s) {
+  return sph::dfsph_alpha_source(f, phys, ff, b, fb, bf, d, s);
 }
 int sph_dfsph_iterate(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
-                      const SphBoundary* b, const SphCSR* fb, const SphCSR* bf, const int* order,
-                      SphDfsph* d, hipStream_t s) {
-  return sph::dfsph_iterate(f, phys, ff, b, fb, bf, order, d, s);
+                      const SphBoundary* b, const SphCSR* fb, const SphCSR* bf, SphDfsph* d,
+                      hipStream_t s) {
+  return sph::dfsph_iterate(f, phys, ff, b, fb, bf, d, s);
 }
 int sph_dfsph_finish(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
-                     const SphBoundary* b, const SphCSR* fb, const int* order, SphDfsph* d,
-                     float* acc, hipStream_t s) {
-  return sph::dfsph_finish(f, phys, ff, b, fb, order, d, acc, s);
+                     const SphCSR* fb, SphDfsph* d, float* acc, hipStream_t s) {
+  return sph::dfsph_finish(f, phys, ff, fb, d, acc, s);
 }
 }

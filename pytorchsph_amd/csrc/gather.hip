@@ -2,9 +2,15 @@
 // models.  Every gather is a CSR row gather: one thread owns one particle,
 // walks its fluid neighbours and then its boundary neighbours, keeps all
 // sums in registers and writes each output once -- no atomics, so results
-// do not depend on scheduling.  Threads visit particles in the grid's sorted
-// order (`order`) so that the neighbours of a wavefront are spatially close
-// and their state is served from L2.
+// do not depend on scheduling.
+//
+// Two forms of the delta-SPH gather share one pair routine:
+//  - sph_dsph_gather reads the structure-of-arrays state directly (the
+//    module-level API, any subset of terms);
+//  - sph_dsph_pack + sph_dsph_gather_packed first write one 32-byte record
+//    per particle {x, y, vx, vy, rho, V, p, rho_hat} so that a neighbour is
+//    two 16-byte loads from one cache line, with V = A / rho_hat and the EOS
+//    evaluated once per particle instead of once per pair.
 #include "sph_common.h"
 
 namespace sph {
@@ -28,14 +34,14 @@ __global__ __launch_bounds__(kBS) void k_fluid_density(SphFluid f, const SphPhys
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= f.np) return;
   const int i = order ? order[t] : t;
-  const SphPhysics P = phys[f.pscene[i]];
+  const KConst K = kconst(phys[f.pscene[i]].h);
   const float2 xi = ld2(f.x, i);
   float sum = 0.f;
   for (int k = ff.rowptr[i]; k < ff.rowptr[i + 1]; ++k) {
     const int j = ff.col[k];
     const float2 xj = ld2(f.x, j);
-    const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, P.h);
-    sum += f.area[j] * wendland(g.q, P.h);
+    const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, K.inv_h);
+    sum += f.area[j] * wendland_k(g.q, K);
   }
   if (vol_out) vol_out[i] = f.area[i] / sum;
   if (fb.rowptr) {
@@ -43,20 +49,25 @@ __global__ __launch_bounds__(kBS) void k_fluid_density(SphFluid f, const SphPhys
     for (int k = fb.rowptr[i]; k < fb.rowptr[i + 1]; ++k) {
       const int j = fb.col[k];
       const float2 xb = ld2(b.x, j);
-      const PairGeo g = pair_geo(xb.x, xb.y, xi.x, xi.y, P.h);
-      bs += wendland(g.q, P.h) * b.vol[j];
+      const PairGeo g = pair_geo(xb.x, xb.y, xi.x, xi.y, K.inv_h);
+      bs += wendland_k(g.q, K) * b.vol[j];
     }
     sum += bs;
   }
   rho_out[i] = sum;
 }
 
-// ------------------------------------------------- delta-SPH fused gather
+// ------------------------------------------------- delta-SPH pair routine
 struct Nbr {
-  float2 x, v;
-  float rhoh;  // normalised density
-  float vol;   // volume A / rhoh (fluid) or V_b / rhoh_b (boundary)
+  float x, y, vx, vy;
+  float rho;   // rho_hat * rho0
+  float vol;   // volume A / rho_hat (fluid) or V_b / rho_hat_b (boundary)
   float p;     // pressure
+};
+
+struct Own {
+  float x, y, vx, vy, rho, p;
+  float gx, gy;  // renormalised density gradient (DDSPH)
 };
 
 struct Acc {
@@ -64,54 +75,86 @@ struct Acc {
   float px, py, vx, vy, lx, ly;
 };
 
-template <unsigned T>
-__device__ __forceinline__ void dsph_pair(const SphPhysics& P, float2 xi, float2 vi, float rho_i,
-                                          float p_i, float2 gri, const Nbr& n, float2 grj,
-                                          bool fluid, unsigned terms, Acc& a) {
-  const PairGeo g = pair_geo(xi.x, xi.y, n.x.x, n.x.y, P.h);
-  const float gw = -wendland_grad(g.q, P.h);
+__device__ __forceinline__ bool has(unsigned terms, unsigned bit) { return (terms & bit) != 0; }
+
+// One pair of the delta-SPH right-hand side (fluid row i, neighbour j).
+// gradW_ij = -g(q) rhat_ij, x_ji = x_j - x_i, r2e = |x_ji|^2 + eps.
+__device__ __forceinline__ void dsph_pair(const SphPhysics& P, const KConst& K, const Own& o,
+                                          const Nbr& n, float grjx, float grjy, bool fluid,
+                                          unsigned terms, Acc& a) {
+  const PairGeo g = pair_geo(o.x, o.y, n.x, n.y, K.inv_h);
+  const float gw = -wendland_grad_k(g.q, K);
   const float gWx = gw * g.hx, gWy = gw * g.hy;
-  const float xjix = -g.rx, xjiy = -g.ry;           // x_j - x_i
-  const float rji2 = g.d * g.d + P.eps;
-  const float vjix = n.v.x - vi.x, vjiy = n.v.y - vi.y;
-  const float rho_j = n.rhoh * P.rho0;
-  const bool cont = fluid ? (terms & SPH_T_CONT) : (terms & SPH_T_CONT_B);
-  const bool press = fluid ? (terms & SPH_T_PRESS) : (terms & SPH_T_PRESS_B);
-  const bool visc = fluid ? (terms & SPH_T_VISC) : (terms & SPH_T_VISC_B);
-  if (cont) a.cont += (vjix * gWx + vjiy * gWy) * n.vol * rho_j;
-  if (fluid && (terms & SPH_T_MOG))
-    a.mog += 2.f * (rho_j - rho_i) / rji2 * (xjix * gWx + xjiy * gWy) * n.vol;
-  if (fluid && (terms & SPH_T_DDSPH)) {
-    const float c = 2.f * (rho_j - rho_i) / rji2;
-    const float psx = c * xjix - (gri.x + grj.x);
-    const float psy = c * xjiy - (gri.y + grj.y);
+  const float xgw = -(g.rx * gWx + g.ry * gWy);     // x_ji . gradW
+  const float inv_r2e = fast_rcp(g.d2 + P.eps);
+  const float vjix = n.vx - o.vx, vjiy = n.vy - o.vy;
+  const bool cont = fluid ? has(terms, SPH_T_CONT) : has(terms, SPH_T_CONT_B);
+  const bool press = fluid ? has(terms, SPH_T_PRESS) : has(terms, SPH_T_PRESS_B);
+  const bool visc = fluid ? has(terms, SPH_T_VISC) : has(terms, SPH_T_VISC_B);
+  // momentum.py:31-38
+  if (cont) a.cont += (vjix * gWx + vjiy * gWy) * n.vol * n.rho;
+  // densityDiffusion.py:74-82
+  if (fluid && has(terms, SPH_T_MOG)) a.mog += 2.f * (n.rho - o.rho) * inv_r2e * xgw * n.vol;
+  // densityDiffusion.py:63-71
+  if (fluid && has(terms, SPH_T_DDSPH)) {
+    const float c = 2.f * (n.rho - o.rho) * inv_r2e;
+    const float psx = -c * g.rx - (o.gx + grjx);
+    const float psy = -c * g.ry - (o.gy + grjy);
     a.dd += (psx * gWx + psy * gWy) * n.vol;
   }
+  // pressure.py:31-38
   if (press) {
-    const float c = (p_i + n.p) * n.vol;
+    const float c = (o.p + n.p) * n.vol;
     a.px += c * gWx;
     a.py += c * gWy;
   }
-  if (visc) {
-    float pij = vjix * xjix + vjiy * xjiy;
-    pij = pij > 0.f ? 0.f : pij;
-    pij = pij / rji2;
-    const float c = pij * n.vol * rho_j / (rho_i + rho_j);
-    a.vx += c * gWx;
-    a.vy += c * gWy;
-  }
-  if (fluid && (terms & SPH_T_LAM)) {
-    // laminar.py:35-45: mu_i + mu_j = 2 rho_i nu (muj indexes rhoj by i)
-    const float mj = rho_j * n.vol;
-    const float num = 4.f * mj * (2.f * rho_i * P.nu) * (xjix * gWx + xjiy * gWy);
-    const float rs = rho_i + rho_j;
-    const float term = num / (rs * rs * rji2);
-    a.lx += term * (vi.x - n.v.x);
-    a.ly += term * (vi.y - n.v.y);
+  if (visc || (fluid && has(terms, SPH_T_LAM))) {
+    const float inv_rs = fast_rcp(o.rho + n.rho);
+    // diffusion.py:30-44
+    if (visc) {
+      float pij = -(vjix * g.rx + vjiy * g.ry);     // v_ji . x_ji
+      pij = pij > 0.f ? 0.f : pij;
+      const float c = pij * inv_r2e * n.vol * n.rho * inv_rs;
+      a.vx += c * gWx;
+      a.vy += c * gWy;
+    }
+    // laminar.py:30-46: mu_i + mu_j = 2 rho_i nu (muj indexes rhoj by i)
+    if (fluid && has(terms, SPH_T_LAM)) {
+      const float mj = n.rho * n.vol;
+      const float term = 4.f * mj * (2.f * o.rho * P.nu) * xgw * inv_rs * inv_rs * inv_r2e;
+      a.lx += term * (-vjix);
+      a.ly += term * (-vjiy);
+    }
   }
 }
 
-template <unsigned T>
+__device__ __forceinline__ void dsph_store(const SphPhysics& P, const Own& o, const Acc& a,
+                                           unsigned terms, int i, float* dpdt, float* acc) {
+  const float dp = -a.cont + P.h * P.delta * P.c0 * (a.mog + a.dd);
+  const float inv_rho = 1.f / o.rho;
+  float ax = -a.px * inv_rho + P.h * P.alpha * P.c0 * a.vx - a.lx;
+  float ay = -a.py * inv_rho + P.h * P.alpha * P.c0 * a.vy - a.ly;
+  if (has(terms, SPH_T_GRAV)) {
+    ax += P.gx;
+    ay += P.gy;
+  }
+  if (has(terms, SPH_T_ACCUM)) {
+    if (dpdt) dpdt[i] += dp;
+    if (acc) {
+      const float2 v = ld2(acc, i);
+      st2(acc, i, make_float2(v.x + ax, v.y + ay));
+    }
+  } else {
+    if (dpdt) dpdt[i] = dp;
+    if (acc) st2(acc, i, make_float2(ax, ay));
+  }
+}
+
+constexpr unsigned kFluidPairTerms =
+    SPH_T_CONT | SPH_T_MOG | SPH_T_DDSPH | SPH_T_PRESS | SPH_T_VISC | SPH_T_LAM;
+constexpr unsigned kBdyPairTerms = SPH_T_CONT_B | SPH_T_PRESS_B | SPH_T_VISC_B;
+
+// structure-of-arrays form (any runtime term mask)
 __global__ __launch_bounds__(kBS) void k_dsph_gather(SphFluid f, const SphPhysics* __restrict__ phys,
                                                      SphCSR ff, SphBoundary b, SphCSR fb,
                                                      const float* __restrict__ gradrho,
@@ -122,55 +165,108 @@ __global__ __launch_bounds__(kBS) void k_dsph_gather(SphFluid f, const SphPhysic
   if (t >= f.np) return;
   const int i = order ? order[t] : t;
   const SphPhysics P = phys[f.pscene[i]];
+  const KConst K = kconst(P.h);
+  const bool dd = has(terms, SPH_T_DDSPH);
+  Own o;
   const float2 xi = ld2(f.x, i), vi = ld2(f.v, i);
-  const float rhoh_i = f.rho[i];
-  const float rho_i = rhoh_i * P.rho0;
-  const float p_i = eos(P, rhoh_i);
-  const bool dd = terms & SPH_T_DDSPH;
+  o.x = xi.x; o.y = xi.y; o.vx = vi.x; o.vy = vi.y;
+  o.rho = f.rho[i] * P.rho0;
+  o.p = eos(P, f.rho[i]);
   const float2 gri = dd ? ld2(gradrho, i) : make_float2(0.f, 0.f);
+  o.gx = gri.x; o.gy = gri.y;
   Acc a = {};
-  const bool pair_terms = terms & (SPH_T_CONT | SPH_T_MOG | SPH_T_DDSPH | SPH_T_PRESS | SPH_T_VISC | SPH_T_LAM);
-  const int kend = pair_terms ? ff.rowptr[i + 1] : 0;
-  for (int k = pair_terms ? ff.rowptr[i] : 0; k < kend; ++k) {
-    const int j = ff.col[k];
-    Nbr n;
-    n.x = ld2(f.x, j);
-    n.v = ld2(f.v, j);
-    n.rhoh = f.rho[j];
-    n.vol = f.vol ? f.vol[j] : f.area[j] / n.rhoh;
-    n.p = (terms & SPH_T_PRESS) ? eos(P, n.rhoh) : 0.f;
-    const float2 grj = dd ? ld2(gradrho, j) : make_float2(0.f, 0.f);
-    dsph_pair<T>(P, xi, vi, rho_i, p_i, gri, n, grj, true, terms, a);
+  if (terms & kFluidPairTerms) {
+    for (int k = ff.rowptr[i]; k < ff.rowptr[i + 1]; ++k) {
+      const int j = ff.col[k];
+      const float2 xj = ld2(f.x, j), vj = ld2(f.v, j);
+      Nbr n;
+      n.x = xj.x; n.y = xj.y; n.vx = vj.x; n.vy = vj.y;
+      n.rho = f.rho[j] * P.rho0;
+      n.vol = f.vol ? f.vol[j] : f.area[j] / f.rho[j];
+      n.p = has(terms, SPH_T_PRESS) ? eos(P, f.rho[j]) : 0.f;
+      const float2 grj = dd ? ld2(gradrho, j) : make_float2(0.f, 0.f);
+      dsph_pair(P, K, o, n, grj.x, grj.y, true, terms, a);
+    }
   }
-  if (fb.rowptr && (terms & (SPH_T_CONT_B | SPH_T_PRESS_B | SPH_T_VISC_B))) {
+  if (fb.rowptr && (terms & kBdyPairTerms)) {
     for (int k = fb.rowptr[i]; k < fb.rowptr[i + 1]; ++k) {
       const int j = fb.col[k];
+      const float2 xb = ld2(b.x, j);
+      const float2 vb = b.v ? ld2(b.v, j) : make_float2(0.f, 0.f);
       Nbr n;
-      n.x = ld2(b.x, j);
-      n.v = b.v ? ld2(b.v, j) : make_float2(0.f, 0.f);
-      n.rhoh = b.rho[j];
-      n.vol = b.vol[j] / n.rhoh;
+      n.x = xb.x; n.y = xb.y; n.vx = vb.x; n.vy = vb.y;
+      n.rho = b.rho[j] * P.rho0;
+      n.vol = b.vol[j] / b.rho[j];
       n.p = b.p ? b.p[j] : 0.f;
-      dsph_pair<T>(P, xi, vi, rho_i, p_i, gri, n, make_float2(0.f, 0.f), false, terms, a);
+      dsph_pair(P, K, o, n, 0.f, 0.f, false, terms, a);
     }
   }
-  const float dp = -a.cont + P.h * P.delta * P.c0 * (a.mog + a.dd);
-  float ax = -a.px / rho_i + P.h * P.alpha * P.c0 * a.vx - a.lx;
-  float ay = -a.py / rho_i + P.h * P.alpha * P.c0 * a.vy - a.ly;
-  if (terms & SPH_T_GRAV) {
-    ax += P.gx;
-    ay += P.gy;
+  dsph_store(P, o, a, terms, i, dpdt, acc);
+}
+
+// packed records: rec[2 i] = {x, y, vx, vy}, rec[2 i + 1] = {rho, V, p, rho_hat}
+__global__ __launch_bounds__(kBS) void k_dsph_pack(SphFluid f, const SphPhysics* __restrict__ phys,
+                                                   SphBoundary b, float4* __restrict__ frec,
+                                                   float4* __restrict__ brec) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < f.np) {
+    const SphPhysics P = phys[f.pscene[t]];
+    const float2 x = ld2(f.x, t), v = ld2(f.v, t);
+    const float rh = f.rho[t];
+    frec[2 * t] = make_float4(x.x, x.y, v.x, v.y);
+    frec[2 * t + 1] = make_float4(rh * P.rho0, f.vol ? f.vol[t] : f.area[t] / rh, eos(P, rh), rh);
+  } else if (t < f.np + b.nb) {
+    const int k = t - f.np;
+    const SphPhysics P = phys[b.bscene[k]];
+    const float2 x = ld2(b.x, k);
+    const float2 v = b.v ? ld2(b.v, k) : make_float2(0.f, 0.f);
+    const float rh = b.rho[k];
+    brec[2 * k] = make_float4(x.x, x.y, v.x, v.y);
+    brec[2 * k + 1] = make_float4(rh * P.rho0, b.vol[k] / rh, b.p ? b.p[k] : 0.f, rh);
   }
-  if (terms & SPH_T_ACCUM) {
-    if (dpdt) dpdt[i] += dp;
-    if (acc) {
-      float2 o = ld2(acc, i);
-      st2(acc, i, make_float2(o.x + ax, o.y + ay));
+}
+
+template <unsigned T>
+__global__ __launch_bounds__(kBS) void k_dsph_gather_packed(
+    const float4* __restrict__ frec, const float4* __restrict__ brec,
+    const int* __restrict__ pscene, int np, const SphPhysics* __restrict__ phys, SphCSR ff,
+    SphCSR fb, const float* __restrict__ gradrho, unsigned rterms, const int* __restrict__ order,
+    float* __restrict__ dpdt, float* __restrict__ acc) {
+  const unsigned terms = T ? T : rterms;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np) return;
+  const int i = order ? order[t] : t;
+  const SphPhysics P = phys[pscene[i]];
+  const KConst K = kconst(P.h);
+  const bool dd = has(terms, SPH_T_DDSPH);
+  const float4 a0 = frec[2 * i], a1 = frec[2 * i + 1];
+  Own o;
+  o.x = a0.x; o.y = a0.y; o.vx = a0.z; o.vy = a0.w;
+  o.rho = a1.x;
+  o.p = a1.z;
+  const float2 gri = dd ? ld2(gradrho, i) : make_float2(0.f, 0.f);
+  o.gx = gri.x; o.gy = gri.y;
+  Acc a = {};
+  if (terms & kFluidPairTerms) {
+    const int k1 = ff.rowptr[i + 1];
+    for (int k = ff.rowptr[i]; k < k1; ++k) {
+      const int j = ff.col[k];
+      const float4 r0 = frec[2 * j], r1 = frec[2 * j + 1];
+      const Nbr n = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z};
+      const float2 grj = dd ? ld2(gradrho, j) : make_float2(0.f, 0.f);
+      dsph_pair(P, K, o, n, grj.x, grj.y, true, terms, a);
     }
-  } else {
-    if (dpdt) dpdt[i] = dp;
-    if (acc) st2(acc, i, make_float2(ax, ay));
   }
+  if (fb.rowptr && (terms & kBdyPairTerms)) {
+    const int k1 = fb.rowptr[i + 1];
+    for (int k = fb.rowptr[i]; k < k1; ++k) {
+      const int j = fb.col[k];
+      const float4 r0 = brec[2 * j], r1 = brec[2 * j + 1];
+      const Nbr n = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z};
+      dsph_pair(P, K, o, n, 0.f, 0.f, false, terms, a);
+    }
+  }
+  dsph_store(P, o, a, terms, i, dpdt, acc);
 }
 
 // ------------------------------------------ renormalised density gradient
@@ -222,6 +318,8 @@ __global__ __launch_bounds__(kBS) void k_pinv2x2(const float* __restrict__ M, in
   }
 }
 
+// M_i = sum_j V_j (x_j - x_i) (x) gradW_ij (densityDiffusion.py:30-43) with
+// the boundary part (solidBoundary.py:684-687); L_i = pinv2x2(M_i).
 __global__ __launch_bounds__(kBS) void k_dsph_normalization(SphFluid f, const SphPhysics* __restrict__ phys,
                                                             SphCSR ff, SphBoundary b, SphCSR fb,
                                                             const int* __restrict__ order,
@@ -231,31 +329,19 @@ __global__ __launch_bounds__(kBS) void k_dsph_normalization(SphFluid f, const Sp
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= f.np) return;
   const int i = order ? order[t] : t;
-  const SphPhysics P = phys[f.pscene[i]];
+  const KConst K = kconst(phys[f.pscene[i]].h);
   const float2 xi = ld2(f.x, i);
   float m00 = 0.f, m01 = 0.f, m10 = 0.f, m11 = 0.f;
-  // M_i = sum_j V_j (x_j - x_i) (x) gradW_ij  (densityDiffusion.py:30-43)
-  for (int k = ff.rowptr[i]; k < ff.rowptr[i + 1]; ++k) {
-    const int j = ff.col[k];
-    const float2 xj = ld2(f.x, j);
-    const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, P.h);
-    const float gw = -wendland_grad(g.q, P.h);
-    const float gx = gw * g.hx, gy = gw * g.hy;
-    const float V = f.area[j] / f.rho[j];
-    const float rx = -g.rx, ry = -g.ry;
-    m00 += rx * gx * V;
-    m01 += rx * gy * V;
-    m10 += ry * gx * V;
-    m11 += ry * gy * V;
-  }
-  if (fb.rowptr) {
-    for (int k = fb.rowptr[i]; k < fb.rowptr[i + 1]; ++k) {
-      const int j = fb.col[k];
-      const float2 xb = ld2(b.x, j);
-      const PairGeo g = pair_geo(xi.x, xi.y, xb.x, xb.y, P.h);
-      const float gw = -wendland_grad(g.q, P.h);
+  for (int pass = 0; pass < 2; ++pass) {
+    const SphCSR& c = pass == 0 ? ff : fb;
+    if (!c.rowptr) continue;
+    for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
+      const int j = c.col[k];
+      const float2 xj = pass == 0 ? ld2(f.x, j) : ld2(b.x, j);
+      const float V = pass == 0 ? f.area[j] / f.rho[j] : b.vol[j] / b.rho[j];
+      const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, K.inv_h);
+      const float gw = -wendland_grad_k(g.q, K);
       const float gx = gw * g.hx, gy = gw * g.hy;
-      const float V = b.vol[j] / b.rho[j];
       const float rx = -g.rx, ry = -g.ry;
       m00 += rx * gx * V;
       m01 += rx * gy * V;
@@ -278,6 +364,8 @@ __global__ __launch_bounds__(kBS) void k_dsph_normalization(SphFluid f, const Sp
   }
 }
 
+// <grad rho>_i = -sum_j 2 (rho_j - rho_i) V_j L_i gradW_ij
+// (densityDiffusion.py:45-61, solidBoundary.py:704-709)
 __global__ __launch_bounds__(kBS) void k_dsph_density_gradient(SphFluid f,
                                                                const SphPhysics* __restrict__ phys,
                                                                SphCSR ff, SphBoundary b, SphCSR fb,
@@ -288,33 +376,26 @@ __global__ __launch_bounds__(kBS) void k_dsph_density_gradient(SphFluid f,
   if (t >= f.np) return;
   const int i = order ? order[t] : t;
   const SphPhysics P = phys[f.pscene[i]];
+  const KConst K = kconst(P.h);
   const float2 xi = ld2(f.x, i);
   const float rho_i = f.rho[i] * P.rho0;
   const float l00 = L[4 * i], l01 = L[4 * i + 1], l10 = L[4 * i + 2], l11 = L[4 * i + 3];
   float sx = 0.f, sy = 0.f;
-  // <grad rho>_i = -sum_j 2 (rho_j - rho_i) V_j L_i gradW_ij (densityDiffusion.py:45-61)
-  for (int k = ff.rowptr[i]; k < ff.rowptr[i + 1]; ++k) {
-    const int j = ff.col[k];
-    const float2 xj = ld2(f.x, j);
-    const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, P.h);
-    const float gw = -wendland_grad(g.q, P.h);
-    const float gx = gw * g.hx, gy = gw * g.hy;
-    const float lx = l00 * gx + l01 * gy, ly = l10 * gx + l11 * gy;
-    const float c = (f.rho[j] * P.rho0 - rho_i) * (f.area[j] / f.rho[j]) * 2.f;
-    sx += c * lx;
-    sy += c * ly;
-  }
-  if (fb.rowptr) {
-    for (int k = fb.rowptr[i]; k < fb.rowptr[i + 1]; ++k) {
-      const int j = fb.col[k];
-      const float2 xb = ld2(b.x, j);
-      const PairGeo g = pair_geo(xi.x, xi.y, xb.x, xb.y, P.h);
-      const float gw = -wendland_grad(g.q, P.h);
+  for (int pass = 0; pass < 2; ++pass) {
+    const SphCSR& c = pass == 0 ? ff : fb;
+    if (!c.rowptr) continue;
+    for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
+      const int j = c.col[k];
+      const float2 xj = pass == 0 ? ld2(f.x, j) : ld2(b.x, j);
+      const float rj = pass == 0 ? f.rho[j] : b.rho[j];
+      const float V = pass == 0 ? f.area[j] / rj : b.vol[j] / rj;
+      const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, K.inv_h);
+      const float gw = -wendland_grad_k(g.q, K);
       const float gx = gw * g.hx, gy = gw * g.hy;
       const float lx = l00 * gx + l01 * gy, ly = l10 * gx + l11 * gy;
-      const float c = (b.rho[j] * P.rho0 - rho_i) * (b.vol[j] / b.rho[j]) * 2.f;
-      sx += c * lx;
-      sy += c * ly;
+      const float c2 = (rj * P.rho0 - rho_i) * V * 2.f;
+      sx += c2 * lx;
+      sy += c2 * ly;
     }
   }
   st2(gradrho, i, make_float2(-sx, -sy));
@@ -382,6 +463,7 @@ __global__ __launch_bounds__(kBS) void k_mdbc_density(SphFluid f, const SphPhysi
   const int bI = blockIdx.x * blockDim.x + threadIdx.x;
   if (bI >= nb) return;
   const SphPhysics P = phys[bscene[bI]];
+  const KConst K = kconst(P.h);
   const float2 g = ld2(gxp, bI);
   float num = 0.f, den = 0.f, gsx = 0.f, gsy = 0.f, vgx = 0.f, vgy = 0.f, psx = 0.f, psy = 0.f;
   float m00 = 0.f, m01 = 0.f, m10 = 0.f, m11 = 0.f;
@@ -390,9 +472,9 @@ __global__ __launch_bounds__(kBS) void k_mdbc_density(SphFluid f, const SphPhysi
     const int j = gf.col[k];
     const float2 xf = ld2(f.x, j);
     // distances of a foreign query: (ghost - fluid) / |.|, q = |.| / h_query
-    const PairGeo geo = pair_geo(g.x, g.y, xf.x, xf.y, P.h);
-    const float W = wendland(geo.q, P.h);
-    const float gw = -wendland_grad(geo.q, P.h);
+    const PairGeo geo = pair_geo(g.x, g.y, xf.x, xf.y, K.inv_h);
+    const float W = wendland_k(geo.q, K);
+    const float gw = -wendland_grad_k(geo.q, K);
     const float gWx = gw * geo.hx, gWy = gw * geo.hy;
     const float mass = f.area[j] * P.rho0;
     const float V = f.area[j] / f.rho[j];
@@ -448,13 +530,14 @@ __global__ __launch_bounds__(kBS) void k_akinci_bdensity(SphFluid f, const SphPh
   const int bI = blockIdx.x * blockDim.x + threadIdx.x;
   if (bI >= nb) return;
   const SphPhysics P = phys[bscene[bI]];
+  const KConst K = kconst(P.h);
   const float2 xb = ld2(bx, bI);
   float s = 0.f;
   for (int k = bf.rowptr[bI]; k < bf.rowptr[bI + 1]; ++k) {
     const int j = bf.col[k];
     const float2 xf = ld2(f.x, j);
-    const PairGeo g = pair_geo(xb.x, xb.y, xf.x, xf.y, P.h);
-    s += wendland(g.q, P.h) * f.area[j];
+    const PairGeo g = pair_geo(xb.x, xb.y, xf.x, xf.y, K.inv_h);
+    s += wendland_k(g.q, K) * f.area[j];
   }
   rho_b[bI] = term[bI] + s + P.beta;
 }
@@ -486,24 +569,78 @@ int fluid_density(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, c
   return launch_check("sph_fluid_density");
 }
 
+static int check_bdy_terms(unsigned terms, const SphBoundary* b, const SphCSR* fb, const char* who) {
+  const bool wants_b = terms & (SPH_T_CONT_B | SPH_T_PRESS_B | SPH_T_VISC_B);
+  if (wants_b && fb && fb->rowptr && !(b && b->x && b->rho && b->vol)) {
+    set_error(std::string(who) + ": boundary terms without boundary state");
+    return SPH_E_ARG;
+  }
+  return SPH_OK;
+}
+
 int dsph_gather(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, const SphBoundary* b,
                 const SphCSR* fb, const float* gradrho, unsigned terms, const int* order,
                 float* dpdt, float* acc, hipStream_t s) {
   SPH_REQUIRE(fluid_ok(f) && f->v && phys && ff && ff->rowptr, "sph_dsph_gather: bad arguments");
   SPH_REQUIRE(!(terms & SPH_T_DDSPH) || gradrho, "sph_dsph_gather: DDSPH needs gradrho");
-  const bool wants_b = terms & (SPH_T_CONT_B | SPH_T_PRESS_B | SPH_T_VISC_B);
-  SPH_REQUIRE(!wants_b || !fb || !fb->rowptr || (b && b->x && b->rho && b->vol),
-              "sph_dsph_gather: boundary terms without boundary state");
+  int rc = check_bdy_terms(terms, b, fb, "sph_dsph_gather");
+  if (rc) return rc;
   if (f->np == 0) return SPH_OK;
-  hipLaunchKernelGGL(k_dsph_gather<0>, dim3(blocks_for(f->np, kBS)), dim3(kBS), 0, s, *f, phys, *ff,
+  hipLaunchKernelGGL(k_dsph_gather, dim3(blocks_for(f->np, kBS)), dim3(kBS), 0, s, *f, phys, *ff,
                      bdy_or_null(b), csr_or_null(fb), gradrho, terms, order, dpdt, acc);
   return launch_check("sph_dsph_gather");
+}
+
+int dsph_pack(const SphFluid* f, const SphPhysics* phys, const SphBoundary* b, float* frec,
+              float* brec, hipStream_t s) {
+  SPH_REQUIRE(fluid_ok(f) && f->v && phys && frec, "sph_dsph_pack: bad arguments");
+  const SphBoundary bb = bdy_or_null(b);
+  SPH_REQUIRE(bb.nb == 0 || (bb.x && bb.rho && bb.vol && bb.bscene && brec),
+              "sph_dsph_pack: bad boundary");
+  const int n = f->np + bb.nb;
+  if (n == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_dsph_pack, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, *f, phys, bb,
+                     reinterpret_cast<float4*>(frec), reinterpret_cast<float4*>(brec));
+  return launch_check("sph_dsph_pack");
+}
+
+// term masks with a compiled specialisation; others use the runtime form
+constexpr unsigned kMaskMOG = SPH_T_CONT | SPH_T_CONT_B | SPH_T_MOG | SPH_T_PRESS | SPH_T_PRESS_B |
+                              SPH_T_VISC | SPH_T_VISC_B | SPH_T_LAM | SPH_T_GRAV;
+constexpr unsigned kMaskVL = SPH_T_VISC | SPH_T_LAM | SPH_T_ACCUM;
+
+int dsph_gather_packed(const float* frec, const float* brec, const int* pscene, int np,
+                       const SphPhysics* phys, const SphCSR* ff, const SphCSR* fb,
+                       const float* gradrho, unsigned terms, const int* order, float* dpdt,
+                       float* acc, hipStream_t s) {
+  SPH_REQUIRE(frec && pscene && phys && ff && ff->rowptr && np >= 0,
+              "sph_dsph_gather_packed: bad arguments");
+  SPH_REQUIRE(!(terms & SPH_T_DDSPH) || gradrho, "sph_dsph_gather_packed: DDSPH needs gradrho");
+  SPH_REQUIRE(!(terms & kBdyPairTerms) || !fb || !fb->rowptr || brec,
+              "sph_dsph_gather_packed: boundary terms without boundary records");
+  if (np == 0) return SPH_OK;
+  const dim3 grid(blocks_for(np, kBS)), block(kBS);
+  const float4* fr = reinterpret_cast<const float4*>(frec);
+  const float4* br = reinterpret_cast<const float4*>(brec);
+  const SphCSR fbc = csr_or_null(fb);
+  if (terms == kMaskMOG)
+    hipLaunchKernelGGL(k_dsph_gather_packed<kMaskMOG>, grid, block, 0, s, fr, br, pscene, np, phys,
+                       *ff, fbc, gradrho, terms, order, dpdt, acc);
+  else if (terms == kMaskVL)
+    hipLaunchKernelGGL(k_dsph_gather_packed<kMaskVL>, grid, block, 0, s, fr, br, pscene, np, phys,
+                       *ff, fbc, gradrho, terms, order, dpdt, acc);
+  else
+    hipLaunchKernelGGL(k_dsph_gather_packed<0>, grid, block, 0, s, fr, br, pscene, np, phys, *ff,
+                       fbc, gradrho, terms, order, dpdt, acc);
+  return launch_check("sph_dsph_gather_packed");
 }
 
 int dsph_normalization(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
                        const SphBoundary* b, const SphCSR* fb, const int* order, float* M,
                        float* L, float* eig, hipStream_t s) {
   SPH_REQUIRE(fluid_ok(f) && phys && ff && ff->rowptr && L, "sph_dsph_normalization: bad arguments");
+  SPH_REQUIRE(!fb || !fb->rowptr || (b && b->x && b->rho && b->vol),
+              "sph_dsph_normalization: boundary list without boundary state");
   if (f->np == 0) return SPH_OK;
   hipLaunchKernelGGL(k_dsph_normalization, dim3(blocks_for(f->np, kBS)), dim3(kBS), 0, s, *f, phys,
                      *ff, bdy_or_null(b), csr_or_null(fb), order, M, L, eig);
@@ -515,6 +652,8 @@ int dsph_density_gradient(const SphFluid* f, const SphPhysics* phys, const SphCS
                           const int* order, float* gradrho, hipStream_t s) {
   SPH_REQUIRE(fluid_ok(f) && phys && ff && ff->rowptr && L && gradrho,
               "sph_dsph_density_gradient: bad arguments");
+  SPH_REQUIRE(!fb || !fb->rowptr || (b && b->x && b->rho && b->vol),
+              "sph_dsph_density_gradient: boundary list without boundary state");
   if (f->np == 0) return SPH_OK;
   hipLaunchKernelGGL(k_dsph_density_gradient, dim3(blocks_for(f->np, kBS)), dim3(kBS), 0, s, *f,
                      phys, *ff, bdy_or_null(b), csr_or_null(fb), L, order, gradrho);
@@ -570,6 +709,17 @@ int sph_dsph_gather(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
                     const SphBoundary* b, const SphCSR* fb, const float* gradrho, unsigned terms,
                     const int* order, float* dpdt, float* acc, hipStream_t s) {
   return sph::dsph_gather(f, phys, ff, b, fb, gradrho, terms, order, dpdt, acc, s);
+}
+int sph_dsph_pack(const SphFluid* f, const SphPhysics* phys, const SphBoundary* b, float* frec,
+                  float* brec, hipStream_t s) {
+  return sph::dsph_pack(f, phys, b, frec, brec, s);
+}
+int sph_dsph_gather_packed(const float* frec, const float* brec, const int* pscene, int np,
+                           const SphPhysics* phys, const SphCSR* ff, const SphCSR* fb,
+                           const float* gradrho, unsigned terms, const int* order, float* dpdt,
+                           float* acc, hipStream_t s) {
+  return sph::dsph_gather_packed(frec, brec, pscene, np, phys, ff, fb, gradrho, terms, order, dpdt,
+                                 acc, s);
 }
 int sph_dsph_normalization(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
                            const SphBoundary* b, const SphCSR* fb, const int* order, float* M,

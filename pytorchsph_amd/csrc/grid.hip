@@ -25,38 +25,34 @@ __global__ __launch_bounds__(kBS) void k_grid_keys(SphGrid g) {
   // reference key: ix + cellsX * iy in int32 (neighborSearch.py:47)
   const int64_t key = (int64_t)ix + (int64_t)d.cells_x * (int64_t)iy;
   const int64_t ncell = (int64_t)d.cells_x * d.cells_y;
+  int k;
   if (key >= 0 && key < ncell) {
-    g.key[p] = (int)key;
-    g.slot[p] = atomicAdd(&g.cell_count[d.cell_begin + key], 1);
+    k = (int)key;
   } else {
-    g.key[p] = -1;
-    g.slot[p] = atomicAdd(g.lost, 1);
+    // the scene's last slot collects points outside its table; stencils
+    // never reach it, and the scene stays contiguous in sorted order
+    k = (int)ncell;
     atomicOr(&g.status[s], SPH_ST_OUTSIDE);
   }
+  g.key[p] = k;
+  g.slot[p] = atomicAdd(&g.cell_count[d.cell_begin + k], 1);
 }
 
 __global__ __launch_bounds__(kBS) void k_grid_scatter(SphGrid g) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= g.np) return;
-  const int key = g.key[p];
-  int pos;
-  if (key >= 0) {
-    const int s = g.pscene[p];
-    pos = g.cell_start[g.scenes[s].cell_begin + key] + g.slot[p];
-  } else {
-    pos = g.cell_start[g.ncells] + g.slot[p];
-  }
+  const int s = g.pscene[p];
+  const int pos = g.cell_start[g.scenes[s].cell_begin + g.key[p]] + g.slot[p];
   g.sorted_idx[pos] = p;
 }
 
 // One thread per cell: order the cell's points by index (atomics gave them
-// in arbitrary order) and copy their positions into sorted order.  The last
-// thread handles the points that fell outside every table.
+// in arbitrary order) and copy their positions into sorted order.
 __global__ __launch_bounds__(kBS) void k_grid_cell_sort(SphGrid g) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c > g.ncells) return;
+  if (c >= g.ncells) return;
   const int a = g.cell_start[c];
-  const int b = (c == g.ncells) ? g.np : g.cell_start[c + 1];
+  const int b = g.cell_start[c + 1];
   int* idx = g.sorted_idx;
   for (int i = a + 1; i < b; ++i) {
     const int v = idx[i];
@@ -210,18 +206,17 @@ int grid_build(const SphGrid* gp, void* ws, size_t ws_bytes, hipStream_t s) {
   const SphGrid& g = *gp;
   SPH_REQUIRE(g.np >= 0 && g.ncells > 0 && g.nscenes > 0, "sph_grid_build: bad sizes");
   SPH_REQUIRE(g.scenes && g.pscene && g.x && g.cell_count && g.cell_start && g.key && g.slot &&
-                  g.sorted_idx && g.sorted_x && g.lost && g.status,
+                  g.sorted_idx && g.sorted_x && g.status,
               "sph_grid_build: null buffer");
   int rc;
   if ((rc = fill_i32(g.cell_count, g.ncells, 0, s))) return rc;
-  if ((rc = fill_i32(g.lost, 1, 0, s))) return rc;
   if (g.np > 0)
     hipLaunchKernelGGL(k_grid_keys, dim3(blocks_for(g.np, kBS)), dim3(kBS), 0, s, g);
   if ((rc = launch_check("sph_grid_build(keys)"))) return rc;
   if ((rc = scan_exclusive(g.cell_count, g.ncells, g.cell_start, ws, ws_bytes, s))) return rc;
   if (g.np > 0)
     hipLaunchKernelGGL(k_grid_scatter, dim3(blocks_for(g.np, kBS)), dim3(kBS), 0, s, g);
-  hipLaunchKernelGGL(k_grid_cell_sort, dim3(blocks_for(g.ncells + 1, kBS)), dim3(kBS), 0, s, g);
+  hipLaunchKernelGGL(k_grid_cell_sort, dim3(blocks_for(g.ncells, kBS)), dim3(kBS), 0, s, g);
   return launch_check("sph_grid_build");
 }
 

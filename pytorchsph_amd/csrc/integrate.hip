@@ -86,7 +86,8 @@ __global__ __launch_bounds__(kBS) void k_permute(const float* __restrict__ in,
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)n * width) return;
   const int k = (int)(t / width), c = (int)(t % width);
-  out[t] = in[(int64_t)perm[k] * width + c];
+  const int64_t src = perm ? (int64_t)perm[k] : (int64_t)k;
+  out[t] = in[src * width + c];
 }
 
 }  // namespace
@@ -118,7 +119,7 @@ int integrate_semi(const int* pscene, const SphPhysics* phys, int np, const floa
 }
 
 int permute_f32(const float* in, const int* perm, int n, int width, float* out, hipStream_t s) {
-  SPH_REQUIRE(in && perm && out && n >= 0 && width > 0, "sph_permute_f32: bad arguments");
+  SPH_REQUIRE(in && out && n >= 0 && width > 0, "sph_permute_f32: bad arguments");
   if (n == 0) return SPH_OK;
   hipLaunchKernelGGL(k_permute, dim3(blocks_for((int64_t)n * width, kBS)), dim3(kBS), 0, s, in,
                      perm, n, width, out);

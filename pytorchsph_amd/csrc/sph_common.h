@@ -90,28 +90,62 @@ __device__ __forceinline__ float spiky_grad(float q, float h) {
   return kC30pi / (h * h * h) * (a * a);
 }
 
+// Hardware reciprocal and reciprocal square root (about 1 ulp); the physics
+// tolerances are far above that, and they replace the ~10-instruction
+// correctly rounded divide sequence in every pair.  The neighbour predicate
+// never uses them.
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// Per-support constants of the kernels, computed once per thread.
+struct KConst {
+  float inv_h;  // 1 / h
+  float w0;     // 7 / (pi h^2)
+  float g0;     // 20 * 7 / (pi h^3)
+  float s0;     // 30 / (pi h^3)
+};
+__device__ __forceinline__ KConst kconst(float h) {
+  KConst k;
+  k.inv_h = 1.f / h;
+  k.w0 = kC7pi * k.inv_h * k.inv_h;
+  k.g0 = 20.f * kC7pi * k.inv_h * k.inv_h * k.inv_h;
+  k.s0 = kC30pi * k.inv_h * k.inv_h * k.inv_h;
+  return k;
+}
+__device__ __forceinline__ float wendland_k(float q, const KConst& k) {
+  const float a = 1.f - q;
+  const float a2 = a * a;
+  return a2 * a2 * (1.f + 4.f * q) * k.w0;
+}
+__device__ __forceinline__ float wendland_grad_k(float q, const KConst& k) {
+  const float a = 1.f - q;
+  return k.g0 * q * a * a * a;
+}
+__device__ __forceinline__ float spiky_grad_k(float q, const KConst& k) {
+  const float a = 1.f - q;
+  return k.s0 * a * a;
+}
+
 // Pair geometry as the reference builds it (neighborSearch.py:218-223):
-// r = x_i - x_j, d = |r|, rhat = r / d (zero when d < 1e-7), q = d / hdiv.
+// r = x_i - x_j, d = |r|, rhat = r / d (zero when d < 1e-7), q = d / h.
 struct PairGeo {
   float rx, ry;    // x_i - x_j
+  float d2;        // |x_i - x_j|^2
   float d;         // |x_i - x_j|
   float hx, hy;    // rhat
   float q;
 };
-__device__ __forceinline__ PairGeo pair_geo(float xi, float yi, float xj, float yj, float hdiv) {
+__device__ __forceinline__ PairGeo pair_geo(float xi, float yi, float xj, float yj, float inv_h) {
   PairGeo g;
   g.rx = xi - xj;
   g.ry = yi - yj;
-  g.d = sqrtf(g.rx * g.rx + g.ry * g.ry);
-  if (g.d >= 1e-7f) {
-    float inv = 1.f / g.d;
-    g.hx = g.rx * inv;
-    g.hy = g.ry * inv;
-  } else {
-    g.hx = 0.f;
-    g.hy = 0.f;
-  }
-  g.q = g.d / hdiv;
+  g.d2 = g.rx * g.rx + g.ry * g.ry;
+  const float inv_d = g.d2 > 0.f ? fast_rsq(g.d2) : 0.f;
+  g.d = g.d2 * inv_d;
+  const float s = g.d >= 1e-7f ? inv_d : 0.f;
+  g.hx = g.rx * s;
+  g.hy = g.ry * s;
+  g.q = g.d * inv_h;
   return g;
 }
 

@@ -10,9 +10,13 @@ particles [b_begin, b_begin + nb_s)).  One step advances every scene:
              right-hand side: summation density with the Akinci boundary,
              the relaxed-Jacobi density solve, viscosity terms
              (dfsph.py:134-202).
-All buffers are allocated up front with fixed pair capacities, so a step
-launches kernels only and can be captured into a hipGraph.  The single
-plain scene is nscenes == 1.
+At the start of every step the particle state is permuted into the cell
+order of the previous step's last neighbour search (each scene keeps its
+index range; `uid` records where every particle came from), so that a
+particle's neighbours sit next to it in memory and every gather reads
+coalesced, cache-local data.  All buffers are allocated up front with fixed
+pair capacities, so a step launches kernels only and can be captured into
+a hipGraph.  A single plain scene is nscenes == 1.
 """
 from __future__ import annotations
 
@@ -61,7 +65,7 @@ class SceneBatch:
     """Many scenes of one scheme ("dsph" or "dfsph"), resident on one GPU."""
 
     def __init__(self, scenes: list[Scene], device="cuda", pair_cap: float = 40.0,
-                 bdy_pair_cap: float = 40.0, dfsph_chunk: int = 8):
+                 bdy_pair_cap: float = 40.0, dfsph_chunk: int = 8, reorder: bool = True):
         assert scenes, "empty batch"
         scheme = scenes[0].scheme
         assert all(s.scheme == scheme for s in scenes), "one scheme per batch"
@@ -77,11 +81,13 @@ class SceneBatch:
         self.NP, self.NB = NP, NB
         self.solid = scenes[0].boundary == "solid"
         self.dfsph_chunk = dfsph_chunk
+        self.reorder = reorder
+        self._sorted_valid = False
         f32 = torch.float32
 
-        def cat(arrs, shape_tail=()):
-            if not arrs:
-                return torch.zeros((1,) + shape_tail, dtype=f32, device=dev)
+        def cat(arrs):
+            if not arrs or sum(len(a) for a in arrs) == 0:
+                return torch.zeros(1, dtype=f32, device=dev)
             return torch.from_numpy(np.ascontiguousarray(np.concatenate(arrs))).to(dev)
 
         # ---- fluid state (perennial)
@@ -94,7 +100,8 @@ class SceneBatch:
         self.pscene = torch.from_numpy(np.repeat(np.arange(self.nscenes, dtype=np.int32),
                                                  self.n)).to(dev)
         self.uid = torch.arange(NP, dtype=torch.int64, device=dev)
-        # ---- boundary state
+        self.tmp = torch.empty(max(2 * NP, 1), dtype=f32, device=dev)
+        # ---- boundary state (static)
         self.bx = cat([s.bx for s in scenes]).float().contiguous()
         self.bgx = cat([s.bgx for s in scenes]).float().contiguous() if self.solid else None
         self.bvol = cat([s.bvol for s in scenes])
@@ -128,6 +135,8 @@ class SceneBatch:
         # ---- step buffers
         self.acc = torch.zeros((NP, 2), dtype=f32, device=dev)
         self.dpdt = torch.zeros(NP, dtype=f32, device=dev)
+        self.frec = torch.zeros((max(NP, 1), 8), dtype=f32, device=dev)
+        self.brec = torch.zeros((max(NB, 1), 8), dtype=f32, device=dev)
         if scheme == "dsph":
             self.S = [(torch.empty_like(self.x), torch.empty_like(self.v), torch.empty_like(self.rho))
                       for _ in range(2)]
@@ -144,24 +153,25 @@ class SceneBatch:
     def _alloc_dfsph(self):
         dev, NP, NB, S = self.dev, self.NP, self.NB, self.nscenes
         z = lambda *s, dt=torch.float32: torch.zeros(*s, dtype=dt, device=dev)  # noqa: E731
+        nb1 = max(NB, 1)
         self.max_hist = 260
+        ints = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).to(dev)  # noqa
         self.df = dict(
             vpred=z(NP, 2), aact=z(NP), alpha=z(NP), src=z(NP), p=z(NP), p2=z(NP), apred=z(NP, 2),
-            resid=z(NP), b_aact=z(max(NB, 1)), b_alpha=z(max(NB, 1)), b_src=z(max(NB, 1)),
-            b_p=z(max(NB, 1)), b_resid=z(max(NB, 1)), err_acc=z(S, dt=torch.float64),
-            err_hist=z(S, self.max_hist, dt=torch.float64), iters=z(S, dt=torch.int32),
-            active=z(S, dt=torch.int32), any_active=z(1, dt=torch.int32),
-            status=z(S, dt=torch.int32),
-            nfluid=torch.from_numpy(self.n.astype(np.int32)).to(dev),
-            nbdy=torch.from_numpy(self.nb.astype(np.int32)).to(dev))
+            resid=z(NP), xrec=z(NP, 4), arec=z(NP, 4), b_aact=z(nb1), b_alpha=z(nb1),
+            b_src=z(nb1), b_p=z(nb1), b_resid=z(nb1), bxrec=z(nb1, 4),
+            err_acc=z(S, dt=torch.float64), err_hist=z(S, self.max_hist, dt=torch.float64),
+            iters=z(S, dt=torch.int32), active=z(S, dt=torch.int32),
+            any_active=z(1, dt=torch.int32), status=z(S, dt=torch.int32),
+            nfluid=ints(self.n), nbdy=ints(self.nb), fbegin=ints(self.p_begin),
+            bbegin=ints(self.b_begin), act_scene=z(S, dt=torch.int32),
+            act_pre=z(2 * (S + 1), dt=torch.int32), act_n=z(3, dt=torch.int32))
         self.any_active_host = torch.zeros(1, dtype=torch.int32, pin_memory=True)
+        self.dfsph_grid = max(1, min(-(-(NP + NB) // 256), 4096))
 
     def _dfsph_struct(self) -> L.SphDfsph:
-        d = self.df
-        names = ("vpred", "aact", "alpha", "src", "p", "p2", "apred", "resid", "b_aact",
-                 "b_alpha", "b_src", "b_p", "b_resid", "err_acc", "err_hist", "iters", "active",
-                 "any_active", "status", "nfluid", "nbdy")
-        return L.SphDfsph(*[L.ptr(d[k]) for k in names], self.nscenes, self.max_hist)
+        return L.SphDfsph(*[L.ptr(self.df[k]) for k in L.DFSPH_FIELDS], self.nscenes,
+                          self.max_hist, self.dfsph_grid)
 
     def fluid(self, x, v, rho, vol=None) -> L.SphFluid:
         return L.SphFluid(L.ptr(x), L.ptr(v), L.ptr(rho), L.ptr(self.area), L.ptr(self.rest),
@@ -191,9 +201,26 @@ class SceneBatch:
             L.call("sph_csr_transpose", C.byref(c), self.NP, L.ptr(self.fb.counts),
                    L.ptr(self.t_slot), L.ptr(self.fb.rowptr), L.ptr(self.fb.col),
                    L.ptr(self.scan_ws), int(self.scan_ws.numel()), L.stream_handle(s))
+        self._sorted_valid = True
 
     def _order(self):
-        return L.ptr(self.grid.sorted_idx)
+        # with the state in cell order, index order is already local
+        return None if self.reorder else L.ptr(self.grid.sorted_idx)
+
+    def _permute(self, t: torch.Tensor, width: int, s):
+        sh = L.stream_handle(s)
+        L.call("sph_permute_f32", L.ptr(t), L.ptr(self.grid.sorted_idx), self.NP, width,
+               L.ptr(self.tmp), sh)
+        L.call("sph_permute_f32", L.ptr(self.tmp), None, self.NP, width, L.ptr(t), sh)
+
+    def reorder_state(self, s=None):
+        """Permutes the perennial particle state into the cell order of the
+        last neighbour search (scene ranges are preserved)."""
+        if not (self.reorder and self._sorted_valid and self.NP):
+            return
+        for t, w in ((self.x, 2), (self.v, 2), (self.rho, 1), (self.area, 1), (self.rest, 1),
+                     (self.h, 1), (self.uid, 2)):
+            self._permute(t, w, s)
 
     # ---------------------------------------------------------- delta-SPH
     def dsph_rhs(self, x, v, rho, s=None):
@@ -215,11 +242,15 @@ class SceneBatch:
                    fbp, self._order(), None, L.ptr(self.Lmat), None, sh)
             L.call("sph_dsph_density_gradient", C.byref(F), L.ptr(self.phys), C.byref(ff),
                    C.byref(B), fbp, L.ptr(self.Lmat), self._order(), L.ptr(self.gradrho), sh)
-        L.call("sph_dsph_gather", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B), fbp,
+        L.call("sph_dsph_pack", C.byref(F), L.ptr(self.phys), C.byref(B), L.ptr(self.frec),
+               L.ptr(self.brec), sh)
+        L.call("sph_dsph_gather_packed", L.ptr(self.frec), L.ptr(self.brec), L.ptr(self.pscene),
+               self.NP, L.ptr(self.phys), C.byref(ff), fbp,
                L.ptr(self.gradrho) if self.terms & L.T_DDSPH else None, self.terms, self._order(),
                L.ptr(self.dpdt), L.ptr(self.acc), sh)
 
     def _dsph_step(self, s=None):
+        self.reorder_state(s)
         P = (self.x, self.v, self.rho)
         S0, S1 = self.S
         stages = [(P, S0), (S0, S1), (S1, S0), (S0, P)]
@@ -234,6 +265,7 @@ class SceneBatch:
     # -------------------------------------------------------------- DFSPH
     def _dfsph_pre(self, s=None):
         sh = L.stream_handle(s)
+        self.reorder_state(s)
         self.search(self.x, s)
         F = self.fluid(self.x, self.v, self.rho)
         ff = self.ff.csr().c()
@@ -253,7 +285,7 @@ class SceneBatch:
                C.byref(D), sh)
         bfc = self.bf.csr().c()
         L.call("sph_dfsph_alpha_source", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B),
-               C.byref(fb) if fb is not None else None, C.byref(bfc), self._order(), C.byref(D), sh)
+               C.byref(fb) if fb is not None else None, C.byref(bfc), C.byref(D), sh)
 
     def _dfsph_iters(self, k, s=None):
         sh = L.stream_handle(s)
@@ -265,23 +297,22 @@ class SceneBatch:
         D = self._dfsph_struct()
         for _ in range(k):
             L.call("sph_dfsph_iterate", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B),
-                   C.byref(fb) if fb is not None else None, C.byref(bfc), self._order(),
-                   C.byref(D), sh)
+                   C.byref(fb) if fb is not None else None, C.byref(bfc), C.byref(D), sh)
 
     def _dfsph_post(self, s=None):
         sh = L.stream_handle(s)
         ff = self.ff.csr().c()
         fb = self.fb.csr().c() if self.NB else None
-        B = self.boundary()
         F = self.fluid(self.x, self.v, self.rho)
         D = self._dfsph_struct()
-        L.call("sph_dfsph_finish", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B),
-               C.byref(fb) if fb is not None else None, self._order(), C.byref(D),
-               L.ptr(self.acc), sh)
+        L.call("sph_dfsph_finish", C.byref(F), L.ptr(self.phys), C.byref(ff),
+               C.byref(fb) if fb is not None else None, C.byref(D), L.ptr(self.acc), sh)
         # velocity diffusion + laminar with V = A / fluid-only density (density.py:53)
         Fv = self.fluid(self.x, self.v, self.rho, self.vol)
-        L.call("sph_dsph_gather", C.byref(Fv), L.ptr(self.phys), C.byref(ff), None, None, None,
-               L.T_VISC | L.T_LAM | L.T_ACCUM, self._order(), None, L.ptr(self.acc), sh)
+        L.call("sph_dsph_pack", C.byref(Fv), L.ptr(self.phys), None, L.ptr(self.frec), None, sh)
+        L.call("sph_dsph_gather_packed", L.ptr(self.frec), None, L.ptr(self.pscene), self.NP,
+               L.ptr(self.phys), C.byref(ff), None, None, L.T_VISC | L.T_LAM | L.T_ACCUM,
+               self._order(), None, L.ptr(self.acc), sh)
         L.call("sph_integrate_semi_implicit", L.ptr(self.pscene), L.ptr(self.phys), self.NP,
                L.ptr(self.x), L.ptr(self.v), None, L.ptr(self.v), L.ptr(self.acc), None,
                L.ptr(self.x), L.ptr(self.v), None, sh)
@@ -312,12 +343,12 @@ class SceneBatch:
 
     def capture(self):
         """Captures the step into hipGraphs (delta-SPH: one graph per step;
-        DFSPH: prologue, a chunk of iterations, epilogue)."""
+        DFSPH: prologue, a chunk of iterations, epilogue).  Runs one eager
+        step first."""
         torch.cuda.synchronize()
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
-            # warm-up launch outside capture
             self.step(s)
         torch.cuda.current_stream().wait_stream(s)
         torch.cuda.synchronize()
@@ -363,9 +394,12 @@ class SceneBatch:
         return st, nnz
 
     def scene_state(self, k: int) -> dict:
+        """State of scene k in its original particle order."""
         a, b = int(self.p_begin[k]), int(self.p_begin[k] + self.n[k])
-        return dict(x=self.x[a:b].cpu().numpy(), v=self.v[a:b].cpu().numpy(),
-                    rho=self.rho[a:b].cpu().numpy())
+        inv = np.empty(b - a, dtype=np.int64)
+        inv[self.uid[a:b].cpu().numpy() - a] = np.arange(b - a)
+        return dict(x=self.x[a:b].cpu().numpy()[inv], v=self.v[a:b].cpu().numpy()[inv],
+                    rho=self.rho[a:b].cpu().numpy()[inv])
 
     def dfsph_iterations(self) -> np.ndarray:
         return self.df["iters"].cpu().numpy()
@@ -396,7 +430,14 @@ class MixedBatch:
             b.capture()
         self.captured = True
 
-    def step(self):
+    def step(self, serial: bool = False):
+        if serial:   # both parts on the current stream, one after the other
+            for b in self.parts:
+                if self.captured:
+                    b.step_graph()
+                else:
+                    b.step()
+            return
         cur = torch.cuda.current_stream()
         for s in self.streams:
             s.wait_stream(cur)

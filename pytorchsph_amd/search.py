@@ -59,7 +59,7 @@ class NeighborGrid:
             qx, qy, cx, cy, h = grid_params(sc["vmin"], sc["vmax"], sc["hmax"])
             desc[s] = (qx, qy, h, cx, cy, cell_begin, sc["p_begin"], sc["p_count"])
             self.scene_params.append((qx, qy, cx, cy, h))
-            cell_begin += cx * cy
+            cell_begin += cx * cy + 1     # + the scene's bucket of outside points
         if cell_begin >= 2**31 - 1:
             raise ValueError("cell table exceeds int32 range")
         self.ncells = cell_begin
@@ -75,7 +75,6 @@ class NeighborGrid:
         self.slot = torch.empty(n1, dtype=torch.int32, device=dev)
         self.sorted_idx = torch.empty(n1, dtype=torch.int32, device=dev)
         self.sorted_x = torch.empty((n1, 2), dtype=torch.float32, device=dev)
-        self.lost = torch.empty(1, dtype=torch.int32, device=dev)
         self.status = torch.zeros(self.nscenes, dtype=torch.int32, device=dev)
         lib = L.load()
         self.ws_bytes = int(lib.sph_grid_workspace_bytes(self.ncells))
@@ -87,8 +86,7 @@ class NeighborGrid:
         return L.SphGrid(
             L.ptr(self.desc), L.ptr(self.pscene), L.ptr(self.x), self.np, self.nscenes,
             self.ncells, L.ptr(self.cell_count), L.ptr(self.cell_start), L.ptr(self.key),
-            L.ptr(self.slot), L.ptr(self.sorted_idx), L.ptr(self.sorted_x), L.ptr(self.lost),
-            L.ptr(self.status))
+            L.ptr(self.slot), L.ptr(self.sorted_idx), L.ptr(self.sorted_x), L.ptr(self.status))
 
     def build(self, x: torch.Tensor, pscene: torch.Tensor, stream=None) -> None:
         """Sorts the points into the cell table (positions x [np, 2] fp32)."""

@@ -46,7 +46,7 @@ def test_argument_errors_are_reported_without_gpu():
 def test_record_layouts():
     assert L.GRID_DESC_DTYPE.itemsize == 32
     assert L.PHYS_DTYPE.itemsize == 80
-    assert C.sizeof(L.SphGrid) == 8 * 3 + 4 * 3 + 4 + 8 * 8
+    assert C.sizeof(L.SphGrid) == 8 * 3 + 4 * 3 + 4 + 8 * 7
     assert C.sizeof(L.SphQuery) == 8 * 3 + 4 * 3 + 4
 
 

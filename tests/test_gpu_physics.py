@@ -145,13 +145,17 @@ def test_dfsph_converged_step(cuda):
     x1, v1, rho, out = so.dfsph_step(sc.x, sc.v)
     it_gpu = int(b.df["iters"][0].item())
     assert abs(it_gpu - out["iters"]) <= 1
-    assert rel_err(b.v.cpu().numpy(), v1) < 1e-3
-    assert np.max(np.abs(b.x.cpu().numpy() - x1)) < 1e-6
+    st = b.scene_state(0)
+    assert rel_err(st["v"], v1) < 1e-3
+    assert np.max(np.abs(st["x"] - x1)) < 1e-6
 
 
-def test_dsph_trajectory_ten_steps(cuda):
+@pytest.mark.parametrize("reorder", [True, False])
+def test_dsph_trajectory_ten_steps(cuda, reorder):
+    """Ten RK4 steps; with reorder the state is permuted into cell order at
+    every step and read back through the particle ids."""
     sc = S.dam_break(dx=0.02, block=(1.0, 0.5), scheme="dsph", seed=6)
-    b = SceneBatch([sc])
+    b = SceneBatch([sc], reorder=reorder)
     so = O.SceneOracle(sc)
     x, v, r = sc.x.astype(np.float64), sc.v.astype(np.float64), np.ones(sc.n)
     for _ in range(10):
@@ -160,9 +164,10 @@ def test_dsph_trajectory_ten_steps(cuda):
     torch.cuda.synchronize()
     st, _ = b.check()
     assert st[0] == 0
-    assert np.max(np.abs(b.x.cpu().numpy() - x)) < 1e-5 * 4.0
-    assert np.max(np.abs(b.rho.cpu().numpy() - r)) < 1e-5
-    assert rel_err(b.v.cpu().numpy(), v) < 1e-3
+    got = b.scene_state(0)
+    assert np.max(np.abs(got["x"] - x)) < 1e-5 * 4.0
+    assert np.max(np.abs(got["rho"] - r)) < 1e-5
+    assert rel_err(got["v"], v) < 1e-3
 
 
 def test_dfsph_trajectory_three_steps(cuda):
@@ -174,8 +179,26 @@ def test_dfsph_trajectory_three_steps(cuda):
         b.step()
         x, v, _, _ = so.dfsph_step(x, v)
     torch.cuda.synchronize()
-    assert np.max(np.abs(b.x.cpu().numpy() - x)) < 1e-5 * 4.0
-    assert rel_err(b.v.cpu().numpy(), v) < 2e-3
+    got = b.scene_state(0)
+    assert np.max(np.abs(got["x"] - x)) < 1e-5 * 4.0
+    assert rel_err(got["v"], v) < 2e-3
+
+
+def test_packed_gather_equals_structure_of_arrays(cuda):
+    """sph_dsph_gather_packed and sph_dsph_gather compute the same terms."""
+    import ctypes as C
+    sc = perturbed_dsph_scene(dx=0.01)
+    rng = np.random.default_rng(3)
+    rho = (1.0 + rng.uniform(-2e-3, 4e-3, sc.n)).astype(np.float32)
+    b = run_rhs(sc, rho)                       # packed path
+    dp_packed, acc_packed = b.dpdt.clone(), b.acc.clone()
+    F, B = b.fluid(b.x, b.v, b.rho), b.boundary()
+    ff, fb = b.ff.csr().c(), b.fb.csr().c()
+    L.call("sph_dsph_gather", C.byref(F), L.ptr(b.phys), C.byref(ff), C.byref(B), C.byref(fb),
+           None, b.terms, None, L.ptr(b.dpdt), L.ptr(b.acc), L.stream_handle())
+    torch.cuda.synchronize()
+    assert rel_err(dp_packed.cpu().numpy(), b.dpdt.cpu().numpy()) < 1e-6
+    assert rel_err(acc_packed.cpu().numpy(), b.acc.cpu().numpy()) < 1e-6
 
 
 def test_pinv2x2_device_matches_oracle(cuda):
