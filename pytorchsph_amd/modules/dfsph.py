@@ -1,0 +1,130 @@
+"""dfsphModule (src/modules/dfsph.py:103-317): the DFSPH density solve on
+the device.
+
+DFSPH() predicts v*, computes alpha and the source term (fluid and Akinci
+PBSPH boundary), runs relaxed-Jacobi iterations until the FP64 mean
+clamped residual drops below densityThreshold (at least
+minDensitySolverIterations, at most maxDensitySolverIterations + 1, the
+reference's loop bounds at :205-207) and leaves the final pressure
+acceleration in simulationState['fluidPredAccel'].  The per-iteration
+.item() of the reference is replaced by one device flag read per chunk of
+iterations.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from .. import _lib as L
+from ..module import Module
+from ..parameter import Parameter
+from ._ctx import byref, csr_c, fluid
+
+
+class dfsphModule(Module):
+    def getParameters(self):
+        return [
+            Parameter("dfsph", "minDensitySolverIterations", "int", 2, required=False, export=True, hint=""),
+            Parameter("dfsph", "minDivergenceSolverIterations", "int", 2, required=False, export=True, hint=""),
+            Parameter("dfsph", "maxDensitySolverIterations", "int", 256, required=False, export=True, hint=""),
+            Parameter("dfsph", "maxDivergenceSolverIterations", "int", 8, required=False, export=True, hint=""),
+            Parameter("dfsph", "densityThreshold", "float", 1e-3, required=False, export=True, hint=""),
+            Parameter("dfsph", "divergenceThreshold", "float", 1e-2, required=False, export=True, hint=""),
+            Parameter("dfsph", "divergenceSolver", "bool", True, required=False, export=True, hint=""),
+            Parameter("dfsph", "relaxedJacobiOmega", "float", 0.5, required=False, export=True, hint=""),
+        ]
+
+    def __init__(self):
+        super().__init__("densityInterpolation", "Evaluates density at the current timestep")
+        self._buf = None
+        self.chunk = 8
+
+    def initialize(self, simulationConfig, simulationState):
+        c = simulationConfig["dfsph"]
+        self.support = simulationConfig["particle"]["support"]
+        self.minDensitySolverIterations = c["minDensitySolverIterations"]
+        self.maxDensitySolverIterations = c["maxDensitySolverIterations"]
+        self.densityThreshold = c["densityThreshold"]
+        self.relaxedJacobiOmega = c["relaxedJacobiOmega"]
+        self.backgroundPressure = simulationConfig["fluid"]["backgroundPressure"]
+        if self.backgroundPressure:
+            raise NotImplementedError("fluid.backgroundPressure is not part of the hot path")
+        self.device = simulationConfig["compute"]["device"]
+
+    def _buffers(self, n, nb):
+        key = (n, nb)
+        if self._buf is not None and self._buf[0] == key:
+            return self._buf[1]
+        dev = self.device
+        z = lambda *s, dt=torch.float32: torch.zeros(*s, dtype=dt, device=dev)  # noqa: E731
+        nb1 = max(nb, 1)
+        ints = lambda a: torch.tensor(a, dtype=torch.int32, device=dev)  # noqa: E731
+        b = dict(vpred=z(n, 2), aact=z(n), alpha=z(n), src=z(n), p=z(n), p2=z(n), apred=z(n, 2),
+                 resid=z(n), xrec=z(n, 4), arec=z(n, 4), b_aact=z(nb1), b_alpha=z(nb1),
+                 b_src=z(nb1), b_p=z(nb1), b_resid=z(nb1), bxrec=z(nb1, 4),
+                 err_acc=z(1, dt=torch.float64), err_hist=z(1, 300, dt=torch.float64),
+                 iters=z(1, dt=torch.int32), active=z(1, dt=torch.int32),
+                 any_active=z(1, dt=torch.int32), status=z(1, dt=torch.int32),
+                 nfluid=ints([n]), nbdy=ints([nb]), fbegin=ints([0]), bbegin=ints([0]),
+                 act_scene=z(1, dt=torch.int32), act_pre=z(4, dt=torch.int32),
+                 act_n=z(3, dt=torch.int32))
+        self._buf = (key, b)
+        return b
+
+    def DFSPH(self, simulationState, simulation, density=True):
+        if not density:
+            raise NotImplementedError("the divergence solve is off in every shipped config")
+        n = simulationState["numParticles"]
+        bm = simulation.boundaryModule
+        nb = bm.numPtcls if getattr(bm, "active", False) else 0
+        buf = self._buffers(n, nb)
+        D = L.SphDfsph(*[L.ptr(buf[k]) for k in L.DFSPH_FIELDS], 1, 300,
+                       max(1, min(-(-(n + nb) // 256), 4096)))
+        F = fluid(simulation, simulationState)
+        ns = simulation.neighborSearch
+        ff = csr_c(ns.csr.rowptr, ns.csr.col, n)
+        B = bm.boundary_struct() if nb else None
+        fb = bm.fb_struct() if nb else None
+        bf = bm.bf_struct() if nb else None
+        sh = L.stream_handle()
+        acc = simulationState["fluidAcceleration"].contiguous()
+        L.call("sph_dfsph_prepare", C.byref(F), L.ptr(acc), L.ptr(simulation.phys), byref(B),
+               C.byref(D), sh)
+        L.call("sph_dfsph_alpha_source", C.byref(F), L.ptr(simulation.phys), C.byref(ff), byref(B),
+               byref(fb), byref(bf), C.byref(D), sh)
+        flag = torch.zeros(1, dtype=torch.int32, pin_memory=True)
+        while True:
+            for _ in range(self.chunk):
+                L.call("sph_dfsph_iterate", C.byref(F), L.ptr(simulation.phys), C.byref(ff),
+                       byref(B), byref(fb), byref(bf), C.byref(D), sh)
+            flag.copy_(buf["any_active"], non_blocking=True)
+            torch.cuda.current_stream().synchronize()
+            if int(flag[0]) == 0:
+                break
+        L.call("sph_dfsph_finish", C.byref(F), L.ptr(simulation.phys), C.byref(ff), byref(fb),
+               C.byref(D), None, sh)
+        iters = int(buf["iters"][0].item())
+        errors = buf["err_hist"][0, :min(iters, 300)].cpu().numpy().tolist()
+        if int(buf["status"][0].item()) & L.ST_NAN:
+            raise Exception("Pressure solver became unstable!")
+        simulationState["fluidPredAccel"] = buf["apred"].clone()
+        simulationState["fluidPredictedVelocity"] = buf["vpred"].clone()
+        simulationState["fluidActualArea"] = buf["aact"].clone()
+        simulationState["fluidAlpha"] = buf["alpha"].clone()
+        simulationState["fluidSourceTerm"] = buf["src"].clone()
+        simulationState["fluidPressure"] = buf["p"].clone()
+        simulationState["fluidPressure2"] = buf["p2"].clone()
+        simulationState["residual"] = buf["resid"].clone()
+        simulationState["densityErrors"] = errors
+        simulationState["densitySolverPressure"] = simulationState["fluidPressure"]
+        if nb:
+            bm.boundaryPressure = buf["b_p"][:nb].clone()
+        return errors
+
+    def incompressibleSolver(self, simulationState, simulation):
+        return self.DFSPH(simulationState, simulation, True)
+
+    def divergenceSolver(self, simulationState, simulation):
+        return self.DFSPH(simulationState, simulation, False)
