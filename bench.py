@@ -36,6 +36,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--serial", action="store_true",
                     help="run the delta-SPH and DFSPH parts one after the other (profiling)")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the kernel timings and the copy calibration (counter runs)")
     return ap.parse_args()
 
 
@@ -43,47 +46,129 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def kernel_roofline(batch):
-    """Average duration of the fused delta-SPH gather (the dominant kernel)
-    over repeated launches on the stream it runs on, and its algorithmic
-    HBM bytes: per fluid particle, own state x, v, rho, A (24 B) + visit
-    order and scene id (8 B) + row pointers of the fluid and boundary lists
-    (8 B) + its K column indices (4 K B) + dpdt and acc written (12 B);
-    neighbour state re-reads are cache traffic, not compulsory bytes."""
+def _events_avg(launch, stream, reps=20, warm=3):
+    """Average duration of `launch` over `reps` launches, timed with events
+    recorded on the stream the kernel runs on."""
     import torch
 
-    from pytorchsph_amd import _lib as L
-
-    b = batch
-    b.dsph_rhs(b.x, b.v, b.rho)          # pair lists of the current state
-    torch.cuda.synchronize()
-    nnz = int(b.ff.rowptr[b.NP].item())
-    nnz_b = int(b.fb.rowptr[b.NP].item())
-    F = b.fluid(b.x, b.v, b.rho)
-    ff, fb, B = b.ff.csr().c(), b.fb.csr().c(), b.boundary()
-    stream = torch.cuda.current_stream()
-    sh = L.stream_handle(stream)
-
-    def launch():
-        L.call("sph_dsph_gather", C.byref(F), L.ptr(b.phys), C.byref(ff), C.byref(B),
-               C.byref(fb), None, b.terms, L.ptr(b.grid.sorted_idx), L.ptr(b.dpdt),
-               L.ptr(b.acc), sh)
-
-    for _ in range(3):
+    for _ in range(warm):
         launch()
-    reps = 20
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(reps):
         launch()
     e1.record(stream)
     e1.synchronize()
-    sec = e0.elapsed_time(e1) / 1e3 / reps
-    n = b.NP
-    bytes_alg = n * (24 + 8 + 8 + 12) + 4 * nnz + 4 * nnz_b
-    achieved = bytes_alg / sec / 1e9
-    return dict(kernel="k_dsph_gather (sph_dsph_gather)", seconds=sec, bytes=bytes_alg,
-                particles=n, pairs=nnz, achieved_gbs=achieved)
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def pressure_roofline(b, reps=20):
+    """sph_dfsph_pressure (k_pressure), the dominant kernel of the batched
+    step, with every DFSPH scene active, and its algorithmic HBM bytes:
+      per fluid row      76 B: fluid and boundary row pointers (4 + 4), own
+                         xrec and arec records (16 + 16), src, p2, alpha,
+                         rho, rest (20), p, p2, resid written (12), xrec.z
+                         written (4)
+      per boundary row   56 B: row pointer (4), bxrec (16), b_aact (4),
+                         b_src, b_p, b_alpha, rho_b, rest_b (20), b_p,
+                         b_resid written (8), bxrec.z written (4)
+      per pair           4 B column index (fluid-fluid, fluid-boundary,
+                         boundary-fluid lists)
+    Neighbour records are counted once (as their owner's row), so re-reads
+    of a neighbour are cache traffic, not algorithmic bytes."""
+    import ctypes as C
+
+    import torch
+
+    from pytorchsph_amd import _lib as L
+
+    b._dfsph_pre()              # pair lists, density, alpha/source; all scenes active
+    torch.cuda.synchronize()
+    nnz_ff = int(b.ff.rowptr[b.NP].item())
+    nnz_fb = int(b.fb.rowptr[b.NP].item()) if b.NB else 0
+    nnz_bf = int(b.bf.rowptr[b.NB].item()) if b.NB else 0
+    F = b.fluid(b.x, b.v, b.rho)
+    ff = b.ff.csr().c()
+    fb = b.fb.csr().c() if b.NB else None
+    bf = b.bf.csr().c()
+    B = b.boundary()
+    D = b._dfsph_struct()
+    stream = torch.cuda.current_stream()
+    sh = L.stream_handle(stream)
+
+    def launch():
+        L.call("sph_dfsph_pressure", C.byref(F), L.ptr(b.phys), C.byref(ff), C.byref(B),
+               C.byref(fb) if fb is not None else None, C.byref(bf), C.byref(D), sh)
+
+    sec = _events_avg(launch, stream, reps)
+    nbytes = 76 * b.NP + 56 * b.NB + 4 * (nnz_ff + nnz_fb + nnz_bf)
+    return dict(kernel="k_pressure (sph_dfsph_pressure)", seconds=sec, bytes=nbytes,
+                rows=b.NP + b.NB, pairs=nnz_ff + nnz_fb + nnz_bf,
+                achieved_gbs=nbytes / sec / 1e9)
+
+
+def gather_roofline(b, reps=20):
+    """sph_dsph_gather_packed (k_dsph_gather_packed, all delta-SPH terms),
+    the gather of every RK4 stage, and its algorithmic HBM bytes: per fluid
+    particle its two 16-byte records (32 B), scene id (4 B), fluid and
+    boundary row pointers (8 B), dpdt and acc written (12 B); per boundary
+    particle its records (32 B); 4 B per pair (column index)."""
+    import ctypes as C
+
+    import torch
+
+    from pytorchsph_amd import _lib as L
+
+    b.dsph_rhs(b.x, b.v, b.rho)      # pair lists, mDBC and records of the current state
+    torch.cuda.synchronize()
+    nnz = int(b.ff.rowptr[b.NP].item())
+    nnz_b = int(b.fb.rowptr[b.NP].item())
+    ff, fb = b.ff.csr().c(), b.fb.csr().c()
+    stream = torch.cuda.current_stream()
+    sh = L.stream_handle(stream)
+
+    def launch():
+        L.call("sph_dsph_gather_packed", L.ptr(b.frec), L.ptr(b.brec), L.ptr(b.pscene), b.NP,
+               L.ptr(b.phys), C.byref(ff), C.byref(fb),
+               L.ptr(b.gradrho) if b.terms & L.T_DDSPH else None, b.terms, b._order(),
+               L.ptr(b.dpdt), L.ptr(b.acc), sh)
+
+    sec = _events_avg(launch, stream, reps)
+    nbytes = 56 * b.NP + 32 * b.NB + 4 * (nnz + nnz_b)
+    return dict(kernel="k_dsph_gather_packed (sph_dsph_gather_packed)", seconds=sec,
+                bytes=nbytes, rows=b.NP, pairs=nnz + nnz_b, achieved_gbs=nbytes / sec / 1e9)
+
+
+def copy_calibration(n_floats=1 << 26):
+    """A plain copy of known size through the library's copy kernel
+    (sph_permute_f32 without a permutation): 256 MiB read, 256 MiB written.
+    Under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE calibrate the counters."""
+    import torch
+
+    from pytorchsph_amd import _lib as L
+
+    src = torch.ones(n_floats, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        L.call("sph_permute_f32", L.ptr(src), None, n_floats, 1, L.ptr(dst), L.stream_handle())
+    torch.cuda.synchronize()
+    return 4 * n_floats
+
+
+def traffic_from_profile(kernel: str):
+    """HBM bytes per launch of `kernel` as measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration), read from the
+    committed summary of that measurement; None when there is none."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "k_pressure_traffic_c4_256_scenes.md")
+    if not os.path.exists(path):
+        return None
+    want = kernel.split()[0]
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch") and f"kernel={want}" in line:
+                return int(line.split("=")[1].split()[0])
+    return None
 
 
 def cpu_baseline(seconds: float):
@@ -121,7 +206,9 @@ def main():
         sys.exit(2)
     import torch
 
+    from pytorchsph_amd import _lib as L
     from pytorchsph_amd import workloads as W
+    from pytorchsph_amd.autotune import tune
     from pytorchsph_amd.engine import MixedBatch, SceneBatch
 
     t0 = time.perf_counter()
@@ -130,6 +217,19 @@ def main():
     log(f"{len(scenes)} scenes, {n_part} fluid particles, generated in "
         f"{time.perf_counter() - t0:.1f}s")
     batch = MixedBatch(scenes)
+    tuned = None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        tuned = tune(batch.parts, log=log)
+        log(f"autotuned in {time.perf_counter() - t0:.1f}s: {tuned['choice']}")
+    if args.roofline_only:
+        batch.step(serial=True)
+        torch.cuda.synchronize()
+        rl = pressure_roofline(next(b for b in batch.parts if b.scheme == "dfsph"))
+        rl_g = gather_roofline(next(b for b in batch.parts if b.scheme == "dsph"))
+        copied = copy_calibration()
+        print(json.dumps({"roofline": rl, "dsph_gather": rl_g, "copy_bytes": copied}), flush=True)
+        return
     batch.capture()
     torch.cuda.synchronize()
     log("captured")
@@ -151,30 +251,40 @@ def main():
     status = batch.status()
     iters = {b.scheme: b.dfsph_iterations().tolist() for b in batch.parts if b.scheme == "dfsph"}
 
-    rl = kernel_roofline(next(b for b in batch.parts if b.scheme == "dsph"))
+    rl = pressure_roofline(next(b for b in batch.parts if b.scheme == "dfsph"))
     log(f"roofline: {rl}")
+    rl_g = gather_roofline(next(b for b in batch.parts if b.scheme == "dsph"))
+    log(f"delta-SPH gather: {rl_g}")
     del batch
     torch.cuda.empty_cache()
 
     plain = None
     if not args.no_plain:
+        # the same schedule as the batch: capture (one eager step), warmup
+        # steps, then the timed steps, one scene at a time
         rng = np.random.default_rng(7)
         pick = rng.choice(len(scenes), size=min(args.plain_sample, len(scenes)), replace=False)
         t_sum, n_sum = 0.0, 0
+        batch_tuning = L.get_tuning()
+        batch_cap = L.DFSPH_GRID_CAP
         for k in pick:
             b = SceneBatch([scenes[k]])
+            if not args.no_tune:        # each scene with its own best launch shapes
+                tune([b])
             b.capture()
-            b.step_graph()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            reps = 2
-            for _ in range(reps):
+            for _ in range(args.warmup):
                 b.step_graph()
             torch.cuda.synchronize()
-            t_sum += (time.perf_counter() - t0) / reps
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                b.step_graph()
+                torch.cuda.synchronize()
+            t_sum += (time.perf_counter() - t0) / args.steps
             n_sum += scenes[k].n
             del b
         plain = n_sum / t_sum
+        L.set_tuning(**batch_tuning)
+        L.DFSPH_GRID_CAP = batch_cap
         log(f"plain one-scene-at-a-time: {plain:.3e} particle-steps/s over {len(pick)} scenes")
 
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
@@ -203,10 +313,16 @@ def main():
         "dfsph_iterations_mean": float(np.mean(iters["dfsph"])) if iters else None,
         "status_flags": int(np.bitwise_or.reduce(status)),
         "roofline": {"bound": "hbm", "achieved": rl["achieved_gbs"], "peak": peak,
-                     "unit": "GB/s", "frac": rl["achieved_gbs"] / peak, "traffic": None,
+                     "unit": "GB/s", "frac": rl["achieved_gbs"] / peak,
+                     "traffic": traffic_from_profile(rl["kernel"]),
                      "kernel": rl["kernel"], "avg_launch_us": rl["seconds"] * 1e6,
-                     "bytes_per_launch": rl["bytes"], "particles": rl["particles"],
+                     "bytes_per_launch": rl["bytes"], "rows": rl["rows"],
                      "pairs": rl["pairs"]},
+        "roofline_dsph_gather": {"achieved": rl_g["achieved_gbs"],
+                                 "frac": rl_g["achieved_gbs"] / peak,
+                                 "avg_launch_us": rl_g["seconds"] * 1e6,
+                                 "bytes_per_launch": rl_g["bytes"], "pairs": rl_g["pairs"]},
+        "tuning": tuned["choice"] if tuned else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -184,6 +184,20 @@ int sph_scan_exclusive(const int* in, int n, int* out, void* ws, size_t ws_bytes
 int sph_fill_i32(int* p, int n, int value, hipStream_t s);
 int sph_fill_f32(float* p, int n, float value, hipStream_t s);
 
+/* Launch-shape knobs (process-wide; a captured graph keeps the values that
+ * were set when it was captured).  The *_LANES knobs give the number of
+ * lanes of a wavefront that share one row of a pair list (1, 2, 4, 8, 16 or
+ * 32): the lanes walk the row's pairs together and combine their partial
+ * sums with shuffles.  Pair sets do not depend on them; sums are re-associated
+ * across the lanes.  No reference counterpart: the reference's gathers are
+ * torch scatter_add over the pair list (util.py:11-40). */
+#define SPH_TUNE_SEARCH_LANES 0   /* sph_neighbor_count / sph_neighbor_fill   */
+#define SPH_TUNE_GATHER_LANES 1   /* density and delta-SPH gathers            */
+#define SPH_TUNE_SOLVE_LANES  2   /* DFSPH alpha/source, iterate and finish   */
+#define SPH_TUNE_COUNT        3
+int sph_set_tuning(int knob, int value);
+int sph_get_tuning(int knob);
+
 /* -------------------------------------------------------- neighbour search */
 /* Counting sort of the grid's points by (scene, cell key, index) and the
  * dense cell-start table.  Replaces the reference's hash map; the pair set
@@ -283,7 +297,9 @@ typedef struct SphDfsph {
   float* b_p;    /* [nb] boundary pressure                            */
   float* b_resid;/* [nb]                                              */
   float* bxrec;  /* [nb][4] {x, y, p_b / (rho_b rho0_b)^2, V_b rho0_b} */
-  double* err_acc;  /* [nscenes] FP64 sum of clamped residuals        */
+  double* err_acc;  /* [SPH_DFSPH_ERR_SLOTS][nscenes] FP64 partial sums
+                       of clamped residuals; workgroups spread their
+                       atomics over the slots                          */
   double* err_hist; /* [nscenes][max_hist] per-iteration errors        */
   int* iters;       /* [nscenes] iterations done                       */
   int* active;      /* [nscenes] 1 while the scene still iterates      */
@@ -300,6 +316,8 @@ typedef struct SphDfsph {
   int nscenes, max_hist;
   int grid_blocks;  /* grid of the iteration kernels (work is strided) */
 } SphDfsph;
+
+#define SPH_DFSPH_ERR_SLOTS 32
 
 /* v* = v + dt a, A^act = A / rho; boundary A^act_b = V_b / rho_b, p_b = 0;
  * pressures zeroed; iteration counters reset; every scene active.  The
@@ -318,6 +336,20 @@ int sph_dfsph_alpha_source(const SphFluid* f, const SphPhysics* phys, const SphC
 int sph_dfsph_iterate(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
                       const SphBoundary* b, const SphCSR* fb, const SphCSR* bf,
                       SphDfsph* d, hipStream_t s);
+/* The three launches of sph_dfsph_iterate, one entry point each, in the
+ * order of the reference loop body (dfsph.py:205-230):
+ *   sph_dfsph_accel    <- computeAcceleration (dfsph.py:186-193, 26-42)
+ *   sph_dfsph_pressure <- computeUpdatedPressure + the clamped residual sum
+ *                         (dfsph.py:164-184, 44-57, 228)
+ *   sph_dfsph_control  <- the while condition of densitySolve (dfsph.py:205-207)
+ * Between sph_dfsph_control calls, accel and pressure touch only the scenes
+ * of the current active list. */
+int sph_dfsph_accel(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                    const SphCSR* fb, SphDfsph* d, hipStream_t s);
+int sph_dfsph_pressure(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                       const SphBoundary* b, const SphCSR* fb, const SphCSR* bf,
+                       SphDfsph* d, hipStream_t s);
+int sph_dfsph_control(const SphPhysics* phys, SphDfsph* d, hipStream_t s);
 /* Final pressure acceleration, v* += dt a*, acc += a*, p = p2. */
 int sph_dfsph_finish(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
                      const SphCSR* fb, SphDfsph* d, float* acc, hipStream_t s);

@@ -85,6 +85,9 @@ ST_OUTSIDE, ST_OVERFLOW, ST_NAN, ST_ITERCAP = 1, 2, 4, 8
 T_CONT, T_CONT_B, T_MOG, T_DDSPH = 1 << 0, 1 << 1, 1 << 2, 1 << 3
 T_PRESS, T_PRESS_B, T_VISC, T_VISC_B = 1 << 4, 1 << 5, 1 << 6, 1 << 7
 T_LAM, T_GRAV, T_ACCUM = 1 << 8, 1 << 9, 1 << 16
+TUNE_SEARCH_LANES, TUNE_GATHER_LANES, TUNE_SOLVE_LANES = 0, 1, 2
+DFSPH_ERR_SLOTS = 32
+TUNE_KNOBS = {"search": TUNE_SEARCH_LANES, "gather": TUNE_GATHER_LANES, "solve": TUNE_SOLVE_LANES}
 
 # exported entry points: name -> argtypes (restype is int unless noted)
 _P = C.POINTER
@@ -96,6 +99,8 @@ _SIGS = {
     "sph_scan_exclusive": ([c_vp, c_int, c_vp, c_vp, C.c_size_t, c_vp], c_int),
     "sph_fill_i32": ([c_vp, c_int, c_int, c_vp], c_int),
     "sph_fill_f32": ([c_vp, c_int, C.c_float, c_vp], c_int),
+    "sph_set_tuning": ([c_int, c_int], c_int),
+    "sph_get_tuning": ([c_int], c_int),
     "sph_grid_build": ([_P(SphGrid), c_vp, C.c_size_t, c_vp], c_int),
     "sph_neighbor_count": ([_P(SphGrid), _P(SphQuery), c_vp, c_vp], c_int),
     "sph_neighbor_fill": ([_P(SphGrid), _P(SphQuery), c_vp, c_vp, C.c_int64, c_vp], c_int),
@@ -122,6 +127,10 @@ _SIGS = {
                                 _P(SphCSR), _P(SphDfsph), c_vp], c_int),
     "sph_dfsph_iterate": ([_P(SphFluid), c_vp, _P(SphCSR), _P(SphBoundary), _P(SphCSR),
                            _P(SphCSR), _P(SphDfsph), c_vp], c_int),
+    "sph_dfsph_accel": ([_P(SphFluid), c_vp, _P(SphCSR), _P(SphCSR), _P(SphDfsph), c_vp], c_int),
+    "sph_dfsph_pressure": ([_P(SphFluid), c_vp, _P(SphCSR), _P(SphBoundary), _P(SphCSR),
+                            _P(SphCSR), _P(SphDfsph), c_vp], c_int),
+    "sph_dfsph_control": ([c_vp, _P(SphDfsph), c_vp], c_int),
     "sph_dfsph_finish": ([_P(SphFluid), c_vp, _P(SphCSR), _P(SphCSR), _P(SphDfsph), c_vp, c_vp],
                          c_int),
     "sph_dsph_pack": ([_P(SphFluid), c_vp, _P(SphBoundary), c_vp, c_vp, c_vp], c_int),
@@ -165,6 +174,31 @@ def call(name: str, *args) -> None:
     if rc != 0:
         msg = lib.sph_last_error().decode(errors="replace")
         raise RuntimeError(f"{name} failed (status {rc}): {msg}")
+
+
+def set_tuning(**lanes: int) -> dict:
+    """Sets lanes-per-row knobs by name (search=, gather=, solve=); returns
+    the previous values.  Host-side only: no device is touched."""
+    old = get_tuning()
+    for name, v in lanes.items():
+        call("sph_set_tuning", TUNE_KNOBS[name], int(v))
+    return old
+
+
+def get_tuning() -> dict:
+    lib = load()
+    return {name: int(lib.sph_get_tuning(k)) for name, k in TUNE_KNOBS.items()}
+
+
+DFSPH_GRID_CAP = 8192
+
+
+def dfsph_grid_blocks(rows: int, cap: int | None = None) -> int:
+    """Workgroups of the DFSPH iteration kernels: one lane group per row
+    (fluid + boundary), capped at DFSPH_GRID_CAP; the kernels stride over
+    larger work."""
+    kl = get_tuning()["solve"]
+    return max(1, min(-(-(rows * kl) // 256), cap or DFSPH_GRID_CAP))
 
 
 def ptr(t) -> int | None:

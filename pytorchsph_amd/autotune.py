@@ -1,0 +1,123 @@
+"""Launch-shape autotuner of the batched step.
+
+The knobs are the lanes per row of the neighbour search, of the kernel-sum
+gathers and of the DFSPH solve (sph_set_tuning), and the workgroup cap of
+the DFSPH iteration kernels.  Each knob is timed on the batch it will run
+(same scenes, same pair lists), one knob at a time with the others held,
+and the fastest value is kept.  Pair sets do not depend on any knob; sums
+are re-associated, so results move within fp32 rounding.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib as L
+
+LANE_VALUES = (1, 2, 4, 8, 16, 32)
+GRID_CAPS = (2048, 4096, 8192, 16384)
+
+
+def _time(fn, reps: int) -> float:
+    """Average seconds per call of fn() on the current stream (events)."""
+    stream = torch.cuda.current_stream()
+    fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def _search_fn(b):
+    return lambda: b.search(b.x)
+
+
+def _gather_fn(b):
+    if b.scheme == "dsph":
+        return lambda: b.dsph_rhs(b.x, b.v, b.rho)
+    F = b.fluid(b.x, b.v, b.rho)
+    ff = b.ff.csr().c()
+    fb = b.fb.csr().c() if b.NB else None
+    B = b.boundary()
+
+    def run():
+        L.call("sph_fluid_density", C.byref(F), L.ptr(b.phys), C.byref(ff), C.byref(B),
+               C.byref(fb) if fb is not None else None, b._order(), L.ptr(b.rho), L.ptr(b.vol),
+               L.stream_handle())
+    return run
+
+
+def _solve_fn(b):
+    """accel + pressure over every scene (the active list after prepare)."""
+    F = b.fluid(b.x, b.v, b.rho)
+    ff = b.ff.csr().c()
+    fb = b.fb.csr().c() if b.NB else None
+    bf = b.bf.csr().c()
+    B = b.boundary()
+    sh = L.stream_handle()
+
+    def run():
+        D = b._dfsph_struct()     # grid size follows the current knobs
+        L.call("sph_dfsph_accel", C.byref(F), L.ptr(b.phys), C.byref(ff),
+               C.byref(fb) if fb is not None else None, C.byref(D), sh)
+        L.call("sph_dfsph_pressure", C.byref(F), L.ptr(b.phys), C.byref(ff), C.byref(B),
+               C.byref(fb) if fb is not None else None, C.byref(bf), C.byref(D), sh)
+    return run
+
+
+def tune(parts, reps: int = 5, log=None) -> dict:
+    """Tunes the knobs on the SceneBatch objects `parts` (their current
+    state; DFSPH parts are left with a fresh prologue, so call this before
+    stepping or capturing).  Returns {"choice": {...}, "times": {...}}."""
+    dsph = [b for b in parts if b.scheme == "dsph"]
+    dfsph = [b for b in parts if b.scheme == "dfsph"]
+    torch.cuda.synchronize()
+    for b in dsph:
+        b.dsph_rhs(b.x, b.v, b.rho)
+    for b in dfsph:
+        b._dfsph_pre()
+    torch.cuda.synchronize()
+    plans = {
+        "search": [_search_fn(b) for b in parts],
+        "gather": [_gather_fn(b) for b in parts],
+        "solve": [_solve_fn(b) for b in dfsph],
+    }
+    times: dict = {}
+    choice = L.get_tuning()
+    for knob, fns in plans.items():
+        if not fns:
+            continue
+        res = {}
+        for v in LANE_VALUES:
+            L.set_tuning(**{knob: v})
+            res[v] = sum(_time(fn, reps) for fn in fns)
+        best = min(res, key=res.get)
+        L.set_tuning(**{knob: best})
+        choice[knob] = best
+        times[knob] = res
+        if log:
+            log(f"autotune {knob}: " + ", ".join(f"{k}:{t * 1e6:.0f}us" for k, t in res.items())
+                + f" -> {best}")
+    if dfsph:
+        fns = plans["solve"]
+        res = {}
+        for cap in GRID_CAPS:
+            L.DFSPH_GRID_CAP = cap
+            res[cap] = sum(_time(fn, reps) for fn in fns)
+        best = min(res, key=res.get)
+        L.DFSPH_GRID_CAP = best
+        choice["grid_cap"] = best
+        times["grid_cap"] = res
+        if log:
+            log("autotune grid_cap: " + ", ".join(f"{k}:{t * 1e6:.0f}us" for k, t in res.items())
+                + f" -> {best}")
+    # leave the DFSPH parts in a consistent prologue state
+    for b in dfsph:
+        b._dfsph_pre()
+    torch.cuda.synchronize()
+    return {"choice": choice, "times": {k: {str(a): c for a, c in v.items()}
+                                        for k, v in times.items()}}

@@ -55,6 +55,14 @@ __device__ __forceinline__ int find_rank(const int* __restrict__ pre, int nact, 
   }
   return lo;
 }
+// The same rank found by walking forward from the rank r of an item
+// t0 <= t.  A workgroup's items are consecutive, so one binary search with
+// a workgroup-uniform argument (scalar loads) plus this walk, usually zero
+// or one step, replaces a binary search per lane.
+__device__ __forceinline__ int advance_rank(const int* __restrict__ pre, int nact, int r, int t) {
+  while (r + 1 < nact && pre[r + 1] <= t) ++r;
+  return r;
+}
 
 __global__ __launch_bounds__(kBS) void k_prepare(SphFluid f, const float* __restrict__ acc,
                                                  const SphPhysics* __restrict__ phys, SphBoundary b,
@@ -80,8 +88,8 @@ __global__ __launch_bounds__(kBS) void k_prepare(SphFluid f, const float* __rest
     const float2 x = ld2(b.x, k);
     st4(d.bxrec, k, make_float4(x.x, x.y, 0.f, b.vol[k] * b.rest[k]));
   }
+  if (t < SPH_DFSPH_ERR_SLOTS * d.nscenes) d.err_acc[t] = 0.0;
   if (t < d.nscenes) {
-    d.err_acc[t] = 0.0;
     d.iters[t] = 0;
     d.active[t] = 1;
   }
@@ -125,17 +133,21 @@ __device__ void build_active_lists(SphDfsph& d) {
 
 __global__ __launch_bounds__(kBS) void k_active_lists(SphDfsph d) { build_active_lists(d); }
 
+// KL lanes share one row (fluid particle or boundary particle).
+template <int KL>
 __global__ __launch_bounds__(kBS) void k_alpha_source(SphFluid f, const SphPhysics* __restrict__ phys,
                                                       SphCSR ff, SphBoundary b, SphCSR fb, SphCSR bf,
                                                       SphDfsph d) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = (int)(gt / KL), sub = (int)(gt % KL);
   if (t < f.np) {
     const int i = t;
     const SphPhysics P = phys[f.pscene[i]];
     const KConst K = kconst(P.h);
     const float2 xi = ld2(f.x, i), vi = ld2(d.vpred, i);
     float k1x = 0.f, k1y = 0.f, k2 = 0.f, src = 0.f;
-    for (int k = ff.rowptr[i]; k < ff.rowptr[i + 1]; ++k) {
+    const int r1 = ff.rowptr[i + 1];
+    for (int k = ff.rowptr[i] + sub; k < r1; k += KL) {
       const int j = ff.col[k];
       const float2 xj = ld2(f.x, j), vj = ld2(d.vpred, j);
       const float2 G = spiky_G(xi.x, xi.y, xj.x, xj.y, K);
@@ -146,7 +158,8 @@ __global__ __launch_bounds__(kBS) void k_alpha_source(SphFluid f, const SphPhysi
       src += -P.dt * aa * ((vi.x - vj.x) * G.x + (vi.y - vj.y) * G.y);
     }
     if (fb.rowptr) {
-      for (int k = fb.rowptr[i]; k < fb.rowptr[i + 1]; ++k) {
+      const int e1 = fb.rowptr[i + 1];
+      for (int k = fb.rowptr[i] + sub; k < e1; k += KL) {
         const int j = fb.col[k];
         const float2 xb = ld2(b.x, j);
         const float2 G = spiky_G(xi.x, xi.y, xb.x, xb.y, K);
@@ -158,6 +171,11 @@ __global__ __launch_bounds__(kBS) void k_alpha_source(SphFluid f, const SphPhysi
         src += -P.dt * aa * (vi.x * G.x + vi.y * G.y);
       }
     }
+    k1x = group_sum<KL>(k1x);
+    k1y = group_sum<KL>(k1y);
+    k2 = group_sum<KL>(k2);
+    src = group_sum<KL>(src);
+    if (sub != 0) return;
     const float ai = d.aact[i];
     const float fac = -P.dt * P.dt * ai;
     const float mass = f.area[i] * f.rest[i];
@@ -171,7 +189,8 @@ __global__ __launch_bounds__(kBS) void k_alpha_source(SphFluid f, const SphPhysi
     const KConst K = kconst(P.h);
     const float2 xb = ld2(b.x, k0);
     float s2 = 0.f, src = 0.f;
-    for (int k = bf.rowptr[k0]; k < bf.rowptr[k0 + 1]; ++k) {
+    const int e1 = bf.rowptr[k0 + 1];
+    for (int k = bf.rowptr[k0] + sub; k < e1; k += KL) {
       const int j = bf.col[k];
       const float2 xf = ld2(f.x, j), vf = ld2(d.vpred, j);
       const float2 G = spiky_G(xb.x, xb.y, xf.x, xf.y, K);
@@ -179,19 +198,25 @@ __global__ __launch_bounds__(kBS) void k_alpha_source(SphFluid f, const SphPhysi
       s2 += aa * aa / (f.area[j] * f.rest[j]) * (G.x * G.x + G.y * G.y);
       src += aa * ((0.f - vf.x) * G.x + (0.f - vf.y) * G.y);
     }
+    s2 = group_sum<KL>(s2);
+    src = group_sum<KL>(src);
+    if (sub != 0) return;
     d.b_alpha[k0] = -P.dt * P.dt * d.b_aact[k0] * s2;   // unclamped (akinciBoundary.py:340)
     d.b_src[k0] = 1.f - b.rho[k0] - P.dt * src;
   }
 }
 
 // a*_i = sum_j -(A_j rho0_j)(pt_i + pt_j) G_ij + boundary (dfsph.py:25-42,
-// akinciBoundary.py:229-252), pt = p / (rho rho0)^2
+// akinciBoundary.py:229-252), pt = p / (rho rho0)^2.  Lane `sub` of a group
+// of KL takes pairs sub, sub + KL, ...; every lane returns the group's sum.
+template <int KL>
 __device__ __forceinline__ float2 pressure_accel(const SphCSR& ff, const SphCSR& fb,
-                                                 const SphDfsph& d, const KConst& K, int i) {
+                                                 const SphDfsph& d, const KConst& K, int i,
+                                                 int sub) {
   const float4 ri = ld4(d.xrec, i);
   float ax = 0.f, ay = 0.f;
   const int k1 = ff.rowptr[i + 1];
-  for (int k = ff.rowptr[i]; k < k1; ++k) {
+  for (int k = ff.rowptr[i] + sub; k < k1; k += KL) {
     const float4 rj = ld4(d.xrec, ff.col[k]);
     const float2 G = spiky_G(ri.x, ri.y, rj.x, rj.y, K);
     const float c = -rj.w * (ri.z + rj.z);
@@ -200,7 +225,7 @@ __device__ __forceinline__ float2 pressure_accel(const SphCSR& ff, const SphCSR&
   }
   if (fb.rowptr) {
     const int e1 = fb.rowptr[i + 1];
-    for (int k = fb.rowptr[i]; k < e1; ++k) {
+    for (int k = fb.rowptr[i] + sub; k < e1; k += KL) {
       const float4 rb = ld4(d.bxrec, fb.col[k]);
       const float2 G = spiky_G(ri.x, ri.y, rb.x, rb.y, K);
       const float c = -rb.w * (ri.z + rb.z);
@@ -208,26 +233,44 @@ __device__ __forceinline__ float2 pressure_accel(const SphCSR& ff, const SphCSR&
       ay += c * G.y;
     }
   }
-  return make_float2(ax, ay);
+  return make_float2(group_sum<KL>(ax), group_sum<KL>(ay));
 }
 
+// Work item (active fluid row) of this lane group and the stride between
+// the items of successive grid passes; group-uniform.
+template <int KL>
+__device__ __forceinline__ int item0() {
+  return (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / KL);
+}
+template <int KL>
+__device__ __forceinline__ int item_stride() {
+  return (int)((int64_t)gridDim.x * blockDim.x / KL);
+}
+
+template <int KL>
 __global__ __launch_bounds__(kBS) void k_accel(SphFluid f, const SphPhysics* __restrict__ phys,
                                                SphCSR ff, SphCSR fb, SphDfsph d) {
   const int nact = d.act_n[2];
   const int nitems = d.act_n[0];
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nitems; t += gridDim.x * blockDim.x) {
-    const int r = find_rank(d.act_pre, nact, t);
+  const int sub = threadIdx.x % KL;
+  const int per_block = blockDim.x / KL;
+  for (int base = blockIdx.x * per_block; base < nitems; base += gridDim.x * per_block) {
+    const int t = base + threadIdx.x / KL;
+    const int r0 = find_rank(d.act_pre, nact, base);
+    if (t >= nitems) break;
+    const int r = advance_rank(d.act_pre, nact, r0, t);
     const int s = d.act_scene[r];
     const int i = d.fbegin[s] + (t - d.act_pre[r]);
     const SphPhysics P = phys[s];
     const KConst K = kconst(P.h);
-    const float2 a = pressure_accel(ff, fb, d, K, i);
-    st4(d.arec, i, make_float4(a.x, a.y, P.dt * P.dt * d.aact[i], 0.f));
+    const float2 a = pressure_accel<KL>(ff, fb, d, K, i, sub);
+    if (sub == 0) st4(d.arec, i, make_float4(a.x, a.y, P.dt * P.dt * d.aact[i], 0.f));
   }
 }
 
 // Adds v (FP64) into acc[scene], one atomic per wavefront when the whole
-// wavefront belongs to one scene.
+// wavefront belongs to one scene.  `acc` is the workgroup's slot row of the
+// partial sums, so concurrent atomics on one scene spread over the slots.
 __device__ __forceinline__ void scene_sum(double v, int s, bool valid, double* acc) {
   const int lane = threadIdx.x & (kWave - 1);
   const int s0 = __shfl(valid ? s : -1, 0, kWave);
@@ -241,22 +284,30 @@ __device__ __forceinline__ void scene_sum(double v, int s, bool valid, double* a
   }
 }
 
+template <int KL>
 __global__ __launch_bounds__(kBS) void k_pressure(SphFluid f, const SphPhysics* __restrict__ phys,
                                                   SphCSR ff, SphBoundary b, SphCSR fb, SphCSR bf,
                                                   SphDfsph d) {
   const int nact = d.act_n[2];
   const int nf = d.act_n[0];
   const int nitems = nf + d.act_n[1];
-  const int stride = gridDim.x * blockDim.x;
+  const int per_block = blockDim.x / KL;
+  const int stride = gridDim.x * per_block;
+  const int sub = threadIdx.x % KL;
+  const int* preb = d.act_pre + d.nscenes + 1;
+  double* acc_slot = d.err_acc + (size_t)(blockIdx.x % SPH_DFSPH_ERR_SLOTS) * d.nscenes;
   // every lane runs the same number of loop trips so the wavefront-wide
   // reductions inside see all lanes
-  for (int base = blockIdx.x * blockDim.x; base < nitems; base += stride) {
-    const int t = base + threadIdx.x;
+  for (int base = blockIdx.x * per_block; base < nitems; base += stride) {
+    const int t = base + threadIdx.x / KL;
     double contrib = 0.0;
     int s = -1;
     bool valid = false;
+    // workgroup-uniform ranks of the pass's first fluid and boundary items
+    const int r0f = base < nf ? find_rank(d.act_pre, nact, base) : 0;
+    const int r0b = base + per_block > nf ? find_rank(preb, nact, base > nf ? base - nf : 0) : 0;
     if (t < nf) {
-      const int r = find_rank(d.act_pre, nact, t);
+      const int r = advance_rank(d.act_pre, nact, r0f, t);
       s = d.act_scene[r];
       const int i = d.fbegin[s] + (t - d.act_pre[r]);
       const SphPhysics P = phys[s];
@@ -266,7 +317,7 @@ __global__ __launch_bounds__(kBS) void k_pressure(SphFluid f, const SphPhysics* 
       const float dt2 = P.dt * P.dt;
       float ks = 0.f;
       const int k1 = ff.rowptr[i + 1];
-      for (int k = ff.rowptr[i]; k < k1; ++k) {
+      for (int k = ff.rowptr[i] + sub; k < k1; k += KL) {
         const int j = ff.col[k];
         const float2 xj = ld2(d.xrec, 2 * j);
         const float4 aj = ld4(d.arec, j);
@@ -275,53 +326,59 @@ __global__ __launch_bounds__(kBS) void k_pressure(SphFluid f, const SphPhysics* 
       }
       if (fb.rowptr) {
         const int e1 = fb.rowptr[i + 1];
-        for (int k = fb.rowptr[i]; k < e1; ++k) {
+        for (int k = fb.rowptr[i] + sub; k < e1; k += KL) {
           const int j = fb.col[k];
           const float2 xb = ld2(d.bxrec, 2 * j);
           const float2 G = spiky_G(xi.x, xi.y, xb.x, xb.y, K);
           ks += dt2 * d.b_aact[j] * (ai.x * G.x + ai.y * G.y);
         }
       }
-      const float rr = ks - d.src[i];
-      const float pold = d.p2[i];
-      d.p[i] = pold;
-      float pn = pold - P.fluid_omega * rr / d.alpha[i];
-      pn = pn < 0.f ? 0.f : pn;  // clamp(min = 0) keeps NaN
-      if (isnan(pn) || isinf(pn)) atomicOr(&d.status[s], SPH_ST_NAN);
-      d.p2[i] = pn;
-      d.resid[i] = rr;
-      const float rho = f.rho[i] * f.rest[i];
-      d.xrec[4 * i + 2] = pn / (rho * rho);
-      contrib = (double)(rr < -P.tau ? -P.tau : rr);
-      valid = true;
+      ks = group_sum<KL>(ks);
+      if (sub == 0) {
+        const float rr = ks - d.src[i];
+        const float pold = d.p2[i];
+        d.p[i] = pold;
+        float pn = pold - P.fluid_omega * rr / d.alpha[i];
+        pn = pn < 0.f ? 0.f : pn;  // clamp(min = 0) keeps NaN
+        if (isnan(pn) || isinf(pn)) atomicOr(&d.status[s], SPH_ST_NAN);
+        d.p2[i] = pn;
+        d.resid[i] = rr;
+        const float rho = f.rho[i] * f.rest[i];
+        d.xrec[4 * i + 2] = pn / (rho * rho);
+        contrib = (double)(rr < -P.tau ? -P.tau : rr);
+        valid = true;
+      }
     } else if (t < nitems) {
       const int tb = t - nf;
-      const int* preb = d.act_pre + d.nscenes + 1;
-      const int r = find_rank(preb, nact, tb);
+      const int r = advance_rank(preb, nact, r0b, tb);
       s = d.act_scene[r];
       const int k0 = d.bbegin[s] + (tb - preb[r]);
       const SphPhysics P = phys[s];
       const KConst K = kconst(P.h);
       const float2 xb = ld2(d.bxrec, 2 * k0);
       float ks = 0.f;
-      for (int k = bf.rowptr[k0]; k < bf.rowptr[k0 + 1]; ++k) {
+      const int e1 = bf.rowptr[k0 + 1];
+      for (int k = bf.rowptr[k0] + sub; k < e1; k += KL) {
         const int j = bf.col[k];
         const float2 xf = ld2(d.xrec, 2 * j);
         const float4 af = ld4(d.arec, j);
         const float2 G = spiky_G(xb.x, xb.y, xf.x, xf.y, K);
         ks += af.z * (af.x * (-G.x) + af.y * (-G.y));
       }
-      const float rr = ks - d.b_src[k0];
-      float pn = d.b_p[k0] - P.omega * rr / d.b_alpha[k0];
-      pn = pn < 0.f ? 0.f : pn;
-      d.b_p[k0] = pn;
-      d.b_resid[k0] = rr;
-      const float rb = b.rho[k0] * b.rest[k0];
-      d.bxrec[4 * k0 + 2] = pn / (rb * rb);
-      contrib = (double)(rr < -P.tau ? -P.tau : rr);
-      valid = true;
+      ks = group_sum<KL>(ks);
+      if (sub == 0) {
+        const float rr = ks - d.b_src[k0];
+        float pn = d.b_p[k0] - P.omega * rr / d.b_alpha[k0];
+        pn = pn < 0.f ? 0.f : pn;
+        d.b_p[k0] = pn;
+        d.b_resid[k0] = rr;
+        const float rb = b.rho[k0] * b.rest[k0];
+        d.bxrec[4 * k0 + 2] = pn / (rb * rb);
+        contrib = (double)(rr < -P.tau ? -P.tau : rr);
+        valid = true;
+      }
     }
-    scene_sum(contrib, s, valid, d.err_acc);
+    scene_sum(contrib, s, valid, acc_slot);
   }
 }
 
@@ -332,12 +389,16 @@ __global__ __launch_bounds__(kBS) void k_control(const SphPhysics* __restrict__ 
     if (!d.active[s]) continue;
     const SphPhysics P = phys[s];
     const double n = (double)(d.nfluid[s] + d.nbdy[s]);
-    const double err = n > 0 ? d.err_acc[s] / n : 0.0;
+    double sum = 0.0;
+    for (int k = 0; k < SPH_DFSPH_ERR_SLOTS; ++k) {
+      sum += d.err_acc[(size_t)k * d.nscenes + s];
+      d.err_acc[(size_t)k * d.nscenes + s] = 0.0;
+    }
+    const double err = n > 0 ? sum / n : 0.0;
     const int it = d.iters[s];
     if (it < d.max_hist) d.err_hist[(size_t)s * d.max_hist + it] = err;
     const int i = it + 1;
     d.iters[s] = i;
-    d.err_acc[s] = 0.0;
     const bool go = (i < P.min_iters || err > (double)P.tau) && i <= P.max_iters;
     if (!go) {
       d.active[s] = 0;
@@ -348,14 +409,17 @@ __global__ __launch_bounds__(kBS) void k_control(const SphPhysics* __restrict__ 
   build_active_lists(d);
 }
 
+template <int KL>
 __global__ __launch_bounds__(kBS) void k_finish(SphFluid f, const SphPhysics* __restrict__ phys,
                                                 SphCSR ff, SphCSR fb, SphDfsph d,
                                                 float* __restrict__ acc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = item0<KL>();
+  const int sub = threadIdx.x % KL;
   if (i >= f.np) return;
   const SphPhysics P = phys[f.pscene[i]];
   const KConst K = kconst(P.h);
-  const float2 a = pressure_accel(ff, fb, d, K, i);
+  const float2 a = pressure_accel<KL>(ff, fb, d, K, i, sub);
+  if (sub != 0) return;
   st2(d.apred, i, a);
   d.p[i] = d.p2[i];
   const float2 vp = ld2(d.vpred, i);
@@ -397,7 +461,7 @@ int dfsph_prepare(const SphFluid* f, const float* acc, const SphPhysics* phys, c
   const SphBoundary bb = bdy_or_null(b);
   SPH_REQUIRE(bdy_ok(bb, d), "sph_dfsph_prepare: bad boundary");
   int n = f->np + bb.nb;
-  if (n < d->nscenes) n = d->nscenes;
+  if (n < SPH_DFSPH_ERR_SLOTS * d->nscenes) n = SPH_DFSPH_ERR_SLOTS * d->nscenes;
   if (n < 1) n = 1;
   hipLaunchKernelGGL(k_prepare, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, *f, acc, phys, bb, *d);
   hipLaunchKernelGGL(k_active_lists, dim3(1), dim3(kBS), 0, s, *d);
@@ -413,23 +477,54 @@ int dfsph_alpha_source(const SphFluid* f, const SphPhysics* phys, const SphCSR* 
               "sph_dfsph_alpha_source: boundary needs its pair list");
   const int n = f->np + bb.nb;
   if (n == 0) return SPH_OK;
-  hipLaunchKernelGGL(k_alpha_source, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, *f, phys, *ff, bb,
-                     csr_or_null(fb), csr_or_null(bf), *d);
+  with_lanes(tuning(SPH_TUNE_SOLVE_LANES), [&](auto kl) {
+    constexpr int KL = decltype(kl)::value;
+    hipLaunchKernelGGL((k_alpha_source<KL>), dim3(blocks_for((int64_t)n * KL, kBS)), dim3(kBS), 0,
+                       s, *f, phys, *ff, bb, csr_or_null(fb), csr_or_null(bf), *d);
+  });
   return launch_check("sph_dfsph_alpha_source");
+}
+
+int dfsph_accel(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, const SphCSR* fb,
+                SphDfsph* d, hipStream_t s) {
+  SPH_REQUIRE(f && phys && ff && ff->rowptr && dfsph_ok(d), "sph_dfsph_accel: bad arguments");
+  SPH_REQUIRE(!fb || !fb->rowptr || d->bxrec, "sph_dfsph_accel: boundary list without records");
+  with_lanes(tuning(SPH_TUNE_SOLVE_LANES), [&](auto kl) {
+    constexpr int KL = decltype(kl)::value;
+    hipLaunchKernelGGL((k_accel<KL>), dim3(d->grid_blocks), dim3(kBS), 0, s, *f, phys, *ff,
+                       csr_or_null(fb), *d);
+  });
+  return launch_check("sph_dfsph_accel");
+}
+
+int dfsph_pressure(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                   const SphBoundary* b, const SphCSR* fb, const SphCSR* bf, SphDfsph* d,
+                   hipStream_t s) {
+  SPH_REQUIRE(f && f->rho && f->rest && phys && ff && ff->rowptr && dfsph_ok(d),
+              "sph_dfsph_pressure: bad arguments");
+  const SphBoundary bb = bdy_or_null(b);
+  SPH_REQUIRE(bdy_ok(bb, d) && (bb.nb == 0 || (bf && bf->rowptr)),
+              "sph_dfsph_pressure: bad boundary");
+  with_lanes(tuning(SPH_TUNE_SOLVE_LANES), [&](auto kl) {
+    constexpr int KL = decltype(kl)::value;
+    hipLaunchKernelGGL((k_pressure<KL>), dim3(d->grid_blocks), dim3(kBS), 0, s, *f, phys, *ff,
+                       bb, csr_or_null(fb), csr_or_null(bf), *d);
+  });
+  return launch_check("sph_dfsph_pressure");
+}
+
+int dfsph_control(const SphPhysics* phys, SphDfsph* d, hipStream_t s) {
+  SPH_REQUIRE(phys && dfsph_ok(d), "sph_dfsph_control: bad arguments");
+  hipLaunchKernelGGL(k_control, dim3(1), dim3(kBS), 0, s, phys, *d);
+  return launch_check("sph_dfsph_control");
 }
 
 int dfsph_iterate(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, const SphBoundary* b,
                   const SphCSR* fb, const SphCSR* bf, SphDfsph* d, hipStream_t s) {
-  SPH_REQUIRE(f && f->rho && f->rest && phys && ff && ff->rowptr && dfsph_ok(d),
-              "sph_dfsph_iterate: bad arguments");
-  const SphBoundary bb = bdy_or_null(b);
-  SPH_REQUIRE(bdy_ok(bb, d) && (bb.nb == 0 || (bf && bf->rowptr)), "sph_dfsph_iterate: bad boundary");
-  const dim3 grid(d->grid_blocks), block(kBS);
-  hipLaunchKernelGGL(k_accel, grid, block, 0, s, *f, phys, *ff, csr_or_null(fb), *d);
-  hipLaunchKernelGGL(k_pressure, grid, block, 0, s, *f, phys, *ff, bb, csr_or_null(fb),
-                     csr_or_null(bf), *d);
-  hipLaunchKernelGGL(k_control, dim3(1), dim3(kBS), 0, s, phys, *d);
-  return launch_check("sph_dfsph_iterate");
+  int rc;
+  if ((rc = dfsph_accel(f, phys, ff, fb, d, s))) return rc;
+  if ((rc = dfsph_pressure(f, phys, ff, b, fb, bf, d, s))) return rc;
+  return dfsph_control(phys, d, s);
 }
 
 int dfsph_finish(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, const SphCSR* fb,
@@ -437,8 +532,11 @@ int dfsph_finish(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, co
   SPH_REQUIRE(f && f->pscene && phys && ff && ff->rowptr && dfsph_ok(d),
               "sph_dfsph_finish: bad arguments");
   if (f->np == 0) return SPH_OK;
-  hipLaunchKernelGGL(k_finish, dim3(blocks_for(f->np, kBS)), dim3(kBS), 0, s, *f, phys, *ff,
-                     csr_or_null(fb), *d, acc);
+  with_lanes(tuning(SPH_TUNE_SOLVE_LANES), [&](auto kl) {
+    constexpr int KL = decltype(kl)::value;
+    hipLaunchKernelGGL((k_finish<KL>), dim3(blocks_for((int64_t)f->np * KL, kBS)), dim3(kBS), 0, s,
+                       *f, phys, *ff, csr_or_null(fb), *d, acc);
+  });
   return launch_check("sph_dfsph_finish");
 }
 
@@ -458,6 +556,18 @@ int sph_dfsph_iterate(const SphFluid* f, const SphPhysics* phys, const SphCSR* f
                       const SphBoundary* b, const SphCSR* fb, const SphCSR* bf, SphDfsph* d,
                       hipStream_t s) {
   return sph::dfsph_iterate(f, phys, ff, b, fb, bf, d, s);
+}
+int sph_dfsph_accel(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                    const SphCSR* fb, SphDfsph* d, hipStream_t s) {
+  return sph::dfsph_accel(f, phys, ff, fb, d, s);
+}
+int sph_dfsph_pressure(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                       const SphBoundary* b, const SphCSR* fb, const SphCSR* bf, SphDfsph* d,
+                       hipStream_t s) {
+  return sph::dfsph_pressure(f, phys, ff, b, fb, bf, d, s);
+}
+int sph_dfsph_control(const SphPhysics* phys, SphDfsph* d, hipStream_t s) {
+  return sph::dfsph_control(phys, d, s);
 }
 int sph_dfsph_finish(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
                      const SphCSR* fb, SphDfsph* d, float* acc, hipStream_t s) {

@@ -1,8 +1,9 @@
 // Kernel-sum gathers of the delta-SPH right-hand side and of the boundary
-// models.  Every gather is a CSR row gather: one thread owns one particle,
-// walks its fluid neighbours and then its boundary neighbours, keeps all
-// sums in registers and writes each output once -- no atomics, so results
-// do not depend on scheduling.
+// models.  Every gather is a CSR row gather: a group of KL lanes owns one
+// particle, walks its fluid neighbours and then its boundary neighbours
+// (lane `sub` takes pairs sub, sub + KL, ...), keeps all sums in registers,
+// combines them with shuffles and writes each output once -- no atomics, so
+// results do not depend on scheduling.
 //
 // Two forms of the delta-SPH gather share one pair routine:
 //  - sph_dsph_gather reads the structure-of-arrays state directly (the
@@ -26,35 +27,51 @@ __device__ __forceinline__ void st2(float* p, int i, float2 v) {
 }
 
 // ------------------------------------------------------------- density
+// Row owner of this lane: particle index t (before `order`) and the lane's
+// position inside its group.
+template <int KL>
+struct RowLane {
+  int t, sub;
+};
+template <int KL>
+__device__ __forceinline__ RowLane<KL> row_lane() {
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  return {(int)(gt / KL), (int)(gt % KL)};
+}
+
+template <int KL>
 __global__ __launch_bounds__(kBS) void k_fluid_density(SphFluid f, const SphPhysics* __restrict__ phys,
                                                        SphCSR ff, SphBoundary b, SphCSR fb,
                                                        const int* __restrict__ order,
                                                        float* __restrict__ rho_out,
                                                        float* __restrict__ vol_out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= f.np) return;
-  const int i = order ? order[t] : t;
+  const RowLane<KL> rl = row_lane<KL>();
+  if (rl.t >= f.np) return;   // whole groups leave together
+  const int i = order ? order[rl.t] : rl.t;
   const KConst K = kconst(phys[f.pscene[i]].h);
   const float2 xi = ld2(f.x, i);
   float sum = 0.f;
-  for (int k = ff.rowptr[i]; k < ff.rowptr[i + 1]; ++k) {
+  const int k1 = ff.rowptr[i + 1];
+  for (int k = ff.rowptr[i] + rl.sub; k < k1; k += KL) {
     const int j = ff.col[k];
     const float2 xj = ld2(f.x, j);
     const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, K.inv_h);
     sum += f.area[j] * wendland_k(g.q, K);
   }
-  if (vol_out) vol_out[i] = f.area[i] / sum;
+  sum = group_sum<KL>(sum);
+  if (vol_out && rl.sub == 0) vol_out[i] = f.area[i] / sum;
   if (fb.rowptr) {
     float bs = 0.f;
-    for (int k = fb.rowptr[i]; k < fb.rowptr[i + 1]; ++k) {
+    const int e1 = fb.rowptr[i + 1];
+    for (int k = fb.rowptr[i] + rl.sub; k < e1; k += KL) {
       const int j = fb.col[k];
       const float2 xb = ld2(b.x, j);
       const PairGeo g = pair_geo(xb.x, xb.y, xi.x, xi.y, K.inv_h);
       bs += wendland_k(g.q, K) * b.vol[j];
     }
-    sum += bs;
+    sum += group_sum<KL>(bs);
   }
-  rho_out[i] = sum;
+  if (rl.sub == 0) rho_out[i] = sum;
 }
 
 // ------------------------------------------------- delta-SPH pair routine
@@ -76,6 +93,28 @@ struct Acc {
 };
 
 __device__ __forceinline__ bool has(unsigned terms, unsigned bit) { return (terms & bit) != 0; }
+
+// Combines the partial sums of a lane group; only the sums that `terms`
+// feeds are exchanged.
+template <int KL>
+__device__ __forceinline__ void reduce_acc(Acc& a, unsigned terms) {
+  if (KL == 1) return;
+  if (terms & (SPH_T_CONT | SPH_T_CONT_B)) a.cont = group_sum<KL>(a.cont);
+  if (terms & SPH_T_MOG) a.mog = group_sum<KL>(a.mog);
+  if (terms & SPH_T_DDSPH) a.dd = group_sum<KL>(a.dd);
+  if (terms & (SPH_T_PRESS | SPH_T_PRESS_B)) {
+    a.px = group_sum<KL>(a.px);
+    a.py = group_sum<KL>(a.py);
+  }
+  if (terms & (SPH_T_VISC | SPH_T_VISC_B)) {
+    a.vx = group_sum<KL>(a.vx);
+    a.vy = group_sum<KL>(a.vy);
+  }
+  if (terms & SPH_T_LAM) {
+    a.lx = group_sum<KL>(a.lx);
+    a.ly = group_sum<KL>(a.ly);
+  }
+}
 
 // One pair of the delta-SPH right-hand side (fluid row i, neighbour j).
 // gradW_ij = -g(q) rhat_ij, x_ji = x_j - x_i, r2e = |x_ji|^2 + eps.
@@ -155,15 +194,16 @@ constexpr unsigned kFluidPairTerms =
 constexpr unsigned kBdyPairTerms = SPH_T_CONT_B | SPH_T_PRESS_B | SPH_T_VISC_B;
 
 // structure-of-arrays form (any runtime term mask)
+template <int KL>
 __global__ __launch_bounds__(kBS) void k_dsph_gather(SphFluid f, const SphPhysics* __restrict__ phys,
                                                      SphCSR ff, SphBoundary b, SphCSR fb,
                                                      const float* __restrict__ gradrho,
                                                      unsigned terms, const int* __restrict__ order,
                                                      float* __restrict__ dpdt,
                                                      float* __restrict__ acc) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= f.np) return;
-  const int i = order ? order[t] : t;
+  const RowLane<KL> rl = row_lane<KL>();
+  if (rl.t >= f.np) return;
+  const int i = order ? order[rl.t] : rl.t;
   const SphPhysics P = phys[f.pscene[i]];
   const KConst K = kconst(P.h);
   const bool dd = has(terms, SPH_T_DDSPH);
@@ -176,7 +216,8 @@ __global__ __launch_bounds__(kBS) void k_dsph_gather(SphFluid f, const SphPhysic
   o.gx = gri.x; o.gy = gri.y;
   Acc a = {};
   if (terms & kFluidPairTerms) {
-    for (int k = ff.rowptr[i]; k < ff.rowptr[i + 1]; ++k) {
+    const int k1 = ff.rowptr[i + 1];
+    for (int k = ff.rowptr[i] + rl.sub; k < k1; k += KL) {
       const int j = ff.col[k];
       const float2 xj = ld2(f.x, j), vj = ld2(f.v, j);
       Nbr n;
@@ -189,7 +230,8 @@ __global__ __launch_bounds__(kBS) void k_dsph_gather(SphFluid f, const SphPhysic
     }
   }
   if (fb.rowptr && (terms & kBdyPairTerms)) {
-    for (int k = fb.rowptr[i]; k < fb.rowptr[i + 1]; ++k) {
+    const int e1 = fb.rowptr[i + 1];
+    for (int k = fb.rowptr[i] + rl.sub; k < e1; k += KL) {
       const int j = fb.col[k];
       const float2 xb = ld2(b.x, j);
       const float2 vb = b.v ? ld2(b.v, j) : make_float2(0.f, 0.f);
@@ -201,7 +243,8 @@ __global__ __launch_bounds__(kBS) void k_dsph_gather(SphFluid f, const SphPhysic
       dsph_pair(P, K, o, n, 0.f, 0.f, false, terms, a);
     }
   }
-  dsph_store(P, o, a, terms, i, dpdt, acc);
+  reduce_acc<KL>(a, terms);
+  if (rl.sub == 0) dsph_store(P, o, a, terms, i, dpdt, acc);
 }
 
 // packed records: rec[2 i] = {x, y, vx, vy}, rec[2 i + 1] = {rho, V, p, rho_hat}
@@ -226,16 +269,16 @@ __global__ __launch_bounds__(kBS) void k_dsph_pack(SphFluid f, const SphPhysics*
   }
 }
 
-template <unsigned T>
+template <unsigned T, int KL>
 __global__ __launch_bounds__(kBS) void k_dsph_gather_packed(
     const float4* __restrict__ frec, const float4* __restrict__ brec,
     const int* __restrict__ pscene, int np, const SphPhysics* __restrict__ phys, SphCSR ff,
     SphCSR fb, const float* __restrict__ gradrho, unsigned rterms, const int* __restrict__ order,
     float* __restrict__ dpdt, float* __restrict__ acc) {
   const unsigned terms = T ? T : rterms;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= np) return;
-  const int i = order ? order[t] : t;
+  const RowLane<KL> rl = row_lane<KL>();
+  if (rl.t >= np) return;
+  const int i = order ? order[rl.t] : rl.t;
   const SphPhysics P = phys[pscene[i]];
   const KConst K = kconst(P.h);
   const bool dd = has(terms, SPH_T_DDSPH);
@@ -249,7 +292,7 @@ __global__ __launch_bounds__(kBS) void k_dsph_gather_packed(
   Acc a = {};
   if (terms & kFluidPairTerms) {
     const int k1 = ff.rowptr[i + 1];
-    for (int k = ff.rowptr[i]; k < k1; ++k) {
+    for (int k = ff.rowptr[i] + rl.sub; k < k1; k += KL) {
       const int j = ff.col[k];
       const float4 r0 = frec[2 * j], r1 = frec[2 * j + 1];
       const Nbr n = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z};
@@ -259,14 +302,15 @@ __global__ __launch_bounds__(kBS) void k_dsph_gather_packed(
   }
   if (fb.rowptr && (terms & kBdyPairTerms)) {
     const int k1 = fb.rowptr[i + 1];
-    for (int k = fb.rowptr[i]; k < k1; ++k) {
+    for (int k = fb.rowptr[i] + rl.sub; k < k1; k += KL) {
       const int j = fb.col[k];
       const float4 r0 = brec[2 * j], r1 = brec[2 * j + 1];
       const Nbr n = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z};
       dsph_pair(P, K, o, n, 0.f, 0.f, false, terms, a);
     }
   }
-  dsph_store(P, o, a, terms, i, dpdt, acc);
+  reduce_acc<KL>(a, terms);
+  if (rl.sub == 0) dsph_store(P, o, a, terms, i, dpdt, acc);
 }
 
 // ------------------------------------------ renormalised density gradient
@@ -564,8 +608,12 @@ int fluid_density(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, c
   SPH_REQUIRE(fluid_ok(f) && phys && ff && ff->rowptr && rho_out, "sph_fluid_density: bad arguments");
   SPH_REQUIRE(!fb || !fb->rowptr || (b && b->x && b->vol), "sph_fluid_density: boundary list without boundary");
   if (f->np == 0) return SPH_OK;
-  hipLaunchKernelGGL(k_fluid_density, dim3(blocks_for(f->np, kBS)), dim3(kBS), 0, s, *f, phys, *ff,
-                     bdy_or_null(b), csr_or_null(fb), order, rho_out, vol_out);
+  with_lanes(tuning(SPH_TUNE_GATHER_LANES), [&](auto kl) {
+    constexpr int KL = decltype(kl)::value;
+    hipLaunchKernelGGL((k_fluid_density<KL>), dim3(blocks_for((int64_t)f->np * KL, kBS)),
+                       dim3(kBS), 0, s, *f, phys, *ff, bdy_or_null(b), csr_or_null(fb), order,
+                       rho_out, vol_out);
+  });
   return launch_check("sph_fluid_density");
 }
 
@@ -586,8 +634,12 @@ int dsph_gather(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, con
   int rc = check_bdy_terms(terms, b, fb, "sph_dsph_gather");
   if (rc) return rc;
   if (f->np == 0) return SPH_OK;
-  hipLaunchKernelGGL(k_dsph_gather, dim3(blocks_for(f->np, kBS)), dim3(kBS), 0, s, *f, phys, *ff,
-                     bdy_or_null(b), csr_or_null(fb), gradrho, terms, order, dpdt, acc);
+  with_lanes(tuning(SPH_TUNE_GATHER_LANES), [&](auto kl) {
+    constexpr int KL = decltype(kl)::value;
+    hipLaunchKernelGGL((k_dsph_gather<KL>), dim3(blocks_for((int64_t)f->np * KL, kBS)),
+                       dim3(kBS), 0, s, *f, phys, *ff, bdy_or_null(b), csr_or_null(fb), gradrho,
+                       terms, order, dpdt, acc);
+  });
   return launch_check("sph_dsph_gather");
 }
 
@@ -619,19 +671,22 @@ int dsph_gather_packed(const float* frec, const float* brec, const int* pscene, 
   SPH_REQUIRE(!(terms & kBdyPairTerms) || !fb || !fb->rowptr || brec,
               "sph_dsph_gather_packed: boundary terms without boundary records");
   if (np == 0) return SPH_OK;
-  const dim3 grid(blocks_for(np, kBS)), block(kBS);
   const float4* fr = reinterpret_cast<const float4*>(frec);
   const float4* br = reinterpret_cast<const float4*>(brec);
   const SphCSR fbc = csr_or_null(fb);
-  if (terms == kMaskMOG)
-    hipLaunchKernelGGL(k_dsph_gather_packed<kMaskMOG>, grid, block, 0, s, fr, br, pscene, np, phys,
-                       *ff, fbc, gradrho, terms, order, dpdt, acc);
-  else if (terms == kMaskVL)
-    hipLaunchKernelGGL(k_dsph_gather_packed<kMaskVL>, grid, block, 0, s, fr, br, pscene, np, phys,
-                       *ff, fbc, gradrho, terms, order, dpdt, acc);
-  else
-    hipLaunchKernelGGL(k_dsph_gather_packed<0>, grid, block, 0, s, fr, br, pscene, np, phys, *ff,
-                       fbc, gradrho, terms, order, dpdt, acc);
+  with_lanes(tuning(SPH_TUNE_GATHER_LANES), [&](auto kl) {
+    constexpr int KL = decltype(kl)::value;
+    const dim3 grid(blocks_for((int64_t)np * KL, kBS)), block(kBS);
+    if (terms == kMaskMOG)
+      hipLaunchKernelGGL((k_dsph_gather_packed<kMaskMOG, KL>), grid, block, 0, s, fr, br, pscene,
+                         np, phys, *ff, fbc, gradrho, terms, order, dpdt, acc);
+    else if (terms == kMaskVL)
+      hipLaunchKernelGGL((k_dsph_gather_packed<kMaskVL, KL>), grid, block, 0, s, fr, br, pscene,
+                         np, phys, *ff, fbc, gradrho, terms, order, dpdt, acc);
+    else
+      hipLaunchKernelGGL((k_dsph_gather_packed<0, KL>), grid, block, 0, s, fr, br, pscene, np,
+                         phys, *ff, fbc, gradrho, terms, order, dpdt, acc);
+  });
   return launch_check("sph_dsph_gather_packed");
 }
 

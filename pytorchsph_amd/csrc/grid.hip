@@ -74,12 +74,19 @@ __global__ __launch_bounds__(kBS) void k_grid_cell_sort(SphGrid g) {
 // row come out in sorted-point order.  Cells with qx < 0 or qy < 0 are
 // skipped as in queryHashMap (neighSearch.cpp:95); keys past the scene's
 // table are clipped.
-template <bool FILL>
+//
+// KL lanes share one query: they test consecutive candidates of a range
+// together (coalesced reads of the sorted positions) and place the hits with
+// a ballot and a popcount of the lower lanes, so the row is written in
+// candidate order with coalesced stores.
+template <bool FILL, int KL>
 __global__ __launch_bounds__(kBS) void k_neighbors(SphGrid g, SphQuery q,
                                                    const int* __restrict__ rowptr,
                                                    int* __restrict__ out, int64_t cap) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= q.nq) return;
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = (int)(gt / KL);
+  const int sub = (int)(gt % KL);
+  if (t >= q.nq) return;   // whole groups leave together
   const int qi = q.self_sorted ? g.sorted_idx[t] : t;
   const int s = q.qscene[qi];
   const SphGridDesc d = g.scenes[s];
@@ -91,9 +98,10 @@ __global__ __launch_bounds__(kBS) void k_neighbors(SphGrid g, SphQuery q,
   const int R = q.radius;
   const float2* __restrict__ sx = reinterpret_cast<const float2*>(g.sorted_x);
   const int* __restrict__ cs = g.cell_start + d.cell_begin;
+  const unsigned long long gmask = group_mask<KL>();
   int64_t base = 0;
   if (FILL) base = rowptr[qi];
-  int cnt = 0;
+  int cnt = 0;   // identical in every lane of the group
   for (int dy = -R; dy <= R; ++dy) {
     const int qy = iy + dy;
     if (qy < 0) continue;
@@ -105,21 +113,27 @@ __global__ __launch_bounds__(kBS) void k_neighbors(SphGrid g, SphQuery q,
     if (k0 >= ncell) break;
     if (k1 >= ncell) k1 = ncell - 1;
     const int a = cs[k0], b = cs[k1 + 1];
-    for (int j = a; j < b; ++j) {
-      const float2 xj = sx[j];
-      if (dist2_nofma(xq, yq, xj.x, xj.y) <= thr) {
-        if (FILL) {
-          const int64_t pos = base + cnt;
-          if (pos < cap) out[pos] = g.sorted_idx[j];
-        }
-        ++cnt;
+    for (int c0 = a; c0 < b; c0 += KL) {
+      const int j = c0 + sub;
+      bool hit = false;
+      if (j < b) {
+        const float2 xj = sx[j];
+        hit = dist2_nofma(xq, yq, xj.x, xj.y) <= thr;
       }
+      const unsigned long long m = __ballot(hit) & gmask;
+      if (FILL && hit) {
+        const int64_t pos = base + cnt + __popcll(m & lanes_below());
+        if (pos < cap) out[pos] = g.sorted_idx[j];
+      }
+      cnt += __popcll(m);
     }
   }
-  if (!FILL) {
-    out[qi] = cnt;
-  } else if (base + cnt > cap) {
-    atomicOr(&g.status[s], SPH_ST_OVERFLOW);
+  if (sub == 0) {
+    if (!FILL) {
+      out[qi] = cnt;
+    } else if (base + cnt > cap) {
+      atomicOr(&g.status[s], SPH_ST_OVERFLOW);
+    }
   }
 }
 
@@ -237,8 +251,11 @@ int neighbor_count(const SphGrid* g, const SphQuery* q, int* counts, hipStream_t
   if (rc) return rc;
   SPH_REQUIRE(counts, "sph_neighbor_count: null counts");
   if (q->nq == 0) return SPH_OK;
-  hipLaunchKernelGGL(k_neighbors<false>, dim3(blocks_for(q->nq, kBS)), dim3(kBS), 0, s, *g, *q,
-                     (const int*)nullptr, counts, (int64_t)0);
+  with_lanes(tuning(SPH_TUNE_SEARCH_LANES), [&](auto kl) {
+    constexpr int KL = decltype(kl)::value;
+    hipLaunchKernelGGL((k_neighbors<false, KL>), dim3(blocks_for((int64_t)q->nq * KL, kBS)),
+                       dim3(kBS), 0, s, *g, *q, (const int*)nullptr, counts, (int64_t)0);
+  });
   return launch_check("sph_neighbor_count");
 }
 
@@ -248,8 +265,11 @@ int neighbor_fill(const SphGrid* g, const SphQuery* q, int* rowptr, int* col, in
   if (rc) return rc;
   SPH_REQUIRE(rowptr && (col || cap == 0) && cap >= 0, "sph_neighbor_fill: bad output");
   if (q->nq == 0) return SPH_OK;
-  hipLaunchKernelGGL(k_neighbors<true>, dim3(blocks_for(q->nq, kBS)), dim3(kBS), 0, s, *g, *q,
-                     rowptr, col, cap);
+  with_lanes(tuning(SPH_TUNE_SEARCH_LANES), [&](auto kl) {
+    constexpr int KL = decltype(kl)::value;
+    hipLaunchKernelGGL((k_neighbors<true, KL>), dim3(blocks_for((int64_t)q->nq * KL, kBS)),
+                       dim3(kBS), 0, s, *g, *q, rowptr, col, cap);
+  });
   hipLaunchKernelGGL(k_clamp_rowptr, dim3(blocks_for(q->nq + 1, kBS)), dim3(kBS), 0, s, rowptr,
                      q->nq + 1, cap);
   return launch_check("sph_neighbor_fill");

@@ -4,6 +4,8 @@
 
 #include <stdio.h>
 
+#include <atomic>
+
 namespace sph {
 
 static thread_local std::string g_last_error;
@@ -21,6 +23,24 @@ int hip_fail(const char* where) {
 int launch_check(const char* where) {
   hipError_t e = hipPeekAtLastError();
   if (e != hipSuccess) return hip_fail(where);
+  return SPH_OK;
+}
+
+// Lanes per row: search, gathers, DFSPH solve.
+static std::atomic<int> g_tuning[SPH_TUNE_COUNT] = {{16}, {8}, {8}};
+
+int tuning(int knob) { return g_tuning[knob].load(std::memory_order_relaxed); }
+
+static int set_tuning(int knob, int value) {
+  if (knob < 0 || knob >= SPH_TUNE_COUNT) {
+    set_error("sph_set_tuning: unknown knob");
+    return SPH_E_ARG;
+  }
+  if (value < 1 || value > 32 || (value & (value - 1))) {
+    set_error("sph_set_tuning: lanes per row must be 1, 2, 4, 8, 16 or 32");
+    return SPH_E_ARG;
+  }
+  g_tuning[knob].store(value, std::memory_order_relaxed);
   return SPH_OK;
 }
 
@@ -138,5 +158,13 @@ int sph_scan_exclusive(const int* in, int n, int* out, void* ws, size_t ws_bytes
 int sph_fill_i32(int* p, int n, int value, hipStream_t s) { return sph::fill_i32(p, n, value, s); }
 int sph_fill_f32(float* p, int n, float value, hipStream_t s) {
   return sph::fill_f32(p, n, value, s);
+}
+int sph_set_tuning(int knob, int value) { return sph::set_tuning(knob, value); }
+int sph_get_tuning(int knob) {
+  if (knob < 0 || knob >= SPH_TUNE_COUNT) {
+    sph::set_error("sph_get_tuning: unknown knob");
+    return -SPH_E_ARG;
+  }
+  return sph::tuning(knob);
 }
 }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <type_traits>
 #include "sph_mi355x.h"
 
 namespace sph {
@@ -14,6 +15,23 @@ int hip_fail(const char* where);
 int launch_check(const char* where);
 
 constexpr int kWave = 64;
+
+// Current value of a launch-shape knob (SPH_TUNE_*).
+int tuning(int knob);
+
+// Calls f(std::integral_constant<int, KL>) for the lanes-per-row value kl
+// (a power of two from 1 to 32; the setter rejects anything else).
+template <typename F>
+inline void with_lanes(int kl, F&& f) {
+  switch (kl) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    default: f(std::integral_constant<int, 8>{}); break;
+  }
+}
 
 inline int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
@@ -161,6 +179,34 @@ __device__ __forceinline__ float eos(const SphPhysics& P, float rhohat) {
   }
   if (P.eos == 1) return P.c0 * P.c0 * (rhohat * P.rho0 - P.rho0);
   return P.kappa * (rhohat * P.rho0 - P.rho0);
+}
+
+// ------------------------------------------------------ lane groups
+// KL consecutive lanes of a wavefront cooperate on one row: lane `sub`
+// takes pairs sub, sub + KL, ...; the group's partial sums are combined with
+// xor shuffles that stay inside the aligned group.
+template <int KL>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int o = KL / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+template <int KL>
+__device__ __forceinline__ int group_sum_i(int v) {
+#pragma unroll
+  for (int o = KL / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+// mask of the lanes of this lane's group
+template <int KL>
+__device__ __forceinline__ unsigned long long group_mask() {
+  const int lane = threadIdx.x & (kWave - 1);
+  const unsigned long long ones = (KL == 64) ? ~0ull : ((1ull << KL) - 1ull);
+  return ones << (lane & ~(KL - 1));
+}
+__device__ __forceinline__ unsigned long long lanes_below() {
+  const int lane = threadIdx.x & (kWave - 1);
+  return (1ull << lane) - 1ull;
 }
 
 // --------------------------------------------------------------- wave scans

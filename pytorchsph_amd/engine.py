@@ -160,18 +160,17 @@ class SceneBatch:
             vpred=z(NP, 2), aact=z(NP), alpha=z(NP), src=z(NP), p=z(NP), p2=z(NP), apred=z(NP, 2),
             resid=z(NP), xrec=z(NP, 4), arec=z(NP, 4), b_aact=z(nb1), b_alpha=z(nb1),
             b_src=z(nb1), b_p=z(nb1), b_resid=z(nb1), bxrec=z(nb1, 4),
-            err_acc=z(S, dt=torch.float64), err_hist=z(S, self.max_hist, dt=torch.float64),
+            err_acc=z(L.DFSPH_ERR_SLOTS, S, dt=torch.float64), err_hist=z(S, self.max_hist, dt=torch.float64),
             iters=z(S, dt=torch.int32), active=z(S, dt=torch.int32),
             any_active=z(1, dt=torch.int32), status=z(S, dt=torch.int32),
             nfluid=ints(self.n), nbdy=ints(self.nb), fbegin=ints(self.p_begin),
             bbegin=ints(self.b_begin), act_scene=z(S, dt=torch.int32),
             act_pre=z(2 * (S + 1), dt=torch.int32), act_n=z(3, dt=torch.int32))
         self.any_active_host = torch.zeros(1, dtype=torch.int32, pin_memory=True)
-        self.dfsph_grid = max(1, min(-(-(NP + NB) // 256), 4096))
 
     def _dfsph_struct(self) -> L.SphDfsph:
         return L.SphDfsph(*[L.ptr(self.df[k]) for k in L.DFSPH_FIELDS], self.nscenes,
-                          self.max_hist, self.dfsph_grid)
+                          self.max_hist, L.dfsph_grid_blocks(self.NP + self.NB))
 
     def fluid(self, x, v, rho, vol=None) -> L.SphFluid:
         return L.SphFluid(L.ptr(x), L.ptr(v), L.ptr(rho), L.ptr(self.area), L.ptr(self.rest),

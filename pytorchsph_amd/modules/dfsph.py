@@ -64,7 +64,7 @@ class dfsphModule(Module):
         b = dict(vpred=z(n, 2), aact=z(n), alpha=z(n), src=z(n), p=z(n), p2=z(n), apred=z(n, 2),
                  resid=z(n), xrec=z(n, 4), arec=z(n, 4), b_aact=z(nb1), b_alpha=z(nb1),
                  b_src=z(nb1), b_p=z(nb1), b_resid=z(nb1), bxrec=z(nb1, 4),
-                 err_acc=z(1, dt=torch.float64), err_hist=z(1, 300, dt=torch.float64),
+                 err_acc=z(L.DFSPH_ERR_SLOTS, 1, dt=torch.float64), err_hist=z(1, 300, dt=torch.float64),
                  iters=z(1, dt=torch.int32), active=z(1, dt=torch.int32),
                  any_active=z(1, dt=torch.int32), status=z(1, dt=torch.int32),
                  nfluid=ints([n]), nbdy=ints([nb]), fbegin=ints([0]), bbegin=ints([0]),
@@ -81,7 +81,7 @@ class dfsphModule(Module):
         nb = bm.numPtcls if getattr(bm, "active", False) else 0
         buf = self._buffers(n, nb)
         D = L.SphDfsph(*[L.ptr(buf[k]) for k in L.DFSPH_FIELDS], 1, 300,
-                       max(1, min(-(-(n + nb) // 256), 4096)))
+                       L.dfsph_grid_blocks(n + nb))
         F = fluid(simulation, simulationState)
         ns = simulation.neighborSearch
         ff = csr_c(ns.csr.rowptr, ns.csr.col, n)

@@ -59,3 +59,21 @@ def rel_err(a, b):
 
 def grid_of(scene):
     return grid_params(scene.vmin, scene.vmax, scene.support)
+
+
+class tuned:
+    """Sets lanes-per-row knobs for the duration of a with-block."""
+
+    def __init__(self, **lanes):
+        self.lanes = lanes
+        self.old = None
+
+    def __enter__(self):
+        from pytorchsph_amd import _lib as L
+        self.old = L.set_tuning(**self.lanes)
+        return self
+
+    def __exit__(self, *exc):
+        from pytorchsph_amd import _lib as L
+        L.set_tuning(**self.old)
+        return False

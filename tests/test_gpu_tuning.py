@@ -1,0 +1,91 @@
+"""Lanes-per-row knobs: the pair lists are bit-identical for every knob value
+(including the order inside each row), and the gathers and the DFSPH solve
+stay within the oracle tolerances for every value."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from pytorchsph_amd import _lib as L
+from pytorchsph_amd import scenes as S
+from pytorchsph_amd.engine import SceneBatch
+from tests.gpu_helpers import assert_same_pairs, gpu_search, rel_err, tuned
+from tests.test_gpu_physics import perturbed_dsph_scene
+from tests.test_gpu_search import oracle_self
+
+pytestmark = pytest.mark.gpu
+
+LANES = [1, 2, 4, 8, 16, 32]
+
+
+def test_knob_validation(cuda):
+    old = L.get_tuning()
+    for bad in (0, 3, 64, -1):
+        with pytest.raises(RuntimeError):
+            L.set_tuning(search=bad)
+    assert L.get_tuning() == old
+
+
+def test_search_rows_identical_across_lanes(cuda):
+    x = S.point_cloud(50_000, seed=11)
+    hmax = S.support_for_mean_neighbors(x, 20.0, sample=5000)
+    h = np.full(len(x), hmax, np.float32)
+    ref = None
+    for kl in LANES:
+        with tuned(search=kl):
+            rp, c, st, _ = gpu_search(x, h, (0, 0), (1, 1), hmax)
+        assert st == 0
+        if ref is None:
+            ref = (rp, c)
+            assert_same_pairs(rp, c, oracle_self(x, h, (0, 0), (1, 1), hmax))
+        else:
+            assert np.array_equal(rp, ref[0]) and np.array_equal(c, ref[1]), f"lanes {kl}"
+
+
+@pytest.mark.parametrize("kl", [1, 8, 32])
+def test_dsph_rhs_lanes(cuda, kl):
+    sc = perturbed_dsph_scene(dx=0.01, seed=2)
+    rng = np.random.default_rng(5)
+    rho = (1.0 + rng.uniform(-2e-3, 4e-3, sc.n)).astype(np.float32)
+    with tuned(gather=kl, search=kl):
+        b = SceneBatch([sc])
+        b.rho.copy_(torch.from_numpy(rho))
+        b.dsph_rhs(b.x, b.v, b.rho)
+        torch.cuda.synchronize()
+    so = O.SceneOracle(sc)
+    ff, _, _, fb = so.pairs(sc.x)
+    f = dict(x=sc.x, v=sc.v, rho=rho, area=so.area, rest=so.rest)
+    bd = dict(x=sc.bx, rho=b.brho[: sc.nb].cpu().numpy(), vol=sc.bvol,
+              p=b.bp[: sc.nb].cpu().numpy())
+    dpdt, acc, _ = O.dsph_rhs(f, sc.phys, ff, bd, fb, O.dsph_terms(sc.phys))
+    assert rel_err(b.dpdt.cpu().numpy(), dpdt) < 2e-4
+    assert rel_err(b.acc.cpu().numpy(), acc) < 2e-4
+
+
+@pytest.mark.parametrize("kl", [1, 4, 16, 32])
+def test_dfsph_lanes(cuda, kl):
+    sc = S.dam_break(dx=0.01, block=(1.0, 0.6), scheme="dfsph", seed=4)
+    rng = np.random.default_rng(4)
+    sc.v = rng.normal(0, 0.5, sc.v.shape).astype(np.float32)
+    with tuned(solve=kl, gather=kl):
+        b = SceneBatch([sc])
+        b._dfsph_pre()
+        b._dfsph_iters(5)
+        torch.cuda.synchronize()
+    so = O.SceneOracle(sc)
+    ff, bf, _, fb = so.pairs(sc.x)
+    rho = b.rho.cpu().numpy().astype(np.float64)
+    brho = b.brho[: sc.nb].cpu().numpy().astype(np.float64)
+    f = dict(x=sc.x, v=sc.v, rho=rho, area=so.area, rest=so.rest)
+    bd = dict(x=sc.bx, vol=sc.bvol, rest=so.brest, rho=brho)
+    acc = np.tile([0.0, -S.G], (sc.n, 1))
+    out = O.dfsph(f, sc.phys, ff, bd, fb, bf, acc, fixed_iters=5)
+    d = b.df
+    rho_o, _ = O.fluid_density(f, sc.phys, ff, bd, fb)
+    assert np.max(np.abs(b.rho.cpu().numpy() / rho_o - 1)) < 1e-5
+    assert rel_err(d["alpha"].cpu().numpy(), out["alpha"]) < 1e-4
+    assert rel_err(d["src"].cpu().numpy(), out["src"]) < 1e-4
+    assert rel_err(d["b_alpha"][: sc.nb].cpu().numpy(), out["b_alpha"]) < 1e-4
+    hist = d["err_hist"][0, :5].cpu().numpy()
+    assert np.allclose(hist, out["err_hist"][:5], rtol=1e-3, atol=1e-7)
+    assert rel_err(d["p2"].cpu().numpy(), out["p2"]) < 1e-3
