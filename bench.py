@@ -37,6 +37,9 @@ def parse():
     ap.add_argument("--serial", action="store_true",
                     help="run the delta-SPH and DFSPH parts one after the other (profiling)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--tuning", default=None,
+                    help="fixed launch shapes instead of autotuning: "
+                         "search,gather,solve lanes per row and the DFSPH grid cap, e.g. 4,4,2,16384")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the kernel timings and the copy calibration (counter runs)")
     return ap.parse_args()
@@ -218,7 +221,13 @@ def main():
         f"{time.perf_counter() - t0:.1f}s")
     batch = MixedBatch(scenes)
     tuned = None
-    if not args.no_tune:
+    if args.tuning:
+        sv, gv, vv, cap = (int(t) for t in args.tuning.split(","))
+        L.set_tuning(search=sv, gather=gv, solve=vv)
+        L.DFSPH_GRID_CAP = cap
+        tuned = {"choice": dict(L.get_tuning(), grid_cap=cap)}
+        args.no_tune = True
+    elif not args.no_tune:
         t0 = time.perf_counter()
         tuned = tune(batch.parts, log=log)
         log(f"autotuned in {time.perf_counter() - t0:.1f}s: {tuned['choice']}")

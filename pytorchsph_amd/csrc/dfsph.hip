@@ -138,7 +138,7 @@ template <int KL>
 __global__ __launch_bounds__(kBS) void k_alpha_source(SphFluid f, const SphPhysics* __restrict__ phys,
                                                       SphCSR ff, SphBoundary b, SphCSR fb, SphCSR bf,
                                                       SphDfsph d) {
-  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gt = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   const int t = (int)(gt / KL), sub = (int)(gt % KL);
   if (t < f.np) {
     const int i = t;
@@ -240,7 +240,7 @@ __device__ __forceinline__ float2 pressure_accel(const SphCSR& ff, const SphCSR&
 // the items of successive grid passes; group-uniform.
 template <int KL>
 __device__ __forceinline__ int item0() {
-  return (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / KL);
+  return (int)(((int64_t)xcd_block() * blockDim.x + threadIdx.x) / KL);
 }
 template <int KL>
 __device__ __forceinline__ int item_stride() {
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(kBS) void k_accel(SphFluid f, const SphPhysics* __r
   const int nitems = d.act_n[0];
   const int sub = threadIdx.x % KL;
   const int per_block = blockDim.x / KL;
-  for (int base = blockIdx.x * per_block; base < nitems; base += gridDim.x * per_block) {
+  for (int base = xcd_block() * per_block; base < nitems; base += gridDim.x * per_block) {
     const int t = base + threadIdx.x / KL;
     const int r0 = find_rank(d.act_pre, nact, base);
     if (t >= nitems) break;
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(kBS) void k_pressure(SphFluid f, const SphPhysics* 
   double* acc_slot = d.err_acc + (size_t)(blockIdx.x % SPH_DFSPH_ERR_SLOTS) * d.nscenes;
   // every lane runs the same number of loop trips so the wavefront-wide
   // reductions inside see all lanes
-  for (int base = blockIdx.x * per_block; base < nitems; base += stride) {
+  for (int base = xcd_block() * per_block; base < nitems; base += stride) {
     const int t = base + threadIdx.x / KL;
     double contrib = 0.0;
     int s = -1;

@@ -35,7 +35,7 @@ struct RowLane {
 };
 template <int KL>
 __device__ __forceinline__ RowLane<KL> row_lane() {
-  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gt = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   return {(int)(gt / KL), (int)(gt % KL)};
 }
 

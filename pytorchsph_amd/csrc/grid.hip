@@ -83,7 +83,7 @@ template <bool FILL, int KL>
 __global__ __launch_bounds__(kBS) void k_neighbors(SphGrid g, SphQuery q,
                                                    const int* __restrict__ rowptr,
                                                    int* __restrict__ out, int64_t cap) {
-  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gt = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   const int t = (int)(gt / KL);
   const int sub = (int)(gt % KL);
   if (t >= q.nq) return;   // whole groups leave together

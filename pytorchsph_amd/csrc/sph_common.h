@@ -181,6 +181,20 @@ __device__ __forceinline__ float eos(const SphPhysics& P, float rhohat) {
   return P.kappa * (rhohat * P.rho0 - P.rho0);
 }
 
+// ---------------------------------------------------- XCD-aware blocks
+// The dispatcher hands consecutive workgroups to the 8 XCDs in turn, and
+// each XCD has its own L2.  Rows in cell order share neighbours, so the
+// logical block index is remapped to give every XCD one contiguous share of
+// the rows: hardware block b runs on XCD b % 8 and becomes logical block
+// (XCD's first share block) + b / 8.  The map is a bijection on
+// [0, gridDim.x) for any grid size.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int x = b % kXcds, q = nb / kXcds, r = nb % kXcds;
+  return x * q + (x < r ? x : r) + b / kXcds;
+}
+
 // ------------------------------------------------------ lane groups
 // KL consecutive lanes of a wavefront cooperate on one row: lane `sub`
 // takes pairs sub, sub + KL, ...; the group's partial sums are combined with
