@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--serial", action="store_true",
                     help="run the delta-SPH and DFSPH parts one after the other (profiling)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="arrival window of the serving run (0 = skip)")
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shapes instead of autotuning: "
                          "search,gather,solve lanes per row and the DFSPH grid cap, e.g. 4,4,2,16384")
@@ -174,6 +176,28 @@ def traffic_from_profile(kernel: str):
     return None
 
 
+def serving_run(window_s: float):
+    """Poisson arrivals (lambda 20 / s, sizes log-uniform 2k-500k, 100-2,000
+    steps, schemes 50/50, default_rng(4)) over `window_s` seconds into the
+    continuous-batching server; the loop stops at the end of the window and
+    reports what it completed and what is still running or waiting."""
+    from pytorchsph_amd import scenes as S
+    from pytorchsph_amd import workloads as W
+    from pytorchsph_amd.serving import Job, Server
+
+    t0 = time.perf_counter()
+    specs = W.c5_jobs(window_s=window_s)
+    jobs = [Job(id=k, scene=S.scene_for_count(j["kind"], j["scheme"], j["n"], j["seed"]),
+                steps=j["steps"], t_arrive=j["t"]) for k, j in enumerate(specs)]
+    log(f"serving: {len(jobs)} jobs generated in {time.perf_counter() - t0:.1f}s")
+    srv = Server(log=log)
+    out = srv.run(jobs, time_limit=window_s)
+    out["offered_particle_steps_per_s"] = sum(j.n * j.steps for j in jobs) / window_s
+    out["workload"] = (f"Poisson lambda=20/s over {window_s:g} s, 2k-500k particles, "
+                       f"100-2000 steps, delta-SPH/DFSPH 50/50 (seed 4)")
+    return out
+
+
 def cpu_baseline(seconds: float):
     """The oracle's delta-SPH RK4 step (fp32 neighbour search, FP64 physics,
     OpenMP) on the 5,000-particle dam break, on this host's cores."""
@@ -296,6 +320,11 @@ def main():
         L.DFSPH_GRID_CAP = batch_cap
         log(f"plain one-scene-at-a-time: {plain:.3e} particle-steps/s over {len(pick)} scenes")
 
+    serve = None
+    if args.serve_seconds > 0:
+        serve = serving_run(args.serve_seconds)
+        log(f"serving: {serve}")
+
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     peak = 8000.0
     out = {
@@ -332,6 +361,7 @@ def main():
                                  "avg_launch_us": rl_g["seconds"] * 1e6,
                                  "bytes_per_launch": rl_g["bytes"], "pairs": rl_g["pairs"]},
         "tuning": tuned["choice"] if tuned else None,
+        "serving": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

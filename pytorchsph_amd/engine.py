@@ -65,7 +65,13 @@ class SceneBatch:
     """Many scenes of one scheme ("dsph" or "dfsph"), resident on one GPU."""
 
     def __init__(self, scenes: list[Scene], device="cuda", pair_cap: float = 40.0,
-                 bdy_pair_cap: float = 40.0, dfsph_chunk: int = 8, reorder: bool = True):
+                 bdy_pair_cap: float = 40.0, dfsph_chunk: int = 8, reorder: bool = True,
+                 carry: list | None = None):
+        """carry[k], when given, continues scene k from another batch: a dict
+        from `export` with device tensors of its positions, velocities and
+        normalised density, its particle ids and the cell order of that
+        batch's last search.  The scene then advances exactly as it would
+        have in the batch it came from (scene k's own x / v are not used)."""
         assert scenes, "empty batch"
         scheme = scenes[0].scheme
         assert all(s.scheme == scheme for s in scenes), "one scheme per batch"
@@ -90,9 +96,20 @@ class SceneBatch:
                 return torch.zeros(1, dtype=f32, device=dev)
             return torch.from_numpy(np.ascontiguousarray(np.concatenate(arrs))).to(dev)
 
+        carry = carry if carry is not None else [None] * self.nscenes
+        assert len(carry) == self.nscenes
+
+        def state(key):
+            if all(c is None for c in carry):
+                return cat([getattr(s, key) for s in scenes]).float().contiguous()
+            parts = [c[key].to(dev) if c is not None
+                     else torch.from_numpy(np.ascontiguousarray(getattr(s, key), np.float32)).to(dev)
+                     for s, c in zip(scenes, carry)]
+            return torch.cat(parts).contiguous()
+
         # ---- fluid state (perennial)
-        self.x = cat([s.x for s in scenes]).float().contiguous()
-        self.v = cat([s.v for s in scenes]).float().contiguous()
+        self.x = state("x")
+        self.v = state("v")
         self.rho = torch.ones(NP, dtype=f32, device=dev)
         self.area = cat([np.full(s.n, s.area, np.float32) for s in scenes])
         self.rest = cat([np.full(s.n, s.rho0, np.float32) for s in scenes])
@@ -100,6 +117,7 @@ class SceneBatch:
         self.pscene = torch.from_numpy(np.repeat(np.arange(self.nscenes, dtype=np.int32),
                                                  self.n)).to(dev)
         self.uid = torch.arange(NP, dtype=torch.int64, device=dev)
+        self.carry = carry
         self.tmp = torch.empty(max(2 * NP, 1), dtype=f32, device=dev)
         # ---- boundary state (static)
         self.bx = cat([s.bx for s in scenes]).float().contiguous()
@@ -148,6 +166,67 @@ class SceneBatch:
             self._alloc_dfsph()
         self.steps_done = 0
         self.graph = None
+        if any(c is not None for c in self.carry):
+            self._apply_carry()
+
+    def _apply_carry(self):
+        """Density, particle ids and pending cell order of carried scenes;
+        fresh scenes get the identity order, so the first reorder leaves
+        them as they are."""
+        order = torch.arange(self.NP, dtype=torch.int32, device=self.dev)
+        for k, c in enumerate(self.carry):
+            if c is None:
+                continue
+            a, n = int(self.p_begin[k]), int(self.n[k])
+            self.rho[a:a + n].copy_(c["rho"])
+            self.uid[a:a + n].copy_(c["uid"] + a)
+            order[a:a + n].copy_(c["order"] + a)
+        self.grid.sorted_idx[: self.NP].copy_(order)
+        self._sorted_valid = True
+
+    @staticmethod
+    def footprint(scenes: list[Scene], **kw) -> int:
+        """Device bytes a batch of these scenes allocates (built on the meta
+        device: no memory is touched)."""
+        b = SceneBatch(scenes, device="meta", **kw)
+        seen, total = set(), 0
+
+        def walk(o, depth):
+            nonlocal total
+            if isinstance(o, torch.Tensor):
+                if id(o) not in seen:
+                    seen.add(id(o))
+                    total += o.numel() * o.element_size()
+            elif isinstance(o, dict):
+                for v in o.values():
+                    walk(v, depth + 1)
+            elif isinstance(o, (list, tuple)):
+                for v in o:
+                    walk(v, depth + 1)
+            elif depth < 3 and hasattr(o, "__dict__") and not isinstance(o, type):
+                for v in vars(o).values():
+                    walk(v, depth + 1)
+
+        walk(b, 0)
+        return total
+
+    def export(self, k: int) -> tuple[Scene, dict]:
+        """Scene k with the carry record (device copies of its state in the
+        current particle order) that lets another batch continue it
+        exactly."""
+        a, n = int(self.p_begin[k]), int(self.n[k])
+        order = (self.grid.sorted_idx[a:a + n] - a if self._sorted_valid
+                 else torch.arange(n, dtype=torch.int32, device=self.dev))
+        carry = dict(x=self.x[a:a + n].clone(), v=self.v[a:a + n].clone(),
+                     rho=self.rho[a:a + n].clone(), uid=self.uid[a:a + n] - a,
+                     order=order.clone(), status=self.scene_status(k))
+        return self.scenes[k], carry
+
+    def scene_status(self, k: int) -> int:
+        st = int(self.grid.status[k].item())
+        if self.scheme == "dfsph":
+            st |= int(self.df["status"][k].item())
+        return st
 
     # ----------------------------------------------------------- helpers
     def _alloc_dfsph(self):
@@ -409,14 +488,16 @@ class MixedBatch:
     SceneBatch; the two run on their own streams so their kernels overlap
     on the device (default stream priority for both)."""
 
-    def __init__(self, scenes: list[Scene], device="cuda", **kw):
+    def __init__(self, scenes: list[Scene], device="cuda", carry: list | None = None, **kw):
         self.scenes = scenes
         self.index = {}   # scene number -> (sub-batch, position)
         self.parts = []
+        carry = carry if carry is not None else [None] * len(scenes)
         for scheme in ("dsph", "dfsph"):
             ids = [k for k, s in enumerate(scenes) if s.scheme == scheme]
             if ids:
-                b = SceneBatch([scenes[k] for k in ids], device, **kw)
+                b = SceneBatch([scenes[k] for k in ids], device,
+                               carry=[carry[k] for k in ids], **kw)
                 for pos, k in enumerate(ids):
                     self.index[k] = (b, pos)
                 self.parts.append(b)
@@ -454,6 +535,10 @@ class MixedBatch:
     def scene_state(self, k):
         b, pos = self.index[k]
         return b.scene_state(pos)
+
+    def export(self, k):
+        b, pos = self.index[k]
+        return b.export(pos)
 
     def status(self):
         return np.concatenate([b.status() for b in self.parts])

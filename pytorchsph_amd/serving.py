@@ -1,0 +1,210 @@
+"""Serving: a request loop with continuous batching over the batched engine.
+
+Jobs (a scene and a number of steps) arrive over time.  The server keeps
+one running batch (a MixedBatch: delta-SPH and DFSPH parts on their own
+streams) whose particle state stays on the device between steps.  Every
+tick advances every job in the batch by one step (one graph replay per
+part).  The batch is rebuilt -- "repacked" -- when
+
+  * jobs are waiting and the HBM budget and the particle cap admit the next
+    one (at most once every `min_repack_s`), or
+  * finished jobs hold more than `waste_frac` of the batch's particles, or
+  * every job in the batch has finished.
+
+A repack exports each unfinished job's state exactly (positions,
+velocities, density, particle ids and the pending cell order, see
+SceneBatch.export) and builds the new batch from those and the newly
+admitted jobs, so a job advances exactly as it would alone: its kernels
+read only its own scene's data.  Admission is by device footprint
+(SceneBatch.footprint, the bytes a batch of the job allocates) against
+`budget_bytes` (default 85 % of the free device memory at start).
+
+A job's result is read back the tick it finishes; a finished job keeps
+being stepped (and ignored) until the next repack drops it.  Per-step
+latency is the wall time of the tick in which a job advanced.
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .engine import MixedBatch, SceneBatch
+from .scenes import Scene
+
+
+@dataclass
+class Job:
+    id: int
+    scene: Scene
+    steps: int
+    t_arrive: float = 0.0          # seconds after the start of the loop
+    bytes: int = 0
+    steps_done: int = 0
+    t_admit: float | None = None
+    t_done: float | None = None
+    result: dict | None = None     # x, v, rho in the original particle order, status
+    carry: dict | None = None      # exported state between repacks
+
+    @property
+    def n(self) -> int:
+        return self.scene.n
+
+    @property
+    def finished(self) -> bool:
+        return self.steps_done >= self.steps
+
+
+class Server:
+    def __init__(self, device="cuda", budget_bytes: int | None = None,
+                 max_particles: int = 12_000_000, min_repack_s: float = 0.5,
+                 waste_frac: float = 0.5, log=None, **batch_kw):
+        self.device = device
+        if budget_bytes is None:
+            free, _ = torch.cuda.mem_get_info()
+            budget_bytes = int(0.85 * free)
+        self.budget = int(budget_bytes)
+        self.max_particles = int(max_particles)
+        self.min_repack_s = float(min_repack_s)
+        self.waste_frac = float(waste_frac)
+        self.batch_kw = batch_kw
+        self.log = log
+        self.batch: MixedBatch | None = None
+        self.live: list[Job] = []
+        self.repacks = 0
+        self.rejected: list[Job] = []
+        self.latency: list[float] = []          # one sample per (job, step)
+        self.ticks: list[tuple[float, int]] = []  # (seconds, particles advanced)
+
+    # ------------------------------------------------------------ helpers
+    def footprint(self, job: Job) -> int:
+        if not job.bytes:
+            job.bytes = SceneBatch.footprint([job.scene], **self.batch_kw)
+        return job.bytes
+
+    def _finish(self, job: Job, k: int, now: float):
+        st = self.batch.scene_state(k)
+        b, pos = self.batch.index[k]
+        st["status"] = b.scene_status(pos)
+        job.result = st
+        job.t_done = now
+
+    def _advance(self, pos_of: dict, seconds: float, now: float):
+        """Book-keeping after one step of the whole batch."""
+        moved = 0
+        for j in self.live:
+            if j.finished:
+                continue
+            j.steps_done += 1
+            moved += j.n
+            self.latency.append(seconds)
+            if j.finished:
+                self._finish(j, pos_of[j.id], now)
+        self.ticks.append((seconds, moved))
+
+    def _repack(self, queue: deque, now: float, clock, start):
+        keep = [j for j in self.live if not j.finished]
+        for k, j in enumerate(self.live):
+            if not j.finished:
+                j.scene, j.carry = self.batch.export(k)
+        self.batch = None
+        used = sum(self.footprint(j) for j in keep)
+        parts = sum(j.n for j in keep)
+        admitted = []
+        while queue:
+            j = queue[0]
+            b = self.footprint(j)
+            if used + b > self.budget or (parts + j.n > self.max_particles and (keep or admitted)):
+                break
+            queue.popleft()
+            used += b
+            parts += j.n
+            j.t_admit = now
+            admitted.append(j)
+        if not keep and not admitted and queue:
+            # larger than the whole budget: it can never run
+            j = queue.popleft()
+            j.result = dict(rejected=True)
+            j.t_done = now
+            self.rejected.append(j)
+        self.live = keep + admitted
+        self.repacks += 1
+        if not self.live:
+            return
+        t0 = clock()
+        self.batch = MixedBatch([j.scene for j in self.live], self.device,
+                                carry=[j.carry for j in self.live], **self.batch_kw)
+        # capture() advances every scene by one eager step before recording
+        self.batch.capture()
+        torch.cuda.synchronize()
+        dt = clock() - t0
+        for j in self.live:
+            j.carry = None
+        self._advance({j.id: k for k, j in enumerate(self.live)}, dt, clock() - start)
+        if self.log:
+            self.log(f"repack {self.repacks}: {len(keep)} kept + {len(admitted)} admitted, "
+                     f"{parts} particles, {used / 2**30:.2f} GiB, {dt * 1e3:.0f} ms")
+
+    # --------------------------------------------------------------- loop
+    def run(self, jobs: list[Job], time_limit: float | None = None, drain: bool = True,
+            clock=time.perf_counter) -> dict:
+        """Serves `jobs` (arrival times relative to the start).  Stops when
+        every job has finished (drain) or at `time_limit` seconds."""
+        pending = deque(sorted(jobs, key=lambda j: j.t_arrive))
+        queue: deque = deque()
+        start = clock()
+        last_repack = -1e30
+        while True:
+            now = clock() - start
+            if time_limit is not None and now >= time_limit:
+                break
+            while pending and pending[0].t_arrive <= now:
+                queue.append(pending.popleft())
+            live_n = sum(j.n for j in self.live)
+            dead_n = sum(j.n for j in self.live if j.finished)
+            running = [j for j in self.live if not j.finished]
+            can_admit = bool(queue) and (
+                sum(self.footprint(j) for j in running) + self.footprint(queue[0]) <= self.budget
+                and sum(j.n for j in running) + queue[0].n <= self.max_particles)
+            if (self.batch is None and queue) \
+                    or (self.batch is not None and dead_n == live_n) \
+                    or (self.batch is not None and dead_n > self.waste_frac * live_n) \
+                    or (can_admit and now - last_repack >= self.min_repack_s):
+                self._repack(queue, now, clock, start)
+                last_repack = clock() - start
+                continue
+            if self.batch is None:
+                if not pending and not queue:
+                    break
+                if not drain and not pending and time_limit is None:
+                    break
+                wait = (pending[0].t_arrive - now) if pending else 0.001
+                time.sleep(max(0.0, min(wait, 0.01)))
+                continue
+            t0 = clock()
+            self.batch.step()
+            torch.cuda.synchronize()
+            self._advance({j.id: k for k, j in enumerate(self.live)}, clock() - t0,
+                          clock() - start)
+        wall = clock() - start
+        done = [j for j in jobs if j.result is not None and not j.result.get("rejected")]
+        lat = np.array(self.latency) if self.latency else np.zeros(1)
+        moved = sum(m for _, m in self.ticks)
+        return dict(
+            wall_s=wall,
+            particle_steps=moved,
+            particle_steps_per_s=moved / wall if wall > 0 else 0.0,
+            p50_step_ms=float(np.percentile(lat, 50) * 1e3),
+            p99_step_ms=float(np.percentile(lat, 99) * 1e3),
+            jobs=len(jobs), jobs_done=len(done),
+            jobs_running=sum(1 for j in self.live if not j.finished),
+            jobs_waiting=len(queue) + len(pending),
+            repacks=self.repacks,
+            jobs_rejected=len(self.rejected),
+            mean_queue_s=float(np.mean([j.t_admit - j.t_arrive for j in jobs
+                                        if j.t_admit is not None])) if any(
+                j.t_admit is not None for j in jobs) else None,
+        )

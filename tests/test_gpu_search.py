@@ -200,3 +200,27 @@ def test_c2_four_million_points(cuda):
     assert st == 0
     orc = oracle_self(x, h, (0, 0), (1, 1), hmax)
     assert_same_pairs(rp, c, orc)
+
+
+GOLD = __import__("pathlib").Path(__file__).resolve().parent / "golden" / "search_ref.npz"
+
+
+@pytest.mark.parametrize("case", ["uniform", "clustered", "variable", "neartie"])
+def test_device_matches_reference_fixture(cuda, case):
+    """Pairs equal to those of the reference's own compiled CPU search."""
+    g = np.load(GOLD)
+    x, h = g[f"{case}_x"], g[f"{case}_h"]
+    rp, c, st, _ = gpu_search(x, h, (0, 0), (1, 1), np.float32(h.max()))
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    o = np.lexsort((c, rows))
+    assert np.array_equal(rows[o], g[f"{case}_rows"]) and np.array_equal(c[o], g[f"{case}_cols"])
+
+
+def test_device_foreign_matches_reference_fixture(cuda):
+    g = np.load(GOLD)
+    x, h = g["uniform_x"], g["uniform_h"]
+    rp, c, _, _ = gpu_search(x, h, (0, 0), (1, 1), np.float32(h.max()),
+                             queries=(g["foreign_xq"], g["foreign_hq"]))
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    o = np.lexsort((c, rows))
+    assert np.array_equal(rows[o], g["foreign_rows"]) and np.array_equal(c[o], g["foreign_cols"])

@@ -1,0 +1,79 @@
+"""Serving with continuous batching: every job's result is the same alone
+and in the mix.  delta-SPH results are bit-identical (each scene's kernels
+read only that scene's data, and a repack carries the exact state and
+pending cell order over).  DFSPH results may differ only through the FP64
+residual sum, whose atomic summation order depends on the batch; the
+tolerance below covers a last-bit change of that sum (the iteration counts
+are compared exactly)."""
+import numpy as np
+import pytest
+import torch
+
+from pytorchsph_amd import scenes as S
+from pytorchsph_amd.engine import SceneBatch
+from pytorchsph_amd.serving import Job, Server
+
+pytestmark = pytest.mark.gpu
+
+
+class FakeClock:
+    """Advances 10 ms per reading, so the arrival schedule (and with it
+    which jobs share a batch at which step) does not depend on the box."""
+
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        self.t += 0.01
+        return self.t
+
+
+def make_jobs():
+    specs = [("dambreak", "dsph", 2000, 11, 6, 0.0), ("drops", "dfsph", 3000, 12, 9, 0.0),
+             ("drops", "dsph", 2500, 13, 5, 0.05), ("dambreak", "dfsph", 2000, 14, 8, 0.12),
+             ("dambreak", "dsph", 1500, 15, 7, 0.2)]
+    return [Job(id=k, scene=S.scene_for_count(kind, sch, n, sd), steps=st, t_arrive=ta)
+            for k, (kind, sch, n, sd, st, ta) in enumerate(specs)]
+
+
+def run_alone(job):
+    b = SceneBatch([job.scene])
+    b.capture()                  # one eager step
+    for _ in range(job.steps - 1):
+        b.step_graph()
+    torch.cuda.synchronize()
+    st = b.scene_state(0)
+    it = b.dfsph_iterations()[0] if b.scheme == "dfsph" else None
+    return st, it
+
+
+def test_alone_equals_in_mix_with_repacks(cuda):
+    jobs = make_jobs()
+    alone = {j.id: run_alone(j) for j in make_jobs()}
+    srv = Server(min_repack_s=0.0, waste_frac=0.2)
+    out = srv.run(jobs, clock=FakeClock())
+    assert out["jobs_done"] == len(jobs)
+    assert srv.repacks >= 3                      # jobs joined and left mid-flight
+    for j in jobs:
+        ref, _ = alone[j.id]
+        got = j.result
+        assert got["status"] & 3 == 0            # no pair overflow, nothing outside
+        if j.scene.scheme == "dsph":
+            for k in ("x", "v", "rho"):
+                assert np.array_equal(got[k], ref[k]), (j.id, k)
+        else:
+            assert np.max(np.abs(got["x"] - ref["x"])) <= 1e-6
+            scale = max(np.max(np.abs(ref["v"])), 1e-3)
+            assert np.max(np.abs(got["v"] - ref["v"])) <= 1e-4 * scale
+
+
+def test_footprint_matches_allocation(cuda):
+    sc = S.scene_for_count("dambreak", "dfsph", 20000, 3)
+    torch.cuda.synchronize()
+    before = torch.cuda.memory_allocated()
+    b = SceneBatch([sc])
+    torch.cuda.synchronize()
+    used = torch.cuda.memory_allocated() - before
+    est = SceneBatch.footprint([sc])
+    assert abs(est - used) <= 0.05 * used + (1 << 20)
+    del b

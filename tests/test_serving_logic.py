@@ -1,0 +1,115 @@
+"""Host logic of the serving loop (admission, repacking, step accounting)
+with a stand-in batch that counts steps per scene instead of simulating."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from pytorchsph_amd import serving
+from pytorchsph_amd.scenes import Scene
+
+
+def tiny_scene(n):
+    z = np.zeros((n, 2), np.float32)
+    return Scene(name="t", scheme="dsph", boundary="solid", x=z, v=z, area=1.0, support=1.0,
+                 rho0=1.0, bx=np.zeros((0, 2), np.float32), bgx=np.zeros((0, 2), np.float32),
+                 bvol=np.zeros(0, np.float32), bterm=None, vmin=(0, 0), vmax=(1, 1))
+
+
+class CountingBatch:
+    """Stands in for MixedBatch: scene k's 'state' is its step count."""
+    built = []
+
+    def __init__(self, scenes, device, carry=None, **kw):
+        carry = carry or [None] * len(scenes)
+        self.scenes = scenes
+        self.count = [c["steps"] if c else 0 for c in carry]
+        CountingBatch.built.append(sum(s.n for s in scenes))
+
+    def capture(self):
+        self.step()
+
+    def step(self):
+        self.count = [c + 1 for c in self.count]
+
+    def scene_state(self, k):
+        return dict(steps=self.count[k])
+
+    @property
+    def index(self):
+        return {k: (self, k) for k in range(len(self.scenes))}
+
+    def scene_status(self, k):
+        return 0
+
+    def export(self, k):
+        return dataclasses.replace(self.scenes[k]), dict(steps=self.count[k], status=0)
+
+
+class Clock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        self.t += 0.01
+        return self.t
+
+
+@pytest.fixture
+def fake(monkeypatch):
+    CountingBatch.built = []
+    monkeypatch.setattr(serving, "MixedBatch", CountingBatch)
+    monkeypatch.setattr(serving.torch.cuda, "synchronize", lambda *a: None)
+    return CountingBatch
+
+
+def jobs_of(specs):
+    out = []
+    for k, (n, steps, t) in enumerate(specs):
+        j = serving.Job(id=k, scene=tiny_scene(n), steps=steps, t_arrive=t)
+        j.bytes = 100 * n
+        out.append(j)
+    return out
+
+
+def test_every_job_gets_exactly_its_steps(fake):
+    jobs = jobs_of([(10, 5, 0.0), (20, 30, 0.0), (15, 3, 0.1), (5, 12, 0.3), (8, 1, 0.35)])
+    srv = serving.Server(budget_bytes=10**9, min_repack_s=0.05, waste_frac=0.5)
+    out = srv.run(jobs, clock=Clock())
+    assert out["jobs_done"] == len(jobs)
+    for j in jobs:
+        assert j.result["steps"] == j.steps, j.id
+        assert j.t_admit >= j.t_arrive
+    assert out["particle_steps"] == sum(j.n * j.steps for j in jobs)
+    assert len(srv.latency) == sum(j.steps for j in jobs)
+
+
+def test_admission_respects_budget_and_particle_cap(fake):
+    jobs = jobs_of([(10, 50, 0.0), (10, 50, 0.0), (10, 50, 0.0), (10, 5, 0.0)])
+    # budget for two jobs at a time
+    srv = serving.Server(budget_bytes=2 * 100 * 10, min_repack_s=0.0, waste_frac=0.9)
+    out = srv.run(jobs, clock=Clock())
+    assert out["jobs_done"] == 4
+    assert max(fake.built) <= 20
+    fake.built = []
+    jobs = jobs_of([(10, 20, 0.0), (10, 20, 0.0), (10, 20, 0.0)])
+    srv = serving.Server(budget_bytes=10**9, max_particles=25, min_repack_s=0.0)
+    srv.run(jobs, clock=Clock())
+    assert max(fake.built) <= 25
+    assert all(j.result["steps"] == 20 for j in jobs)
+
+
+def test_job_larger_than_budget_is_rejected(fake):
+    jobs = jobs_of([(1000, 5, 0.0), (10, 5, 0.0)])
+    srv = serving.Server(budget_bytes=100 * 500, min_repack_s=0.0)
+    out = srv.run(jobs, clock=Clock())
+    assert jobs[0].result == {"rejected": True}
+    assert jobs[1].result["steps"] == 5
+    assert out["jobs_rejected"] == 1 and out["jobs_done"] == 1
+
+
+def test_time_limit_stops_with_backlog(fake):
+    jobs = jobs_of([(10, 10**6, 0.0), (10, 10, 5.0)])
+    srv = serving.Server(budget_bytes=10**9, min_repack_s=0.0)
+    out = srv.run(jobs, time_limit=1.0, clock=Clock())
+    assert out["jobs_done"] == 0 and out["jobs_running"] == 1 and out["jobs_waiting"] == 1
