@@ -256,8 +256,6 @@ def main():
         tuned = tune(batch.parts, log=log)
         log(f"autotuned in {time.perf_counter() - t0:.1f}s: {tuned['choice']}")
     if args.roofline_only:
-        batch.step(serial=True)
-        torch.cuda.synchronize()
         rl = pressure_roofline(next(b for b in batch.parts if b.scheme == "dfsph"))
         rl_g = gather_roofline(next(b for b in batch.parts if b.scheme == "dsph"))
         copied = copy_calibration()
