@@ -85,9 +85,10 @@ ST_OUTSIDE, ST_OVERFLOW, ST_NAN, ST_ITERCAP = 1, 2, 4, 8
 T_CONT, T_CONT_B, T_MOG, T_DDSPH = 1 << 0, 1 << 1, 1 << 2, 1 << 3
 T_PRESS, T_PRESS_B, T_VISC, T_VISC_B = 1 << 4, 1 << 5, 1 << 6, 1 << 7
 T_LAM, T_GRAV, T_ACCUM = 1 << 8, 1 << 9, 1 << 16
-TUNE_SEARCH_LANES, TUNE_GATHER_LANES, TUNE_SOLVE_LANES = 0, 1, 2
+TUNE_SEARCH_LANES, TUNE_GATHER_LANES, TUNE_SOLVE_LANES, TUNE_SOLVE_TILES = 0, 1, 2, 3
 DFSPH_ERR_SLOTS = 32
-TUNE_KNOBS = {"search": TUNE_SEARCH_LANES, "gather": TUNE_GATHER_LANES, "solve": TUNE_SOLVE_LANES}
+TUNE_KNOBS = {"search": TUNE_SEARCH_LANES, "gather": TUNE_GATHER_LANES, "solve": TUNE_SOLVE_LANES,
+              "solve_tiles": TUNE_SOLVE_TILES}
 
 # exported entry points: name -> argtypes (restype is int unless noted)
 _P = C.POINTER
@@ -177,8 +178,9 @@ def call(name: str, *args) -> None:
 
 
 def set_tuning(**lanes: int) -> dict:
-    """Sets lanes-per-row knobs by name (search=, gather=, solve=); returns
-    the previous values.  Host-side only: no device is touched."""
+    """Sets launch-shape knobs by name (search=, gather=, solve= lanes per
+    row; solve_tiles= 0/1); returns the previous values.  Host-side only:
+    no device is touched."""
     old = get_tuning()
     for name, v in lanes.items():
         call("sph_set_tuning", TUNE_KNOBS[name], int(v))
@@ -197,8 +199,10 @@ def dfsph_grid_blocks(rows: int, cap: int | None = None) -> int:
     """Workgroups of the DFSPH iteration kernels: one lane group per row
     (fluid + boundary), capped at DFSPH_GRID_CAP; the kernels stride over
     larger work."""
-    kl = get_tuning()["solve"]
-    return max(1, min(-(-(rows * kl) // 256), cap or DFSPH_GRID_CAP))
+    t = get_tuning()
+    per_block = 128 if t["solve_tiles"] else 256 // t["solve"]   # rows per workgroup pass
+    # (workgroup tiles: 128 rows; wavefront tiles: 4 waves x 32 rows)
+    return max(1, min(-(-rows // per_block), cap or DFSPH_GRID_CAP))
 
 
 def ptr(t) -> int | None:

@@ -27,7 +27,7 @@ int launch_check(const char* where) {
 }
 
 // Lanes per row: search, gathers, DFSPH solve.
-static std::atomic<int> g_tuning[SPH_TUNE_COUNT] = {{16}, {8}, {8}};
+static std::atomic<int> g_tuning[SPH_TUNE_COUNT] = {{4}, {4}, {2}, {1}};
 
 int tuning(int knob) { return g_tuning[knob].load(std::memory_order_relaxed); }
 
@@ -36,7 +36,12 @@ static int set_tuning(int knob, int value) {
     set_error("sph_set_tuning: unknown knob");
     return SPH_E_ARG;
   }
-  if (value < 1 || value > 32 || (value & (value - 1))) {
+  if (knob == SPH_TUNE_SOLVE_TILES) {
+    if (value < 0 || value > 2) {
+      set_error("sph_set_tuning: SPH_TUNE_SOLVE_TILES must be 0, 1 or 2");
+      return SPH_E_ARG;
+    }
+  } else if (value < 1 || value > 32 || (value & (value - 1))) {
     set_error("sph_set_tuning: lanes per row must be 1, 2, 4, 8, 16 or 32");
     return SPH_E_ARG;
   }

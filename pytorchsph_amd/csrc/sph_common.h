@@ -223,6 +223,60 @@ __device__ __forceinline__ unsigned long long lanes_below() {
   return (1ull << lane) - 1ull;
 }
 
+// ------------------------------------------------ wave-staged column slices
+// A wavefront's lane groups hold consecutive rows, whose column indices
+// form one contiguous slice [rowptr[first], rowptr[last + 1]) of the CSR.
+// The wave copies that slice into its own LDS window with coalesced loads
+// (64 consecutive indices per instruction) and the row loops read columns
+// from LDS; a per-lane strided walk of the CSR would touch one cache line
+// per row per instruction.  Pairs beyond the window, and every pair of a
+// wave whose rows are not consecutive, are read from global memory.
+constexpr int kWaveCols = 1024;
+
+struct WaveCols {
+  int base;   // CSR position of window slot 0
+  int n;      // staged slots (0: none)
+  bool full;  // every pair of the wave's rows is staged (wave-uniform)
+};
+
+// Every lane of the wave calls this with the same control flow.  `row` is
+// the lane's row (any value when !valid), [k0, k1) its CSR range; valid
+// lanes form a prefix of the wave and lane group g holds row first + g.
+template <int KL>
+__device__ __forceinline__ WaveCols stage_wave_cols(const int* __restrict__ col, int row, int k0,
+                                                    int k1, bool valid, int* wcols) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const unsigned long long vm = __ballot(valid);
+  WaveCols w = {0, 0, false};
+  if (vm == 0) return w;
+  const int last = 63 - __clzll(vm);
+  const int first_row = __shfl(row, 0, kWave);
+  const bool in_seq = !valid || row == first_row + lane / KL;
+  if (__ballot(!in_seq) != 0) return w;
+  w.base = __shfl(k0, 0, kWave);
+  const int end = __shfl(k1, last, kWave);
+  w.n = min(end - w.base, kWaveCols);
+  w.full = w.n == end - w.base;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (int k = lane; k < w.n; k += kWave) wcols[k] = col[w.base + k];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return w;
+}
+
+// Column k from the wave's window (STAGED, the wave-uniform w.full case)
+// or from global memory.  Callers pick the variant per wave: a per-pair
+// choice between the two address spaces compiles to a flat load, whose wait
+// drains every outstanding global load and defeats load batching.
+template <bool STAGED>
+__device__ __forceinline__ int wave_col(const int* __restrict__ col, const WaveCols& w,
+                                        const int* wcols, int k) {
+  if (STAGED) return wcols[k - w.base];
+  return col[k];
+}
+
 // --------------------------------------------------------------- wave scans
 __device__ __forceinline__ int wave_inclusive_scan(int v) {
   const int lane = threadIdx.x & (kWave - 1);
