@@ -18,6 +18,10 @@ namespace sph {
 namespace {
 
 constexpr int kBS = 256;
+// neighbour loads a lane keeps in flight: a row loop takes kBatch pairs per
+// trip, issues all their loads (slots past the row re-read pair k0), then
+// adds the pairs in order -- the order of the one-pair loop
+constexpr int kBatch = 4;
 
 __device__ __forceinline__ float2 ld2(const float* p, int i) {
   return reinterpret_cast<const float2*>(p)[i];
@@ -51,12 +55,30 @@ __global__ __launch_bounds__(kBS) void k_fluid_density(SphFluid f, const SphPhys
   const KConst K = kconst(phys[f.pscene[i]].h);
   const float2 xi = ld2(f.x, i);
   float sum = 0.f;
-  const int k1 = ff.rowptr[i + 1];
-  for (int k = ff.rowptr[i] + rl.sub; k < k1; k += KL) {
-    const int j = ff.col[k];
-    const float2 xj = ld2(f.x, j);
-    const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, K.inv_h);
-    sum += f.area[j] * wendland_k(g.q, K);
+  const int k0 = ff.rowptr[i], k1 = ff.rowptr[i + 1];
+  for (int kb = k0 + rl.sub; kb < k1; kb += KL * kBatch) {
+    bool ok[kBatch];
+    int jj[kBatch];
+    float2 xj[kBatch];
+    float aj[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int k = kb + u * KL;
+      ok[u] = k < k1;
+      jj[u] = ff.col[ok[u] ? k : k0];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      xj[u] = ld2(f.x, jj[u]);
+      aj[u] = f.area[jj[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (ok[u]) {
+        const PairGeo g = pair_geo(xi.x, xi.y, xj[u].x, xj[u].y, K.inv_h);
+        sum += aj[u] * wendland_k(g.q, K);
+      }
+    }
   }
   sum = group_sum<KL>(sum);
   if (vol_out && rl.sub == 0) vol_out[i] = f.area[i] / sum;
@@ -291,13 +313,31 @@ __global__ __launch_bounds__(kBS) void k_dsph_gather_packed(
   o.gx = gri.x; o.gy = gri.y;
   Acc a = {};
   if (terms & kFluidPairTerms) {
-    const int k1 = ff.rowptr[i + 1];
-    for (int k = ff.rowptr[i] + rl.sub; k < k1; k += KL) {
-      const int j = ff.col[k];
-      const float4 r0 = frec[2 * j], r1 = frec[2 * j + 1];
-      const Nbr n = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z};
-      const float2 grj = dd ? ld2(gradrho, j) : make_float2(0.f, 0.f);
-      dsph_pair(P, K, o, n, grj.x, grj.y, true, terms, a);
+    const int k0 = ff.rowptr[i], k1 = ff.rowptr[i + 1];
+    for (int kb = k0 + rl.sub; kb < k1; kb += KL * kBatch) {
+      bool ok[kBatch];
+      int jj[kBatch];
+      float4 r0[kBatch], r1[kBatch];
+      float2 grj[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int k = kb + u * KL;
+        ok[u] = k < k1;
+        jj[u] = ff.col[ok[u] ? k : k0];
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        r0[u] = frec[2 * jj[u]];
+        r1[u] = frec[2 * jj[u] + 1];
+        grj[u] = dd ? ld2(gradrho, jj[u]) : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        if (ok[u]) {
+          const Nbr n = {r0[u].x, r0[u].y, r0[u].z, r0[u].w, r1[u].x, r1[u].y, r1[u].z};
+          dsph_pair(P, K, o, n, grj[u].x, grj[u].y, true, terms, a);
+        }
+      }
     }
   }
   if (fb.rowptr && (terms & kBdyPairTerms)) {
