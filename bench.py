@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--serial", action="store_true",
                     help="run the delta-SPH and DFSPH parts one after the other (profiling)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--no-paths", action="store_true",
+                    help="skip the single-scene workloads (C2 search, 4M delta-SPH, C3 DFSPH)")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="arrival window of the serving run (0 = skip)")
     ap.add_argument("--tuning", default=None,
@@ -175,6 +177,108 @@ def traffic_from_profile(kernel: str):
             if line.startswith("traffic_bytes_per_launch") and f"kernel={want}" in line:
                 return int(line.split("=")[1].split()[0])
     return None
+
+
+def _timed_steps(b, warmup: int, steps: int):
+    import torch
+    b.capture()
+    for _ in range(warmup):
+        b.step_graph()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.step_graph()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def plain_paths(steps: int = 3, warmup: int = 1):
+    """The single-scene workloads, each on its own:
+      search_c2     neighbour search of 4,000,000 points (half uniform, half
+                    8 Gaussian clusters, ~20 mean neighbours, default_rng(1)):
+                    grid build + count + scan + fill, timed with events
+      dsph_4m       plain delta-SPH RK4 step of a 2000 x 2000 dam break
+                    (dx = 0.0005, 4,000,000 fluid particles, mDBC walls)
+      dfsph_c3      plain DFSPH step of the 1,000,000-particle dam break
+                    (dx = 0.001, Akinci walls), solved to tau = 1e-3
+    Algorithmic bytes (compulsory, each array once; K = mean neighbours,
+    c = cells per point):
+      search        x, h read (12) + sorted x, index written (12) + key (4)
+                    + cell tables (8 c) + rowptr (4) + columns (4 K)
+      delta-SPH     per right-hand side: search + the gather's row pointers
+                    (4) + columns (4 K) + x, v, rho, A (24) + dpdt, a (12);
+                    per RK stage combine 52; four of each per step"""
+    import torch
+
+    from pytorchsph_amd import _lib as L
+    from pytorchsph_amd import scenes as S
+    from pytorchsph_amd import workloads as W
+    from pytorchsph_amd.engine import SceneBatch
+    from pytorchsph_amd.search import NeighborGrid, grid_params, scan
+
+    out = {}
+    dev = torch.device("cuda")
+    peak = 8000.0
+    # ---- C2: search alone
+    x, h = W.c2_points()
+    n = len(x)
+    g = NeighborGrid([dict(vmin=(0.0, 0.0), vmax=(1.0, 1.0), hmax=np.float32(h), p_begin=0,
+                           p_count=n)], n, dev)
+    xt = torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(dev)
+    ht = torch.full((n,), float(h), dtype=torch.float32, device=dev)
+    ps = torch.zeros(n, dtype=torch.int32, device=dev)
+    counts = torch.empty(n, dtype=torch.int32, device=dev)
+    rowptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    g.build(xt, ps)
+    q = g.query_struct(xt, ht, ps, 1, True)
+    g.count(q, counts)
+    scan(counts, n, rowptr)
+    nnz = int(rowptr[n].item())
+    col = torch.empty(nnz, dtype=torch.int32, device=dev)
+
+    def one_search():
+        g.build(xt, ps)
+        g.count(q, counts)
+        scan(counts, n, rowptr)
+        g.fill(q, rowptr, col)
+
+    sec = _events_avg(one_search, torch.cuda.current_stream(), reps=10, warm=2)
+    cells = g.ncells / n
+    k_mean = nnz / n
+    nbytes = n * (12 + 12 + 4 + 8 * cells + 4 + 4 * k_mean)
+    out["search_c2"] = dict(points=n, pairs=nnz, ms=sec * 1e3, searches_per_s=1 / sec,
+                            achieved_gbs=nbytes / sec / 1e9, frac=nbytes / sec / 1e9 / peak,
+                            bytes=int(nbytes))
+    del g, xt, ht, ps, counts, rowptr, col
+    torch.cuda.empty_cache()
+    log(f"search C2: {out['search_c2']}")
+    # ---- plain delta-SPH at 4M
+    sc = S.dam_break(dx=0.0005, block=(1.0, 1.0), scheme="dsph", seed=5)
+    b = SceneBatch([sc])
+    sec = _timed_steps(b, warmup, steps)
+    nnz = int(b.ff.rowptr[b.NP].item())
+    k_mean = nnz / b.NP
+    cells = b.grid.ncells / b.NP
+    rhs = 12 + 12 + 4 + 8 * cells + 4 + 4 * k_mean + 4 + 4 * k_mean + 24 + 12
+    nbytes = b.NP * 4 * (rhs + 52)
+    out["dsph_4m"] = dict(particles=b.NP, ms_per_step=sec * 1e3,
+                          particle_steps_per_s=b.NP / sec, achieved_gbs=nbytes / sec / 1e9,
+                          frac=nbytes / sec / 1e9 / peak, bytes_per_step=int(nbytes),
+                          mean_neighbours=k_mean)
+    del b
+    torch.cuda.empty_cache()
+    log(f"plain delta-SPH 4M: {out['dsph_4m']}")
+    # ---- C3: plain DFSPH at 1M
+    sc = W.c3_scene()
+    b = SceneBatch([sc])
+    sec = _timed_steps(b, warmup, steps)
+    out["dfsph_c3"] = dict(particles=b.NP, ms_per_step=sec * 1e3,
+                           particle_steps_per_s=b.NP / sec,
+                           iterations_last_step=int(b.dfsph_iterations()[0]))
+    del b
+    torch.cuda.empty_cache()
+    log(f"plain DFSPH C3: {out['dfsph_c3']}")
+    return out
 
 
 def serving_run(window_s: float):
@@ -321,6 +425,8 @@ def main():
         L.DFSPH_GRID_CAP = batch_cap
         log(f"plain one-scene-at-a-time: {plain:.3e} particle-steps/s over {len(pick)} scenes")
 
+    paths = None if args.no_paths else plain_paths()
+
     serve = None
     if args.serve_seconds > 0:
         serve = serving_run(args.serve_seconds)
@@ -363,6 +469,7 @@ def main():
                                  "bytes_per_launch": rl_g["bytes"], "pairs": rl_g["pairs"]},
         "tuning": tuned["choice"] if tuned else None,
         "serving": serve,
+        "plain_paths": paths,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -26,8 +26,10 @@ int launch_check(const char* where) {
   return SPH_OK;
 }
 
-// Lanes per row: search, gathers, DFSPH solve.
-static std::atomic<int> g_tuning[SPH_TUNE_COUNT] = {{4}, {4}, {2}, {1}};
+// Defaults (the shapes the autotuner picks on a 256-scene mixed batch):
+// lanes per row of the search, the gathers and the DFSPH solve; lane groups
+// (not row tiles) for the DFSPH iteration.
+static std::atomic<int> g_tuning[SPH_TUNE_COUNT] = {{2}, {4}, {2}, {0}};
 
 int tuning(int knob) { return g_tuning[knob].load(std::memory_order_relaxed); }
 

@@ -225,10 +225,12 @@ def support_for_mean_neighbors(pts: np.ndarray, target: float = 20.0, sample: in
     rng = np.random.default_rng(seed)
     tree = cKDTree(pts.astype(np.float64))
     qs = pts[rng.choice(len(pts), size=min(sample, len(pts)), replace=False)].astype(np.float64)
-    lo, hi = 1e-6, 0.1
+    # bracket around the radius of a uniform cloud of the same size
+    r0 = float(np.sqrt(target / (np.pi * len(pts))))
+    lo, hi = r0 / 8, r0 * 8
     for _ in range(40):
         mid = 0.5 * (lo + hi)
-        cnt = np.mean([len(c) for c in tree.query_ball_point(qs, mid)])
+        cnt = float(np.mean(tree.query_ball_point(qs, mid, return_length=True)))
         if cnt < target:
             lo = mid
         else:
