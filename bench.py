@@ -43,8 +43,7 @@ def parse():
                     help="arrival window of the serving run (0 = skip)")
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shapes instead of autotuning: "
-                         "search,gather,solve lanes per row, the DFSPH grid cap and optionally "
-                         "DFSPH row tiles (0/1), e.g. 4,4,2,16384,1")
+                         "search,gather,solve lanes per row and the DFSPH grid cap, e.g. 2,4,2,16384")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the kernel timings and the copy calibration (counter runs)")
     return ap.parse_args()
@@ -351,10 +350,8 @@ def main():
     batch = MixedBatch(scenes)
     tuned = None
     if args.tuning:
-        vals = [int(t) for t in args.tuning.split(",")]
-        sv, gv, vv, cap = vals[:4]
-        tiles = vals[4] if len(vals) > 4 else 0
-        L.set_tuning(search=sv, gather=gv, solve=vv, solve_tiles=tiles)
+        sv, gv, vv, cap = (int(t) for t in args.tuning.split(","))
+        L.set_tuning(search=sv, gather=gv, solve=vv)
         L.DFSPH_GRID_CAP = cap
         tuned = {"choice": dict(L.get_tuning(), grid_cap=cap)}
         args.no_tune = True

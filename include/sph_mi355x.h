@@ -194,13 +194,7 @@ int sph_fill_f32(float* p, int n, float value, hipStream_t s);
 #define SPH_TUNE_SEARCH_LANES 0   /* sph_neighbor_count / sph_neighbor_fill   */
 #define SPH_TUNE_GATHER_LANES 1   /* density and delta-SPH gathers            */
 #define SPH_TUNE_SOLVE_LANES  2   /* DFSPH alpha/source, iterate and finish   */
-/* 0, 1 or 2.  1 / 2: the DFSPH iteration kernels (accel, pressure) run
- * pair-parallel over tiles of consecutive rows, one tile per workgroup (1)
- * or per wavefront (2): consecutive lanes take consecutive pairs of the
- * tile's rows and each row's sum is taken in pair order from LDS (the order
- * of SOLVE_LANES = 1).  0: lane groups of SOLVE_LANES lanes per row. */
-#define SPH_TUNE_SOLVE_TILES  3
-#define SPH_TUNE_COUNT        4
+#define SPH_TUNE_COUNT        3
 int sph_set_tuning(int knob, int value);
 int sph_get_tuning(int knob);
 
@@ -297,6 +291,9 @@ typedef struct SphDfsph {
   float* resid;  /* [np] residual                                     */
   float* xrec;   /* [np][4] {x, y, p2 / (rho rho0)^2, A rho0}         */
   float* arec;   /* [np][4] {a*_x, a*_y, dt^2 A^act, 0}               */
+  float* prec;   /* [np][4] {x, y, w a*_x, w a*_y}, w = dt^2 A^act    */
+  float* wvec;   /* [np][2] W_i = dt^2 sum_j A^act_j gradWs_ij over fluid
+                    and boundary neighbours (fixed during the solve)  */
   float* b_aact; /* [nb]                                              */
   float* b_alpha;/* [nb]                                              */
   float* b_src;  /* [nb]                                              */

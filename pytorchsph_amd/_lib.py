@@ -55,7 +55,7 @@ class SphCSR(C.Structure):
 
 
 DFSPH_FIELDS = ("vpred", "aact", "alpha", "src", "p", "p2", "apred", "resid", "xrec", "arec",
-                "b_aact", "b_alpha", "b_src", "b_p", "b_resid", "bxrec", "err_acc", "err_hist",
+                "prec", "wvec", "b_aact", "b_alpha", "b_src", "b_p", "b_resid", "bxrec", "err_acc", "err_hist",
                 "iters", "active", "any_active", "status", "nfluid", "nbdy", "fbegin", "bbegin",
                 "act_scene", "act_pre", "act_n")
 
@@ -85,10 +85,9 @@ ST_OUTSIDE, ST_OVERFLOW, ST_NAN, ST_ITERCAP = 1, 2, 4, 8
 T_CONT, T_CONT_B, T_MOG, T_DDSPH = 1 << 0, 1 << 1, 1 << 2, 1 << 3
 T_PRESS, T_PRESS_B, T_VISC, T_VISC_B = 1 << 4, 1 << 5, 1 << 6, 1 << 7
 T_LAM, T_GRAV, T_ACCUM = 1 << 8, 1 << 9, 1 << 16
-TUNE_SEARCH_LANES, TUNE_GATHER_LANES, TUNE_SOLVE_LANES, TUNE_SOLVE_TILES = 0, 1, 2, 3
+TUNE_SEARCH_LANES, TUNE_GATHER_LANES, TUNE_SOLVE_LANES = 0, 1, 2
 DFSPH_ERR_SLOTS = 32
-TUNE_KNOBS = {"search": TUNE_SEARCH_LANES, "gather": TUNE_GATHER_LANES, "solve": TUNE_SOLVE_LANES,
-              "solve_tiles": TUNE_SOLVE_TILES}
+TUNE_KNOBS = {"search": TUNE_SEARCH_LANES, "gather": TUNE_GATHER_LANES, "solve": TUNE_SOLVE_LANES}
 
 # exported entry points: name -> argtypes (restype is int unless noted)
 _P = C.POINTER
@@ -178,9 +177,8 @@ def call(name: str, *args) -> None:
 
 
 def set_tuning(**lanes: int) -> dict:
-    """Sets launch-shape knobs by name (search=, gather=, solve= lanes per
-    row; solve_tiles= 0/1); returns the previous values.  Host-side only:
-    no device is touched."""
+    """Sets lanes-per-row knobs by name (search=, gather=, solve=); returns
+    the previous values.  Host-side only: no device is touched."""
     old = get_tuning()
     for name, v in lanes.items():
         call("sph_set_tuning", TUNE_KNOBS[name], int(v))
@@ -199,9 +197,7 @@ def dfsph_grid_blocks(rows: int, cap: int | None = None) -> int:
     """Workgroups of the DFSPH iteration kernels: one lane group per row
     (fluid + boundary), capped at DFSPH_GRID_CAP; the kernels stride over
     larger work."""
-    t = get_tuning()
-    per_block = 128 if t["solve_tiles"] else 256 // t["solve"]   # rows per workgroup pass
-    # (workgroup tiles: 128 rows; wavefront tiles: 4 waves x 32 rows)
+    per_block = 256 // get_tuning()["solve"]   # rows per workgroup pass
     return max(1, min(-(-rows // per_block), cap or DFSPH_GRID_CAP))
 
 

@@ -1,8 +1,7 @@
 """Launch-shape autotuner of the batched step.
 
 The knobs are the lanes per row of the neighbour search, of the kernel-sum
-gathers and of the DFSPH solve, whether the DFSPH iteration kernels run as
-pair-parallel workgroup tiles (sph_set_tuning), and the workgroup cap of
+gathers and of the DFSPH solve (sph_set_tuning), and the workgroup cap of
 the DFSPH iteration kernels.  Each knob is timed on the batch it will run
 (same scenes, same pair lists), one knob at a time with the others held,
 and the fastest value is kept.  Pair sets do not depend on any knob; sums
@@ -88,7 +87,6 @@ def tune(parts, reps: int = 5, log=None) -> dict:
         "solve": [_solve_fn(b) for b in dfsph],
     }
     times: dict = {}
-    L.set_tuning(solve_tiles=0)
     choice = L.get_tuning()
     for knob, fns in plans.items():
         if not fns:
@@ -101,19 +99,9 @@ def tune(parts, reps: int = 5, log=None) -> dict:
         L.set_tuning(**{knob: best})
         choice[knob] = best
         times[knob] = res
-        if knob == "solve":
-            # workgroup and wavefront row tiles against the best lane-group shape
-            opts = {0: res[best]}
-            for tv in (1, 2):
-                L.set_tuning(solve_tiles=tv)
-                opts[tv] = sum(_time(fn, reps) for fn in fns)
-                res[f"tiles{tv}"] = opts[tv]
-            use = min(opts, key=opts.get)
-            L.set_tuning(solve_tiles=use)
-            choice["solve_tiles"] = use
         if log:
             log(f"autotune {knob}: " + ", ".join(f"{k}:{t * 1e6:.0f}us" for k, t in res.items())
-                + f" -> {best}" + (f", tiles={choice['solve_tiles']}" if knob == "solve" else ""))
+                + f" -> {best}")
     if dfsph:
         fns = plans["solve"]
         res = {}

@@ -82,6 +82,7 @@ __global__ __launch_bounds__(kBS) void k_prepare(SphFluid f, const float* __rest
     const float2 x = ld2(f.x, t);
     st4(d.xrec, t, make_float4(x.x, x.y, 0.f, f.area[t] * f.rest[t]));
     st4(d.arec, t, make_float4(0.f, 0.f, P.dt * P.dt * aact, 0.f));
+    st4(d.prec, t, make_float4(x.x, x.y, 0.f, 0.f));
   } else if (t < f.np + b.nb) {
     const int k = t - f.np;
     d.b_aact[k] = b.vol[k] / b.rho[k];
@@ -184,6 +185,9 @@ __global__ __launch_bounds__(kBS) void k_alpha_source(SphFluid f, const SphPhysi
     al = al < -1.f ? -1.f : (al > -1e-7f ? -1e-7f : al);  // dfsph.py:82
     d.alpha[i] = al;
     d.src[i] = 1.f - f.rho[i] + src;
+    // W_i = dt^2 k1_i: the part of the pressure-update sum that does not
+    // change during the solve
+    st2(d.wvec, i, make_float2(P.dt * P.dt * k1x, P.dt * P.dt * k1y));
   } else if (t < f.np + b.nb) {
     const int k0 = t - f.np;
     const SphPhysics P = phys[b.bscene[k0]];
@@ -274,6 +278,15 @@ __device__ __forceinline__ int item_stride() {
   return (int)((int64_t)gridDim.x * blockDim.x / KL);
 }
 
+// a*_i and the record the pressure update gathers: {x, y, w a*} with
+// w = dt^2 A^act.
+__device__ __forceinline__ void store_accel(const SphDfsph& d, int i, float2 a, float dt2) {
+  const float w = dt2 * d.aact[i];
+  st4(d.arec, i, make_float4(a.x, a.y, w, 0.f));
+  const float2 x = ld2(d.xrec, 2 * i);
+  st4(d.prec, i, make_float4(x.x, x.y, w * a.x, w * a.y));
+}
+
 template <int KL>
 __global__ __launch_bounds__(kBS) void k_accel(SphFluid f, const SphPhysics* __restrict__ phys,
                                                SphCSR ff, SphCSR fb, SphDfsph d) {
@@ -308,12 +321,12 @@ __global__ __launch_bounds__(kBS) void k_accel(SphFluid f, const SphPhysics* __r
       const SphPhysics P = phys[d.act_scene[r0]];
       const KConst K = kconst(P.h);
       const float2 a = pressure_accel<KL>(ff, fb, d, K, i, sub, w, wcols);
-      if (sub == 0) st4(d.arec, i, make_float4(a.x, a.y, P.dt * P.dt * d.aact[i], 0.f));
+      if (sub == 0) store_accel(d, i, a, P.dt * P.dt);
     } else if (valid) {
       const SphPhysics P = phys[s];
       const KConst K = kconst(P.h);
       const float2 a = pressure_accel<KL>(ff, fb, d, K, i, sub, w, wcols);
-      if (sub == 0) st4(d.arec, i, make_float4(a.x, a.y, P.dt * P.dt * d.aact[i], 0.f));
+      if (sub == 0) store_accel(d, i, a, P.dt * P.dt);
     }
   }
 }
@@ -334,24 +347,26 @@ __device__ __forceinline__ void scene_sum(double v, int s, bool valid, double* a
   }
 }
 
-// The fluid-row part of k_pressure: r_i = sum_j w_j (a*_i - a*_j) . G_ij
-// (+ boundary neighbours) - s_i, relaxed update of p2 and the clamped
-// residual.  Returns the residual contribution for the scene sum.
+// The fluid-row part of k_pressure (dfsph.py:44-57,164-184 with the Akinci
+// terms of akinciBoundary.py:280-315):
+//   r_i = sum_j w_j (a*_i - a*_j) . G_ij + sum_b w_b a*_i . G_ib - s_i
+//       = a*_i . W_i - sum_j (w_j a*_j) . G_ij - s_i,
+// W_i = sum_j w_j G_ij + sum_b w_b G_ib being fixed during the solve (it is
+// dt^2 k1_i of the alpha term).  One 16-byte record {x_j, y_j, w_j a*_j}
+// per pair; boundary neighbours only enter through W_i.  Then the relaxed
+// update of p2 and the clamped residual for the scene sum.
 template <int KL, bool STAGED>
 __device__ __forceinline__ double pressure_fluid_row(const SphFluid& f, const SphCSR& ff,
-                                                     const SphCSR& fb, const SphDfsph& d,
-                                                     const KConst& K, float dt2, float omega,
-                                                     float tau, int s, int i, int k0, int k1,
-                                                     int sub, const WaveCols& w, const int* wcols,
-                                                     bool* wrote) {
+                                                     const SphDfsph& d, const KConst& K,
+                                                     float omega, float tau, int s, int i, int k0,
+                                                     int k1, int sub, const WaveCols& w,
+                                                     const int* wcols, bool* wrote) {
   const float2 xi = ld2(d.xrec, 2 * i);          // {x, y} of xrec[i]
-  const float4 ai = ld4(d.arec, i);
   float ks = 0.f;
   for (int kb = k0 + sub; kb < k1; kb += KL * kPairBatch) {
     bool ok[kPairBatch];
     int jj[kPairBatch];
-    float2 xj[kPairBatch];
-    float4 aj[kPairBatch];
+    float4 pj[kPairBatch];
 #pragma unroll
     for (int u = 0; u < kPairBatch; ++u) {
       const int k = kb + u * KL;
@@ -359,31 +374,21 @@ __device__ __forceinline__ double pressure_fluid_row(const SphFluid& f, const Sp
       jj[u] = wave_col<STAGED>(ff.col, w, wcols, ok[u] ? k : k0);
     }
 #pragma unroll
-    for (int u = 0; u < kPairBatch; ++u) {
-      xj[u] = ld2(d.xrec, 2 * jj[u]);
-      aj[u] = ld4(d.arec, jj[u]);
-    }
+    for (int u = 0; u < kPairBatch; ++u) pj[u] = ld4(d.prec, jj[u]);
 #pragma unroll
     for (int u = 0; u < kPairBatch; ++u) {
       if (ok[u]) {
-        const float2 G = spiky_G(xi.x, xi.y, xj[u].x, xj[u].y, K);
-        ks += aj[u].z * ((ai.x - aj[u].x) * G.x + (ai.y - aj[u].y) * G.y);
+        const float2 G = spiky_G(xi.x, xi.y, pj[u].x, pj[u].y, K);
+        ks += pj[u].z * G.x + pj[u].w * G.y;
       }
-    }
-  }
-  if (fb.rowptr) {
-    const int e1 = fb.rowptr[i + 1];
-    for (int k = fb.rowptr[i] + sub; k < e1; k += KL) {
-      const int j = fb.col[k];
-      const float2 xb = ld2(d.bxrec, 2 * j);
-      const float2 G = spiky_G(xi.x, xi.y, xb.x, xb.y, K);
-      ks += dt2 * d.b_aact[j] * (ai.x * G.x + ai.y * G.y);
     }
   }
   ks = group_sum<KL>(ks);
   *wrote = false;
   if (sub != 0) return 0.0;
-  const float rr = ks - d.src[i];
+  const float2 ai = ld2(d.arec, 2 * i);
+  const float2 wi = ld2(d.wvec, i);
+  const float rr = (ai.x * wi.x + ai.y * wi.y) - ks - d.src[i];
   const float pold = d.p2[i];
   d.p[i] = pold;
   float pn = pold - omega * rr / d.alpha[i];
@@ -440,17 +445,17 @@ __global__ __launch_bounds__(kBS) void k_pressure(SphFluid f, const SphPhysics* 
     if (uni) {
       const SphPhysics P = phys[d.act_scene[r0f]];
       const KConst K = kconst(P.h);
-      contrib = w.full ? pressure_fluid_row<KL, true>(f, ff, fb, d, K, P.dt * P.dt, P.fluid_omega,
-                                                      P.tau, s, i, k0, k1, sub, w, wcols, &valid)
-                       : pressure_fluid_row<KL, false>(f, ff, fb, d, K, P.dt * P.dt, P.fluid_omega,
-                                                       P.tau, s, i, k0, k1, sub, w, wcols, &valid);
+      contrib = w.full ? pressure_fluid_row<KL, true>(f, ff, d, K, P.fluid_omega, P.tau, s, i, k0,
+                                                      k1, sub, w, wcols, &valid)
+                       : pressure_fluid_row<KL, false>(f, ff, d, K, P.fluid_omega, P.tau, s, i, k0,
+                                                       k1, sub, w, wcols, &valid);
     } else if (t < nf) {
       const SphPhysics P = phys[s];
       const KConst K = kconst(P.h);
-      contrib = w.full ? pressure_fluid_row<KL, true>(f, ff, fb, d, K, P.dt * P.dt, P.fluid_omega,
-                                                      P.tau, s, i, k0, k1, sub, w, wcols, &valid)
-                       : pressure_fluid_row<KL, false>(f, ff, fb, d, K, P.dt * P.dt, P.fluid_omega,
-                                                       P.tau, s, i, k0, k1, sub, w, wcols, &valid);
+      contrib = w.full ? pressure_fluid_row<KL, true>(f, ff, d, K, P.fluid_omega, P.tau, s, i, k0,
+                                                      k1, sub, w, wcols, &valid)
+                       : pressure_fluid_row<KL, false>(f, ff, d, K, P.fluid_omega, P.tau, s, i, k0,
+                                                       k1, sub, w, wcols, &valid);
     } else if (t < nitems) {
       const int tb = t - nf;
       const int r = advance_rank(preb, nact, r0b, tb);
@@ -461,12 +466,11 @@ __global__ __launch_bounds__(kBS) void k_pressure(SphFluid f, const SphPhysics* 
       const float2 xb = ld2(d.bxrec, 2 * k0);
       float ks = 0.f;
       const int e1 = bf.rowptr[k0 + 1];
+      // sum_f w_f a*_f . (-G_bf) with {x_f, y_f, w_f a*_f} from prec
       for (int k = bf.rowptr[k0] + sub; k < e1; k += KL) {
-        const int j = bf.col[k];
-        const float2 xf = ld2(d.xrec, 2 * j);
-        const float4 af = ld4(d.arec, j);
-        const float2 G = spiky_G(xb.x, xb.y, xf.x, xf.y, K);
-        ks += af.z * (af.x * (-G.x) + af.y * (-G.y));
+        const float4 pf = ld4(d.prec, bf.col[k]);
+        const float2 G = spiky_G(xb.x, xb.y, pf.x, pf.y, K);
+        ks -= pf.z * G.x + pf.w * G.y;
       }
       ks = group_sum<KL>(ks);
       if (sub == 0) {
@@ -479,542 +483,6 @@ __global__ __launch_bounds__(kBS) void k_pressure(SphFluid f, const SphPhysics* 
         d.bxrec[4 * k0 + 2] = pn / (rb * rb);
         contrib = (double)(rr < -P.tau ? -P.tau : rr);
         valid = true;
-      }
-    }
-    scene_sum(contrib, s, valid, acc_slot);
-  }
-}
-
-// ------------------------------------------------ pair-parallel row tiles
-// A workgroup takes kTileRows consecutive work items (rows) per pass.  The
-// rows' pair ranges are laid end to end in LDS order: slot u belongs to row
-// q with off[q] <= u < off[q + 1].  Lanes take consecutive slots, so one
-// wave-wide load reads consecutive column indices and, in cell order,
-// consecutive neighbour records.  Each slot's contribution goes to LDS and
-// the row's owner thread adds its slots in pair order -- the order of the
-// one-lane-per-row kernels, independent of the batch a scene runs in.
-// Rows with more pairs than fit are taken in chunks of kTileSlots.
-constexpr int kTileRows = 128;
-constexpr int kTileSlots = 4096;
-constexpr int kTileBatch = 8;   // slots a lane has in flight at once
-
-struct TileRows {
-  int b1[kTileRows];    // first pair of the row's first list
-  int n1[kTileRows];    // pairs in the first list
-  int b2[kTileRows];    // first pair of the second (boundary) list
-  int off[kTileRows + 1];
-  float4 own[kTileRows];
-  float4 kc[kTileRows];  // inv_h, s0, dt^2, row kind (0 fluid, 1 boundary)
-  unsigned char row[kTileSlots];
-  int scan[kBS / kWave + 1];
-};
-
-// Lays out the rows' slots (exclusive scan of their lengths); returns the
-// total.  Every thread of the workgroup calls it.
-__device__ __forceinline__ int tile_layout(TileRows& T, int len) {
-  int tot;
-  const int ex = block_exclusive_scan(threadIdx.x < kTileRows ? len : 0, T.scan, &tot);
-  if (threadIdx.x < kTileRows) T.off[threadIdx.x] = ex;
-  if (threadIdx.x == 0) T.off[kTileRows] = tot;
-  __syncthreads();
-  return tot;
-}
-
-// Marks the slots [c0, c1) with their rows.
-__device__ __forceinline__ void tile_mark(TileRows& T, int c0, int c1) {
-  if (threadIdx.x < kTileRows) {
-    const int q = threadIdx.x;
-    const int a = max(T.off[q], c0), b = min(T.off[q + 1], c1);
-    for (int u = a; u < b; ++u) T.row[u - c0] = (unsigned char)q;
-  }
-  __syncthreads();
-}
-
-// a*_i of kTileRows active fluid rows per pass (dfsph.py:26-42).
-__global__ __launch_bounds__(kBS) void k_accel_tiles(SphFluid f, const SphPhysics* __restrict__ phys,
-                                                     SphCSR ff, SphCSR fb, SphDfsph d) {
-  __shared__ TileRows T;
-  __shared__ float2 c[kTileSlots];
-  const int nact = d.act_n[2];
-  const int nitems = d.act_n[0];
-  const int tid = threadIdx.x;
-  for (int base = xcd_block() * kTileRows; base < nitems; base += gridDim.x * kTileRows) {
-    int i = -1, len = 0;
-    float dt2 = 0.f;
-    if (tid < kTileRows && base + tid < nitems) {
-      const int t = base + tid;
-      const int r = advance_rank(d.act_pre, nact, find_rank(d.act_pre, nact, base), t);
-      const int s = d.act_scene[r];
-      i = d.fbegin[s] + (t - d.act_pre[r]);
-      const SphPhysics P = phys[s];
-      const KConst K = kconst(P.h);
-      dt2 = P.dt * P.dt;
-      const int b1 = ff.rowptr[i], n1 = ff.rowptr[i + 1] - b1;
-      int b2 = 0, n2 = 0;
-      if (fb.rowptr) {
-        b2 = fb.rowptr[i];
-        n2 = fb.rowptr[i + 1] - b2;
-      }
-      T.b1[tid] = b1;
-      T.n1[tid] = n1;
-      T.b2[tid] = b2;
-      T.own[tid] = ld4(d.xrec, i);
-      T.kc[tid] = make_float4(K.inv_h, K.s0, 0.f, 0.f);
-      len = n1 + n2;
-    }
-    const int total = tile_layout(T, len);
-    float ax = 0.f, ay = 0.f;
-    for (int c0 = 0; c0 < total; c0 += kTileSlots) {
-      const int c1 = min(total, c0 + kTileSlots);
-      tile_mark(T, c0, c1);
-      for (int u0 = c0 + tid; u0 < c1; u0 += kBS * kTileBatch) {
-        float4 rj[kTileBatch];
-        int qq[kTileBatch];
-#pragma unroll
-        for (int m = 0; m < kTileBatch; ++m) {
-          const int u = u0 + m * kBS;
-          qq[m] = -1;
-          if (u < c1) {
-            const int q = T.row[u - c0];
-            const int l = u - T.off[q];
-            const bool bnd = l >= T.n1[q];
-            const int* colp = bnd ? fb.col : ff.col;
-            const float* recp = bnd ? d.bxrec : d.xrec;
-            const int j = colp[bnd ? T.b2[q] + l - T.n1[q] : T.b1[q] + l];
-            rj[m] = ld4(recp, j);
-            qq[m] = q;
-          }
-        }
-#pragma unroll
-        for (int m = 0; m < kTileBatch; ++m) {
-          if (qq[m] >= 0) {
-            const int q = qq[m];
-            const float4 ri = T.own[q];
-            const float4 kk = T.kc[q];
-            KConst K;
-            K.inv_h = kk.x;
-            K.s0 = kk.y;
-            const float2 G = spiky_G(ri.x, ri.y, rj[m].x, rj[m].y, K);
-            const float cc = -rj[m].w * (ri.z + rj[m].z);
-            c[u0 + m * kBS - c0] = make_float2(cc * G.x, cc * G.y);
-          }
-        }
-      }
-      __syncthreads();
-      if (tid < kTileRows) {
-        const int a = max(T.off[tid], c0), b = min(T.off[tid + 1], c1);
-        for (int u = a; u < b; ++u) {
-          const float2 v = c[u - c0];
-          ax += v.x;
-          ay += v.y;
-        }
-      }
-      __syncthreads();
-    }
-    if (i >= 0) st4(d.arec, i, make_float4(ax, ay, dt2 * d.aact[i], 0.f));
-  }
-}
-
-// Pressure update and residual of kTileRows active rows per pass (fluid
-// rows first, then boundary rows; dfsph.py:44-57,164-184, akinciBoundary.py:
-// 280-315).  Every pair is w_j ((a_i - a_j) . G_ij) with a_i = 0 for a
-// boundary row and, for a boundary neighbour, a_j = 0 and w_j = dt^2 A^act_b.
-__global__ __launch_bounds__(kBS) void k_pressure_tiles(SphFluid f, const SphPhysics* __restrict__ phys,
-                                                        SphCSR ff, SphBoundary b, SphCSR fb,
-                                                        SphCSR bf, SphDfsph d) {
-  __shared__ TileRows T;
-  __shared__ float c[kTileSlots];
-  const int nact = d.act_n[2];
-  const int nf = d.act_n[0];
-  const int nitems = nf + d.act_n[1];
-  const int* preb = d.act_pre + d.nscenes + 1;
-  double* acc_slot = d.err_acc + (size_t)(blockIdx.x % SPH_DFSPH_ERR_SLOTS) * d.nscenes;
-  const int tid = threadIdx.x;
-  for (int base = xcd_block() * kTileRows; base < nitems; base += gridDim.x * kTileRows) {
-    int row = -1, s = -1, len = 0;
-    bool fluid = true;
-    if (tid < kTileRows && base + tid < nitems) {
-      const int t = base + tid;
-      float4 own;
-      if (t < nf) {
-        const int r = advance_rank(d.act_pre, nact, find_rank(d.act_pre, nact, base), t);
-        s = d.act_scene[r];
-        row = d.fbegin[s] + (t - d.act_pre[r]);
-        const float2 x = ld2(d.xrec, 2 * row);
-        const float4 a = ld4(d.arec, row);
-        own = make_float4(x.x, x.y, a.x, a.y);
-        const int b1 = ff.rowptr[row];
-        T.b1[tid] = b1;
-        T.n1[tid] = ff.rowptr[row + 1] - b1;
-        len = T.n1[tid];
-        if (fb.rowptr) {
-          const int b2 = fb.rowptr[row];
-          T.b2[tid] = b2;
-          len += fb.rowptr[row + 1] - b2;
-        }
-      } else {
-        const int tb = t - nf;
-        const int tb0 = base > nf ? base - nf : 0;
-        const int r = advance_rank(preb, nact, find_rank(preb, nact, tb0), tb);
-        s = d.act_scene[r];
-        row = d.bbegin[s] + (tb - preb[r]);
-        fluid = false;
-        const float2 x = ld2(d.bxrec, 2 * row);
-        own = make_float4(x.x, x.y, 0.f, 0.f);
-        const int b1 = bf.rowptr[row];
-        T.b1[tid] = b1;
-        T.n1[tid] = bf.rowptr[row + 1] - b1;
-        len = T.n1[tid];
-      }
-      const SphPhysics P = phys[s];
-      const KConst K = kconst(P.h);
-      T.own[tid] = own;
-      T.kc[tid] = make_float4(K.inv_h, K.s0, P.dt * P.dt, fluid ? 0.f : 1.f);
-    }
-    const int total = tile_layout(T, len);
-    float ks = 0.f;
-    for (int c0 = 0; c0 < total; c0 += kTileSlots) {
-      const int c1 = min(total, c0 + kTileSlots);
-      tile_mark(T, c0, c1);
-      for (int u0 = c0 + tid; u0 < c1; u0 += kBS * kTileBatch) {
-        float2 xj[kTileBatch];
-        float4 aj[kTileBatch];
-        int qq[kTileBatch];
-#pragma unroll
-        for (int m = 0; m < kTileBatch; ++m) {
-          const int u = u0 + m * kBS;
-          qq[m] = -1;
-          if (u < c1) {
-            const int q = T.row[u - c0];
-            const int l = u - T.off[q];
-            const bool brow = T.kc[q].w != 0.f;
-            const bool bnd = !brow && l >= T.n1[q];      // boundary neighbour of a fluid row
-            const int* colp = brow ? bf.col : (bnd ? fb.col : ff.col);
-            const int j = colp[bnd ? T.b2[q] + l - T.n1[q] : T.b1[q] + l];
-            xj[m] = ld2(bnd ? d.bxrec : d.xrec, 2 * j);
-            aj[m] = bnd ? make_float4(0.f, 0.f, T.kc[q].z * d.b_aact[j], 0.f) : ld4(d.arec, j);
-            qq[m] = q;
-          }
-        }
-#pragma unroll
-        for (int m = 0; m < kTileBatch; ++m) {
-          if (qq[m] >= 0) {
-            const int q = qq[m];
-            const float4 oi = T.own[q];
-            const float4 kk = T.kc[q];
-            KConst K;
-            K.inv_h = kk.x;
-            K.s0 = kk.y;
-            const float2 G = spiky_G(oi.x, oi.y, xj[m].x, xj[m].y, K);
-            c[u0 + m * kBS - c0] = aj[m].z * ((oi.z - aj[m].x) * G.x + (oi.w - aj[m].y) * G.y);
-          }
-        }
-      }
-      __syncthreads();
-      if (tid < kTileRows) {
-        const int a = max(T.off[tid], c0), e = min(T.off[tid + 1], c1);
-        for (int u = a; u < e; ++u) ks += c[u - c0];
-      }
-      __syncthreads();
-    }
-    if (tid < kTileRows) {      // waves 0 and 1, whole
-      double contrib = 0.0;
-      const bool valid = row >= 0;
-      if (valid) {
-        const SphPhysics P = phys[s];
-        if (fluid) {
-          const float rr = ks - d.src[row];
-          const float pold = d.p2[row];
-          d.p[row] = pold;
-          float pn = pold - P.fluid_omega * rr / d.alpha[row];
-          pn = pn < 0.f ? 0.f : pn;  // clamp(min = 0) keeps NaN
-          if (isnan(pn) || isinf(pn)) atomicOr(&d.status[s], SPH_ST_NAN);
-          d.p2[row] = pn;
-          d.resid[row] = rr;
-          const float rho = f.rho[row] * f.rest[row];
-          d.xrec[4 * row + 2] = pn / (rho * rho);
-          contrib = (double)(rr < -P.tau ? -P.tau : rr);
-        } else {
-          const float rr = ks - d.b_src[row];
-          float pn = d.b_p[row] - P.omega * rr / d.b_alpha[row];
-          pn = pn < 0.f ? 0.f : pn;
-          d.b_p[row] = pn;
-          d.b_resid[row] = rr;
-          const float rb = b.rho[row] * b.rest[row];
-          d.bxrec[4 * row + 2] = pn / (rb * rb);
-          contrib = (double)(rr < -P.tau ? -P.tau : rr);
-        }
-      }
-      scene_sum(contrib, s, valid, acc_slot);
-    }
-    __syncthreads();   // T is rewritten by the next pass
-  }
-}
-
-// ------------------------------------------------------ wave row tiles
-// The same pair-parallel scheme per wavefront instead of per workgroup: a
-// wave takes kWTRows consecutive work items, lays their pairs end to end
-// (slot u of row q for off[q] <= u < off[q + 1], offsets held in lanes
-// 0..kWTRows-1), walks the slots 64 at a time (lane = slot, so a wave-wide
-// load reads consecutive columns and consecutive neighbours of a few rows)
-// and stores each slot's contribution in its own LDS window.  Lane q then
-// adds row q's slots in pair order.  No workgroup barrier: every wave runs
-// on its own, so the CU keeps many independent waves in flight.
-constexpr int kWTRows = 32;
-constexpr int kWTSlots = 512;    // contribution slots per wave and chunk
-constexpr int kWTRounds = 4;     // 64-slot rounds with their loads in flight together
-
-struct WaveTile {
-  int b1[kWTRows], n1[kWTRows], b2[kWTRows];
-  float4 own[kWTRows];
-  float4 kc[kWTRows];   // inv_h, s0, dt^2, row kind (0 fluid, 1 boundary)
-};
-
-__device__ __forceinline__ void wave_sync_lds() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Row of slot u: the largest q with off[q] <= u (off in lanes 0..31; the
-// lanes above hold the total).  All lanes take part.
-__device__ __forceinline__ int wt_row_of(int off_reg, int u) {
-  int lo = 0, hi = kWTRows;
-#pragma unroll
-  for (int step = 0; step < 5; ++step) {
-    const int m = (lo + hi) >> 1;
-    const int om = __shfl(off_reg, m, kWave);
-    if (om <= u) lo = m; else hi = m;
-  }
-  return lo;
-}
-
-__global__ __launch_bounds__(kBS) void k_accel_wt(SphFluid f, const SphPhysics* __restrict__ phys,
-                                                  SphCSR ff, SphCSR fb, SphDfsph d) {
-  __shared__ WaveTile s_tile[kBS / kWave];
-  __shared__ float2 s_c[kBS / kWave][kWTSlots];
-  const int wv = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  WaveTile& T = s_tile[wv];
-  float2* c = s_c[wv];
-  const int nact = d.act_n[2];
-  const int nitems = d.act_n[0];
-  const int waves = gridDim.x * (kBS / kWave);
-  for (int tile = xcd_block() * (kBS / kWave) + wv; tile * kWTRows < nitems; tile += waves) {
-    const int base = tile * kWTRows;
-    const int t = base + lane;
-    const bool valid = lane < kWTRows && t < nitems;
-    int i = 0, len = 0;
-    float dt2 = 0.f;
-    const int r0 = find_rank(d.act_pre, nact, base);
-    if (valid) {
-      const int r = advance_rank(d.act_pre, nact, r0, t);
-      const int s = d.act_scene[r];
-      i = d.fbegin[s] + (t - d.act_pre[r]);
-      const SphPhysics P = phys[s];
-      const KConst K = kconst(P.h);
-      dt2 = P.dt * P.dt;
-      const int b1 = ff.rowptr[i], n1 = ff.rowptr[i + 1] - b1;
-      int b2 = 0, n2 = 0;
-      if (fb.rowptr) {
-        b2 = fb.rowptr[i];
-        n2 = fb.rowptr[i + 1] - b2;
-      }
-      T.b1[lane] = b1;
-      T.n1[lane] = n1;
-      T.b2[lane] = b2;
-      T.own[lane] = ld4(d.xrec, i);
-      T.kc[lane] = make_float4(K.inv_h, K.s0, 0.f, 0.f);
-      len = n1 + n2;
-    }
-    const int inc = wave_inclusive_scan(len);
-    const int total = __shfl(inc, kWave - 1, kWave);
-    const int off = lane < kWTRows ? inc - len : total;
-    wave_sync_lds();
-    float ax = 0.f, ay = 0.f;
-    for (int c0 = 0; c0 < total; c0 += kWTSlots) {
-      const int c1 = min(total, c0 + kWTSlots);
-      for (int u0 = c0; u0 < c1; u0 += kWave * kWTRounds) {
-        float4 rj[kWTRounds];
-        int qq[kWTRounds];
-#pragma unroll
-        for (int m = 0; m < kWTRounds; ++m) {
-          const int u = u0 + m * kWave + lane;
-          const int q = wt_row_of(off, u);
-          const int oq = __shfl(off, q, kWave);   // every lane takes part
-          qq[m] = -1;
-          if (u < c1) {
-            const int l = u - oq;
-            const bool bnd = l >= T.n1[q];
-            const int* colp = bnd ? fb.col : ff.col;
-            const float* recp = bnd ? d.bxrec : d.xrec;
-            const int j = colp[bnd ? T.b2[q] + l - T.n1[q] : T.b1[q] + l];
-            rj[m] = ld4(recp, j);
-            qq[m] = q;
-          }
-        }
-#pragma unroll
-        for (int m = 0; m < kWTRounds; ++m) {
-          if (qq[m] >= 0) {
-            const float4 ri = T.own[qq[m]];
-            const float4 kk = T.kc[qq[m]];
-            KConst K;
-            K.inv_h = kk.x;
-            K.s0 = kk.y;
-            const float2 G = spiky_G(ri.x, ri.y, rj[m].x, rj[m].y, K);
-            const float cc = -rj[m].w * (ri.z + rj[m].z);
-            c[u0 + m * kWave + lane - c0] = make_float2(cc * G.x, cc * G.y);
-          }
-        }
-      }
-      wave_sync_lds();
-      if (valid) {
-        const int a = max(off, c0), b = min(off + len, c1);
-        for (int u = a; u < b; ++u) {
-          const float2 v = c[u - c0];
-          ax += v.x;
-          ay += v.y;
-        }
-      }
-      wave_sync_lds();
-    }
-    if (valid) st4(d.arec, i, make_float4(ax, ay, dt2 * d.aact[i], 0.f));
-  }
-}
-
-__global__ __launch_bounds__(kBS) void k_pressure_wt(SphFluid f, const SphPhysics* __restrict__ phys,
-                                                     SphCSR ff, SphBoundary b, SphCSR fb, SphCSR bf,
-                                                     SphDfsph d) {
-  __shared__ WaveTile s_tile[kBS / kWave];
-  __shared__ float s_c[kBS / kWave][kWTSlots];
-  const int wv = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  WaveTile& T = s_tile[wv];
-  float* c = s_c[wv];
-  const int nact = d.act_n[2];
-  const int nf = d.act_n[0];
-  const int nitems = nf + d.act_n[1];
-  const int* preb = d.act_pre + d.nscenes + 1;
-  double* acc_slot = d.err_acc + (size_t)(blockIdx.x % SPH_DFSPH_ERR_SLOTS) * d.nscenes;
-  const int waves = gridDim.x * (kBS / kWave);
-  for (int tile = xcd_block() * (kBS / kWave) + wv; tile * kWTRows < nitems; tile += waves) {
-    const int base = tile * kWTRows;
-    const int t = base + lane;
-    const bool valid = lane < kWTRows && t < nitems;
-    int row = -1, s = -1, len = 0;
-    bool fluid = true;
-    const int r0f = base < nf ? find_rank(d.act_pre, nact, base) : 0;
-    const int r0b = base + kWTRows > nf ? find_rank(preb, nact, base > nf ? base - nf : 0) : 0;
-    if (valid) {
-      float4 own;
-      if (t < nf) {
-        const int r = advance_rank(d.act_pre, nact, r0f, t);
-        s = d.act_scene[r];
-        row = d.fbegin[s] + (t - d.act_pre[r]);
-        const float2 x = ld2(d.xrec, 2 * row);
-        const float4 a = ld4(d.arec, row);
-        own = make_float4(x.x, x.y, a.x, a.y);
-        const int b1 = ff.rowptr[row];
-        const int n1 = ff.rowptr[row + 1] - b1;
-        T.b1[lane] = b1;
-        T.n1[lane] = n1;
-        len = n1;
-        if (fb.rowptr) {
-          const int b2 = fb.rowptr[row];
-          T.b2[lane] = b2;
-          len += fb.rowptr[row + 1] - b2;
-        }
-      } else {
-        const int tb = t - nf;
-        const int r = advance_rank(preb, nact, r0b, tb);
-        s = d.act_scene[r];
-        row = d.bbegin[s] + (tb - preb[r]);
-        fluid = false;
-        const float2 x = ld2(d.bxrec, 2 * row);
-        own = make_float4(x.x, x.y, 0.f, 0.f);
-        const int b1 = bf.rowptr[row];
-        const int n1 = bf.rowptr[row + 1] - b1;
-        T.b1[lane] = b1;
-        T.n1[lane] = n1;
-        len = n1;
-      }
-      const SphPhysics P = phys[s];
-      const KConst K = kconst(P.h);
-      T.own[lane] = own;
-      T.kc[lane] = make_float4(K.inv_h, K.s0, P.dt * P.dt, fluid ? 0.f : 1.f);
-    }
-    const int inc = wave_inclusive_scan(len);
-    const int total = __shfl(inc, kWave - 1, kWave);
-    const int off = lane < kWTRows ? inc - len : total;
-    wave_sync_lds();
-    float ks = 0.f;
-    for (int c0 = 0; c0 < total; c0 += kWTSlots) {
-      const int c1 = min(total, c0 + kWTSlots);
-      for (int u0 = c0; u0 < c1; u0 += kWave * kWTRounds) {
-        float2 xj[kWTRounds];
-        float4 aj[kWTRounds];
-        int qq[kWTRounds];
-#pragma unroll
-        for (int m = 0; m < kWTRounds; ++m) {
-          const int u = u0 + m * kWave + lane;
-          const int q = wt_row_of(off, u);
-          const int oq = __shfl(off, q, kWave);   // every lane takes part
-          qq[m] = -1;
-          if (u < c1) {
-            const int l = u - oq;
-            const float4 kk = T.kc[q];
-            const bool brow = kk.w != 0.f;
-            const bool bnd = !brow && l >= T.n1[q];   // boundary neighbour of a fluid row
-            const int* colp = brow ? bf.col : (bnd ? fb.col : ff.col);
-            const int j = colp[bnd ? T.b2[q] + l - T.n1[q] : T.b1[q] + l];
-            xj[m] = ld2(bnd ? d.bxrec : d.xrec, 2 * j);
-            aj[m] = bnd ? make_float4(0.f, 0.f, kk.z * d.b_aact[j], 0.f) : ld4(d.arec, j);
-            qq[m] = q;
-          }
-        }
-#pragma unroll
-        for (int m = 0; m < kWTRounds; ++m) {
-          if (qq[m] >= 0) {
-            const float4 oi = T.own[qq[m]];
-            const float4 kk = T.kc[qq[m]];
-            KConst K;
-            K.inv_h = kk.x;
-            K.s0 = kk.y;
-            const float2 G = spiky_G(oi.x, oi.y, xj[m].x, xj[m].y, K);
-            c[u0 + m * kWave + lane - c0] =
-                aj[m].z * ((oi.z - aj[m].x) * G.x + (oi.w - aj[m].y) * G.y);
-          }
-        }
-      }
-      wave_sync_lds();
-      if (valid) {
-        const int a = max(off, c0), e = min(off + len, c1);
-        for (int u = a; u < e; ++u) ks += c[u - c0];
-      }
-      wave_sync_lds();
-    }
-    double contrib = 0.0;
-    if (valid) {
-      const SphPhysics P = phys[s];
-      if (fluid) {
-        const float rr = ks - d.src[row];
-        const float pold = d.p2[row];
-        d.p[row] = pold;
-        float pn = pold - P.fluid_omega * rr / d.alpha[row];
-        pn = pn < 0.f ? 0.f : pn;  // clamp(min = 0) keeps NaN
-        if (isnan(pn) || isinf(pn)) atomicOr(&d.status[s], SPH_ST_NAN);
-        d.p2[row] = pn;
-        d.resid[row] = rr;
-        const float rho = f.rho[row] * f.rest[row];
-        d.xrec[4 * row + 2] = pn / (rho * rho);
-        contrib = (double)(rr < -P.tau ? -P.tau : rr);
-      } else {
-        const float rr = ks - d.b_src[row];
-        float pn = d.b_p[row] - P.omega * rr / d.b_alpha[row];
-        pn = pn < 0.f ? 0.f : pn;
-        d.b_p[row] = pn;
-        d.b_resid[row] = rr;
-        const float rb = b.rho[row] * b.rest[row];
-        d.bxrec[4 * row + 2] = pn / (rb * rb);
-        contrib = (double)(rr < -P.tau ? -P.tau : rr);
       }
     }
     scene_sum(contrib, s, valid, acc_slot);
@@ -1074,7 +542,7 @@ __global__ __launch_bounds__(kBS) void k_finish(SphFluid f, const SphPhysics* __
 
 static bool dfsph_ok(const SphDfsph* d) {
   return d && d->vpred && d->aact && d->alpha && d->src && d->p && d->p2 && d->apred &&
-         d->resid && d->xrec && d->arec && d->err_acc && d->iters && d->active &&
+         d->resid && d->xrec && d->arec && d->prec && d->wvec && d->err_acc && d->iters && d->active &&
          d->any_active && d->status && d->nfluid && d->nbdy && d->fbegin && d->bbegin &&
          d->act_scene && d->act_pre && d->act_n && d->nscenes > 0 && d->grid_blocks > 0;
 }
@@ -1129,17 +597,6 @@ int dfsph_accel(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, con
                 SphDfsph* d, hipStream_t s) {
   SPH_REQUIRE(f && phys && ff && ff->rowptr && dfsph_ok(d), "sph_dfsph_accel: bad arguments");
   SPH_REQUIRE(!fb || !fb->rowptr || d->bxrec, "sph_dfsph_accel: boundary list without records");
-  const int tiles = tuning(SPH_TUNE_SOLVE_TILES);
-  if (tiles == 1) {
-    hipLaunchKernelGGL(k_accel_tiles, dim3(d->grid_blocks), dim3(kBS), 0, s, *f, phys, *ff,
-                       csr_or_null(fb), *d);
-    return launch_check("sph_dfsph_accel");
-  }
-  if (tiles == 2) {
-    hipLaunchKernelGGL(k_accel_wt, dim3(d->grid_blocks), dim3(kBS), 0, s, *f, phys, *ff,
-                       csr_or_null(fb), *d);
-    return launch_check("sph_dfsph_accel");
-  }
   with_lanes(tuning(SPH_TUNE_SOLVE_LANES), [&](auto kl) {
     constexpr int KL = decltype(kl)::value;
     hipLaunchKernelGGL((k_accel<KL>), dim3(d->grid_blocks), dim3(kBS), 0, s, *f, phys, *ff,
@@ -1156,17 +613,6 @@ int dfsph_pressure(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
   const SphBoundary bb = bdy_or_null(b);
   SPH_REQUIRE(bdy_ok(bb, d) && (bb.nb == 0 || (bf && bf->rowptr)),
               "sph_dfsph_pressure: bad boundary");
-  const int tiles = tuning(SPH_TUNE_SOLVE_TILES);
-  if (tiles == 1) {
-    hipLaunchKernelGGL(k_pressure_tiles, dim3(d->grid_blocks), dim3(kBS), 0, s, *f, phys, *ff, bb,
-                       csr_or_null(fb), csr_or_null(bf), *d);
-    return launch_check("sph_dfsph_pressure");
-  }
-  if (tiles == 2) {
-    hipLaunchKernelGGL(k_pressure_wt, dim3(d->grid_blocks), dim3(kBS), 0, s, *f, phys, *ff, bb,
-                       csr_or_null(fb), csr_or_null(bf), *d);
-    return launch_check("sph_dfsph_pressure");
-  }
   with_lanes(tuning(SPH_TUNE_SOLVE_LANES), [&](auto kl) {
     constexpr int KL = decltype(kl)::value;
     hipLaunchKernelGGL((k_pressure<KL>), dim3(d->grid_blocks), dim3(kBS), 0, s, *f, phys, *ff,

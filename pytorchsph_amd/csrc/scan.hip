@@ -27,9 +27,8 @@ int launch_check(const char* where) {
 }
 
 // Defaults (the shapes the autotuner picks on a 256-scene mixed batch):
-// lanes per row of the search, the gathers and the DFSPH solve; lane groups
-// (not row tiles) for the DFSPH iteration.
-static std::atomic<int> g_tuning[SPH_TUNE_COUNT] = {{2}, {4}, {2}, {0}};
+// lanes per row of the search, the gathers and the DFSPH solve.
+static std::atomic<int> g_tuning[SPH_TUNE_COUNT] = {{2}, {4}, {2}};
 
 int tuning(int knob) { return g_tuning[knob].load(std::memory_order_relaxed); }
 
@@ -38,12 +37,7 @@ static int set_tuning(int knob, int value) {
     set_error("sph_set_tuning: unknown knob");
     return SPH_E_ARG;
   }
-  if (knob == SPH_TUNE_SOLVE_TILES) {
-    if (value < 0 || value > 2) {
-      set_error("sph_set_tuning: SPH_TUNE_SOLVE_TILES must be 0, 1 or 2");
-      return SPH_E_ARG;
-    }
-  } else if (value < 1 || value > 32 || (value & (value - 1))) {
+  if (value < 1 || value > 32 || (value & (value - 1))) {
     set_error("sph_set_tuning: lanes per row must be 1, 2, 4, 8, 16 or 32");
     return SPH_E_ARG;
   }

@@ -237,7 +237,7 @@ class SceneBatch:
         ints = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).to(dev)  # noqa
         self.df = dict(
             vpred=z(NP, 2), aact=z(NP), alpha=z(NP), src=z(NP), p=z(NP), p2=z(NP), apred=z(NP, 2),
-            resid=z(NP), xrec=z(NP, 4), arec=z(NP, 4), b_aact=z(nb1), b_alpha=z(nb1),
+            resid=z(NP), xrec=z(NP, 4), arec=z(NP, 4), prec=z(NP, 4), wvec=z(NP, 2), b_aact=z(nb1), b_alpha=z(nb1),
             b_src=z(nb1), b_p=z(nb1), b_resid=z(nb1), bxrec=z(nb1, 4),
             err_acc=z(L.DFSPH_ERR_SLOTS, S, dt=torch.float64), err_hist=z(S, self.max_hist, dt=torch.float64),
             iters=z(S, dt=torch.int32), active=z(S, dt=torch.int32),

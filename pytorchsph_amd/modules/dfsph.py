@@ -62,7 +62,8 @@ class dfsphModule(Module):
         nb1 = max(nb, 1)
         ints = lambda a: torch.tensor(a, dtype=torch.int32, device=dev)  # noqa: E731
         b = dict(vpred=z(n, 2), aact=z(n), alpha=z(n), src=z(n), p=z(n), p2=z(n), apred=z(n, 2),
-                 resid=z(n), xrec=z(n, 4), arec=z(n, 4), b_aact=z(nb1), b_alpha=z(nb1),
+                 resid=z(n), xrec=z(n, 4), arec=z(n, 4), prec=z(n, 4), wvec=z(n, 2),
+                 b_aact=z(nb1), b_alpha=z(nb1),
                  b_src=z(nb1), b_p=z(nb1), b_resid=z(nb1), bxrec=z(nb1, 4),
                  err_acc=z(L.DFSPH_ERR_SLOTS, 1, dt=torch.float64), err_hist=z(1, 300, dt=torch.float64),
                  iters=z(1, dt=torch.int32), active=z(1, dt=torch.int32),

@@ -89,51 +89,3 @@ def test_dfsph_lanes(cuda, kl):
     hist = d["err_hist"][0, :5].cpu().numpy()
     assert np.allclose(hist, out["err_hist"][:5], rtol=1e-3, atol=1e-7)
     assert rel_err(d["p2"].cpu().numpy(), out["p2"]) < 1e-3
-
-
-def _dfsph_five_iterations(sc, **knobs):
-    with tuned(**knobs):
-        b = SceneBatch([sc])
-        b._dfsph_pre()
-        b._dfsph_iters(5)
-        torch.cuda.synchronize()
-    return b
-
-
-def test_dfsph_tiles_equal_one_lane_per_row(cuda):
-    """Row tiles add each row's pairs in pair order, like one lane per row;
-    only FMA contraction of the running sum can differ (last bits)."""
-    sc = S.dam_break(dx=0.01, block=(1.0, 0.6), scheme="dfsph", seed=8)
-    rng = np.random.default_rng(8)
-    sc.v = rng.normal(0, 0.5, sc.v.shape).astype(np.float32)
-    a = _dfsph_five_iterations(sc, solve=1, solve_tiles=0)
-    for tiles in (1, 2):
-        b = _dfsph_five_iterations(sc, solve=1, solve_tiles=tiles)
-        assert int(a.df["iters"][0]) == int(b.df["iters"][0])
-        for k in ("p2", "arec", "resid", "b_p", "b_resid"):
-            assert rel_err(a.df[k].cpu().numpy(), b.df[k].cpu().numpy()) < 1e-5, (tiles, k)
-
-
-@pytest.mark.parametrize("tiles", [1, 2])
-def test_dfsph_tiles_multi_scene_against_oracle(cuda, tiles):
-    """Tiles straddle scene boundaries and the fluid/boundary item border."""
-    scenes = [S.scene_for_count("dambreak", "dfsph", n, 40 + n) for n in (700, 1300, 2100)]
-    with tuned(solve_tiles=tiles, solve=2):
-        b = SceneBatch(scenes)
-        b._dfsph_pre()
-        b._dfsph_iters(4)
-        torch.cuda.synchronize()
-    for k, sc in enumerate(scenes):
-        so = O.SceneOracle(sc)
-        ff, bf, _, fb = so.pairs(sc.x)
-        a0, n = int(b.p_begin[k]), sc.n
-        c0, nb = int(b.b_begin[k]), sc.nb
-        rho = b.rho[a0:a0 + n].cpu().numpy().astype(np.float64)
-        brho = b.brho[c0:c0 + nb].cpu().numpy().astype(np.float64)
-        f = dict(x=sc.x, v=sc.v, rho=rho, area=so.area, rest=so.rest)
-        bd = dict(x=sc.bx, vol=sc.bvol, rest=so.brest, rho=brho)
-        acc = np.tile([0.0, -S.G], (n, 1))
-        out = O.dfsph(f, sc.phys, ff, bd, fb, bf, acc, fixed_iters=4)
-        d = b.df
-        assert rel_err(d["p2"][a0:a0 + n].cpu().numpy(), out["p2"]) < 1e-3
-        assert rel_err(d["b_p"][c0:c0 + nb].cpu().numpy(), out["b_p"]) < 1e-3
