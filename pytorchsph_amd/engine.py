@@ -245,7 +245,8 @@ class SceneBatch:
             nfluid=ints(self.n), nbdy=ints(self.nb), fbegin=ints(self.p_begin),
             bbegin=ints(self.b_begin), act_scene=z(S, dt=torch.int32),
             act_pre=z(2 * (S + 1), dt=torch.int32), act_n=z(3, dt=torch.int32))
-        self.any_active_host = torch.zeros(1, dtype=torch.int32, pin_memory=True)
+        self.any_active_host = torch.zeros(1, dtype=torch.int32,
+                                           pin_memory=self.dev.type == "cuda")
 
     def _dfsph_struct(self) -> L.SphDfsph:
         return L.SphDfsph(*[L.ptr(self.df[k]) for k in L.DFSPH_FIELDS], self.nscenes,

@@ -76,6 +76,7 @@ class Server:
         self.live: list[Job] = []
         self.repacks = 0
         self.rejected: list[Job] = []
+        self.repack_times: list[dict] = []      # seconds per phase of each repack
         self.latency: list[float] = []          # one sample per (job, step)
         self.ticks: list[tuple[float, int]] = []  # (seconds, particles advanced)
 
@@ -106,11 +107,13 @@ class Server:
         self.ticks.append((seconds, moved))
 
     def _repack(self, queue: deque, now: float, clock, start):
+        t_begin = clock()
         keep = [j for j in self.live if not j.finished]
         for k, j in enumerate(self.live):
             if not j.finished:
                 j.scene, j.carry = self.batch.export(k)
         self.batch = None
+        t_export = clock()
         used = sum(self.footprint(j) for j in keep)
         parts = sum(j.n for j in keep)
         admitted = []
@@ -137,16 +140,22 @@ class Server:
         t0 = clock()
         self.batch = MixedBatch([j.scene for j in self.live], self.device,
                                 carry=[j.carry for j in self.live], **self.batch_kw)
+        torch.cuda.synchronize()
+        t_build = clock()
         # capture() advances every scene by one eager step before recording
         self.batch.capture()
         torch.cuda.synchronize()
         dt = clock() - t0
+        self.repack_times.append(dict(export=t_export - t_begin, admit=t0 - t_export,
+                                      build=t_build - t0, capture=clock() - t_build))
         for j in self.live:
             j.carry = None
         self._advance({j.id: k for k, j in enumerate(self.live)}, dt, clock() - start)
         if self.log:
+            rt = self.repack_times[-1]
             self.log(f"repack {self.repacks}: {len(keep)} kept + {len(admitted)} admitted, "
-                     f"{parts} particles, {used / 2**30:.2f} GiB, {dt * 1e3:.0f} ms")
+                     f"{parts} particles, {used / 2**30:.2f} GiB, {dt * 1e3:.0f} ms ("
+                     + ", ".join(f"{k} {v * 1e3:.0f}" for k, v in rt.items()) + ")")
 
     # --------------------------------------------------------------- loop
     def run(self, jobs: list[Job], time_limit: float | None = None, drain: bool = True,

@@ -113,3 +113,15 @@ def test_time_limit_stops_with_backlog(fake):
     srv = serving.Server(budget_bytes=10**9, min_repack_s=0.0)
     out = srv.run(jobs, time_limit=1.0, clock=Clock())
     assert out["jobs_done"] == 0 and out["jobs_running"] == 1 and out["jobs_waiting"] == 1
+
+
+def test_footprint_on_meta_device():
+    """Admission sizes come from a meta-device build: no device memory and
+    no GPU needed, for both schemes, growing with the particle count."""
+    from pytorchsph_amd import scenes as S
+    from pytorchsph_amd.engine import SceneBatch
+    for scheme in ("dsph", "dfsph"):
+        small = SceneBatch.footprint([S.scene_for_count("dambreak", scheme, 2000, 1)])
+        large = SceneBatch.footprint([S.scene_for_count("dambreak", scheme, 8000, 1)])
+        assert 0 < small < large
+        assert 200 * 8000 < large < 4000 * 8000
