@@ -218,30 +218,34 @@ __device__ __forceinline__ float2 pressure_accel_t(const SphCSR& ff, const SphCS
   const float4 ri = ld4(d.xrec, i);
   float ax = 0.f, ay = 0.f;
   const int k0 = ff.rowptr[i], k1 = ff.rowptr[i + 1];
-  // kPairBatch pairs per lane per trip: their loads are issued together and
-  // unconditionally (a slot past the row reads pair k0 again), then the
-  // pairs are added in order (the order of the one-pair loop)
-  for (int kb = k0 + sub; kb < k1; kb += KL * kPairBatch) {
-    bool ok[kPairBatch];
-    int jj[kPairBatch];
-    float4 rj[kPairBatch];
+  // kPairBatch pairs per lane per trip, loads issued unconditionally (a slot
+  // past the row reads pair k0 again) and one batch ahead of the pairs being
+  // evaluated; the pairs are added in order (the order of the one-pair loop)
+  float4 rj[kPairBatch], rn[kPairBatch];
+  int kb = k0 + sub;
+#pragma unroll
+  for (int u = 0; u < kPairBatch; ++u) {
+    const int k = kb + u * KL;
+    rj[u] = ld4(d.xrec, wave_col<STAGED>(ff.col, w, wcols, k < k1 ? k : k0));
+  }
+  for (; kb < k1; kb += KL * kPairBatch) {
+    const int kn = kb + KL * kPairBatch;
 #pragma unroll
     for (int u = 0; u < kPairBatch; ++u) {
-      const int k = kb + u * KL;
-      ok[u] = k < k1;
-      jj[u] = wave_col<STAGED>(ff.col, w, wcols, ok[u] ? k : k0);
+      const int k = kn + u * KL;
+      rn[u] = ld4(d.xrec, wave_col<STAGED>(ff.col, w, wcols, k < k1 ? k : k0));
     }
 #pragma unroll
-    for (int u = 0; u < kPairBatch; ++u) rj[u] = ld4(d.xrec, jj[u]);
-#pragma unroll
     for (int u = 0; u < kPairBatch; ++u) {
-      if (ok[u]) {
+      if (kb + u * KL < k1) {
         const float2 G = spiky_G(ri.x, ri.y, rj[u].x, rj[u].y, K);
         const float c = -rj[u].w * (ri.z + rj[u].z);
         ax += c * G.x;
         ay += c * G.y;
       }
     }
+#pragma unroll
+    for (int u = 0; u < kPairBatch; ++u) rj[u] = rn[u];
   }
   if (fb.rowptr) {
     const int e1 = fb.rowptr[i + 1];
@@ -363,25 +367,31 @@ __device__ __forceinline__ double pressure_fluid_row(const SphFluid& f, const Sp
                                                      const int* wcols, bool* wrote) {
   const float2 xi = ld2(d.xrec, 2 * i);          // {x, y} of xrec[i]
   float ks = 0.f;
-  for (int kb = k0 + sub; kb < k1; kb += KL * kPairBatch) {
-    bool ok[kPairBatch];
-    int jj[kPairBatch];
-    float4 pj[kPairBatch];
+  // software pipeline: the records of batch b + 1 are requested before the
+  // pairs of batch b are evaluated
+  float4 pj[kPairBatch], pq[kPairBatch];
+  int kb = k0 + sub;
+#pragma unroll
+  for (int u = 0; u < kPairBatch; ++u) {
+    const int k = kb + u * KL;
+    pj[u] = ld4(d.prec, wave_col<STAGED>(ff.col, w, wcols, k < k1 ? k : k0));
+  }
+  for (; kb < k1; kb += KL * kPairBatch) {
+    const int kn = kb + KL * kPairBatch;
 #pragma unroll
     for (int u = 0; u < kPairBatch; ++u) {
-      const int k = kb + u * KL;
-      ok[u] = k < k1;
-      jj[u] = wave_col<STAGED>(ff.col, w, wcols, ok[u] ? k : k0);
+      const int k = kn + u * KL;
+      pq[u] = ld4(d.prec, wave_col<STAGED>(ff.col, w, wcols, k < k1 ? k : k0));
     }
 #pragma unroll
-    for (int u = 0; u < kPairBatch; ++u) pj[u] = ld4(d.prec, jj[u]);
-#pragma unroll
     for (int u = 0; u < kPairBatch; ++u) {
-      if (ok[u]) {
+      if (kb + u * KL < k1) {
         const float2 G = spiky_G(xi.x, xi.y, pj[u].x, pj[u].y, K);
         ks += pj[u].z * G.x + pj[u].w * G.y;
       }
     }
+#pragma unroll
+    for (int u = 0; u < kPairBatch; ++u) pj[u] = pq[u];
   }
   ks = group_sum<KL>(ks);
   *wrote = false;
