@@ -14,28 +14,50 @@ namespace {
 
 constexpr int kBS = 256;
 
+// Cell key and a slot inside the cell.  Points in cell order (the engine
+// keeps its state that way) put runs of equal keys in neighbouring lanes:
+// each run takes its slots with one atomic, issued by the run's first lane,
+// and its lanes take consecutive slots.  (Which points fill which slots of
+// a cell still depends on the order the atomics land; k_grid_cell_sort
+// makes it deterministic.)
 __global__ __launch_bounds__(kBS) void k_grid_keys(SphGrid g) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= g.np) return;
-  const int s = g.pscene[p];
-  const SphGridDesc d = g.scenes[s];
-  const float x = g.x[2 * p], y = g.x[2 * p + 1];
-  const int ix = cell_coord(x, d.qmin_x, d.hmax);
-  const int iy = cell_coord(y, d.qmin_y, d.hmax);
-  // reference key: ix + cellsX * iy in int32 (neighborSearch.py:47)
-  const int64_t key = (int64_t)ix + (int64_t)d.cells_x * (int64_t)iy;
-  const int64_t ncell = (int64_t)d.cells_x * d.cells_y;
-  int k;
-  if (key >= 0 && key < ncell) {
-    k = (int)key;
-  } else {
-    // the scene's last slot collects points outside its table; stencils
-    // never reach it, and the scene stays contiguous in sorted order
-    k = (int)ncell;
-    atomicOr(&g.status[s], SPH_ST_OUTSIDE);
+  const int lane = threadIdx.x & (kWave - 1);
+  const bool valid = p < g.np;
+  int kg = -1 - lane;          // distinct negative keys for lanes past the end
+  int s = 0, k = 0;
+  if (valid) {
+    s = g.pscene[p];
+    const SphGridDesc d = g.scenes[s];
+    const float x = g.x[2 * p], y = g.x[2 * p + 1];
+    const int ix = cell_coord(x, d.qmin_x, d.hmax);
+    const int iy = cell_coord(y, d.qmin_y, d.hmax);
+    // reference key: ix + cellsX * iy in int32 (neighborSearch.py:47)
+    const int64_t key = (int64_t)ix + (int64_t)d.cells_x * (int64_t)iy;
+    const int64_t ncell = (int64_t)d.cells_x * d.cells_y;
+    if (key >= 0 && key < ncell) {
+      k = (int)key;
+    } else {
+      // the scene's last slot collects points outside its table; stencils
+      // never reach it, and the scene stays contiguous in sorted order
+      k = (int)ncell;
+      atomicOr(&g.status[s], SPH_ST_OUTSIDE);
+    }
+    kg = d.cell_begin + k;     // global cell slot
+    g.key[p] = k;
   }
-  g.key[p] = k;
-  g.slot[p] = atomicAdd(&g.cell_count[d.cell_begin + k], 1);
+  // runs of equal global cells among neighbouring lanes
+  const int prev = __shfl_up(kg, 1, kWave);
+  const bool head = lane == 0 || prev != kg;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  const int start = 63 - __clzll(heads & upto);              // first lane of my run
+  const unsigned long long after = heads & ~upto;
+  const int end = after ? __ffsll((long long)after) - 1 : kWave;  // first lane of the next run
+  int base = 0;
+  if (valid && head) base = atomicAdd(&g.cell_count[kg], end - start);
+  base = __shfl(base, start, kWave);
+  if (valid) g.slot[p] = base + (lane - start);
 }
 
 __global__ __launch_bounds__(kBS) void k_grid_scatter(SphGrid g) {
@@ -48,20 +70,41 @@ __global__ __launch_bounds__(kBS) void k_grid_scatter(SphGrid g) {
 
 // One thread per cell: order the cell's points by index (atomics gave them
 // in arbitrary order) and copy their positions into sorted order.
+constexpr int kRegSort = 16;   // cells up to this size are sorted in registers
+
 __global__ __launch_bounds__(kBS) void k_grid_cell_sort(SphGrid g) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.ncells) return;
   const int a = g.cell_start[c];
   const int b = g.cell_start[c + 1];
   int* idx = g.sorted_idx;
-  for (int i = a + 1; i < b; ++i) {
-    const int v = idx[i];
-    int j = i - 1;
-    while (j >= a && idx[j] > v) {
-      idx[j + 1] = idx[j];
-      --j;
+  if (b - a <= kRegSort) {
+    // sorting network over a register array (unused entries hold INT_MAX)
+    int v[kRegSort];
+#pragma unroll
+    for (int u = 0; u < kRegSort; ++u) v[u] = a + u < b ? idx[a + u] : 0x7fffffff;
+#pragma unroll
+    for (int i = 1; i < kRegSort; ++i) {
+#pragma unroll
+      for (int j = i; j > 0; --j) {
+        const int lo = min(v[j - 1], v[j]), hi = max(v[j - 1], v[j]);
+        v[j - 1] = lo;
+        v[j] = hi;
+      }
     }
-    idx[j + 1] = v;
+#pragma unroll
+    for (int u = 0; u < kRegSort; ++u)
+      if (a + u < b) idx[a + u] = v[u];
+  } else {
+    for (int i = a + 1; i < b; ++i) {
+      const int v = idx[i];
+      int j = i - 1;
+      while (j >= a && idx[j] > v) {
+        idx[j + 1] = idx[j];
+        --j;
+      }
+      idx[j + 1] = v;
+    }
   }
   const float2* x = reinterpret_cast<const float2*>(g.x);
   float2* sx = reinterpret_cast<float2*>(g.sorted_x);
