@@ -42,8 +42,8 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="arrival window of the serving run (0 = skip)")
     ap.add_argument("--tuning", default=None,
-                    help="fixed launch shapes instead of autotuning: "
-                         "search,gather,solve lanes per row and the DFSPH grid cap, e.g. 2,4,2,16384")
+                    help="fixed launch shapes instead of autotuning: count,gather,solve[,fill] "
+                         "lanes per row and the DFSPH grid cap, e.g. 2,4,2,8,16384")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the kernel timings and the copy calibration (counter runs)")
     return ap.parse_args()
@@ -350,8 +350,11 @@ def main():
     batch = MixedBatch(scenes)
     tuned = None
     if args.tuning:
-        sv, gv, vv, cap = (int(t) for t in args.tuning.split(","))
-        L.set_tuning(search=sv, gather=gv, solve=vv)
+        vals = [int(t) for t in args.tuning.split(",")]
+        if len(vals) not in (4, 5):
+            raise SystemExit("--tuning takes count,gather,solve[,fill],cap")
+        L.set_tuning(**dict(zip(("search", "gather", "solve", "fill"), vals[:-1])))
+        cap = vals[-1]
         L.DFSPH_GRID_CAP = cap
         tuned = {"choice": dict(L.get_tuning(), grid_cap=cap)}
         args.no_tune = True

@@ -191,10 +191,11 @@ int sph_fill_f32(float* p, int n, float value, hipStream_t s);
  * sums with shuffles.  Pair sets do not depend on them; sums are re-associated
  * across the lanes.  No reference counterpart: the reference's gathers are
  * torch scatter_add over the pair list (util.py:11-40). */
-#define SPH_TUNE_SEARCH_LANES 0   /* sph_neighbor_count / sph_neighbor_fill   */
+#define SPH_TUNE_SEARCH_LANES 0   /* sph_neighbor_count                       */
 #define SPH_TUNE_GATHER_LANES 1   /* density and delta-SPH gathers            */
 #define SPH_TUNE_SOLVE_LANES  2   /* DFSPH alpha/source, iterate and finish   */
-#define SPH_TUNE_COUNT        3
+#define SPH_TUNE_FILL_LANES   3   /* sph_neighbor_fill                        */
+#define SPH_TUNE_COUNT        4
 int sph_set_tuning(int knob, int value);
 int sph_get_tuning(int knob);
 

@@ -85,9 +85,10 @@ ST_OUTSIDE, ST_OVERFLOW, ST_NAN, ST_ITERCAP = 1, 2, 4, 8
 T_CONT, T_CONT_B, T_MOG, T_DDSPH = 1 << 0, 1 << 1, 1 << 2, 1 << 3
 T_PRESS, T_PRESS_B, T_VISC, T_VISC_B = 1 << 4, 1 << 5, 1 << 6, 1 << 7
 T_LAM, T_GRAV, T_ACCUM = 1 << 8, 1 << 9, 1 << 16
-TUNE_SEARCH_LANES, TUNE_GATHER_LANES, TUNE_SOLVE_LANES = 0, 1, 2
+TUNE_SEARCH_LANES, TUNE_GATHER_LANES, TUNE_SOLVE_LANES, TUNE_FILL_LANES = 0, 1, 2, 3
 DFSPH_ERR_SLOTS = 32
-TUNE_KNOBS = {"search": TUNE_SEARCH_LANES, "gather": TUNE_GATHER_LANES, "solve": TUNE_SOLVE_LANES}
+TUNE_KNOBS = {"search": TUNE_SEARCH_LANES, "gather": TUNE_GATHER_LANES, "solve": TUNE_SOLVE_LANES,
+              "fill": TUNE_FILL_LANES}
 
 # exported entry points: name -> argtypes (restype is int unless noted)
 _P = C.POINTER
@@ -177,7 +178,7 @@ def call(name: str, *args) -> None:
 
 
 def set_tuning(**lanes: int) -> dict:
-    """Sets lanes-per-row knobs by name (search=, gather=, solve=); returns
+    """Sets lanes-per-row knobs by name (search=, gather=, solve=, fill=); returns
     the previous values.  Host-side only: no device is touched."""
     old = get_tuning()
     for name, v in lanes.items():

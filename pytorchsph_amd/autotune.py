@@ -1,7 +1,7 @@
 """Launch-shape autotuner of the batched step.
 
-The knobs are the lanes per row of the neighbour search, of the kernel-sum
-gathers and of the DFSPH solve (sph_set_tuning), and the workgroup cap of
+The knobs are the lanes per row of the neighbour count and fill, of the
+kernel-sum gathers and of the DFSPH solve (sph_set_tuning), and the workgroup cap of
 the DFSPH iteration kernels.  Each knob is timed on the batch it will run
 (same scenes, same pair lists), one knob at a time with the others held,
 and the fastest value is kept.  Pair sets do not depend on any knob; sums
@@ -83,6 +83,7 @@ def tune(parts, reps: int = 5, log=None) -> dict:
     torch.cuda.synchronize()
     plans = {
         "search": [_search_fn(b) for b in parts],
+        "fill": [_search_fn(b) for b in parts],
         "gather": [_gather_fn(b) for b in parts],
         "solve": [_solve_fn(b) for b in dfsph],
     }

@@ -71,6 +71,7 @@ __global__ __launch_bounds__(kBS) void k_grid_scatter(SphGrid g) {
 // One thread per cell: order the cell's points by index (atomics gave them
 // in arbitrary order) and copy their positions into sorted order.
 constexpr int kRegSort = 16;   // cells up to this size are sorted in registers
+constexpr int kCandBatch = 4;  // candidates each lane loads at once in a search pass
 
 __global__ __launch_bounds__(kBS) void k_grid_cell_sort(SphGrid g) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -111,17 +112,69 @@ __global__ __launch_bounds__(kBS) void k_grid_cell_sort(SphGrid g) {
   for (int i = a; i < b; ++i) sx[i] = x[idx[i]];
 }
 
+// Candidate range [a, b) of sorted points in stencil row dy of a query in
+// cell (ix, iy): cells qx in [max(ix - R, 0), ix + R] of row qy = iy + dy,
+// skipped when qy < 0 (queryHashMap's negative-index rule,
+// neighSearch.cpp:95), keys past the scene's table clipped.  Returns false
+// when the row, and every later one, lies past the table.
+__device__ __forceinline__ bool stencil_row(const SphGridDesc& d, const int* __restrict__ cs,
+                                            int ix, int iy, int dy, int R, int& a, int& b) {
+  a = b = 0;
+  const int qy = iy + dy;
+  if (qy < 0) return true;
+  const int qx0 = ix - R < 0 ? 0 : ix - R;
+  const int qx1 = ix + R;
+  if (qx1 < qx0) return true;
+  const int64_t ncell = (int64_t)d.cells_x * d.cells_y;
+  const int64_t k0 = qx0 + (int64_t)d.cells_x * qy;
+  int64_t k1 = qx1 + (int64_t)d.cells_x * qy;
+  if (k0 >= ncell) return false;
+  if (k1 >= ncell) k1 = ncell - 1;
+  a = cs[k0];
+  b = cs[k1 + 1];
+  return true;
+}
+
+// The radius-1 stencil of a query as one sequence of candidates: rows
+// dy = -1, 0, 1 are three contiguous runs of sorted points, and candidate p
+// is the p-th point of their concatenation.
+struct Stencil3 {
+  int ra[3], rb[3];
+  int n0, n01, ncand;
+  __device__ __forceinline__ void init(const SphGridDesc& d, const int* __restrict__ cs, int ix,
+                                       int iy) {
+    bool more = true;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      if (more) more = stencil_row(d, cs, ix, iy, r - 1, 1, ra[r], rb[r]);
+      if (!more) ra[r] = rb[r] = 0;
+    }
+    n0 = rb[0] - ra[0];
+    n01 = n0 + rb[1] - ra[1];
+    ncand = n01 + rb[2] - ra[2];
+  }
+  // Sorted slot of candidate p, p clamped into [0, ncand) (ncand > 0) so a
+  // batch of loads needs no guard.
+  __device__ __forceinline__ int slot(int p) const {
+    p = p < ncand ? p : ncand - 1;
+    return p < n0 ? ra[0] + p : (p < n01 ? ra[1] + (p - n0) : ra[2] + (p - n01));
+  }
+};
+
 // Count (FILL = false) or emit (FILL = true) the pairs of every query.
 // Stencil rows are walked bottom to top and each row is one contiguous range
-// [cell_start[k0], cell_start[k1 + 1]) of sorted points, so the columns of a
-// row come out in sorted-point order.  Cells with qx < 0 or qy < 0 are
-// skipped as in queryHashMap (neighSearch.cpp:95); keys past the scene's
-// table are clipped.
+// of sorted points, so the columns of a row come out in sorted-point order.
 //
-// KL lanes share one query: they test consecutive candidates of a range
-// together (coalesced reads of the sorted positions) and place the hits with
-// a ballot and a popcount of the lower lanes, so the row is written in
-// candidate order with coalesced stores.
+// KL lanes share one query: lane `sub` takes candidates p = p0 + u KL + sub,
+// u < kCandBatch, so the group reads consecutive sorted points (coalesced)
+// and every lane has kCandBatch independent loads in flight before its first
+// test.  Hits are placed with a ballot and a popcount of the lower lanes, so
+// the row is written in candidate order.  Wider stencils (radius > 1) walk
+// their rows one load at a time.
+//
+// Self-sorted queries read their position from the sorted copy (coalesced,
+// no wait on the index load), and a one-scene grid needs no scene lookup, so
+// the first stencil loads wait on one load only.
 template <bool FILL, int KL>
 __global__ __launch_bounds__(kBS) void k_neighbors(SphGrid g, SphQuery q,
                                                    const int* __restrict__ rowptr,
@@ -131,44 +184,64 @@ __global__ __launch_bounds__(kBS) void k_neighbors(SphGrid g, SphQuery q,
   const int sub = (int)(gt % KL);
   if (t >= q.nq) return;   // whole groups leave together
   const int qi = q.self_sorted ? g.sorted_idx[t] : t;
-  const int s = q.qscene[qi];
+  const int s = g.nscenes == 1 ? 0 : q.qscene[qi];
   const SphGridDesc d = g.scenes[s];
-  const float xq = q.x[2 * qi], yq = q.x[2 * qi + 1];
-  const float thr = sq_threshold(q.h[qi]);
+  const float* __restrict__ xp = q.self_sorted ? g.sorted_x + 2 * (int64_t)t : q.x + 2 * (int64_t)qi;
+  const float xq = xp[0], yq = xp[1];
   const int ix = cell_coord(xq, d.qmin_x, d.hmax);
   const int iy = cell_coord(yq, d.qmin_y, d.hmax);
-  const int64_t ncell = (int64_t)d.cells_x * d.cells_y;
   const int R = q.radius;
-  const float2* __restrict__ sx = reinterpret_cast<const float2*>(g.sorted_x);
   const int* __restrict__ cs = g.cell_start + d.cell_begin;
+  const float2* __restrict__ sx = reinterpret_cast<const float2*>(g.sorted_x);
   const unsigned long long gmask = group_mask<KL>();
   int64_t base = 0;
   if (FILL) base = rowptr[qi];
-  int cnt = 0;   // identical in every lane of the group
-  for (int dy = -R; dy <= R; ++dy) {
-    const int qy = iy + dy;
-    if (qy < 0) continue;
-    const int qx0 = ix - R < 0 ? 0 : ix - R;
-    const int qx1 = ix + R;
-    if (qx1 < qx0) continue;
-    const int64_t k0 = qx0 + (int64_t)d.cells_x * qy;
-    int64_t k1 = qx1 + (int64_t)d.cells_x * qy;
-    if (k0 >= ncell) break;
-    if (k1 >= ncell) k1 = ncell - 1;
-    const int a = cs[k0], b = cs[k1 + 1];
-    for (int c0 = a; c0 < b; c0 += KL) {
-      const int j = c0 + sub;
-      bool hit = false;
-      if (j < b) {
-        const float2 xj = sx[j];
-        hit = dist2_nofma(xq, yq, xj.x, xj.y) <= thr;
+  int cnt = 0;   // hits so far, identical in every lane of the group
+  if (R == 1) {
+    Stencil3 st;
+    st.init(d, cs, ix, iy);
+    const int ncand = st.ncand;
+    const float thr = sq_threshold(q.h[qi]);
+    for (int p0 = 0; p0 < ncand; p0 += KL * kCandBatch) {
+      float2 xj[kCandBatch];
+      int idx[kCandBatch];
+#pragma unroll
+      for (int u = 0; u < kCandBatch; ++u) {
+        const int j = st.slot(p0 + u * KL + sub);
+        xj[u] = sx[j];
+        if (FILL) idx[u] = g.sorted_idx[j];
       }
-      const unsigned long long m = __ballot(hit) & gmask;
-      if (FILL && hit) {
-        const int64_t pos = base + cnt + __popcll(m & lanes_below());
-        if (pos < cap) out[pos] = g.sorted_idx[j];
+#pragma unroll
+      for (int u = 0; u < kCandBatch; ++u) {
+        const int p = p0 + u * KL + sub;
+        const bool hit = p < ncand && dist2_nofma(xq, yq, xj[u].x, xj[u].y) <= thr;
+        const unsigned long long m = __ballot(hit) & gmask;
+        if (FILL && hit) {
+          const int64_t pos = base + cnt + __popcll(m & lanes_below());
+          if (pos < cap) out[pos] = idx[u];
+        }
+        cnt += __popcll(m);
       }
-      cnt += __popcll(m);
+    }
+  } else {
+    const float thr = sq_threshold(q.h[qi]);
+    for (int dy = -R; dy <= R; ++dy) {
+      int a, b;
+      if (!stencil_row(d, cs, ix, iy, dy, R, a, b)) break;
+      for (int c0 = a; c0 < b; c0 += KL) {
+        const int j = c0 + sub;
+        bool hit = false;
+        if (j < b) {
+          const float2 xj = sx[j];
+          hit = dist2_nofma(xq, yq, xj.x, xj.y) <= thr;
+        }
+        const unsigned long long m = __ballot(hit) & gmask;
+        if (FILL && hit) {
+          const int64_t pos = base + cnt + __popcll(m & lanes_below());
+          if (pos < cap) out[pos] = g.sorted_idx[j];
+        }
+        cnt += __popcll(m);
+      }
     }
   }
   if (sub == 0) {
@@ -180,8 +253,6 @@ __global__ __launch_bounds__(kBS) void k_neighbors(SphGrid g, SphQuery q,
   }
 }
 
-// Clamps row pointers to the column capacity so that every consumer of a
-// truncated list stays inside the column buffer.
 __global__ __launch_bounds__(kBS) void k_clamp_rowptr(int* __restrict__ rowptr, int n, int64_t cap) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -308,7 +379,7 @@ int neighbor_fill(const SphGrid* g, const SphQuery* q, int* rowptr, int* col, in
   if (rc) return rc;
   SPH_REQUIRE(rowptr && (col || cap == 0) && cap >= 0, "sph_neighbor_fill: bad output");
   if (q->nq == 0) return SPH_OK;
-  with_lanes(tuning(SPH_TUNE_SEARCH_LANES), [&](auto kl) {
+  with_lanes(tuning(SPH_TUNE_FILL_LANES), [&](auto kl) {
     constexpr int KL = decltype(kl)::value;
     hipLaunchKernelGGL((k_neighbors<true, KL>), dim3(blocks_for((int64_t)q->nq * KL, kBS)),
                        dim3(kBS), 0, s, *g, *q, rowptr, col, cap);

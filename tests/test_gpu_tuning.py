@@ -32,7 +32,7 @@ def test_search_rows_identical_across_lanes(cuda):
     h = np.full(len(x), hmax, np.float32)
     ref = None
     for kl in LANES:
-        with tuned(search=kl):
+        with tuned(search=kl, fill=kl):
             rp, c, st, _ = gpu_search(x, h, (0, 0), (1, 1), hmax)
         assert st == 0
         if ref is None:
