@@ -39,6 +39,37 @@ __device__ __forceinline__ void st4(float* p, int i, float4 v) {
   reinterpret_cast<float4*>(p)[i] = v;
 }
 
+// Calls f(rec[col(k)]) for this lane's pairs k = k0 + sub, k0 + sub + KL,
+// ... < k1, in that order (the order of the one-pair loop), with kPairBatch
+// record loads in flight: loads are issued unconditionally (a slot past the
+// row reads pair k0 again) and one batch ahead of the pairs being evaluated.
+// An empty row reads nothing.  k0, k1 are the same in every lane of a group.
+template <int KL, typename Col, typename F>
+__device__ __forceinline__ void pair_pipeline(int k0, int k1, int sub, const float* rec, Col col,
+                                              F f) {
+  if (k0 >= k1) return;
+  float4 rj[kPairBatch], rn[kPairBatch];
+  int kb = k0 + sub;
+#pragma unroll
+  for (int u = 0; u < kPairBatch; ++u) {
+    const int k = kb + u * KL;
+    rj[u] = ld4(rec, col(k < k1 ? k : k0));
+  }
+  for (; kb < k1; kb += KL * kPairBatch) {
+    const int kn = kb + KL * kPairBatch;
+#pragma unroll
+    for (int u = 0; u < kPairBatch; ++u) {
+      const int k = kn + u * KL;
+      rn[u] = ld4(rec, col(k < k1 ? k : k0));
+    }
+#pragma unroll
+    for (int u = 0; u < kPairBatch; ++u)
+      if (kb + u * KL < k1) f(rj[u]);
+#pragma unroll
+    for (int u = 0; u < kPairBatch; ++u) rj[u] = rn[u];
+  }
+}
+
 __device__ __forceinline__ float2 spiky_G(float xi, float yi, float xj, float yj, const KConst& K) {
   const PairGeo g = pair_geo(xi, yi, xj, yj, K.inv_h);
   const float s = -spiky_grad_k(g.q, K);
@@ -217,46 +248,17 @@ __device__ __forceinline__ float2 pressure_accel_t(const SphCSR& ff, const SphCS
                                                    int sub, const WaveCols& w, const int* wcols) {
   const float4 ri = ld4(d.xrec, i);
   float ax = 0.f, ay = 0.f;
-  const int k0 = ff.rowptr[i], k1 = ff.rowptr[i + 1];
-  // kPairBatch pairs per lane per trip, loads issued unconditionally (a slot
-  // past the row reads pair k0 again) and one batch ahead of the pairs being
-  // evaluated; the pairs are added in order (the order of the one-pair loop)
-  float4 rj[kPairBatch], rn[kPairBatch];
-  int kb = k0 + sub;
-#pragma unroll
-  for (int u = 0; u < kPairBatch; ++u) {
-    const int k = kb + u * KL;
-    rj[u] = ld4(d.xrec, wave_col<STAGED>(ff.col, w, wcols, k < k1 ? k : k0));
-  }
-  for (; kb < k1; kb += KL * kPairBatch) {
-    const int kn = kb + KL * kPairBatch;
-#pragma unroll
-    for (int u = 0; u < kPairBatch; ++u) {
-      const int k = kn + u * KL;
-      rn[u] = ld4(d.xrec, wave_col<STAGED>(ff.col, w, wcols, k < k1 ? k : k0));
-    }
-#pragma unroll
-    for (int u = 0; u < kPairBatch; ++u) {
-      if (kb + u * KL < k1) {
-        const float2 G = spiky_G(ri.x, ri.y, rj[u].x, rj[u].y, K);
-        const float c = -rj[u].w * (ri.z + rj[u].z);
-        ax += c * G.x;
-        ay += c * G.y;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kPairBatch; ++u) rj[u] = rn[u];
-  }
-  if (fb.rowptr) {
-    const int e1 = fb.rowptr[i + 1];
-    for (int k = fb.rowptr[i] + sub; k < e1; k += KL) {
-      const float4 rb = ld4(d.bxrec, fb.col[k]);
-      const float2 G = spiky_G(ri.x, ri.y, rb.x, rb.y, K);
-      const float c = -rb.w * (ri.z + rb.z);
-      ax += c * G.x;
-      ay += c * G.y;
-    }
-  }
+  auto add = [&](const float4& rj) {
+    const float2 G = spiky_G(ri.x, ri.y, rj.x, rj.y, K);
+    const float c = -rj.w * (ri.z + rj.z);
+    ax += c * G.x;
+    ay += c * G.y;
+  };
+  pair_pipeline<KL>(ff.rowptr[i], ff.rowptr[i + 1], sub, d.xrec,
+                    [&](int k) { return wave_col<STAGED>(ff.col, w, wcols, k); }, add);
+  if (fb.rowptr)
+    pair_pipeline<KL>(fb.rowptr[i], fb.rowptr[i + 1], sub, d.bxrec,
+                      [&](int k) { return fb.col[k]; }, add);
   return make_float2(group_sum<KL>(ax), group_sum<KL>(ay));
 }
 
@@ -367,32 +369,11 @@ __device__ __forceinline__ double pressure_fluid_row(const SphFluid& f, const Sp
                                                      const int* wcols, bool* wrote) {
   const float2 xi = ld2(d.xrec, 2 * i);          // {x, y} of xrec[i]
   float ks = 0.f;
-  // software pipeline: the records of batch b + 1 are requested before the
-  // pairs of batch b are evaluated
-  float4 pj[kPairBatch], pq[kPairBatch];
-  int kb = k0 + sub;
-#pragma unroll
-  for (int u = 0; u < kPairBatch; ++u) {
-    const int k = kb + u * KL;
-    pj[u] = ld4(d.prec, wave_col<STAGED>(ff.col, w, wcols, k < k1 ? k : k0));
-  }
-  for (; kb < k1; kb += KL * kPairBatch) {
-    const int kn = kb + KL * kPairBatch;
-#pragma unroll
-    for (int u = 0; u < kPairBatch; ++u) {
-      const int k = kn + u * KL;
-      pq[u] = ld4(d.prec, wave_col<STAGED>(ff.col, w, wcols, k < k1 ? k : k0));
-    }
-#pragma unroll
-    for (int u = 0; u < kPairBatch; ++u) {
-      if (kb + u * KL < k1) {
-        const float2 G = spiky_G(xi.x, xi.y, pj[u].x, pj[u].y, K);
-        ks += pj[u].z * G.x + pj[u].w * G.y;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kPairBatch; ++u) pj[u] = pq[u];
-  }
+  pair_pipeline<KL>(k0, k1, sub, d.prec, [&](int k) { return wave_col<STAGED>(ff.col, w, wcols, k); },
+                    [&](const float4& pj) {
+                      const float2 G = spiky_G(xi.x, xi.y, pj.x, pj.y, K);
+                      ks += pj.z * G.x + pj.w * G.y;
+                    });
   ks = group_sum<KL>(ks);
   *wrote = false;
   if (sub != 0) return 0.0;
@@ -475,13 +456,12 @@ __global__ __launch_bounds__(kBS) void k_pressure(SphFluid f, const SphPhysics* 
       const KConst K = kconst(P.h);
       const float2 xb = ld2(d.bxrec, 2 * k0);
       float ks = 0.f;
-      const int e1 = bf.rowptr[k0 + 1];
       // sum_f w_f a*_f . (-G_bf) with {x_f, y_f, w_f a*_f} from prec
-      for (int k = bf.rowptr[k0] + sub; k < e1; k += KL) {
-        const float4 pf = ld4(d.prec, bf.col[k]);
-        const float2 G = spiky_G(xb.x, xb.y, pf.x, pf.y, K);
-        ks -= pf.z * G.x + pf.w * G.y;
-      }
+      pair_pipeline<KL>(bf.rowptr[k0], bf.rowptr[k0 + 1], sub, d.prec,
+                        [&](int k) { return bf.col[k]; }, [&](const float4& pf) {
+                          const float2 G = spiky_G(xb.x, xb.y, pf.x, pf.y, K);
+                          ks -= pf.z * G.x + pf.w * G.y;
+                        });
       ks = group_sum<KL>(ks);
       if (sub == 0) {
         const float rr = ks - d.b_src[k0];
