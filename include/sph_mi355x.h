@@ -301,9 +301,10 @@ typedef struct SphDfsph {
   float* b_p;    /* [nb] boundary pressure                            */
   float* b_resid;/* [nb]                                              */
   float* bxrec;  /* [nb][4] {x, y, p_b / (rho_b rho0_b)^2, V_b rho0_b} */
-  double* err_acc;  /* [SPH_DFSPH_ERR_SLOTS][nscenes] FP64 partial sums
-                       of clamped residuals; workgroups spread their
-                       atomics over the slots                          */
+  double* err_acc;  /* [SPH_DFSPH_ERR_SLOTS][SPH_DFSPH_ERR_STRIDE(nscenes)]
+                       FP64 partial sums of clamped residuals (scene s
+                       of slot k at k * stride + s); workgroups spread
+                       their atomics over the slots                    */
   double* err_hist; /* [nscenes][max_hist] per-iteration errors        */
   int* iters;       /* [nscenes] iterations done                       */
   int* active;      /* [nscenes] 1 while the scene still iterates      */
@@ -321,7 +322,10 @@ typedef struct SphDfsph {
   int grid_blocks;  /* grid of the iteration kernels (work is strided) */
 } SphDfsph;
 
-#define SPH_DFSPH_ERR_SLOTS 32
+#define SPH_DFSPH_ERR_SLOTS 64
+/* Row stride of err_acc in doubles: a multiple of 16 (128 bytes), so no two
+ * slots share a cache line even for one scene. */
+#define SPH_DFSPH_ERR_STRIDE(nscenes) ((((nscenes) + 15) / 16) * 16)
 
 /* v* = v + dt a, A^act = A / rho; boundary A^act_b = V_b / rho_b, p_b = 0;
  * pressures zeroed; iteration counters reset; every scene active.  The

@@ -86,7 +86,12 @@ T_CONT, T_CONT_B, T_MOG, T_DDSPH = 1 << 0, 1 << 1, 1 << 2, 1 << 3
 T_PRESS, T_PRESS_B, T_VISC, T_VISC_B = 1 << 4, 1 << 5, 1 << 6, 1 << 7
 T_LAM, T_GRAV, T_ACCUM = 1 << 8, 1 << 9, 1 << 16
 TUNE_SEARCH_LANES, TUNE_GATHER_LANES, TUNE_SOLVE_LANES, TUNE_FILL_LANES = 0, 1, 2, 3
-DFSPH_ERR_SLOTS = 32
+DFSPH_ERR_SLOTS = 64
+
+
+def dfsph_err_stride(nscenes: int) -> int:
+    """Row stride (doubles) of SphDfsph.err_acc: SPH_DFSPH_ERR_STRIDE."""
+    return (nscenes + 15) // 16 * 16
 TUNE_KNOBS = {"search": TUNE_SEARCH_LANES, "gather": TUNE_GATHER_LANES, "solve": TUNE_SOLVE_LANES,
               "fill": TUNE_FILL_LANES}
 

@@ -121,7 +121,7 @@ __global__ __launch_bounds__(kBS) void k_prepare(SphFluid f, const float* __rest
     const float2 x = ld2(b.x, k);
     st4(d.bxrec, k, make_float4(x.x, x.y, 0.f, b.vol[k] * b.rest[k]));
   }
-  if (t < SPH_DFSPH_ERR_SLOTS * d.nscenes) d.err_acc[t] = 0.0;
+  if (t < SPH_DFSPH_ERR_SLOTS * SPH_DFSPH_ERR_STRIDE(d.nscenes)) d.err_acc[t] = 0.0;
   if (t < d.nscenes) {
     d.iters[t] = 0;
     d.active[t] = 1;
@@ -337,20 +337,47 @@ __global__ __launch_bounds__(kBS) void k_accel(SphFluid f, const SphPhysics* __r
   }
 }
 
-// Adds v (FP64) into acc[scene], one atomic per wavefront when the whole
-// wavefront belongs to one scene.  `acc` is the workgroup's slot row of the
-// partial sums, so concurrent atomics on one scene spread over the slots.
-__device__ __forceinline__ void scene_sum(double v, int s, bool valid, double* acc) {
+// Adds v (FP64) into acc[scene] with as few atomics as the scene layout
+// allows: one per workgroup when all its valid items belong to one scene,
+// else one per wavefront whose valid items share a scene, else one per
+// item.  Atomics on one address serialise, so per-wavefront atomics of a
+// large single scene would queue tens of thousands deep.  `acc` is the
+// workgroup's slot row.  Every thread of the workgroup calls this the same
+// number of times (it holds two workgroup barriers).
+__device__ __forceinline__ void scene_sum(double v, int s, bool valid, double* acc, double* wsum,
+                                          int* wscene) {
   const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
   const int s0 = __shfl(valid ? s : -1, 0, kWave);
-  const unsigned long long ball = __ballot(valid && s == s0);
   const unsigned long long bv = __ballot(valid);
-  if (ball == bv && s0 >= 0) {
-    const double w = wave_sum(valid ? v : 0.0);
+  const bool wuni = bv != 0 && s0 >= 0 && __ballot(valid && s == s0) == bv;
+  const double w = wave_sum(valid ? v : 0.0);
+  // per wave: its scene, -1 when it has no valid item, -2 when mixed
+  if (lane == 0) {
+    wsum[wv] = w;
+    wscene[wv] = bv == 0 ? -1 : (wuni ? s0 : -2);
+  }
+  __syncthreads();
+  int bs = -1;
+  bool buni = true;
+  for (int k = 0; k < (int)(blockDim.x / kWave); ++k) {
+    const int ws = wscene[k];
+    if (ws == -2 || (ws >= 0 && bs >= 0 && ws != bs)) buni = false;
+    if (ws >= 0 && bs < 0) bs = ws;
+  }
+  if (buni) {
+    if (threadIdx.x == 0 && bs >= 0) {
+      double tot = 0.0;
+      for (int k = 0; k < (int)(blockDim.x / kWave); ++k)
+        if (wscene[k] >= 0) tot += wsum[k];
+      atomicAdd(&acc[bs], tot);
+    }
+  } else if (wuni) {
     if (lane == 0) atomicAdd(&acc[s0], w);
   } else if (valid) {
     atomicAdd(&acc[s], v);
   }
+  __syncthreads();   // the next pass rewrites wsum / wscene
 }
 
 // The fluid-row part of k_pressure (dfsph.py:44-57,164-184 with the Akinci
@@ -404,7 +431,10 @@ __global__ __launch_bounds__(kBS) void k_pressure(SphFluid f, const SphPhysics* 
   const int stride = gridDim.x * per_block;
   const int sub = threadIdx.x % KL;
   const int* preb = d.act_pre + d.nscenes + 1;
-  double* acc_slot = d.err_acc + (size_t)(blockIdx.x % SPH_DFSPH_ERR_SLOTS) * d.nscenes;
+  double* acc_slot =
+      d.err_acc + (size_t)(blockIdx.x % SPH_DFSPH_ERR_SLOTS) * SPH_DFSPH_ERR_STRIDE(d.nscenes);
+  __shared__ double s_wsum[kBS / kWave];
+  __shared__ int s_wscene[kBS / kWave];
   __shared__ int s_cols[kBS / kWave][kWaveCols];
   int* wcols = s_cols[threadIdx.x / kWave];
   // every lane runs the same number of loop trips so the wavefront-wide
@@ -475,7 +505,7 @@ __global__ __launch_bounds__(kBS) void k_pressure(SphFluid f, const SphPhysics* 
         valid = true;
       }
     }
-    scene_sum(contrib, s, valid, acc_slot);
+    scene_sum(contrib, s, valid, acc_slot, s_wsum, s_wscene);
   }
 }
 
@@ -487,9 +517,11 @@ __global__ __launch_bounds__(kBS) void k_control(const SphPhysics* __restrict__ 
     const SphPhysics P = phys[s];
     const double n = (double)(d.nfluid[s] + d.nbdy[s]);
     double sum = 0.0;
+    const size_t stride = SPH_DFSPH_ERR_STRIDE(d.nscenes);
+#pragma unroll 8
     for (int k = 0; k < SPH_DFSPH_ERR_SLOTS; ++k) {
-      sum += d.err_acc[(size_t)k * d.nscenes + s];
-      d.err_acc[(size_t)k * d.nscenes + s] = 0.0;
+      sum += d.err_acc[k * stride + s];
+      d.err_acc[k * stride + s] = 0.0;
     }
     const double err = n > 0 ? sum / n : 0.0;
     const int it = d.iters[s];
@@ -559,7 +591,8 @@ int dfsph_prepare(const SphFluid* f, const float* acc, const SphPhysics* phys, c
   const SphBoundary bb = bdy_or_null(b);
   SPH_REQUIRE(bdy_ok(bb, d), "sph_dfsph_prepare: bad boundary");
   int n = f->np + bb.nb;
-  if (n < SPH_DFSPH_ERR_SLOTS * d->nscenes) n = SPH_DFSPH_ERR_SLOTS * d->nscenes;
+  const int nacc = SPH_DFSPH_ERR_SLOTS * SPH_DFSPH_ERR_STRIDE(d->nscenes);
+  if (n < nacc) n = nacc;
   if (n < 1) n = 1;
   hipLaunchKernelGGL(k_prepare, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, *f, acc, phys, bb, *d);
   hipLaunchKernelGGL(k_active_lists, dim3(1), dim3(kBS), 0, s, *d);

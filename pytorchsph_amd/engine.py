@@ -239,7 +239,7 @@ class SceneBatch:
             vpred=z(NP, 2), aact=z(NP), alpha=z(NP), src=z(NP), p=z(NP), p2=z(NP), apred=z(NP, 2),
             resid=z(NP), xrec=z(NP, 4), arec=z(NP, 4), prec=z(NP, 4), wvec=z(NP, 2), b_aact=z(nb1), b_alpha=z(nb1),
             b_src=z(nb1), b_p=z(nb1), b_resid=z(nb1), bxrec=z(nb1, 4),
-            err_acc=z(L.DFSPH_ERR_SLOTS, S, dt=torch.float64), err_hist=z(S, self.max_hist, dt=torch.float64),
+            err_acc=z(L.DFSPH_ERR_SLOTS, L.dfsph_err_stride(S), dt=torch.float64), err_hist=z(S, self.max_hist, dt=torch.float64),
             iters=z(S, dt=torch.int32), active=z(S, dt=torch.int32),
             any_active=z(1, dt=torch.int32), status=z(S, dt=torch.int32),
             nfluid=ints(self.n), nbdy=ints(self.nb), fbegin=ints(self.p_begin),

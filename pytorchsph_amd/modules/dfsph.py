@@ -65,7 +65,7 @@ class dfsphModule(Module):
                  resid=z(n), xrec=z(n, 4), arec=z(n, 4), prec=z(n, 4), wvec=z(n, 2),
                  b_aact=z(nb1), b_alpha=z(nb1),
                  b_src=z(nb1), b_p=z(nb1), b_resid=z(nb1), bxrec=z(nb1, 4),
-                 err_acc=z(L.DFSPH_ERR_SLOTS, 1, dt=torch.float64), err_hist=z(1, 300, dt=torch.float64),
+                 err_acc=z(L.DFSPH_ERR_SLOTS, L.dfsph_err_stride(1), dt=torch.float64), err_hist=z(1, 300, dt=torch.float64),
                  iters=z(1, dt=torch.int32), active=z(1, dt=torch.int32),
                  any_active=z(1, dt=torch.int32), status=z(1, dt=torch.int32),
                  nfluid=ints([n]), nbdy=ints([nb]), fbegin=ints([0]), bbegin=ints([0]),
