@@ -35,7 +35,12 @@ __global__ __launch_bounds__(kBS) void k_grid_keys(SphGrid g) {
     // reference key: ix + cellsX * iy in int32 (neighborSearch.py:47)
     const int64_t key = (int64_t)ix + (int64_t)d.cells_x * (int64_t)iy;
     const int64_t ncell = (int64_t)d.cells_x * d.cells_y;
-    if (key >= 0 && key < ncell) {
+    // a NaN / Inf position has no neighbour under the fp32 predicate; it
+    // goes to the outside slot rather than to whatever cell the float to
+    // int conversion gives (0 for NaN), where a blown-up scene would pile
+    // all its points into one cell that stencils do visit
+    const bool finite = isfinite(x) && isfinite(y);
+    if (finite && key >= 0 && key < ncell) {
       k = (int)key;
     } else {
       // the scene's last slot collects points outside its table; stencils
@@ -73,6 +78,18 @@ __global__ __launch_bounds__(kBS) void k_grid_scatter(SphGrid g) {
 constexpr int kRegSort = 16;   // cells up to this size are sorted in registers
 constexpr int kCandBatch = 4;  // candidates each lane loads at once in a search pass
 
+// Whether global cell c is its scene's slot of outside points (the scene's
+// last slot): stencils never visit it, so its order does not matter.
+__device__ bool outside_slot(const SphGrid& g, int c) {
+  int lo = 0, hi = g.nscenes;   // scene s with cell_begin[s] <= c
+  while (hi - lo > 1) {
+    const int m = (lo + hi) >> 1;
+    if (g.scenes[m].cell_begin <= c) lo = m; else hi = m;
+  }
+  const SphGridDesc& d = g.scenes[lo];
+  return (int64_t)c == (int64_t)d.cell_begin + (int64_t)d.cells_x * d.cells_y;
+}
+
 __global__ __launch_bounds__(kBS) void k_grid_cell_sort(SphGrid g) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.ncells) return;
@@ -96,7 +113,7 @@ __global__ __launch_bounds__(kBS) void k_grid_cell_sort(SphGrid g) {
 #pragma unroll
     for (int u = 0; u < kRegSort; ++u)
       if (a + u < b) idx[a + u] = v[u];
-  } else {
+  } else if (!outside_slot(g, c)) {
     for (int i = a + 1; i < b; ++i) {
       const int v = idx[i];
       int j = i - 1;

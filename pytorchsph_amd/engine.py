@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import time
 
 import numpy as np
 import torch
@@ -420,17 +421,18 @@ class SceneBatch:
             self._dfsph_post(s)
         self.steps_done += 1
 
-    def capture(self):
+    def capture(self, warm: bool = True):
         """Captures the step into hipGraphs (delta-SPH: one graph per step;
-        DFSPH: prologue, a chunk of iterations, epilogue).  Runs one eager
-        step first."""
+        DFSPH: prologue, a chunk of iterations, epilogue).  With `warm`, runs
+        one eager step first (every scene advances by one step)."""
         torch.cuda.synchronize()
-        s = torch.cuda.Stream()
-        s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            self.step(s)
-        torch.cuda.current_stream().wait_stream(s)
-        torch.cuda.synchronize()
+        if warm:
+            s = torch.cuda.Stream()
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                self.step(s)
+            torch.cuda.current_stream().wait_stream(s)
+            torch.cuda.synchronize()
         if self.scheme == "dsph":
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
@@ -506,10 +508,17 @@ class MixedBatch:
         self.n_particles = int(sum(s.n for s in scenes))
         self.captured = False
 
-    def capture(self):
+    def capture(self) -> dict:
+        """One eager step of every scene (both parts on their own streams, as
+        in step()), then each part's graphs.  Returns seconds per phase."""
+        t0 = time.perf_counter()
+        self.step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
         for b in self.parts:
-            b.capture()
+            b.capture(warm=False)
         self.captured = True
+        return dict(eager=t1 - t0, record=time.perf_counter() - t1)
 
     def step(self, serial: bool = False):
         if serial:   # both parts on the current stream, one after the other

@@ -16,6 +16,7 @@ reference's geometric contract rather than its init code:
 """
 from __future__ import annotations
 
+import functools
 import math
 from dataclasses import dataclass, field
 
@@ -54,6 +55,54 @@ class Scene:
 def support_for(dx: float, target_neighbors: int = 20) -> np.float32:
     area = dx * dx
     return np.float32(math.sqrt(area / math.pi * target_neighbors))
+
+
+# Rest-state calibration of the lattice set-ups, the counterpart of the
+# reference's packing and boundary-spacing optimisation
+# (simulationBase.py:857-880, evalPacking / evalBoundarySpacing): there the
+# lattice spacing and the fluid-to-wall gap are solved so that a particle of
+# the initial lattice has SPH density 1.  Here the spacing dx is given (it
+# sets the particle count), so the particle area is scaled instead, and the
+# Akinci wall layer is placed at the distance that gives the first fluid row
+# density 1.  Both are ratios independent of dx (h ~ dx, areas ~ dx^2).
+AKINCI_GAMMA = 0.7
+
+
+@functools.lru_cache(maxsize=None)
+def lattice_calibration(target_neighbors: int = 20, gamma: float = AKINCI_GAMMA):
+    """(area / dx^2, Akinci wall offset / dx): the area that gives a square
+    lattice of spacing dx density sum_j A W_ij = 1, and the distance below the
+    fluid's edge (first row at dx / 2) of a wall line of spacing dx, with
+    volumes gamma / sum W over the line, that gives the first row density 1."""
+    dx = 1.0
+    h = float(support_for(dx, target_neighbors))
+    r = int(math.ceil(h / dx)) + 2
+
+    def w(d):
+        q = np.minimum(d / h, 1.0)
+        return np.where(d < h, (1.0 - q) ** 4 * (1.0 + 4.0 * q) * 7.0 / (math.pi * h * h), 0.0)
+
+    I, J = np.meshgrid(np.arange(-r, r + 1), np.arange(-r, r + 1), indexing="xy")
+    lat = np.stack([I.ravel(), J.ravel()], 1) * dx
+    area = 1.0 / w(np.sqrt((lat ** 2).sum(1))).sum()
+    xs = np.arange(-r, r + 1) * dx
+    vb = gamma / w(np.abs(xs)).sum()
+    half = lat[lat[:, 1] >= 0] + np.array([0.0, 0.5 * dx])    # rows dx/2, 3dx/2, ...
+    q = np.array([0.0, 0.5 * dx])
+    rho_fluid = area * w(np.sqrt(((half - q) ** 2).sum(1))).sum()
+
+    def rho_first(off):
+        wall = np.stack([xs, np.full_like(xs, -off)], 1)
+        return rho_fluid + vb * w(np.sqrt(((wall - q) ** 2).sum(1))).sum()
+
+    lo, hi = 0.0, 2.0 * dx     # rho_first decreases with the offset
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        if rho_first(mid) > 1.0:
+            lo = mid
+        else:
+            hi = mid
+    return area / (dx * dx), 0.5 * (lo + hi) / dx
 
 
 def default_phys(scheme: str, h: float, rho0: float = 1000.0) -> dict:
@@ -99,7 +148,7 @@ def _walls_solid(W, H, dx, layers=3):
 
 def _walls_akinci(W, H, dx):
     nx, ny = int(round(W / dx)), int(round(H / dx))
-    off = 0.5 * dx
+    off = lattice_calibration()[1] * dx
     xs = (np.arange(-1, nx + 1) + 0.5) * dx
     ys = (np.arange(0, ny) + 0.5) * dx
     bottom = np.stack([xs, np.full_like(xs, -off)], 1)
@@ -136,7 +185,7 @@ def akinci_volumes(bx: np.ndarray, h: float, gamma: float = 0.7):
 
 def _finish(name, scheme, boundary, fluid, vel, dx, W, H, seed, h=None):
     h = support_for(dx) if h is None else np.float32(h)
-    area = np.float32(dx * dx)
+    area = np.float32(dx * dx * lattice_calibration()[0])
     if boundary == "solid":
         bx, bgx = _walls_solid(W, H, dx)
         bvol = np.full(len(bx), area, dtype=np.float32)   # pi r^2 = A
@@ -144,7 +193,7 @@ def _finish(name, scheme, boundary, fluid, vel, dx, W, H, seed, h=None):
     else:
         bx = _walls_akinci(W, H, dx)
         bgx = None
-        bvol, bterm = akinci_volumes(bx, h)
+        bvol, bterm = akinci_volumes(bx, h, AKINCI_GAMMA)
     phys = default_phys(scheme, float(h))
     buf = 2.0 * float(h)
     return Scene(name=name, scheme=scheme, boundary=boundary, x=fluid.astype(np.float32),

@@ -143,19 +143,23 @@ class Server:
         torch.cuda.synchronize()
         t_build = clock()
         # capture() advances every scene by one eager step before recording
-        self.batch.capture()
+        cap = self.batch.capture()
         torch.cuda.synchronize()
         dt = clock() - t0
         self.repack_times.append(dict(export=t_export - t_begin, admit=t0 - t_export,
-                                      build=t_build - t0, capture=clock() - t_build))
+                                      build=t_build - t0, eager=cap["eager"],
+                                      record=cap["record"]))
         for j in self.live:
             j.carry = None
         self._advance({j.id: k for k, j in enumerate(self.live)}, dt, clock() - start)
         if self.log:
             rt = self.repack_times[-1]
+            its = [b.dfsph_iterations() for b in self.batch.parts if b.scheme == "dfsph"]
+            it_s = (f", DFSPH iterations max {int(its[0].max())} mean {its[0].mean():.0f}"
+                    if its and its[0].size else "")
             self.log(f"repack {self.repacks}: {len(keep)} kept + {len(admitted)} admitted, "
                      f"{parts} particles, {used / 2**30:.2f} GiB, {dt * 1e3:.0f} ms ("
-                     + ", ".join(f"{k} {v * 1e3:.0f}" for k, v in rt.items()) + ")")
+                     + ", ".join(f"{k} {v * 1e3:.0f}" for k, v in rt.items()) + ")" + it_s)
 
     # --------------------------------------------------------------- loop
     def run(self, jobs: list[Job], time_limit: float | None = None, drain: bool = True,

@@ -28,6 +28,7 @@ class CountingBatch:
 
     def capture(self):
         self.step()
+        return dict(eager=0.0, record=0.0)
 
     def step(self):
         self.count = [c + 1 for c in self.count]
