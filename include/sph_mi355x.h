@@ -385,6 +385,20 @@ int sph_integrate_semi_implicit(const int* pscene, const SphPhysics* phys, int n
 int sph_permute_f32(const float* in, const int* perm, int n, int width, float* out,
                     hipStream_t s);
 
+/* Up to SPH_PERMUTE_MAX_FIELDS arrays of 32-bit words permuted in place
+ * together: field[k] = old field[perm[k]] for every field, through `tmp`
+ * (n * sum of widths words).  Two launches whatever the number of fields:
+ * one gathers every field of element perm[k] into row k of tmp, one writes
+ * the rows back.  The engine's per-step move of the particle state into
+ * cell order; no counterpart in the reference, which does not reorder. */
+#define SPH_PERMUTE_MAX_FIELDS 8
+typedef struct SphFields {
+  float* ptr[SPH_PERMUTE_MAX_FIELDS];  /* device arrays [n][width[k]]          */
+  int width[SPH_PERMUTE_MAX_FIELDS];   /* words per element, 1..4             */
+  int nfields;
+} SphFields;
+int sph_permute_fields(const SphFields* f, const int* perm, int n, float* tmp, hipStream_t s);
+
 #ifdef __cplusplus
 }
 #endif

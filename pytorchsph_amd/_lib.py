@@ -20,6 +20,14 @@ c_vp = C.c_void_p
 c_int = C.c_int
 
 
+PERMUTE_MAX_FIELDS = 8
+
+
+class SphFields(C.Structure):
+    _fields_ = [("ptr", c_vp * PERMUTE_MAX_FIELDS), ("width", c_int * PERMUTE_MAX_FIELDS),
+                ("nfields", c_int)]
+
+
 class SphGrid(C.Structure):
     _fields_ = [
         ("scenes", c_vp), ("pscene", c_vp), ("x", c_vp),
@@ -147,6 +155,7 @@ _SIGS = {
     "sph_integrate_semi_implicit": ([c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                      c_vp, c_vp, c_vp], c_int),
     "sph_permute_f32": ([c_vp, c_vp, c_int, c_int, c_vp, c_vp], c_int),
+    "sph_permute_fields": ([_P(SphFields), c_vp, c_int, c_vp, c_vp], c_int),
 }
 
 _lib = None

@@ -90,7 +90,54 @@ __global__ __launch_bounds__(kBS) void k_permute(const float* __restrict__ in,
   out[t] = in[src * width + c];
 }
 
+// Row k of tmp = every field of element perm[k], fields one after the other.
+__global__ __launch_bounds__(kBS) void k_fields_gather(SphFields f, const int* __restrict__ perm,
+                                                       int n, int rw, float* __restrict__ tmp) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t src = perm[k];
+  float* __restrict__ row = tmp + (int64_t)k * rw;
+  int o = 0;
+  for (int q = 0; q < f.nfields; ++q) {
+    const int w = f.width[q];
+    const float* __restrict__ in = f.ptr[q] + src * w;
+    for (int c = 0; c < w; ++c) row[o + c] = in[c];
+    o += w;
+  }
+}
+
+__global__ __launch_bounds__(kBS) void k_fields_scatter(SphFields f, int n, int rw,
+                                                        const float* __restrict__ tmp) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float* __restrict__ row = tmp + (int64_t)k * rw;
+  int o = 0;
+  for (int q = 0; q < f.nfields; ++q) {
+    const int w = f.width[q];
+    float* __restrict__ out = f.ptr[q] + (int64_t)k * w;
+    for (int c = 0; c < w; ++c) out[c] = row[o + c];
+    o += w;
+  }
+}
+
 }  // namespace
+
+int permute_fields(const SphFields* f, const int* perm, int n, float* tmp, hipStream_t s) {
+  SPH_REQUIRE(f && perm && tmp && n >= 0 && f->nfields >= 0 &&
+                  f->nfields <= SPH_PERMUTE_MAX_FIELDS,
+              "sph_permute_fields: bad arguments");
+  int rw = 0;
+  for (int q = 0; q < f->nfields; ++q) {
+    SPH_REQUIRE(f->ptr[q] && f->width[q] >= 1 && f->width[q] <= 4,
+                "sph_permute_fields: each field needs a pointer and a width of 1..4");
+    rw += f->width[q];
+  }
+  if (n == 0 || rw == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_fields_gather, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, *f, perm, n, rw,
+                     tmp);
+  hipLaunchKernelGGL(k_fields_scatter, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, *f, n, rw, tmp);
+  return launch_check("sph_permute_fields");
+}
 
 int integrate_rk4(int stage, const int* pscene, const SphPhysics* phys, int np, const float* x0,
                   const float* v0, const float* rho0, const float* vs, const float* acc,
@@ -129,6 +176,9 @@ int permute_f32(const float* in, const int* perm, int n, int width, float* out, 
 }  // namespace sph
 
 extern "C" {
+int sph_permute_fields(const SphFields* f, const int* perm, int n, float* tmp, hipStream_t s) {
+  return sph::permute_fields(f, perm, n, tmp, s);
+}
 int sph_integrate_rk4(int stage, const int* pscene, const SphPhysics* phys, int np,
                       const float* x0, const float* v0, const float* rho0, const float* v_stage,
                       const float* acc, const float* dpdt, float* kacc, float* x_out,

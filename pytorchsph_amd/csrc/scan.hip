@@ -2,6 +2,7 @@
 // single-workgroup scan of the tile sums, and the tile-local scans.
 #include "sph_common.h"
 
+#include <stdint.h>
 #include <stdio.h>
 
 #include <atomic>
@@ -48,18 +49,36 @@ static int set_tuning(int knob, int value) {
 
 namespace {
 constexpr int kScanBS = 256;
-constexpr int kScanItems = 8;
+constexpr int kScanItems = 8;   // the vector path moves them as two int4
+static_assert(kScanItems == 8, "load_items and k_scan_apply move 8 items as two int4");
 constexpr int kTile = kScanBS * kScanItems;
 
+// Each thread owns kScanItems consecutive items.  With 16-byte aligned
+// arrays (VEC) it moves them as two int4 (a wave reads 2 KiB contiguous per
+// instruction pair); otherwise one int at a time.
+template <bool VEC>
+__device__ __forceinline__ void load_items(const int* __restrict__ in, int n, int base,
+                                           int (&v)[kScanItems]) {
+  if (VEC && base + kScanItems <= n) {
+    const int4 a = *reinterpret_cast<const int4*>(in + base);
+    const int4 b = *reinterpret_cast<const int4*>(in + base + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) v[k] = base + k < n ? in[base + k] : 0;
+}
+
+template <bool VEC>
 __global__ __launch_bounds__(kScanBS) void k_scan_reduce(const int* __restrict__ in, int n,
                                                          int* __restrict__ partial) {
   const int base = blockIdx.x * kTile + threadIdx.x * kScanItems;
+  int v[kScanItems];
+  load_items<VEC>(in, n, base, v);
   int s = 0;
 #pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    int i = base + k;
-    if (i < n) s += in[i];
-  }
+  for (int k = 0; k < kScanItems; ++k) s += v[k];
   s = wave_sum(s);
   __shared__ int lds[kScanBS / kWave];
   if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = s;
@@ -86,27 +105,33 @@ __global__ __launch_bounds__(1024) void k_scan_partials(int* __restrict__ partia
   if (threadIdx.x == 0) out[n] = carry;
 }
 
+template <bool VEC>
 __global__ __launch_bounds__(kScanBS) void k_scan_apply(const int* __restrict__ in, int n,
                                                         const int* __restrict__ partial,
                                                         int* __restrict__ out) {
   __shared__ int lds[kScanBS / kWave + 1];
   const int base = blockIdx.x * kTile + threadIdx.x * kScanItems;
   int v[kScanItems];
+  load_items<VEC>(in, n, base, v);
   int s = 0;
 #pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    int i = base + k;
-    v[k] = i < n ? in[i] : 0;
-    s += v[k];
-  }
+  for (int k = 0; k < kScanItems; ++k) s += v[k];
   int tot;
   int ex = block_exclusive_scan(s, lds, &tot) + partial[blockIdx.x];
+  int o[kScanItems];
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    int i = base + k;
-    if (i < n) out[i] = ex;
+    o[k] = ex;
     ex += v[k];
   }
+  if (VEC && base + kScanItems <= n) {
+    *reinterpret_cast<int4*>(out + base) = make_int4(o[0], o[1], o[2], o[3]);
+    *reinterpret_cast<int4*>(out + base + 4) = make_int4(o[4], o[5], o[6], o[7]);
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k)
+    if (base + k < n) out[base + k] = o[k];
 }
 
 template <typename T>
@@ -128,9 +153,16 @@ int scan_exclusive(const int* in, int n, int* out, void* ws, size_t ws_bytes, hi
     return SPH_E_SPACE;
   }
   int* partial = (int*)ws;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBS), 0, s, in, n, partial);
+  const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+  if (vec)
+    hipLaunchKernelGGL(k_scan_reduce<true>, dim3(nb), dim3(kScanBS), 0, s, in, n, partial);
+  else
+    hipLaunchKernelGGL(k_scan_reduce<false>, dim3(nb), dim3(kScanBS), 0, s, in, n, partial);
   hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, partial, nb, out, n);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBS), 0, s, in, n, partial, out);
+  if (vec)
+    hipLaunchKernelGGL(k_scan_apply<true>, dim3(nb), dim3(kScanBS), 0, s, in, n, partial, out);
+  else
+    hipLaunchKernelGGL(k_scan_apply<false>, dim3(nb), dim3(kScanBS), 0, s, in, n, partial, out);
   return launch_check("sph_scan_exclusive");
 }
 

@@ -119,7 +119,7 @@ class SceneBatch:
                                                  self.n)).to(dev)
         self.uid = torch.arange(NP, dtype=torch.int64, device=dev)
         self.carry = carry
-        self.tmp = torch.empty(max(2 * NP, 1), dtype=f32, device=dev)
+        self.tmp = torch.empty(max(7 * NP, 1), dtype=f32, device=dev)   # x, v, rho, uid rows
         # ---- boundary state (static)
         self.bx = cat([s.bx for s in scenes]).float().contiguous()
         self.bgx = cat([s.bgx for s in scenes]).float().contiguous() if self.solid else None
@@ -287,20 +287,20 @@ class SceneBatch:
         # with the state in cell order, index order is already local
         return None if self.reorder else L.ptr(self.grid.sorted_idx)
 
-    def _permute(self, t: torch.Tensor, width: int, s):
-        sh = L.stream_handle(s)
-        L.call("sph_permute_f32", L.ptr(t), L.ptr(self.grid.sorted_idx), self.NP, width,
-               L.ptr(self.tmp), sh)
-        L.call("sph_permute_f32", L.ptr(self.tmp), None, self.NP, width, L.ptr(t), sh)
-
     def reorder_state(self, s=None):
         """Permutes the perennial particle state into the cell order of the
-        last neighbour search (scene ranges are preserved)."""
+        last neighbour search.  The permutation keeps every scene's range, so
+        the per-scene constants (area, rest density, support) stay put; x, v,
+        rho and the ids move together in two launches."""
         if not (self.reorder and self._sorted_valid and self.NP):
             return
-        for t, w in ((self.x, 2), (self.v, 2), (self.rho, 1), (self.area, 1), (self.rest, 1),
-                     (self.h, 1), (self.uid, 2)):
-            self._permute(t, w, s)
+        fl = L.SphFields()
+        for q, (t, w) in enumerate(((self.x, 2), (self.v, 2), (self.rho, 1), (self.uid, 2))):
+            fl.ptr[q] = L.ptr(t)
+            fl.width[q] = w
+        fl.nfields = 4
+        L.call("sph_permute_fields", C.byref(fl), L.ptr(self.grid.sorted_idx), self.NP,
+               L.ptr(self.tmp), L.stream_handle(s))
 
     # ---------------------------------------------------------- delta-SPH
     def dsph_rhs(self, x, v, rho, s=None):

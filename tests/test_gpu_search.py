@@ -224,3 +224,27 @@ def test_device_foreign_matches_reference_fixture(cuda):
     rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
     o = np.lexsort((c, rows))
     assert np.array_equal(rows[o], g["foreign_rows"]) and np.array_equal(c[o], g["foreign_cols"])
+
+
+@pytest.mark.parametrize("n", [0, 1, 7, 2047, 2048, 2049, 100_003, 5_000_000])
+@pytest.mark.parametrize("offset", [0, 1])
+def test_scan_matches_numpy(cuda, n, offset):
+    """Exclusive prefix sum with the total at out[n], for aligned arrays (the
+    int4 path) and arrays one int off alignment (the scalar path)."""
+    from pytorchsph_amd.search import scan
+
+    rng = np.random.default_rng(n + offset)
+    vals = rng.integers(0, 60, size=n).astype(np.int32)
+    dev = torch.device("cuda")
+    inb = torch.zeros(n + offset + 1, dtype=torch.int32, device=dev)
+    outb = torch.full((n + offset + 2,), -7, dtype=torch.int32, device=dev)
+    inp = inb[offset:offset + n]
+    inp.copy_(torch.from_numpy(vals))
+    out = outb[offset:offset + n + 1]
+    scan(inp, n, out)
+    torch.cuda.synchronize()
+    want = np.zeros(n + 1, dtype=np.int64)
+    want[1:] = np.cumsum(vals)
+    assert np.array_equal(out.cpu().numpy(), want.astype(np.int32))
+    rest = outb.cpu().numpy()
+    assert (rest[:offset] == -7).all() and (rest[offset + n + 1:] == -7).all()
