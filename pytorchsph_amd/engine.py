@@ -424,7 +424,10 @@ class SceneBatch:
     def capture(self, warm: bool = True):
         """Captures the step into hipGraphs (delta-SPH: one graph per step;
         DFSPH: prologue, a chunk of iterations, epilogue).  With `warm`, runs
-        one eager step first (every scene advances by one step)."""
+        one eager step first (every scene advances by one step).  Without
+        it nothing advances; a batch that has no cell order yet gets the
+        identity order, so the recorded step reorders (a no-op the first
+        time) exactly like every later one."""
         torch.cuda.synchronize()
         if warm:
             s = torch.cuda.Stream()
@@ -432,6 +435,11 @@ class SceneBatch:
             with torch.cuda.stream(s):
                 self.step(s)
             torch.cuda.current_stream().wait_stream(s)
+            torch.cuda.synchronize()
+        elif not self._sorted_valid and self.NP:
+            self.grid.sorted_idx[: self.NP].copy_(
+                torch.arange(self.NP, dtype=torch.int32, device=self.dev))
+            self._sorted_valid = True
             torch.cuda.synchronize()
         if self.scheme == "dsph":
             g = torch.cuda.CUDAGraph()
@@ -508,12 +516,14 @@ class MixedBatch:
         self.n_particles = int(sum(s.n for s in scenes))
         self.captured = False
 
-    def capture(self) -> dict:
-        """One eager step of every scene (both parts on their own streams, as
-        in step()), then each part's graphs.  Returns seconds per phase."""
+    def capture(self, warm: bool = True) -> dict:
+        """Records each part's graphs.  With `warm`, one eager step of every
+        scene comes first (both parts on their own streams, as in step());
+        without it no scene advances.  Returns seconds per phase."""
         t0 = time.perf_counter()
-        self.step()
-        torch.cuda.synchronize()
+        if warm:
+            self.step()
+            torch.cuda.synchronize()
         t1 = time.perf_counter()
         for b in self.parts:
             b.capture(warm=False)

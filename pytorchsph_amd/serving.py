@@ -142,13 +142,15 @@ class Server:
                                 carry=[j.carry for j in self.live], **self.batch_kw)
         torch.cuda.synchronize()
         t_build = clock()
-        # capture() advances every scene by one eager step before recording
-        cap = self.batch.capture()
+        # record the graphs, then the tick's step is an ordinary replay
+        cap = self.batch.capture(warm=False)
+        t_step = clock()
+        self.batch.step()
         torch.cuda.synchronize()
         dt = clock() - t0
         self.repack_times.append(dict(export=t_export - t_begin, admit=t0 - t_export,
-                                      build=t_build - t0, eager=cap["eager"],
-                                      record=cap["record"]))
+                                      build=t_build - t0, record=cap["record"],
+                                      step=clock() - t_step))
         for j in self.live:
             j.carry = None
         self._advance({j.id: k for k, j in enumerate(self.live)}, dt, clock() - start)

@@ -69,6 +69,29 @@ def test_graph_replay_matches_eager(cuda, scheme):
     assert np.all(st & (L.ST_OUTSIDE | L.ST_OVERFLOW | L.ST_NAN) == 0)
 
 
+@pytest.mark.parametrize("scheme", ["dsph", "dfsph"])
+def test_cold_capture_matches_eager(cuda, scheme):
+    """capture(warm=False) advances nothing; replays from there follow the
+    eager steps of a fresh batch (the first reorder is the identity)."""
+    scenes = mix(scheme)
+    eager = SceneBatch(scenes)
+    graph = SceneBatch(scenes)
+    graph.capture(warm=False)
+    torch.cuda.synchronize()
+    for k in range(len(scenes)):
+        assert np.array_equal(graph.scene_state(k)["x"], scenes[k].x)
+    for _ in range(3):
+        graph.step_graph()
+        eager.step()
+        torch.cuda.synchronize()
+        for k in range(len(scenes)):
+            g, e = graph.scene_state(k), eager.scene_state(k)
+            if scheme == "dsph":
+                assert np.array_equal(g["x"], e["x"]) and np.array_equal(g["v"], e["v"])
+            else:
+                assert np.abs(g["x"] - e["x"]).max() < 1e-6
+
+
 def test_permute_fields_matches_numpy(cuda):
     """sph_permute_fields moves every field by the same permutation, int64
     ids bit for bit, with fields of widths 1, 2 and 4."""

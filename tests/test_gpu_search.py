@@ -248,3 +248,30 @@ def test_scan_matches_numpy(cuda, n, offset):
     assert np.array_equal(out.cpu().numpy(), want.astype(np.int32))
     rest = outb.cpu().numpy()
     assert (rest[:offset] == -7).all() and (rest[offset + n + 1:] == -7).all()
+
+
+def test_non_finite_points_have_no_neighbours(cuda):
+    """NaN / Inf positions fail the fp32 predicate against every point, as in
+    the reference: their rows are empty and they appear in no row.  They go
+    to the scene's outside slot (SPH_ST_OUTSIDE) instead of piling into one
+    cell, and the finite points' rows are the search without them."""
+    rng = np.random.default_rng(21)
+    x = rng.uniform(0.0, 1.0, size=(20_000, 2)).astype(np.float32)
+    hmax = np.float32(0.02)
+    bad = rng.choice(len(x), 3000, replace=False)
+    x[bad[:1000], 0] = np.nan
+    x[bad[1000:2000], 1] = np.inf
+    x[bad[2000:], :] = -np.inf
+    h = np.full(len(x), hmax, np.float32)
+    rp, c, st, _ = gpu_search(x, h, (0, 0), (1, 1), hmax)
+    assert st & L.ST_OUTSIDE
+    cnt = np.diff(rp)
+    assert (cnt[bad] == 0).all()
+    assert not np.isin(c, bad).any()
+    good = np.setdiff1d(np.arange(len(x)), bad)
+    rp2, c2, st2, _ = gpu_search(x[good], h[good], (0, 0), (1, 1), hmax)
+    s1 = sorted_rows(rp, c)
+    s2 = sorted_rows(rp2, c2)
+    for k, i in enumerate(good[:2000]):
+        # good is ascending, so mapping the reduced indices keeps rows sorted
+        assert np.array_equal(s1[rp[i]:rp[i + 1]], good[s2[rp2[k]:rp2[k + 1]]])

@@ -26,8 +26,9 @@ class CountingBatch:
         self.count = [c["steps"] if c else 0 for c in carry]
         CountingBatch.built.append(sum(s.n for s in scenes))
 
-    def capture(self):
-        self.step()
+    def capture(self, warm=True):
+        if warm:
+            self.step()
         return dict(eager=0.0, record=0.0)
 
     def step(self):
