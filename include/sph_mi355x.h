@@ -292,7 +292,10 @@ typedef struct SphDfsph {
   float* resid;  /* [np] residual                                     */
   float* xrec;   /* [np][4] {x, y, p2 / (rho rho0)^2, A rho0}         */
   float* arec;   /* [np][4] {a*_x, a*_y, dt^2 A^act, 0}               */
-  float* prec;   /* [np][4] {x, y, w a*_x, w a*_y}, w = dt^2 A^act    */
+  float* prec;   /* [np][4] {x, y, w a*_x, w a*_y}, w = dt^2 A^act, as
+                    written by sph_dfsph_accel and read by
+                    sph_dfsph_pressure; from sph_dfsph_prepare until the
+                    first accel it holds {v*, A^act, A rho0}            */
   float* wvec;   /* [np][2] W_i = dt^2 sum_j A^act_j gradWs_ij over fluid
                     and boundary neighbours (fixed during the solve)  */
   float* b_aact; /* [nb]                                              */

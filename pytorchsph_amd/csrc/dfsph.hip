@@ -39,28 +39,30 @@ __device__ __forceinline__ void st4(float* p, int i, float4 v) {
   reinterpret_cast<float4*>(p)[i] = v;
 }
 
-// Calls f(rec[col(k)]) for this lane's pairs k = k0 + sub, k0 + sub + KL,
+// Calls f(load(col(k))) for this lane's pairs k = k0 + sub, k0 + sub + KL,
 // ... < k1, in that order (the order of the one-pair loop), with kPairBatch
-// record loads in flight: loads are issued unconditionally (a slot past the
-// row reads pair k0 again) and one batch ahead of the pairs being evaluated.
-// An empty row reads nothing.  k0, k1 are the same in every lane of a group.
-template <int KL, typename Col, typename F>
-__device__ __forceinline__ void pair_pipeline(int k0, int k1, int sub, const float* rec, Col col,
-                                              F f) {
+// neighbour loads in flight: loads are issued unconditionally (a slot past
+// the row loads pair k0 again) and one batch ahead of the pairs being
+// evaluated.  An empty row loads nothing.  k0, k1 are the same in every lane
+// of a group.
+template <int KL, typename Col, typename Load, typename F>
+__device__ __forceinline__ void pair_pipeline_with(int k0, int k1, int sub, Col col, Load load,
+                                                   F f) {
   if (k0 >= k1) return;
-  float4 rj[kPairBatch], rn[kPairBatch];
+  using R = decltype(load(0));
+  R rj[kPairBatch], rn[kPairBatch];
   int kb = k0 + sub;
 #pragma unroll
   for (int u = 0; u < kPairBatch; ++u) {
     const int k = kb + u * KL;
-    rj[u] = ld4(rec, col(k < k1 ? k : k0));
+    rj[u] = load(col(k < k1 ? k : k0));
   }
   for (; kb < k1; kb += KL * kPairBatch) {
     const int kn = kb + KL * kPairBatch;
 #pragma unroll
     for (int u = 0; u < kPairBatch; ++u) {
       const int k = kn + u * KL;
-      rn[u] = ld4(rec, col(k < k1 ? k : k0));
+      rn[u] = load(col(k < k1 ? k : k0));
     }
 #pragma unroll
     for (int u = 0; u < kPairBatch; ++u)
@@ -69,6 +71,18 @@ __device__ __forceinline__ void pair_pipeline(int k0, int k1, int sub, const flo
     for (int u = 0; u < kPairBatch; ++u) rj[u] = rn[u];
   }
 }
+
+// The same with one 16-byte record per neighbour, rec[col(k)].
+template <int KL, typename Col, typename F>
+__device__ __forceinline__ void pair_pipeline(int k0, int k1, int sub, const float* rec, Col col,
+                                              F f) {
+  pair_pipeline_with<KL>(k0, k1, sub, col, [&](int j) { return ld4(rec, j); }, f);
+}
+
+// Two records of one neighbour.
+struct Rec2 {
+  float4 a, b;
+};
 
 __device__ __forceinline__ float2 spiky_G(float xi, float yi, float xj, float yj, const KConst& K) {
   const PairGeo g = pair_geo(xi, yi, xj, yj, K.inv_h);
@@ -111,9 +125,12 @@ __global__ __launch_bounds__(kBS) void k_prepare(SphFluid f, const float* __rest
     d.p2[t] = 0.f;
     st2(d.apred, t, make_float2(0.f, 0.f));
     const float2 x = ld2(f.x, t);
-    st4(d.xrec, t, make_float4(x.x, x.y, 0.f, f.area[t] * f.rest[t]));
+    const float mass = f.area[t] * f.rest[t];
+    st4(d.xrec, t, make_float4(x.x, x.y, 0.f, mass));
     st4(d.arec, t, make_float4(0.f, 0.f, P.dt * P.dt * aact, 0.f));
-    st4(d.prec, t, make_float4(x.x, x.y, 0.f, 0.f));
+    // until the first k_accel rewrites it, prec carries what the alpha /
+    // source gather needs besides the position: {v*, A^act, A rho0}
+    st4(d.prec, t, make_float4(v.x + P.dt * a.x, v.y + P.dt * a.y, aact, mass));
   } else if (t < f.np + b.nb) {
     const int k = t - f.np;
     d.b_aact[k] = b.vol[k] / b.rho[k];
@@ -179,30 +196,31 @@ __global__ __launch_bounds__(kBS) void k_alpha_source(SphFluid f, const SphPhysi
     const KConst K = kconst(P.h);
     const float2 xi = ld2(f.x, i), vi = ld2(d.vpred, i);
     float k1x = 0.f, k1y = 0.f, k2 = 0.f, src = 0.f;
-    const int r1 = ff.rowptr[i + 1];
-    for (int k = ff.rowptr[i] + sub; k < r1; k += KL) {
-      const int j = ff.col[k];
-      const float2 xj = ld2(f.x, j), vj = ld2(d.vpred, j);
-      const float2 G = spiky_G(xi.x, xi.y, xj.x, xj.y, K);
-      const float aa = d.aact[j];
-      k1x += aa * G.x;
-      k1y += aa * G.y;
-      k2 += aa * aa / (f.area[j] * f.rest[j]) * (G.x * G.x + G.y * G.y);
-      src += -P.dt * aa * ((vi.x - vj.x) * G.x + (vi.y - vj.y) * G.y);
-    }
+    // fluid neighbours: xrec = {x, y, -, A rho0}, prec = {v*, A^act, A rho0}
+    pair_pipeline_with<KL>(
+        ff.rowptr[i], ff.rowptr[i + 1], sub, [&](int k) { return ff.col[k]; },
+        [&](int j) { return Rec2{ld4(d.xrec, j), ld4(d.prec, j)}; }, [&](const Rec2& r) {
+          const float2 G = spiky_G(xi.x, xi.y, r.a.x, r.a.y, K);
+          const float aa = r.b.z;
+          k1x += aa * G.x;
+          k1y += aa * G.y;
+          k2 += aa * aa / r.a.w * (G.x * G.x + G.y * G.y);
+          src += -P.dt * aa * ((vi.x - r.b.x) * G.x + (vi.y - r.b.y) * G.y);
+        });
     if (fb.rowptr) {
-      const int e1 = fb.rowptr[i + 1];
-      for (int k = fb.rowptr[i] + sub; k < e1; k += KL) {
-        const int j = fb.col[k];
-        const float2 xb = ld2(b.x, j);
-        const float2 G = spiky_G(xi.x, xi.y, xb.x, xb.y, K);
-        const float aa = d.b_aact[j];
-        k1x += aa * G.x;
-        k1y += aa * G.y;
-        k2 += aa * aa / (b.vol[j] * b.rest[j]) * (G.x * G.x + G.y * G.y);
-        // boundary predicted velocity is zero (static walls)
-        src += -P.dt * aa * (vi.x * G.x + vi.y * G.y);
-      }
+      // boundary neighbours: bxrec = {x, y, -, V rho0}; predicted velocity 0
+      // (static walls)
+      pair_pipeline_with<KL>(
+          fb.rowptr[i], fb.rowptr[i + 1], sub, [&](int k) { return fb.col[k]; },
+          [&](int j) { return Rec2{ld4(d.bxrec, j), make_float4(d.b_aact[j], 0.f, 0.f, 0.f)}; },
+          [&](const Rec2& r) {
+            const float2 G = spiky_G(xi.x, xi.y, r.a.x, r.a.y, K);
+            const float aa = r.b.x;
+            k1x += aa * G.x;
+            k1y += aa * G.y;
+            k2 += aa * aa / r.a.w * (G.x * G.x + G.y * G.y);
+            src += -P.dt * aa * (vi.x * G.x + vi.y * G.y);
+          });
     }
     k1x = group_sum<KL>(k1x);
     k1y = group_sum<KL>(k1y);
@@ -225,15 +243,14 @@ __global__ __launch_bounds__(kBS) void k_alpha_source(SphFluid f, const SphPhysi
     const KConst K = kconst(P.h);
     const float2 xb = ld2(b.x, k0);
     float s2 = 0.f, src = 0.f;
-    const int e1 = bf.rowptr[k0 + 1];
-    for (int k = bf.rowptr[k0] + sub; k < e1; k += KL) {
-      const int j = bf.col[k];
-      const float2 xf = ld2(f.x, j), vf = ld2(d.vpred, j);
-      const float2 G = spiky_G(xb.x, xb.y, xf.x, xf.y, K);
-      const float aa = d.aact[j];
-      s2 += aa * aa / (f.area[j] * f.rest[j]) * (G.x * G.x + G.y * G.y);
-      src += aa * ((0.f - vf.x) * G.x + (0.f - vf.y) * G.y);
-    }
+    pair_pipeline_with<KL>(
+        bf.rowptr[k0], bf.rowptr[k0 + 1], sub, [&](int k) { return bf.col[k]; },
+        [&](int j) { return Rec2{ld4(d.xrec, j), ld4(d.prec, j)}; }, [&](const Rec2& r) {
+          const float2 G = spiky_G(xb.x, xb.y, r.a.x, r.a.y, K);
+          const float aa = r.b.z;
+          s2 += aa * aa / r.a.w * (G.x * G.x + G.y * G.y);
+          src += aa * ((0.f - r.b.x) * G.x + (0.f - r.b.y) * G.y);
+        });
     s2 = group_sum<KL>(s2);
     src = group_sum<KL>(src);
     if (sub != 0) return;
