@@ -245,7 +245,8 @@ int sph_dsph_gather(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
 /* Packs one 32-byte record per particle for sph_dsph_gather_packed:
  * frec[np][8] = {x, y, vx, vy, rho_hat rho0, V, p, rho_hat} with
  * V = A / rho_hat (or f->vol) and p = EOS(rho_hat); brec[nb][8] likewise for
- * the boundary with V = V_b / rho_b and p = b->p. */
+ * the boundary with V = V_b / rho_b and p = b->p.  frec == NULL packs the
+ * boundary records only. */
 int sph_dsph_pack(const SphFluid* f, const SphPhysics* phys, const SphBoundary* b,
                   float* frec, float* brec, hipStream_t s);
 /* sph_dsph_gather on packed records (same terms, same results). */
@@ -253,6 +254,34 @@ int sph_dsph_gather_packed(const float* frec, const float* brec, const int* psce
                            const SphPhysics* phys, const SphCSR* ff, const SphCSR* fb,
                            const float* gradrho, unsigned terms, const int* order,
                            float* dpdt, float* acc, hipStream_t s);
+
+/* One RK4 stage combined into the packed gather (simulationBase.py:679-705
+ * applied to the rates of deltaSPH.py:146-200). */
+typedef struct SphRk4Stage {
+  int stage;            /* 1..4                                             */
+  const float* x0;      /* [np][2] state at the start of the step           */
+  const float* v0;      /* [np][2]                                          */
+  const float* r0;      /* [np]                                             */
+  const float* area;    /* [np] particle areas (for frec_next's volumes)    */
+  float* K;             /* [np][5] running sums, as sph_integrate_rk4's kacc */
+  float* x_out;         /* [np][2] next stage's state; the step's result at */
+  float* v_out;         /*         stage 4 (may alias x0 / v0 / r0 there)   */
+  float* rho_out;       /* [np]                                             */
+  float* acc_out;       /* stage 4: summed acceleration (as acc_out of
+                           sph_integrate_rk4), or NULL                      */
+  float* frec_next;     /* stages 1-3: packed records of the next stage's
+                           state (not frec), or NULL                        */
+} SphRk4Stage;
+/* sph_dsph_gather_packed followed, per particle, by sph_integrate_rk4 of the
+ * stage (stage velocity = the record's) and, for stages 1-3, the fluid
+ * records of the new state (as sph_dsph_pack with V = A / rho_hat): one
+ * launch instead of three, no dp/dt or acceleration arrays.  The same
+ * arithmetic as the three calls.  Replaces the integrator step around
+ * deltaSPH.timestep (simulationBase.py:679-705); SPH_T_ACCUM not allowed. */
+int sph_dsph_gather_rk4(const float* frec, const float* brec, const int* pscene, int np,
+                        const SphPhysics* phys, const SphCSR* ff, const SphCSR* fb,
+                        const float* gradrho, unsigned terms, const int* order,
+                        const SphRk4Stage* rk, hipStream_t s);
 /* Renormalisation matrix M_i (fluid + boundary neighbours) and L_i =
  * pinv2x2(M_i); M, L are [np][2][2]. */
 int sph_dsph_normalization(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,

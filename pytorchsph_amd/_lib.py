@@ -20,6 +20,12 @@ c_vp = C.c_void_p
 c_int = C.c_int
 
 
+class SphRk4Stage(C.Structure):
+    _fields_ = [("stage", c_int), ("x0", c_vp), ("v0", c_vp), ("r0", c_vp), ("area", c_vp),
+                ("K", c_vp), ("x_out", c_vp), ("v_out", c_vp), ("rho_out", c_vp),
+                ("acc_out", c_vp), ("frec_next", c_vp)]
+
+
 PERMUTE_MAX_FIELDS = 8
 
 
@@ -150,6 +156,8 @@ _SIGS = {
     "sph_dsph_pack": ([_P(SphFluid), c_vp, _P(SphBoundary), c_vp, c_vp, c_vp], c_int),
     "sph_dsph_gather_packed": ([c_vp, c_vp, c_vp, c_int, c_vp, _P(SphCSR), _P(SphCSR), c_vp,
                                 C.c_uint, c_vp, c_vp, c_vp, c_vp], c_int),
+    "sph_dsph_gather_rk4": ([c_vp, c_vp, c_vp, c_int, c_vp, _P(SphCSR), _P(SphCSR), c_vp,
+                             C.c_uint, c_vp, _P(SphRk4Stage), c_vp], c_int),
     "sph_integrate_rk4": ([c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                            c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "sph_integrate_semi_implicit": ([c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,

@@ -189,16 +189,26 @@ __device__ __forceinline__ void dsph_pair(const SphPhysics& P, const KConst& K, 
   }
 }
 
-__device__ __forceinline__ void dsph_store(const SphPhysics& P, const Own& o, const Acc& a,
-                                           unsigned terms, int i, float* dpdt, float* acc) {
-  const float dp = -a.cont + P.h * P.delta * P.c0 * (a.mog + a.dd);
+// dp/dt and acceleration of a row from its sums.  No FMA contraction, so
+// the values are the same whichever kernel (plain or fused with the RK4
+// combine) evaluates them.
+__device__ __forceinline__ void dsph_rates(const SphPhysics& P, const Own& o, const Acc& a,
+                                           unsigned terms, float& dp, float& ax, float& ay) {
+#pragma clang fp contract(off)
+  dp = -a.cont + P.h * P.delta * P.c0 * (a.mog + a.dd);
   const float inv_rho = 1.f / o.rho;
-  float ax = -a.px * inv_rho + P.h * P.alpha * P.c0 * a.vx - a.lx;
-  float ay = -a.py * inv_rho + P.h * P.alpha * P.c0 * a.vy - a.ly;
+  ax = -a.px * inv_rho + P.h * P.alpha * P.c0 * a.vx - a.lx;
+  ay = -a.py * inv_rho + P.h * P.alpha * P.c0 * a.vy - a.ly;
   if (has(terms, SPH_T_GRAV)) {
     ax += P.gx;
     ay += P.gy;
   }
+}
+
+__device__ __forceinline__ void dsph_store(const SphPhysics& P, const Own& o, const Acc& a,
+                                           unsigned terms, int i, float* dpdt, float* acc) {
+  float dp, ax, ay;
+  dsph_rates(P, o, a, terms, dp, ax, ay);
   if (has(terms, SPH_T_ACCUM)) {
     if (dpdt) dpdt[i] += dp;
     if (acc) {
@@ -270,16 +280,24 @@ __global__ __launch_bounds__(kBS) void k_dsph_gather(SphFluid f, const SphPhysic
 }
 
 // packed records: rec[2 i] = {x, y, vx, vy}, rec[2 i + 1] = {rho, V, p, rho_hat}
+__device__ __forceinline__ void pack_fluid(float4* __restrict__ frec, int t, const SphPhysics& P,
+                                           float x, float y, float vx, float vy, float rh,
+                                           float vol) {
+  frec[2 * t] = make_float4(x, y, vx, vy);
+  frec[2 * t + 1] = make_float4(rh * P.rho0, vol, eos(P, rh), rh);
+}
+
+// Threads t0 + thread id: fluid rows below f.np (skipped when t0 = f.np),
+// boundary rows above.
 __global__ __launch_bounds__(kBS) void k_dsph_pack(SphFluid f, const SphPhysics* __restrict__ phys,
                                                    SphBoundary b, float4* __restrict__ frec,
-                                                   float4* __restrict__ brec) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+                                                   float4* __restrict__ brec, int t0) {
+  const int t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (t < f.np) {
     const SphPhysics P = phys[f.pscene[t]];
     const float2 x = ld2(f.x, t), v = ld2(f.v, t);
     const float rh = f.rho[t];
-    frec[2 * t] = make_float4(x.x, x.y, v.x, v.y);
-    frec[2 * t + 1] = make_float4(rh * P.rho0, f.vol ? f.vol[t] : f.area[t] / rh, eos(P, rh), rh);
+    pack_fluid(frec, t, P, x.x, x.y, v.x, v.y, rh, f.vol ? f.vol[t] : f.area[t] / rh);
   } else if (t < f.np + b.nb) {
     const int k = t - f.np;
     const SphPhysics P = phys[b.bscene[k]];
@@ -291,12 +309,16 @@ __global__ __launch_bounds__(kBS) void k_dsph_pack(SphFluid f, const SphPhysics*
   }
 }
 
-template <unsigned T, int KL>
+// FUSE: the row's RK4 stage combine (rk4_particle) and the next stage's
+// record run in the epilogue instead of storing dp/dt and a (see
+// sph_dsph_gather_rk4); the record buffers of successive stages alternate,
+// since neighbours still read this stage's records.
+template <unsigned T, int KL, bool FUSE>
 __global__ __launch_bounds__(kBS) void k_dsph_gather_packed(
     const float4* __restrict__ frec, const float4* __restrict__ brec,
     const int* __restrict__ pscene, int np, const SphPhysics* __restrict__ phys, SphCSR ff,
     SphCSR fb, const float* __restrict__ gradrho, unsigned rterms, const int* __restrict__ order,
-    float* __restrict__ dpdt, float* __restrict__ acc) {
+    float* __restrict__ dpdt, float* __restrict__ acc, SphRk4Stage rk) {
   const unsigned terms = T ? T : rterms;
   const RowLane<KL> rl = row_lane<KL>();
   if (rl.t >= np) return;
@@ -350,7 +372,26 @@ __global__ __launch_bounds__(kBS) void k_dsph_gather_packed(
     }
   }
   reduce_acc<KL>(a, terms);
-  if (rl.sub == 0) dsph_store(P, o, a, terms, i, dpdt, acc);
+  if (rl.sub != 0) return;
+  if (!FUSE) {
+    dsph_store(P, o, a, terms, i, dpdt, acc);
+    return;
+  }
+  float dp, ax, ay;
+  dsph_rates(P, o, a, terms, dp, ax, ay);
+  const Rk4State s0 = {rk.x0[2 * i], rk.x0[2 * i + 1], rk.v0[2 * i], rk.v0[2 * i + 1], rk.r0[i]};
+  float2 asum;
+  const Rk4State n = rk4_particle(rk.stage, P, ax, ay, o.vx, o.vy, dp, true,
+                                  rk.K + 5 * (size_t)i, s0, &asum);
+  st2(rk.x_out, i, make_float2(n.x, n.y));
+  st2(rk.v_out, i, make_float2(n.vx, n.vy));
+  rk.rho_out[i] = n.r;
+  if (rk.stage == 4) {
+    if (rk.acc_out) st2(rk.acc_out, i, asum);
+  } else if (rk.frec_next) {
+    pack_fluid(reinterpret_cast<float4*>(rk.frec_next), i, P, n.x, n.y, n.vx, n.vy, n.r,
+               rk.area[i] / n.r);
+  }
 }
 
 // ------------------------------------------ renormalised density gradient
@@ -685,14 +726,15 @@ int dsph_gather(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, con
 
 int dsph_pack(const SphFluid* f, const SphPhysics* phys, const SphBoundary* b, float* frec,
               float* brec, hipStream_t s) {
-  SPH_REQUIRE(fluid_ok(f) && f->v && phys && frec, "sph_dsph_pack: bad arguments");
+  SPH_REQUIRE(fluid_ok(f) && f->v && phys, "sph_dsph_pack: bad arguments");
   const SphBoundary bb = bdy_or_null(b);
   SPH_REQUIRE(bb.nb == 0 || (bb.x && bb.rho && bb.vol && bb.bscene && brec),
               "sph_dsph_pack: bad boundary");
-  const int n = f->np + bb.nb;
-  if (n == 0) return SPH_OK;
+  const int t0 = frec ? 0 : f->np;   // no fluid records: boundary rows only
+  const int n = f->np + bb.nb - t0;
+  if (n <= 0) return SPH_OK;
   hipLaunchKernelGGL(k_dsph_pack, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, *f, phys, bb,
-                     reinterpret_cast<float4*>(frec), reinterpret_cast<float4*>(brec));
+                     reinterpret_cast<float4*>(frec), reinterpret_cast<float4*>(brec), t0);
   return launch_check("sph_dsph_pack");
 }
 
@@ -700,6 +742,26 @@ int dsph_pack(const SphFluid* f, const SphPhysics* phys, const SphBoundary* b, f
 constexpr unsigned kMaskMOG = SPH_T_CONT | SPH_T_CONT_B | SPH_T_MOG | SPH_T_PRESS | SPH_T_PRESS_B |
                               SPH_T_VISC | SPH_T_VISC_B | SPH_T_LAM | SPH_T_GRAV;
 constexpr unsigned kMaskVL = SPH_T_VISC | SPH_T_LAM | SPH_T_ACCUM;
+
+template <bool FUSE>
+static void launch_gather(const float4* fr, const float4* br, const int* pscene, int np,
+                          const SphPhysics* phys, const SphCSR& ff, const SphCSR& fb,
+                          const float* gradrho, unsigned terms, const int* order, float* dpdt,
+                          float* acc, const SphRk4Stage& rk, hipStream_t s) {
+  with_lanes(tuning(SPH_TUNE_GATHER_LANES), [&](auto kl) {
+    constexpr int KL = decltype(kl)::value;
+    const dim3 grid(blocks_for((int64_t)np * KL, kBS)), block(kBS);
+    if (terms == kMaskMOG)
+      hipLaunchKernelGGL((k_dsph_gather_packed<kMaskMOG, KL, FUSE>), grid, block, 0, s, fr, br,
+                         pscene, np, phys, ff, fb, gradrho, terms, order, dpdt, acc, rk);
+    else if (!FUSE && terms == kMaskVL)
+      hipLaunchKernelGGL((k_dsph_gather_packed<kMaskVL, KL, false>), grid, block, 0, s, fr, br,
+                         pscene, np, phys, ff, fb, gradrho, terms, order, dpdt, acc, rk);
+    else
+      hipLaunchKernelGGL((k_dsph_gather_packed<0, KL, FUSE>), grid, block, 0, s, fr, br, pscene,
+                         np, phys, ff, fb, gradrho, terms, order, dpdt, acc, rk);
+  });
+}
 
 int dsph_gather_packed(const float* frec, const float* brec, const int* pscene, int np,
                        const SphPhysics* phys, const SphCSR* ff, const SphCSR* fb,
@@ -711,23 +773,33 @@ int dsph_gather_packed(const float* frec, const float* brec, const int* pscene, 
   SPH_REQUIRE(!(terms & kBdyPairTerms) || !fb || !fb->rowptr || brec,
               "sph_dsph_gather_packed: boundary terms without boundary records");
   if (np == 0) return SPH_OK;
-  const float4* fr = reinterpret_cast<const float4*>(frec);
-  const float4* br = reinterpret_cast<const float4*>(brec);
-  const SphCSR fbc = csr_or_null(fb);
-  with_lanes(tuning(SPH_TUNE_GATHER_LANES), [&](auto kl) {
-    constexpr int KL = decltype(kl)::value;
-    const dim3 grid(blocks_for((int64_t)np * KL, kBS)), block(kBS);
-    if (terms == kMaskMOG)
-      hipLaunchKernelGGL((k_dsph_gather_packed<kMaskMOG, KL>), grid, block, 0, s, fr, br, pscene,
-                         np, phys, *ff, fbc, gradrho, terms, order, dpdt, acc);
-    else if (terms == kMaskVL)
-      hipLaunchKernelGGL((k_dsph_gather_packed<kMaskVL, KL>), grid, block, 0, s, fr, br, pscene,
-                         np, phys, *ff, fbc, gradrho, terms, order, dpdt, acc);
-    else
-      hipLaunchKernelGGL((k_dsph_gather_packed<0, KL>), grid, block, 0, s, fr, br, pscene, np,
-                         phys, *ff, fbc, gradrho, terms, order, dpdt, acc);
-  });
+  const SphRk4Stage none = {};
+  launch_gather<false>(reinterpret_cast<const float4*>(frec), reinterpret_cast<const float4*>(brec),
+                       pscene, np, phys, *ff, csr_or_null(fb), gradrho, terms, order, dpdt, acc,
+                       none, s);
   return launch_check("sph_dsph_gather_packed");
+}
+
+int dsph_gather_rk4(const float* frec, const float* brec, const int* pscene, int np,
+                    const SphPhysics* phys, const SphCSR* ff, const SphCSR* fb,
+                    const float* gradrho, unsigned terms, const int* order, const SphRk4Stage* rk,
+                    hipStream_t s) {
+  SPH_REQUIRE(frec && pscene && phys && ff && ff->rowptr && np >= 0 && rk,
+              "sph_dsph_gather_rk4: bad arguments");
+  SPH_REQUIRE(rk->stage >= 1 && rk->stage <= 4, "sph_dsph_gather_rk4: stage must be 1..4");
+  SPH_REQUIRE(rk->x0 && rk->v0 && rk->r0 && rk->K && rk->x_out && rk->v_out && rk->rho_out,
+              "sph_dsph_gather_rk4: missing state arrays");
+  SPH_REQUIRE(!rk->frec_next || (rk->area && rk->frec_next != frec),
+              "sph_dsph_gather_rk4: next records need the areas and a buffer of their own");
+  SPH_REQUIRE(!(terms & SPH_T_ACCUM), "sph_dsph_gather_rk4: SPH_T_ACCUM has no meaning here");
+  SPH_REQUIRE(!(terms & SPH_T_DDSPH) || gradrho, "sph_dsph_gather_rk4: DDSPH needs gradrho");
+  SPH_REQUIRE(!(terms & kBdyPairTerms) || !fb || !fb->rowptr || brec,
+              "sph_dsph_gather_rk4: boundary terms without boundary records");
+  if (np == 0) return SPH_OK;
+  launch_gather<true>(reinterpret_cast<const float4*>(frec), reinterpret_cast<const float4*>(brec),
+                      pscene, np, phys, *ff, csr_or_null(fb), gradrho, terms, order, nullptr,
+                      nullptr, *rk, s);
+  return launch_check("sph_dsph_gather_rk4");
 }
 
 int dsph_normalization(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
@@ -815,6 +887,12 @@ int sph_dsph_gather_packed(const float* frec, const float* brec, const int* psce
                            float* acc, hipStream_t s) {
   return sph::dsph_gather_packed(frec, brec, pscene, np, phys, ff, fb, gradrho, terms, order, dpdt,
                                  acc, s);
+}
+int sph_dsph_gather_rk4(const float* frec, const float* brec, const int* pscene, int np,
+                        const SphPhysics* phys, const SphCSR* ff, const SphCSR* fb,
+                        const float* gradrho, unsigned terms, const int* order,
+                        const SphRk4Stage* rk, hipStream_t s) {
+  return sph::dsph_gather_rk4(frec, brec, pscene, np, phys, ff, fb, gradrho, terms, order, rk, s);
 }
 int sph_dsph_normalization(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
                            const SphBoundary* b, const SphCSR* fb, const int* order, float* M,

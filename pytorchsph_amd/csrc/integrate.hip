@@ -21,39 +21,19 @@ __global__ __launch_bounds__(kBS) void k_rk4(int stage, const int* __restrict__ 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const SphPhysics P = phys[pscene[i]];
-  const float kax = acc[2 * i], kay = acc[2 * i + 1];
-  const float kvx = vs[2 * i], kvy = vs[2 * i + 1];
-  const float kr = dpdt ? dpdt[i] : 0.f;
-  float* k = K + 5 * (size_t)i;
-  const float w = (stage == 1 || stage == 4) ? 1.f : 2.f;
-  float s0, s1, s2, s3, s4;
-  if (stage == 1) {
-    s0 = kax; s1 = kay; s2 = kvx; s3 = kvy; s4 = kr;
-  } else {
-    s0 = k[0] + w * kax; s1 = k[1] + w * kay; s2 = k[2] + w * kvx; s3 = k[3] + w * kvy;
-    s4 = k[4] + w * kr;
-  }
-  if (stage < 4) {
-    k[0] = s0; k[1] = s1; k[2] = s2; k[3] = s3; k[4] = s4;
-    const float c = (stage == 3 ? 1.f : 0.5f) * P.dt;
-    vo[2 * i] = v0[2 * i] + c * kax;
-    vo[2 * i + 1] = v0[2 * i + 1] + c * kay;
-    xo[2 * i] = x0[2 * i] + c * kvx;
-    xo[2 * i + 1] = x0[2 * i + 1] + c * kvy;
-    if (dpdt) ro[i] = r0[i] + c * kr / P.rho0;
-    else if (ro != r0) ro[i] = r0[i];
-  } else {
-    const float c = (1.f / 6.f) * P.dt;
-    vo[2 * i] = v0[2 * i] + c * s0;
-    vo[2 * i + 1] = v0[2 * i + 1] + c * s1;
-    xo[2 * i] = x0[2 * i] + c * s2;
-    xo[2 * i + 1] = x0[2 * i + 1] + c * s3;
-    if (dpdt) ro[i] = r0[i] + c * s4 / P.rho0;
-    else if (ro != r0) ro[i] = r0[i];
-    if (ao) {
-      ao[2 * i] = s0;
-      ao[2 * i + 1] = s1;
-    }
+  const Rk4State s0 = {x0[2 * i], x0[2 * i + 1], v0[2 * i], v0[2 * i + 1], r0[i]};
+  float2 asum;
+  const Rk4State o = rk4_particle(stage, P, acc[2 * i], acc[2 * i + 1], vs[2 * i], vs[2 * i + 1],
+                                  dpdt ? dpdt[i] : 0.f, dpdt != nullptr, K + 5 * (size_t)i, s0,
+                                  &asum);
+  vo[2 * i] = o.vx;
+  vo[2 * i + 1] = o.vy;
+  xo[2 * i] = o.x;
+  xo[2 * i + 1] = o.y;
+  if (dpdt || ro != r0) ro[i] = o.r;
+  if (stage == 4 && ao) {
+    ao[2 * i] = asum.x;
+    ao[2 * i + 1] = asum.y;
   }
 }
 

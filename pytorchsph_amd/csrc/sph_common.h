@@ -317,4 +317,50 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int* tot) {
   return res;
 }
 
+// ------------------------------------------------------------- RK4 stage
+// One particle of RK4 stage `stage` = 1..4 (simulationBase.py:679-705).
+// (ka, kv, kr) are the stage's rates (acceleration, velocity, dp/dt), s0 the
+// state at the start of the step, k the particle's five running weighted
+// sums.  Stages 1-3 return the next stage's state (c = dt/2, dt/2, dt);
+// stage 4 returns the step's result (c = dt/6) and the summed acceleration
+// in *asum.  Without a density rate (rate_r false) the density is carried.
+// Shared by k_rk4 and the gather that fuses the stage combine; no FMA
+// contraction, so both round the same way.
+struct Rk4State {
+  float x, y, vx, vy, r;
+};
+__device__ __forceinline__ Rk4State rk4_particle(int stage, const SphPhysics& P, float kax,
+                                                 float kay, float kvx, float kvy, float kr,
+                                                 bool rate_r, float* __restrict__ k,
+                                                 const Rk4State& s0, float2* asum) {
+#pragma clang fp contract(off)
+  const float w = (stage == 1 || stage == 4) ? 1.f : 2.f;
+  float q0, q1, q2, q3, q4;
+  if (stage == 1) {
+    q0 = kax; q1 = kay; q2 = kvx; q3 = kvy; q4 = kr;
+  } else {
+    q0 = k[0] + w * kax; q1 = k[1] + w * kay; q2 = k[2] + w * kvx; q3 = k[3] + w * kvy;
+    q4 = k[4] + w * kr;
+  }
+  Rk4State o;
+  if (stage < 4) {
+    k[0] = q0; k[1] = q1; k[2] = q2; k[3] = q3; k[4] = q4;
+    const float c = (stage == 3 ? 1.f : 0.5f) * P.dt;
+    o.vx = s0.vx + c * kax;
+    o.vy = s0.vy + c * kay;
+    o.x = s0.x + c * kvx;
+    o.y = s0.y + c * kvy;
+    o.r = rate_r ? s0.r + c * kr / P.rho0 : s0.r;
+  } else {
+    const float c = (1.f / 6.f) * P.dt;
+    o.vx = s0.vx + c * q0;
+    o.vy = s0.vy + c * q1;
+    o.x = s0.x + c * q2;
+    o.y = s0.y + c * q3;
+    o.r = rate_r ? s0.r + c * q4 / P.rho0 : s0.r;
+    if (asum) *asum = make_float2(q0, q1);
+  }
+  return o;
+}
+
 }  // namespace sph

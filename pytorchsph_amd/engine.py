@@ -67,8 +67,14 @@ class SceneBatch:
 
     def __init__(self, scenes: list[Scene], device="cuda", pair_cap: float = 40.0,
                  bdy_pair_cap: float = 40.0, dfsph_chunk: int = 8, reorder: bool = True,
-                 carry: list | None = None):
-        """carry[k], when given, continues scene k from another batch: a dict
+                 carry: list | None = None, fuse_rk4: bool = False):
+        """fuse_rk4: delta-SPH stages run the RK4 combine and the next
+        stage's records inside the gather (sph_dsph_gather_rk4) instead of
+        as separate kernels; the results are the same, bit for bit.  Off by
+        default: the fused gather needs 70 registers instead of 63 (seven
+        wavefronts per SIMD instead of eight) and the step takes as long.
+
+        carry[k], when given, continues scene k from another batch: a dict
         from `export` with device tensors of its positions, velocities and
         normalised density, its particle ids and the cell order of that
         batch's last search.  The scene then advances exactly as it would
@@ -89,6 +95,7 @@ class SceneBatch:
         self.solid = scenes[0].boundary == "solid"
         self.dfsph_chunk = dfsph_chunk
         self.reorder = reorder
+        self.fuse_rk4 = fuse_rk4
         self._sorted_valid = False
         f32 = torch.float32
 
@@ -160,6 +167,8 @@ class SceneBatch:
             self.S = [(torch.empty_like(self.x), torch.empty_like(self.v), torch.empty_like(self.rho))
                       for _ in range(2)]
             self.K = torch.zeros((NP, 5), dtype=f32, device=dev)
+            # records of the next RK4 stage while the gather reads this one's
+            self.frec2 = torch.zeros_like(self.frec) if fuse_rk4 else None
             self.gradrho = torch.zeros((NP, 2), dtype=f32, device=dev)
             self.Lmat = torch.zeros((NP, 4), dtype=f32, device=dev)
         else:
@@ -303,8 +312,10 @@ class SceneBatch:
                L.ptr(self.tmp), L.stream_handle(s))
 
     # ---------------------------------------------------------- delta-SPH
-    def dsph_rhs(self, x, v, rho, s=None):
-        """dpdt / acc of the state (x, v, rho) into self.dpdt / self.acc."""
+    def _dsph_prepare(self, x, v, rho, frec, s=None):
+        """Search, mDBC boundary density, delta-SPH density gradient and the
+        records (fluid records into frec unless it is None) of the state
+        (x, v, rho).  Returns the gather's arguments."""
         self.search(x, s)
         F = self.fluid(x, v, rho)
         sh = L.stream_handle(s)
@@ -322,12 +333,17 @@ class SceneBatch:
                    fbp, self._order(), None, L.ptr(self.Lmat), None, sh)
             L.call("sph_dsph_density_gradient", C.byref(F), L.ptr(self.phys), C.byref(ff),
                    C.byref(B), fbp, L.ptr(self.Lmat), self._order(), L.ptr(self.gradrho), sh)
-        L.call("sph_dsph_pack", C.byref(F), L.ptr(self.phys), C.byref(B), L.ptr(self.frec),
-               L.ptr(self.brec), sh)
+        L.call("sph_dsph_pack", C.byref(F), L.ptr(self.phys), C.byref(B),
+               L.ptr(frec) if frec is not None else None, L.ptr(self.brec), sh)
+        return ff, fbp
+
+    def dsph_rhs(self, x, v, rho, s=None):
+        """dpdt / acc of the state (x, v, rho) into self.dpdt / self.acc."""
+        ff, fbp = self._dsph_prepare(x, v, rho, self.frec, s)
         L.call("sph_dsph_gather_packed", L.ptr(self.frec), L.ptr(self.brec), L.ptr(self.pscene),
                self.NP, L.ptr(self.phys), C.byref(ff), fbp,
                L.ptr(self.gradrho) if self.terms & L.T_DDSPH else None, self.terms, self._order(),
-               L.ptr(self.dpdt), L.ptr(self.acc), sh)
+               L.ptr(self.dpdt), L.ptr(self.acc), L.stream_handle(s))
 
     def _dsph_step(self, s=None):
         self.reorder_state(s)
@@ -335,6 +351,22 @@ class SceneBatch:
         S0, S1 = self.S
         stages = [(P, S0), (S0, S1), (S1, S0), (S0, P)]
         sh = L.stream_handle(s)
+        if self.fuse_rk4:
+            # stage k gathers recs[(k - 1) % 2] and writes the next stage's
+            # records into recs[k % 2]; stage 1 packs its own
+            recs = (self.frec, self.frec2)
+            for k, (cur, out) in enumerate(stages, start=1):
+                frec, nxt = recs[(k - 1) % 2], recs[k % 2]
+                ff, fbp = self._dsph_prepare(*cur, frec if k == 1 else None, s)
+                rk = L.SphRk4Stage(k, L.ptr(self.x), L.ptr(self.v), L.ptr(self.rho),
+                                   L.ptr(self.area), L.ptr(self.K), L.ptr(out[0]), L.ptr(out[1]),
+                                   L.ptr(out[2]), L.ptr(self.acc) if k == 4 else None,
+                                   L.ptr(nxt) if k < 4 else None)
+                L.call("sph_dsph_gather_rk4", L.ptr(frec), L.ptr(self.brec), L.ptr(self.pscene),
+                       self.NP, L.ptr(self.phys), C.byref(ff), fbp,
+                       L.ptr(self.gradrho) if self.terms & L.T_DDSPH else None, self.terms,
+                       self._order(), C.byref(rk), sh)
+            return
         for k, (cur, out) in enumerate(stages, start=1):
             self.dsph_rhs(*cur, s=s)
             L.call("sph_integrate_rk4", k, L.ptr(self.pscene), L.ptr(self.phys), self.NP,

@@ -49,6 +49,7 @@ def test_record_layouts():
     assert C.sizeof(L.SphGrid) == 8 * 3 + 4 * 3 + 4 + 8 * 7
     assert C.sizeof(L.SphQuery) == 8 * 3 + 4 * 3 + 4
     assert C.sizeof(L.SphFields) == 8 * 8 + 4 * 8 + 4 + 4
+    assert C.sizeof(L.SphRk4Stage) == 8 + 8 * 10
 
 
 def test_scan_workspace_sizes():

@@ -92,6 +92,26 @@ def test_cold_capture_matches_eager(cuda, scheme):
                 assert np.abs(g["x"] - e["x"]).max() < 1e-6
 
 
+@pytest.mark.parametrize("reorder", [True, False])
+def test_fused_rk4_step_matches_separate_kernels(cuda, reorder):
+    """sph_dsph_gather_rk4 (gather + RK4 stage combine + next records in one
+    launch) gives the state of the separate gather, integrate and pack
+    kernels, bit for bit, over several steps."""
+    scenes = mix("dsph")
+    fused = SceneBatch(scenes, reorder=reorder, fuse_rk4=True)
+    split = SceneBatch(scenes, reorder=reorder, fuse_rk4=False)
+    for _ in range(4):
+        fused.step()
+        split.step()
+    torch.cuda.synchronize()
+    for k in range(len(scenes)):
+        a, b = fused.scene_state(k), split.scene_state(k)
+        for key in ("x", "v", "rho"):
+            d = np.abs(a[key].astype(np.float64) - b[key])
+            assert np.array_equal(a[key], b[key]), (k, key, int((d > 0).sum()), a[key].size,
+                                                    float(d.max()), float(np.abs(b[key]).max()))
+
+
 def test_permute_fields_matches_numpy(cuda):
     """sph_permute_fields moves every field by the same permutation, int64
     ids bit for bit, with fields of widths 1, 2 and 4."""
