@@ -70,8 +70,9 @@ def _events_avg(launch, stream, reps=20, warm=3):
 
 
 def pressure_roofline(b, reps=20):
-    """sph_dfsph_pressure (k_pressure), the dominant kernel of the batched
-    step, with every DFSPH scene active, and its algorithmic HBM bytes:
+    """sph_dfsph_pressure (k_pressure), the DFSPH pressure update, with every
+    DFSPH scene active (after one sph_dfsph_accel, whose records it reads),
+    and its algorithmic HBM bytes:
       per fluid row      76 B: fluid and boundary row pointers (4 + 4), own
                          xrec and arec records (16 + 16), src, p2, alpha,
                          rho, rest (20), p, p2, resid written (12), xrec.z
@@ -102,6 +103,9 @@ def pressure_roofline(b, reps=20):
     D = b._dfsph_struct()
     stream = torch.cuda.current_stream()
     sh = L.stream_handle(stream)
+
+    L.call("sph_dfsph_accel", C.byref(F), L.ptr(b.phys), C.byref(ff),
+           C.byref(fb) if fb is not None else None, C.byref(D), sh)
 
     def launch():
         L.call("sph_dfsph_pressure", C.byref(F), L.ptr(b.phys), C.byref(ff), C.byref(B),
@@ -199,7 +203,11 @@ def plain_paths(steps: int = 3, warmup: int = 1):
       dsph_4m       plain delta-SPH RK4 step of a 2000 x 2000 dam break
                     (dx = 0.0005, 4,000,000 fluid particles, mDBC walls)
       dfsph_c3      plain DFSPH step of the 1,000,000-particle dam break
-                    (dx = 0.001, Akinci walls), solved to tau = 1e-3
+                    (dx = 0.001, Akinci walls), solved to tau = 1e-3, and
+                    the pressure update's roofline on it (every row active)
+      dsph_c1       the 5,000-particle delta-SPH dam break of the CPU
+                    baseline, one graph-captured step per synchronise
+                    (latency-bound at this size)
     Algorithmic bytes (compulsory, each array once; K = mean neighbours,
     c = cells per point):
       search        x, h read (12) + sorted x, index written (12) + key (4)
@@ -274,9 +282,34 @@ def plain_paths(steps: int = 3, warmup: int = 1):
     out["dfsph_c3"] = dict(particles=b.NP, ms_per_step=sec * 1e3,
                            particle_steps_per_s=b.NP / sec,
                            iterations_last_step=int(b.dfsph_iterations()[0]))
+    rl = pressure_roofline(b)
+    out["dfsph_c3"]["k_pressure"] = dict(avg_launch_us=rl["seconds"] * 1e6,
+                                         bytes_per_launch=rl["bytes"],
+                                         achieved_gbs=rl["achieved_gbs"],
+                                         frac=rl["achieved_gbs"] / peak)
     del b
     torch.cuda.empty_cache()
     log(f"plain DFSPH C3: {out['dfsph_c3']}")
+    # ---- C1: the 5,000-particle delta-SPH dam break, one scene on the GPU
+    sc = W.c1_scene()
+    b = SceneBatch([sc])
+    b.capture()
+    for _ in range(warmup):
+        b.step_graph()
+    torch.cuda.synchronize()
+    lat = []
+    for _ in range(50):
+        t0 = time.perf_counter()
+        b.step_graph()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+    lat = np.array(lat)
+    out["dsph_c1"] = dict(particles=b.NP, ms_per_step=float(lat.mean() * 1e3),
+                          particle_steps_per_s=float(b.NP / lat.mean()),
+                          p50_step_ms=float(np.percentile(lat, 50) * 1e3),
+                          p99_step_ms=float(np.percentile(lat, 99) * 1e3), steps=len(lat))
+    del b
+    log(f"plain delta-SPH C1: {out['dsph_c1']}")
     return out
 
 
@@ -325,6 +358,53 @@ def cpu_baseline(seconds: float):
     return dict(value=sc.n / med, unit="particle-steps/s", cores=threads, kind="port",
                 sample=f"oracle delta-SPH RK4 dam break, {sc.n} fluid + {sc.nb} boundary "
                        f"particles, median of {len(times)} steps after 3 warm-up")
+
+
+def cpu_paths():
+    """The oracle (fp32 search, FP64 physics, OpenMP) on the other
+    single-scene workloads, on this host:
+      dsph_c1_1thread   the CPU baseline's C1 step on one thread (median of
+                        20 steps after 3 warm-up)
+      search_c2         the 4,000,000-point search, all threads, one search
+      dfsph_c3          one DFSPH step of the 1,000,000-particle dam break,
+                        all threads (the search included)"""
+    from oracle import oracle as O
+    from pytorchsph_amd import workloads as W
+
+    out = {}
+    threads = os.cpu_count() or 1
+    sc = W.c1_scene()
+    so = O.SceneOracle(sc)
+    O.set_threads(1)
+    x, v, r = sc.x.astype(np.float64), sc.v.astype(np.float64), np.ones(sc.n)
+    for _ in range(3):
+        x, v, r = so.dsph_step(x, v, r)
+    times = []
+    for _ in range(20):
+        t0 = time.perf_counter()
+        x, v, r = so.dsph_step(x, v, r)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    out["dsph_c1_1thread"] = dict(particle_steps_per_s=sc.n / med, ms_per_step=med * 1e3,
+                                  threads=1)
+    O.set_threads(threads)
+    x, h = W.c2_points()
+    qx, qy, cx, _ = O.grid_params((0.0, 0.0), (1.0, 1.0), h)
+    hq = np.full(len(x), h, np.float32)
+    t0 = time.perf_counter()
+    csr = O.search(x, hq, x, (qx, qy), h, cx, 1, self_mode=True)
+    sec = time.perf_counter() - t0
+    out["search_c2"] = dict(seconds=sec, searches_per_s=1.0 / sec, pairs=int(csr.nnz),
+                            threads=threads)
+    sc = W.c3_scene()
+    so = O.SceneOracle(sc)
+    t0 = time.perf_counter()
+    _, _, _, res = so.dfsph_step(sc.x, sc.v)
+    sec = time.perf_counter() - t0
+    out["dfsph_c3"] = dict(seconds_per_step=sec, particle_steps_per_s=sc.n / sec,
+                           iterations=int(res["iters"]), threads=threads)
+    log(f"cpu paths: {out}")
+    return out
 
 
 def main():
@@ -433,6 +513,7 @@ def main():
         log(f"serving: {serve}")
 
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
+    cpu_more = None if (args.no_cpu or args.no_paths) else cpu_paths()
     peak = 8000.0
     out = {
         "metric": "particle-steps/s on 1 GPU, batched vs plain; p50/p99 step latency under load",
@@ -477,6 +558,7 @@ def main():
         "serving": serve,
         "plain_paths": paths,
         "cpu_baseline": cpu,
+        "cpu_paths": cpu_more,
     }
     print(json.dumps(out), flush=True)
 
