@@ -22,6 +22,15 @@ read only its own scene's data.  Admission is by device footprint
 A job's result is read back the tick it finishes; a finished job keeps
 being stepped (and ignored) until the next repack drops it.  Per-step
 latency is the wall time of the tick in which a job advanced.
+
+The particle cap (`max_particles`, default 2M) bounds the tick.  A tick
+lasts as long as its slowest scene: the DFSPH iterations of a tick are the
+maximum over its scenes, and the early ones run over nearly every row, so
+a batch of more scenes costs more per particle.  Below ~1.5M particles the
+per-tick overheads take over instead.  On the 10 s Poisson stream of the
+benchmark, caps of 1, 1.5, 2, 3, 6 and 12M particles gave 2.8, 3.1, 3.1,
+2.5, 1.6 and 1.7e8 completed particle-steps/s with p99 ticks of 11, 15,
+19, 59, 92 and 128 ms (more jobs wait instead).
 """
 from __future__ import annotations
 
@@ -60,7 +69,7 @@ class Job:
 
 class Server:
     def __init__(self, device="cuda", budget_bytes: int | None = None,
-                 max_particles: int = 12_000_000, min_repack_s: float = 0.5,
+                 max_particles: int = 2_000_000, min_repack_s: float = 0.5,
                  waste_frac: float = 0.5, log=None, **batch_kw):
         self.device = device
         if budget_bytes is None:
