@@ -53,6 +53,12 @@
  *                                akinciBoundary.py:166-389)
  *   sph_integrate_*           <- integrateValues / integrate
  *                                (simulationBase.py:564-569,632-705)
+ *   sph_normalization_rows    <- computeNormalizationMatrix + pinv2x2 for
+ *                                boundary rows (solidBoundary.py:676-693)
+ *   sph_dplus_*               <- deltaPlusModule: detectSurface, shift
+ *                                (modules/deltaPlus.py:81-251) and
+ *                                computePressureAccelDeltaPlus
+ *                                (pressure.py:40-52, solidBoundary.py:882-887)
  */
 #ifndef SPH_MI355X_H
 #define SPH_MI355X_H
@@ -430,6 +436,73 @@ typedef struct SphFields {
   int nfields;
 } SphFields;
 int sph_permute_fields(const SphFields* f, const int* perm, int n, float* tmp, hipStream_t s);
+
+/* ------------------------------------------------ delta-plus shifting */
+/* Renormalisation matrices of arbitrary query rows over up to two pair
+ * lists (computeNormalizationMatrix, densityDiffusion.py:30-43, as the solid
+ * boundary calls it for its own rows, solidBoundary.py:676-693):
+ *   M_q = sum_{j in a(q)} va_j (xa_j - xq_q) (x) gradW_qj
+ *       + sum_{j in b(q)} vb_j (xb_j - xq_q) (x) gradW_qj,
+ * L = pinv2x2(M), eig = its two singular values, larger first (util.py:
+ * 386-433).  A list whose rowptr is NULL is skipped.  M and eig may be NULL.
+ * qscene[q] selects the support. */
+int sph_normalization_rows(const float* xq, const int* qscene, int nq, const SphPhysics* phys,
+                           const SphCSR* a, const float* xa, const float* va,
+                           const SphCSR* b, const float* xb, const float* vb,
+                           float* M, float* L, float* eig, hipStream_t s);
+
+/* Gradient of lambda and the fluid normal (deltaPlus.py:132-163).  lambda is
+ * the smaller singular value of M: eig[i][1] for fluid rows, beig[b][1] for
+ * boundary points.
+ *   g_i = -sum_{j in ff(i)} V_j (lam_j - lam_i) L_i gradW_ij
+ *         -sum_{b in fb(i)} bvol_n[b] (lam_b - lam_i) L_i gradW_ib,
+ *   normal_i = g_i / (|g_i| + eps),
+ * with V_j = f->vol[j] (or A_j / rho_j) and eps = 0.1 h^2.  bvol_n is the
+ * volume the reference uses for boundary neighbour b: fluidArea[b] /
+ * fluidDensity[b], the fluid arrays indexed by the boundary index
+ * (deltaPlus.py:151); the caller passes it.  grad may be NULL. */
+int sph_dplus_normal(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                     const float* L, const float* eig, const float* bx, const float* beig,
+                     const float* bvol_n, const SphCSR* fb, float* grad, float* normal,
+                     hipStream_t s);
+
+/* Free-surface classification (deltaPlus.py:165-209).  Over the non-self
+ * fluid pairs d_ij = (x_j - x_i)/|x_j - x_i| . n_i and over the boundary
+ * pairs d_ib = (x_b - x_i)/|x_b - x_i| . n_i; c_i = max(max_j d_ij,
+ * max_b d_ib), each max 0 when the row has no such pair; mask_i = 1 when
+ * acos(c_i) > pi/6 and lam_i < 0.6, else 0; m2_i = max over non-self fluid
+ * neighbours of mask_j.  surface_mask_i = 0 where mask_i = 1, else 1 where
+ * m2_i = 1 and lam_i >= 0.5, else 2/3 where m2_i = 1, else 2.
+ * mask_tmp is [np] scratch.  angle_min / angle_max (acos of the fluid-pair
+ * min / max, 0 taken for a row without pairs) may be NULL. */
+int sph_dplus_surface(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                      const float* normal, const float* eig, const float* bx, const SphCSR* fb,
+                      float* mask_tmp, float* angle_min, float* angle_max,
+                      float* surface_mask, hipStream_t s);
+
+/* Particle shift (deltaPlus.py:81-115 and 211-236):
+ *   s_i = -1.5 Ma 4h^2 [ sum_{j in ff(i)} k_ij A_j / (rho_i + rho_j) gradW_ij
+ *                      + sum_{b in fb(i)} k_ib V_b / (rho_i + rho_b) gradW_ib ],
+ *   k_ij = 1 + 0.2 (W_ij / W(packing))^4,  Ma = max_k |v_k| / c0 over the
+ * particle's scene (rho normalised, V_b = b->vol, rho_b = b->rho); then
+ * |s_i| is clamped to 0.05 h, s_i <- (I - n_i n_i^T) s_i where
+ * surface_mask_i < 1.5 and s_i <- 0 where surface_mask_i < 0.5.
+ * packing[scene] is the lattice spacing over the support.  vmax_ws is
+ * [nscenes] scratch words (cleared here by a kernel).  b / fb may be NULL. */
+int sph_dplus_shift(const SphFluid* f, const SphPhysics* phys, int nscenes, const float* packing,
+                    const SphCSR* ff, const SphBoundary* b, const SphCSR* fb,
+                    const float* normal, const float* surface_mask, unsigned* vmax_ws,
+                    float* shift, hipStream_t s);
+
+/* delta-plus pressure acceleration (pressure.py:40-52; boundary part
+ * solidBoundary.py:882-887): a_i = -(1/rho_i) sum_j p_ij V_j gradW_ij with
+ * p_ij = p_i + p_j where p_i >= 0 or surface_mask_i < 1.5, else p_j - p_i;
+ * p_i = EOS(rho_i), boundary p_b = b->p, V_b = b->vol / b->rho.  terms:
+ * SPH_T_PRESS (fluid pairs), SPH_T_PRESS_B (boundary pairs), SPH_T_ACCUM
+ * (add into acc). */
+int sph_dplus_pressure(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                       const SphBoundary* b, const SphCSR* fb, const float* surface_mask,
+                       unsigned terms, float* acc, hipStream_t s);
 
 #ifdef __cplusplus
 }

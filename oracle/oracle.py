@@ -8,6 +8,7 @@ always as the checker.  It restates the reference's algorithms:
                   pressure.py, diffusion.py, laminar.py, gravity.py and the
                   solid boundary terms of solidBoundary.py:676-895
   mdbc            solidBoundary.py:561-616
+  dplus_*         deltaPlus.py:81-251, pressure.py:40-52 (delta-plus shifting)
   dfsph           modules/dfsph.py:25-311 + akinciBoundary.py:166-389
   rk4 / semi-implicit steps   simulationBase.py:564-569, 632-705
 Physics is FP64; the neighbour predicate is the reference's fp32 one.
@@ -79,6 +80,9 @@ def lib():
         _lib.orc_pinv2x2.argtypes = [_vp, C.c_int, _vp, _vp]
         _lib.orc_set_threads.argtypes = [C.c_int]
         _lib.orc_get_threads.restype = C.c_int
+        _lib.orc_norm_rows.argtypes = [_vp, C.c_int, C.c_double] + [_vp] * 9
+        _lib.orc_dplus_shift.argtypes = [_vp, _vp, C.c_double] + [_vp] * 7
+        _lib.orc_dplus_pressure.argtypes = [_vp] * 6 + [C.c_uint, _vp]
         for name in ("orc_W", "orc_dW", "orc_dWs"):
             fn = getattr(_lib, name)
             fn.restype = C.c_double
@@ -295,6 +299,87 @@ def dfsph(f, phys, ff: CSR, b, fb: CSR, bf: CSR, acc, fixed_iters=0, max_hist=51
     return out
 
 
+
+# ------------------------------------------------------ delta-plus shifting
+def _csr_ref(c: CSR | None):
+    return _byref(c.c() if c is not None else None)
+
+
+def norm_rows(xq, h, a: CSR | None, xa, va, b: CSR | None = None, xb=None, vb=None):
+    """M, L, eig of arbitrary query rows over two lists (densityDiffusion.py:
+    30-43 for the boundary rows, solidBoundary.py:676-693)."""
+    xq = _f64(xq)
+    nq = len(xq)
+    M, Lm, e = np.zeros((nq, 4)), np.zeros((nq, 4)), np.zeros((nq, 2))
+    keep = _Keep()
+    lib().orc_norm_rows(keep(xq), nq, float(h), _csr_ref(a), keep(_f64(xa)), keep(_f64(va)),
+                        _csr_ref(b), keep(_f64(xb)), keep(_f64(vb)), _p(M), _p(Lm), _p(e))
+    return M.reshape(nq, 2, 2), Lm.reshape(nq, 2, 2), e
+
+
+def fluid_norm(f, phys, ff: CSR, b=None, fb: CSR | None = None):
+    """M, L, eig of the fluid rows with the boundary part (deltaPlus.py:116-131)."""
+    keep = _Keep()
+    F, B = fluid_struct(f, keep), bdy_struct(b, keep)
+    n = F.n
+    M, Lm, e = np.zeros((n, 4)), np.zeros((n, 4)), np.zeros((n, 2))
+    P = phys_struct(phys)
+    lib().orc_fluid_norm(C.byref(F), C.byref(P), C.byref(ff.c()), _byref(B), _csr_ref(fb), _p(M),
+                         _p(Lm), _p(e))
+    return M.reshape(n, 2, 2), Lm.reshape(n, 2, 2), e
+
+
+def dplus_normal(f, phys, ff: CSR, Lm, eig, bx=None, beig=None, bvol_n=None, fb: CSR | None = None):
+    """lambda gradient and fluid normal (deltaPlus.py:132-163)."""
+    keep = _Keep()
+    F = fluid_struct(f, keep)
+    n = F.n
+    grad, normal = np.zeros((n, 2)), np.zeros((n, 2))
+    P = phys_struct(phys)
+    lib().orc_dplus_normal(C.byref(F), C.byref(P), C.byref(ff.c()), keep(_f64(Lm)),
+                           keep(_f64(eig)), keep(_f64(bx)), keep(_f64(beig)), keep(_f64(bvol_n)),
+                           _csr_ref(fb), _p(grad), _p(normal))
+    return grad, normal
+
+
+def dplus_surface(f, phys, ff: CSR, normal, eig, bx=None, fb: CSR | None = None):
+    """detectFluidSurface (deltaPlus.py:165-209): (cmax, mask, surface)."""
+    keep = _Keep()
+    F = fluid_struct(f, keep)
+    n = F.n
+    cmax, mask, surf = np.zeros(n), np.zeros(n), np.zeros(n)
+    P = phys_struct(phys)
+    lib().orc_dplus_surface(C.byref(F), C.byref(P), C.byref(ff.c()), keep(_f64(normal)),
+                            keep(_f64(eig)), keep(_f64(bx)), _csr_ref(fb), _p(cmax), _p(mask),
+                            _p(surf))
+    return cmax, mask, surf
+
+
+def dplus_shift(f, phys, packing, ff: CSR, normal, surface, b=None, fb: CSR | None = None):
+    """computeShiftAmount + adjustShiftingAmount (deltaPlus.py:81-115,
+    211-236): (raw shift, adjusted shift)."""
+    keep = _Keep()
+    F, B = fluid_struct(f, keep), bdy_struct(b, keep)
+    n = F.n
+    raw, shift = np.zeros((n, 2)), np.zeros((n, 2))
+    P = phys_struct(phys)
+    lib().orc_dplus_shift(C.byref(F), C.byref(P), float(packing), C.byref(ff.c()), _byref(B),
+                          _csr_ref(fb), keep(_f64(normal)), keep(_f64(surface)), _p(raw),
+                          _p(shift))
+    return raw, shift
+
+
+def dplus_pressure(f, phys, ff: CSR, surface, b=None, fb: CSR | None = None,
+                   terms: int = T_PRESS | T_PRESS_B):
+    """computePressureAccelDeltaPlus (pressure.py:40-52, solidBoundary.py:882-887)."""
+    keep = _Keep()
+    F, B = fluid_struct(f, keep), bdy_struct(b, keep)
+    acc = np.zeros((F.n, 2))
+    P = phys_struct(phys)
+    lib().orc_dplus_pressure(C.byref(F), C.byref(P), C.byref(ff.c()), _byref(B), _csr_ref(fb),
+                             keep(_f64(surface)), int(terms), _p(acc))
+    return acc
+
 # ------------------------------------------------------------ full steps
 DSPH_TERMS_MOG = (T_CONT | T_CONT_B | T_MOG | T_PRESS | T_PRESS_B | T_VISC | T_VISC_B | T_LAM |
                   T_GRAV)
@@ -358,6 +443,72 @@ class SceneOracle:
         kv = sum(wi * k[1] for wi, k in zip(w, ks))
         kr = sum(wi * k[2] for wi, k in zip(w, ks))
         return x0 + dt / 6 * kv, v0 + dt / 6 * ka, p0 + dt / 6 * kr / r0
+
+    # ------------------------------------------------------- delta-plus
+    def boundary_pairs(self):
+        """Static boundary->boundary pairs (solidBoundary.py:180), brute force."""
+        if getattr(self, "_bb", None) is None:
+            bx = np.asarray(self.sc.bx, dtype=np.float32)
+            self._bb = brute(bx, self.bh, bx)
+        return self._bb
+
+    def dplus_surface(self, x, v, rho, lists=None):
+        """detectSurface of one right-hand side (deltaSPH.py:146-169 then
+        deltaPlus.py:242-245): returns (f, b, lists, eig, beig, normal,
+        surface)."""
+        sc, P = self.sc, self.sc.phys
+        ff, bf, gf, fb = lists if lists is not None else self.pairs(x)
+        f = dict(x=x, v=v, rho=rho, area=self.area, rest=self.rest)
+        brho, bp, _ = mdbc(f, P, dict(x=sc.bx, gx=sc.bgx, vol=sc.bvol), gf)
+        b = dict(x=sc.bx, rho=brho, vol=sc.bvol, p=bp)
+        _, Lm, eig = fluid_norm(f, P, ff, b, fb)
+        vf = self.area / np.asarray(rho, dtype=np.float64)
+        _, _, beig = norm_rows(sc.bx, P["h"], bf, x, vf, self.boundary_pairs(), sc.bx,
+                               sc.bvol / brho)
+        n, nb = len(x), len(sc.bx)
+        m = min(n, nb)
+        bvol_n = np.zeros(nb)
+        bvol_n[:m] = vf[:m]          # deltaPlus.py:151 indexes the fluid arrays by b
+        _, normal = dplus_normal(f, P, ff, Lm, eig, sc.bx, beig, bvol_n, fb)
+        _, _, surf = dplus_surface(f, P, ff, normal, eig, sc.bx, fb)
+        return f, b, (ff, bf, gf, fb), eig, beig, normal, surf
+
+    def dplus_rhs(self, x, v, rho):
+        """delta-plus right-hand side: the delta-SPH terms with the pressure
+        switch of pressure.py:40-52.  Returns dpdt, acc, normal, surface."""
+        P = self.sc.phys
+        f, b, (ff, bf, gf, fb), _, _, normal, surf = self.dplus_surface(x, v, rho)
+        terms = dsph_terms(P) & ~(T_PRESS | T_PRESS_B)
+        dpdt, acc, _ = dsph_rhs(f, P, ff, b, fb, terms)
+        acc = acc + dplus_pressure(f, P, ff, surf, b, fb)
+        return dpdt, acc, normal, surf
+
+    def dplus_step(self, x, v, rho, packing, shifting=True):
+        """RK4 step of the delta-plus scheme, then the shift
+        (simulationBase.py:679-732) with the last stage's normal and mask."""
+        sc, P = self.sc, self.sc.phys
+        dt, r0 = P["dt"], sc.rho0
+        x0, v0, p0 = (np.asarray(a, dtype=np.float64) for a in (x, v, rho))
+        ks = []
+        cx, cv, cr = x0, v0, p0
+        for c in (0.5, 0.5, 1.0, None):
+            dpdt, acc, normal, surf = self.dplus_rhs(cx, cv, cr)
+            ks.append((acc, cv, dpdt))
+            if c is not None:
+                cv, cx, cr = v0 + c * dt * acc, x0 + c * dt * cv, p0 + c * dt * dpdt / r0
+        w = (1, 2, 2, 1)
+        ka = sum(wi * k[0] for wi, k in zip(w, ks))
+        kv = sum(wi * k[1] for wi, k in zip(w, ks))
+        kr = sum(wi * k[2] for wi, k in zip(w, ks))
+        x1, v1, r1 = x0 + dt / 6 * kv, v0 + dt / 6 * ka, p0 + dt / 6 * kr / r0
+        if shifting:
+            ff, bf, gf, fb = self.pairs(x1)
+            f = dict(x=x1, v=v1, rho=r1, area=self.area, rest=self.rest)
+            brho, bp, _ = mdbc(f, P, dict(x=sc.bx, gx=sc.bgx, vol=sc.bvol), gf)
+            b = dict(x=sc.bx, rho=brho, vol=sc.bvol, p=bp)
+            _, sh = dplus_shift(f, P, packing, ff, normal, surf, b, fb)
+            x1 = x1 + sh
+        return x1, v1, r1
 
     def dfsph_step(self, x, v, fixed_iters=0):
         """Semi-implicit Euler DFSPH step (dfsph.py:134-202,

@@ -858,4 +858,248 @@ void orc_dfsph(const OrcFluid* F, const OrcPhys* P, const OrcCSR* ff, const OrcB
   free(p);
 }
 
+/* ====================================================== delta-plus shifting */
+/* Renormalisation matrices of arbitrary query rows over two lists
+ * (computeNormalizationMatrix, densityDiffusion.py:30-43, called for the
+ * solid boundary's own rows at solidBoundary.py:676-693) and pinv2x2. */
+void orc_norm_rows(const double* xq, int nq, double h, const OrcCSR* a, const double* xa,
+                   const double* va, const OrcCSR* b, const double* xb, const double* vb,
+                   double* M, double* L, double* eig) {
+#pragma omp parallel for schedule(static)
+  for (int q = 0; q < nq; ++q) {
+    double m[4] = {0, 0, 0, 0};
+    for (int pass = 0; pass < 2; ++pass) {
+      const OrcCSR* c = pass == 0 ? a : b;
+      const double* xs = pass == 0 ? xa : xb;
+      const double* vs = pass == 0 ? va : vb;
+      if (!c || !c->rowptr) continue;
+      for (int64_t k = c->rowptr[q]; k < c->rowptr[q + 1]; ++k) {
+        int j = c->col[k];
+        Geo g = geo(xq[2 * q], xq[2 * q + 1], xs[2 * j], xs[2 * j + 1], h);
+        double gw = -dW(g.q, h), gx = gw * g.hx, gy = gw * g.hy;
+        double rx = xs[2 * j] - xq[2 * q], ry = xs[2 * j + 1] - xq[2 * q + 1];
+        m[0] += rx * gx * vs[j];
+        m[1] += rx * gy * vs[j];
+        m[2] += ry * gx * vs[j];
+        m[3] += ry * gy * vs[j];
+      }
+    }
+    double e[2];
+    if (M) memcpy(M + 4 * q, m, sizeof m);
+    pinv2(m[0], m[1], m[2], m[3], L + 4 * q, e);
+    if (eig) {
+      eig[2 * q] = e[0];
+      eig[2 * q + 1] = e[1];
+    }
+  }
+}
+
+/* Fluid rows: M_i over fluid and boundary neighbours (deltaPlus.py:116-131
+ * with solidBoundary.py:684-687), L = pinv2x2(M), eig. */
+void orc_fluid_norm(const OrcFluid* F, const OrcPhys* P, const OrcCSR* ff, const OrcBdy* B,
+                    const OrcCSR* fb, double* M, double* L, double* eig) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < F->n; ++i) {
+    double m[4], e[2];
+    norm_matrix(F, P, ff, B, fb, i, m);
+    if (M) memcpy(M + 4 * i, m, sizeof m);
+    pinv2(m[0], m[1], m[2], m[3], L + 4 * i, e);
+    eig[2 * i] = e[0];
+    eig[2 * i + 1] = e[1];
+  }
+}
+
+/* computeFluidNormal (deltaPlus.py:132-163): lambda = eig[.][1];
+ * g_i = -sum_j V_j (lam_j - lam_i) L_i gradW_ij (fluid), the same over the
+ * boundary neighbours with volume bvol_n[b] and lambda beig[b][1];
+ * n_i = g_i / (|g_i| + eps). */
+void orc_dplus_normal(const OrcFluid* F, const OrcPhys* P, const OrcCSR* ff, const double* L,
+                      const double* eig, const double* bx, const double* beig,
+                      const double* bvol_n, const OrcCSR* fb, double* grad, double* normal) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < F->n; ++i) {
+    const double* l = L + 4 * i;
+    double lam_i = eig[2 * i + 1], gx = 0, gy = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      const OrcCSR* c = pass == 0 ? ff : fb;
+      if (!c || !c->rowptr) continue;
+      for (int64_t k = c->rowptr[i]; k < c->rowptr[i + 1]; ++k) {
+        int j = c->col[k];
+        double xj, yj, V, lam_j;
+        if (pass == 0) {
+          xj = F->x[2 * j];
+          yj = F->x[2 * j + 1];
+          V = F->vol ? F->vol[j] : F->area[j] / F->rho[j];
+          lam_j = eig[2 * j + 1];
+        } else {
+          xj = bx[2 * j];
+          yj = bx[2 * j + 1];
+          V = bvol_n[j];
+          lam_j = beig[2 * j + 1];
+        }
+        Geo g = geo(F->x[2 * i], F->x[2 * i + 1], xj, yj, P->h);
+        double gw = -dW(g.q, P->h), wx = gw * g.hx, wy = gw * g.hy;
+        double cx = l[0] * wx + l[1] * wy, cy = l[2] * wx + l[3] * wy;
+        gx -= V * (lam_j - lam_i) * cx;
+        gy -= V * (lam_j - lam_i) * cy;
+      }
+    }
+    if (grad) {
+      grad[2 * i] = gx;
+      grad[2 * i + 1] = gy;
+    }
+    double nrm = sqrt(gx * gx + gy * gy) + P->eps;
+    normal[2 * i] = gx / nrm;
+    normal[2 * i + 1] = gy / nrm;
+  }
+}
+
+/* detectFluidSurface (deltaPlus.py:165-209).  cmax_i = max(max over the
+ * non-self fluid pairs of (x_j - x_i)/|x_j - x_i| . n_i, max over the
+ * boundary pairs of (x_b - x_i)/|x_b - x_i| . n_i), an empty max being 0
+ * (torch_scatter's fill); mask_i = [acos(cmax_i) > pi/6 and lam_i < 0.6];
+ * m2_i = max of mask_j over the non-self fluid pairs; classes 0, 2/3, 1, 2.
+ * The thresholds are the fp32 values the reference compares against. */
+void orc_dplus_surface(const OrcFluid* F, const OrcPhys* P, const OrcCSR* ff,
+                       const double* normal, const double* eig, const double* bx,
+                       const OrcCSR* fb, double* cmax, double* mask, double* surface) {
+  const int n = F->n;
+  const double pi6 = (double)(float)(ORC_PI / 6), lam_s = (double)0.6f, lam_k = (double)0.5f;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    double nx = normal[2 * i], ny = normal[2 * i + 1], fmax = 0, bmax = 0;
+    int any = 0;
+    for (int64_t k = ff->rowptr[i]; k < ff->rowptr[i + 1]; ++k) {
+      int j = ff->col[k];
+      if (j == i) continue;
+      Geo g = geo(F->x[2 * j], F->x[2 * j + 1], F->x[2 * i], F->x[2 * i + 1], P->h);
+      double d = g.hx * nx + g.hy * ny;
+      fmax = any ? (d > fmax ? d : fmax) : d;
+      any = 1;
+    }
+    any = 0;
+    if (fb && fb->rowptr)
+      for (int64_t k = fb->rowptr[i]; k < fb->rowptr[i + 1]; ++k) {
+        int b = fb->col[k];
+        Geo g = geo(bx[2 * b], bx[2 * b + 1], F->x[2 * i], F->x[2 * i + 1], P->h);
+        double d = g.hx * nx + g.hy * ny;
+        bmax = any ? (d > bmax ? d : bmax) : d;
+        any = 1;
+      }
+    double c = fmax > bmax ? fmax : bmax;
+    cmax[i] = c;
+    mask[i] = (acos(c) > pi6 && eig[2 * i + 1] < lam_s) ? 1.0 : 0.0;
+  }
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    double m2 = 0;
+    for (int64_t k = ff->rowptr[i]; k < ff->rowptr[i + 1]; ++k) {
+      int j = ff->col[k];
+      if (j != i && mask[j] > m2) m2 = mask[j];
+    }
+    double r;
+    if (mask[i] > 0)
+      r = 0;
+    else if (m2 > 0)
+      r = eig[2 * i + 1] >= lam_k ? 1.0 : 2.0 / 3.0;
+    else
+      r = 2;
+    surface[i] = r;
+  }
+}
+
+/* computeShiftAmount + adjustShiftingAmount (deltaPlus.py:81-115,
+ * 211-236).  Ma = max_k |v_k| / c0; k0 = W(packing); raw (optional) is the
+ * shift before the clamp and the projection. */
+void orc_dplus_shift(const OrcFluid* F, const OrcPhys* P, double packing, const OrcCSR* ff,
+                     const OrcBdy* B, const OrcCSR* fb, const double* normal,
+                     const double* surface, double* raw, double* shift) {
+  const int n = F->n;
+  double vmax = 0;
+  for (int i = 0; i < n; ++i) {
+    double s = sqrt(F->v[2 * i] * F->v[2 * i] + F->v[2 * i + 1] * F->v[2 * i + 1]);
+    if (s > vmax) vmax = s;
+  }
+  const double Ma = vmax / P->c0, k0 = W(packing, P->h), h = P->h;
+  const double c = -1.5 * Ma * 4 * h * h;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    double sx = 0, sy = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      const OrcCSR* cs = pass == 0 ? ff : fb;
+      if (!cs || !cs->rowptr) continue;
+      for (int64_t k = cs->rowptr[i]; k < cs->rowptr[i + 1]; ++k) {
+        int j = cs->col[k];
+        double xj, yj, mass;
+        if (pass == 0) {
+          xj = F->x[2 * j];
+          yj = F->x[2 * j + 1];
+          mass = F->area[j] / (F->rho[i] + F->rho[j]);
+        } else {
+          xj = B->x[2 * j];
+          yj = B->x[2 * j + 1];
+          mass = B->vol[j] / (F->rho[i] + B->rho[j]);
+        }
+        Geo g = geo(F->x[2 * i], F->x[2 * i + 1], xj, yj, h);
+        double w = W(g.q, h) / k0;
+        double kt = 1 + 0.2 * w * w * w * w;
+        double gw = -dW(g.q, h);
+        sx += kt * mass * gw * g.hx;
+        sy += kt * mass * gw * g.hy;
+      }
+    }
+    sx *= c;
+    sy *= c;
+    if (raw) {
+      raw[2 * i] = sx;
+      raw[2 * i + 1] = sy;
+    }
+    double len = sqrt(sx * sx + sy * sy), cap = 0.05 * h;
+    if (len > cap) {
+      sx = sx / len * cap;
+      sy = sy / len * cap;
+    }
+    double m = surface[i];
+    if (m < 0.5) {
+      sx = sy = 0;
+    } else if (m < 1.5) {
+      double nx = normal[2 * i], ny = normal[2 * i + 1];
+      double px = (1 - nx * nx) * sx - nx * ny * sy;
+      double py = -ny * nx * sx + (1 - ny * ny) * sy;
+      sx = px;
+      sy = py;
+    }
+    shift[2 * i] = sx;
+    shift[2 * i + 1] = sy;
+  }
+}
+
+/* computePressureAccelDeltaPlus (pressure.py:40-52) for the fluid pairs
+ * (T_PRESS) and the boundary pairs (T_PRESS_B, solidBoundary.py:882-887):
+ * p_ij = p_i + p_j if p_i >= 0 or surface_i < 1.5, else p_j - p_i. */
+void orc_dplus_pressure(const OrcFluid* F, const OrcPhys* P, const OrcCSR* ff, const OrcBdy* B,
+                        const OrcCSR* fb, const double* surface, unsigned terms, double* acc) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < F->n; ++i) {
+    double pi = eos(P, F->rho[i]), ax = 0, ay = 0;
+    int sum = pi >= 0 || surface[i] < 1.5;
+    for (int pass = 0; pass < 2; ++pass) {
+      const OrcCSR* c = pass == 0 ? ff : fb;
+      if (!c || !c->rowptr || !(terms & (pass == 0 ? T_PRESS : T_PRESS_B))) continue;
+      for (int64_t k = c->rowptr[i]; k < c->rowptr[i + 1]; ++k) {
+        int j = c->col[k];
+        Nb nb = pass == 0 ? fluid_nb(F, P, j) : bdy_nb(B, P, j);
+        double pij = sum ? pi + nb.p : nb.p - pi;
+        Geo g = geo(F->x[2 * i], F->x[2 * i + 1], nb.x, nb.y, P->h);
+        double gw = -dW(g.q, P->h);
+        ax += pij * nb.vol * gw * g.hx;
+        ay += pij * nb.vol * gw * g.hy;
+      }
+    }
+    double rho_i = F->rho[i] * P->rho0;
+    acc[2 * i] = -ax / rho_i;
+    acc[2 * i + 1] = -ay / rho_i;
+  }
+}
+
 void orc_free(void* p) { free(p); }

@@ -164,6 +164,16 @@ _SIGS = {
                                      c_vp, c_vp, c_vp], c_int),
     "sph_permute_f32": ([c_vp, c_vp, c_int, c_int, c_vp, c_vp], c_int),
     "sph_permute_fields": ([_P(SphFields), c_vp, c_int, c_vp, c_vp], c_int),
+    "sph_normalization_rows": ([c_vp, c_vp, c_int, c_vp, _P(SphCSR), c_vp, c_vp, _P(SphCSR), c_vp,
+                                c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
+    "sph_dplus_normal": ([_P(SphFluid), c_vp, _P(SphCSR), c_vp, c_vp, c_vp, c_vp, c_vp,
+                          _P(SphCSR), c_vp, c_vp, c_vp], c_int),
+    "sph_dplus_surface": ([_P(SphFluid), c_vp, _P(SphCSR), c_vp, c_vp, c_vp, _P(SphCSR), c_vp,
+                           c_vp, c_vp, c_vp, c_vp], c_int),
+    "sph_dplus_shift": ([_P(SphFluid), c_vp, c_int, c_vp, _P(SphCSR), _P(SphBoundary), _P(SphCSR),
+                         c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
+    "sph_dplus_pressure": ([_P(SphFluid), c_vp, _P(SphCSR), _P(SphBoundary), _P(SphCSR), c_vp,
+                            C.c_uint, c_vp, c_vp], c_int),
 }
 
 _lib = None

@@ -395,41 +395,6 @@ __global__ __launch_bounds__(kBS) void k_dsph_gather_packed(
 }
 
 // ------------------------------------------ renormalised density gradient
-// Closed-form 2x2 pseudo-inverse through the SVD angles (util.py:386-433).
-__device__ __forceinline__ void pinv2x2_dev(float a, float b, float c, float d, float L[4],
-                                            float eig[2]) {
-  const float theta = 0.5f * atan2f(2.f * a * c + 2.f * b * d, a * a + b * b - c * c - d * d);
-  const float ct = cosf(theta), st = sinf(theta);
-  const float S1 = a * a + b * b + c * c + d * d;
-  const float e = a * a + b * b - c * c - d * d;
-  const float f2 = a * c + b * d;
-  const float S2 = sqrtf(e * e + 4.f * f2 * f2);
-  const float o1 = sqrtf((S1 + S2) / 2.f);
-  const float o2 = sqrtf((S1 - S2) / 2.f);
-  const float phi = 0.5f * atan2f(2.f * a * b + 2.f * c * d, a * a - b * b + c * c - d * d);
-  const float cp = cosf(phi), sp = sinf(phi);
-  const float t11 = (a * ct + c * st) * cp + (b * ct + d * st) * sp;
-  const float t22 = (a * st - c * ct) * sp + (-b * st + d * ct) * cp;
-  const float s11 = (t11 > 0.f) ? 1.f : ((t11 < 0.f) ? -1.f : 0.f);
-  const float s22 = (t22 > 0.f) ? 1.f : ((t22 < 0.f) ? -1.f : 0.f);
-  const float V00 = cp * s11, V01 = -sp * s22, V10 = sp * s11, V11 = cp * s22;
-  const float U00 = ct, U01 = -st, U10 = st, U11 = ct;
-  const float i1 = fabsf(o1) > 1e-5f ? 1.f / o1 : 0.f;
-  const float i2 = fabsf(o2) > 1e-5f ? 1.f / o2 : 0.f;
-  // (V S^-1 U^T)[r][c] = sum_k V[r][k] i_k U[c][k]
-  L[0] = V00 * i1 * U00 + V01 * i2 * U01;
-  L[1] = V00 * i1 * U10 + V01 * i2 * U11;
-  L[2] = V10 * i1 * U00 + V11 * i2 * U01;
-  L[3] = V10 * i1 * U10 + V11 * i2 * U11;
-  if (fabsf(o2) > fabsf(o1)) {
-    eig[0] = o2;
-    eig[1] = o1;
-  } else {
-    eig[0] = o1;
-    eig[1] = o2;
-  }
-}
-
 __global__ __launch_bounds__(kBS) void k_pinv2x2(const float* __restrict__ M, int n,
                                                  float* __restrict__ L, float* __restrict__ eig) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

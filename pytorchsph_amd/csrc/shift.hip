@@ -1,0 +1,477 @@
+// delta-plus particle shifting: free-surface detection through the smaller
+// singular value lambda of the renormalisation matrix, the fluid normal from
+// grad lambda, the shift vector and its surface projection
+// (modules/deltaPlus.py:81-251), and the delta-plus pressure switch
+// (pressure.py:40-52, solidBoundary.py:882-887).
+//
+// One thread per row.  These passes run once per right-hand side next to the
+// delta-SPH gathers and read the same CSR lists.
+#include "sph_common.h"
+
+namespace sph {
+namespace {
+
+constexpr int kBS = 256;
+
+// deltaPlus.py:86-92 and 215-216
+constexpr float kShiftCFL = 1.5f;
+constexpr float kShiftR = 0.2f;
+constexpr float kShiftClamp = 0.05f;
+// deltaPlus.py:195, 204: thresholds compared in fp32
+constexpr float kPi6 = 0.52359877559829887f;
+constexpr float kLamSurface = 0.6f;
+constexpr float kLamKeep = 0.5f;
+constexpr float kTwoThirds = 0.66666666666666663f;
+
+__device__ __forceinline__ float2 ld2(const float* p, int i) {
+  return reinterpret_cast<const float2*>(p)[i];
+}
+__device__ __forceinline__ void st2(float* p, int i, float2 v) {
+  reinterpret_cast<float2*>(p)[i] = v;
+}
+
+__device__ __forceinline__ int row_index() {
+  return xcd_block() * blockDim.x + threadIdx.x;
+}
+
+__device__ __forceinline__ float fluid_vol(const SphFluid& f, int j) {
+  return f.vol ? f.vol[j] : f.area[j] / f.rho[j];
+}
+
+// M_q over two lists (densityDiffusion.py:30-43), L = pinv2x2(M).
+__global__ __launch_bounds__(kBS) void k_norm_rows(const float* __restrict__ xq,
+                                                   const int* __restrict__ qscene, int nq,
+                                                   const SphPhysics* __restrict__ phys, SphCSR a,
+                                                   const float* __restrict__ xa,
+                                                   const float* __restrict__ va, SphCSR b,
+                                                   const float* __restrict__ xb,
+                                                   const float* __restrict__ vb,
+                                                   float* __restrict__ Mout,
+                                                   float* __restrict__ Lout,
+                                                   float* __restrict__ eig) {
+  const int q = row_index();
+  if (q >= nq) return;
+  const KConst K = kconst(phys[qscene[q]].h);
+  const float2 xi = ld2(xq, q);
+  float m[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int pass = 0; pass < 2; ++pass) {
+    const SphCSR& c = pass == 0 ? a : b;
+    if (!c.rowptr) continue;
+    const float* __restrict__ xs = pass == 0 ? xa : xb;
+    const float* __restrict__ vs = pass == 0 ? va : vb;
+    float s00 = 0.f, s01 = 0.f, s10 = 0.f, s11 = 0.f;
+    for (int k = c.rowptr[q]; k < c.rowptr[q + 1]; ++k) {
+      const int j = c.col[k];
+      const float2 xj = ld2(xs, j);
+      const float V = vs[j];
+      const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, K.inv_h);
+      const float gw = -wendland_grad_k(g.q, K);
+      const float gx = gw * g.hx, gy = gw * g.hy;
+      const float rx = -g.rx, ry = -g.ry;
+      s00 += rx * gx * V;
+      s01 += rx * gy * V;
+      s10 += ry * gx * V;
+      s11 += ry * gy * V;
+    }
+    m[0] += s00;
+    m[1] += s01;
+    m[2] += s10;
+    m[3] += s11;
+  }
+  if (Mout)
+    for (int k = 0; k < 4; ++k) Mout[4 * q + k] = m[k];
+  float l[4], e[2];
+  pinv2x2_dev(m[0], m[1], m[2], m[3], l, e);
+  for (int k = 0; k < 4; ++k) Lout[4 * q + k] = l[k];
+  if (eig) st2(eig, q, make_float2(e[0], e[1]));
+}
+
+// g_i = -sum V_j (lam_j - lam_i) L_i gradW_ij; n_i = g_i / (|g_i| + eps)
+// (deltaPlus.py:132-163).
+__global__ __launch_bounds__(kBS) void k_dplus_normal(SphFluid f, const SphPhysics* __restrict__ phys,
+                                                      SphCSR ff, const float* __restrict__ Lm,
+                                                      const float* __restrict__ eig,
+                                                      const float* __restrict__ bx,
+                                                      const float* __restrict__ beig,
+                                                      const float* __restrict__ bvol_n, SphCSR fb,
+                                                      float* __restrict__ grad,
+                                                      float* __restrict__ normal) {
+  const int i = row_index();
+  if (i >= f.np) return;
+  const SphPhysics& P = phys[f.pscene[i]];
+  const KConst K = kconst(P.h);
+  const float2 xi = ld2(f.x, i);
+  const float lam_i = eig[2 * i + 1];
+  const float l00 = Lm[4 * i], l01 = Lm[4 * i + 1], l10 = Lm[4 * i + 2], l11 = Lm[4 * i + 3];
+  float g[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
+  for (int pass = 0; pass < 2; ++pass) {
+    const SphCSR& c = pass == 0 ? ff : fb;
+    if (!c.rowptr) continue;
+    float sx = 0.f, sy = 0.f;
+    for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
+      const int j = c.col[k];
+      float2 xj;
+      float V, lam_j;
+      if (pass == 0) {
+        xj = ld2(f.x, j);
+        V = fluid_vol(f, j);
+        lam_j = eig[2 * j + 1];
+      } else {
+        xj = ld2(bx, j);
+        V = bvol_n[j];
+        lam_j = beig[2 * j + 1];
+      }
+      const PairGeo pg = pair_geo(xi.x, xi.y, xj.x, xj.y, K.inv_h);
+      const float gw = -wendland_grad_k(pg.q, K);
+      const float gx = gw * pg.hx, gy = gw * pg.hy;
+      const float cx = l00 * gx + l01 * gy, cy = l10 * gx + l11 * gy;
+      const float w = -(V * (lam_j - lam_i));
+      sx += w * cx;
+      sy += w * cy;
+    }
+    g[pass][0] = sx;
+    g[pass][1] = sy;
+  }
+  const float gx = g[0][0] + g[1][0], gy = g[0][1] + g[1][1];
+  if (grad) st2(grad, i, make_float2(gx, gy));
+  const float inv = 1.f / (sqrtf(gx * gx + gy * gy) + P.eps);
+  st2(normal, i, make_float2(gx * inv, gy * inv));
+}
+
+// First pass of detectFluidSurface (deltaPlus.py:165-195): the angle test.
+__global__ __launch_bounds__(kBS) void k_dplus_angle(SphFluid f, const SphPhysics* __restrict__ phys,
+                                                     SphCSR ff, const float* __restrict__ normal,
+                                                     const float* __restrict__ eig,
+                                                     const float* __restrict__ bx, SphCSR fb,
+                                                     float* __restrict__ mask,
+                                                     float* __restrict__ amin,
+                                                     float* __restrict__ amax) {
+  const int i = row_index();
+  if (i >= f.np) return;
+  const KConst K = kconst(phys[f.pscene[i]].h);
+  const float2 xi = ld2(f.x, i);
+  const float2 n = ld2(normal, i);
+  float dmin = 0.f, dmax = 0.f;
+  bool any = false;
+  for (int k = ff.rowptr[i]; k < ff.rowptr[i + 1]; ++k) {
+    const int j = ff.col[k];
+    if (j == i) continue;
+    const float2 xj = ld2(f.x, j);
+    const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, K.inv_h);
+    const float d = -g.hx * n.x + -g.hy * n.y;   // (-rhat_ij) . n_i
+    dmin = any ? fminf(dmin, d) : d;
+    dmax = any ? fmaxf(dmax, d) : d;
+    any = true;
+  }
+  float bmax = 0.f;
+  if (fb.rowptr) {
+    bool anyb = false;
+    for (int k = fb.rowptr[i]; k < fb.rowptr[i + 1]; ++k) {
+      const float2 xb = ld2(bx, fb.col[k]);
+      const PairGeo g = pair_geo(xi.x, xi.y, xb.x, xb.y, K.inv_h);
+      const float d = -g.hx * n.x + -g.hy * n.y;   // rhat_bi . n_i
+      bmax = anyb ? fmaxf(bmax, d) : d;
+      anyb = true;
+    }
+  }
+  const float c = fmaxf(dmax, bmax);
+  const float ang = acosf(c);
+  mask[i] = (ang > kPi6 && eig[2 * i + 1] < kLamSurface) ? 1.f : 0.f;
+  if (amin) amin[i] = acosf(dmin);
+  if (amax) amax[i] = acosf(dmax);
+}
+
+// Second pass (deltaPlus.py:197-209): one-ring spread of the mask and the
+// final classes 0 / 2/3 / 1 / 2.
+__global__ __launch_bounds__(kBS) void k_dplus_classify(int np, SphCSR ff,
+                                                        const float* __restrict__ mask,
+                                                        const float* __restrict__ eig,
+                                                        float* __restrict__ out) {
+  const int i = row_index();
+  if (i >= np) return;
+  float m2 = 0.f;
+  for (int k = ff.rowptr[i]; k < ff.rowptr[i + 1]; ++k) {
+    const int j = ff.col[k];
+    if (j != i) m2 = fmaxf(m2, mask[j]);
+  }
+  float fm = 0.f;
+  if (m2 > 0.f) fm = kTwoThirds;
+  if (mask[i] > 0.f) fm = 1.f;
+  float r = fm;
+  if (eig[2 * i + 1] >= kLamKeep && fm > 0.f) r = 1.f;
+  if (fm > 0.7f) r = 0.f;
+  if (fm < 0.1f) r = 2.f;
+  out[i] = r;
+}
+
+__global__ void k_clear_u32(unsigned* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// max_k |v_k| per scene; non-negative floats order like their bit patterns.
+// A wavefront whose rows share one scene reduces with shuffles and issues
+// one atomic; a wavefront that straddles scenes issues one per lane.
+__global__ __launch_bounds__(kBS) void k_speed_max(SphFluid f, unsigned* __restrict__ vmax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = i < f.np;
+  float s = 0.f;
+  int sc = -1;
+  if (in) {
+    const float2 v = ld2(f.v, i);
+    s = sqrtf(v.x * v.x + v.y * v.y);
+    sc = f.pscene[i];
+  }
+  const int sc0 = __shfl(sc, 0);   // lane 0 holds the wavefront's first row
+  if (__all(!in || sc == sc0)) {
+    float m = s;
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0 && sc0 >= 0 && m > 0.f) atomicMax(vmax + sc0, __float_as_uint(m));
+  } else if (in && s > 0.f) {
+    atomicMax(vmax + sc, __float_as_uint(s));
+  }
+}
+
+// computeShiftAmount + adjustShiftingAmount (deltaPlus.py:81-115, 211-236).
+__global__ __launch_bounds__(kBS) void k_dplus_shift(SphFluid f, const SphPhysics* __restrict__ phys,
+                                                     const float* __restrict__ packing, SphCSR ff,
+                                                     SphBoundary b, SphCSR fb,
+                                                     const float* __restrict__ normal,
+                                                     const float* __restrict__ smask,
+                                                     const unsigned* __restrict__ vmax,
+                                                     float* __restrict__ shift) {
+  const int i = row_index();
+  if (i >= f.np) return;
+  const int sc = f.pscene[i];
+  const SphPhysics& P = phys[sc];
+  const KConst K = kconst(P.h);
+  const float inv_k0 = 1.f / wendland(packing[sc], P.h);
+  const float2 xi = ld2(f.x, i);
+  const float ri = f.rho[i];
+  float part[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
+  for (int pass = 0; pass < 2; ++pass) {
+    const SphCSR& c = pass == 0 ? ff : fb;
+    if (!c.rowptr) continue;
+    float sx = 0.f, sy = 0.f;
+    for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
+      const int j = c.col[k];
+      const float2 xj = pass == 0 ? ld2(f.x, j) : ld2(b.x, j);
+      const float mass = pass == 0 ? f.area[j] / (ri + f.rho[j]) : b.vol[j] / (ri + b.rho[j]);
+      const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, K.inv_h);
+      const float w = wendland_k(g.q, K) * inv_k0;
+      const float w2 = w * w;
+      const float kt = 1.f + kShiftR * (w2 * w2);
+      const float gw = -wendland_grad_k(g.q, K);
+      const float t = kt * mass;
+      sx += t * (gw * g.hx);
+      sy += t * (gw * g.hy);
+    }
+    part[pass][0] = sx;
+    part[pass][1] = sy;
+  }
+  const float Ma = __uint_as_float(vmax[sc]) / P.c0;
+  const float c = -kShiftCFL * Ma * (4.f * P.h * P.h);
+  float sx = c * part[0][0] + c * part[1][0];
+  float sy = c * part[0][1] + c * part[1][1];
+  const float len = sqrtf(sx * sx + sy * sy);
+  const float cap = P.h * kShiftClamp;
+  if (len > cap) {
+    sx = sx / len * cap;
+    sy = sy / len * cap;
+  }
+  const float m = smask[i];
+  if (m < 0.5f) {
+    sx = 0.f;
+    sy = 0.f;
+  } else if (m < 1.5f) {
+    const float2 n = ld2(normal, i);
+    const float a00 = 1.f - n.x * n.x, a01 = -(n.x * n.y), a10 = -(n.y * n.x), a11 = 1.f - n.y * n.y;
+    const float px = a00 * sx + a01 * sy, py = a10 * sx + a11 * sy;
+    sx = px;
+    sy = py;
+  }
+  st2(shift, i, make_float2(sx, sy));
+}
+
+// computePressureAccelDeltaPlus (pressure.py:40-52) for fluid and boundary
+// neighbours.
+__global__ __launch_bounds__(kBS) void k_dplus_pressure(SphFluid f, const SphPhysics* __restrict__ phys,
+                                                        SphCSR ff, SphBoundary b, SphCSR fb,
+                                                        const float* __restrict__ smask,
+                                                        unsigned terms, float* __restrict__ acc) {
+  const int i = row_index();
+  if (i >= f.np) return;
+  const SphPhysics& P = phys[f.pscene[i]];
+  const KConst K = kconst(P.h);
+  const float2 xi = ld2(f.x, i);
+  const float pi = eos(P, f.rho[i]);
+  const bool sum = pi >= 0.f || smask[i] < 1.5f;
+  float part[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
+  for (int pass = 0; pass < 2; ++pass) {
+    const SphCSR& c = pass == 0 ? ff : fb;
+    if (!c.rowptr || !(terms & (pass == 0 ? SPH_T_PRESS : SPH_T_PRESS_B))) continue;
+    float sx = 0.f, sy = 0.f;
+    for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
+      const int j = c.col[k];
+      float2 xj;
+      float pj, V;
+      if (pass == 0) {
+        xj = ld2(f.x, j);
+        pj = eos(P, f.rho[j]);
+        V = fluid_vol(f, j);
+      } else {
+        xj = ld2(b.x, j);
+        pj = b.p[j];
+        V = b.vol[j] / b.rho[j];
+      }
+      const float pij = sum ? pi + pj : pj - pi;
+      const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, K.inv_h);
+      const float gw = -wendland_grad_k(g.q, K);
+      const float t = pij * V;
+      sx += t * (gw * g.hx);
+      sy += t * (gw * g.hy);
+    }
+    part[pass][0] = sx;
+    part[pass][1] = sy;
+  }
+  const float inv_rho = 1.f / (f.rho[i] * P.rho0);
+  float ax = -inv_rho * part[0][0] + -inv_rho * part[1][0];
+  float ay = -inv_rho * part[0][1] + -inv_rho * part[1][1];
+  if (terms & SPH_T_ACCUM) {
+    const float2 o = ld2(acc, i);
+    ax += o.x;
+    ay += o.y;
+  }
+  st2(acc, i, make_float2(ax, ay));
+}
+
+bool csr_ok(const SphCSR* c) { return c && c->rowptr && c->col; }
+
+bool fluid_ok(const SphFluid* f) {
+  return f && f->np >= 0 && f->x && f->pscene && (f->np == 0 || (f->rho && f->area));
+}
+
+}  // namespace
+
+int normalization_rows(const float* xq, const int* qscene, int nq, const SphPhysics* phys,
+                       const SphCSR* a, const float* xa, const float* va, const SphCSR* b,
+                       const float* xb, const float* vb, float* M, float* L, float* eig,
+                       hipStream_t s) {
+  SPH_REQUIRE(nq >= 0 && phys && L && (nq == 0 || (xq && qscene)),
+              "sph_normalization_rows: bad arguments");
+  const SphCSR none = {nullptr, nullptr, 0};
+  const bool ua = a && a->rowptr, ub = b && b->rowptr;
+  SPH_REQUIRE(!ua || (a->col && xa && va), "sph_normalization_rows: list a without its points");
+  SPH_REQUIRE(!ub || (b->col && xb && vb), "sph_normalization_rows: list b without its points");
+  if (nq == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_norm_rows, dim3(blocks_for(nq, kBS)), dim3(kBS), 0, s, xq, qscene, nq,
+                     phys, ua ? *a : none, xa, va, ub ? *b : none, xb, vb, M, L, eig);
+  return launch_check("sph_normalization_rows");
+}
+
+int dplus_normal(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, const float* L,
+                 const float* eig, const float* bx, const float* beig, const float* bvol_n,
+                 const SphCSR* fb, float* grad, float* normal, hipStream_t s) {
+  SPH_REQUIRE(fluid_ok(f) && phys && csr_ok(ff) && L && eig && normal,
+              "sph_dplus_normal: bad arguments");
+  const SphCSR none = {nullptr, nullptr, 0};
+  const bool ub = fb && fb->rowptr;
+  SPH_REQUIRE(!ub || (fb->col && bx && beig && bvol_n),
+              "sph_dplus_normal: boundary list without boundary points, lambda or volumes");
+  if (f->np == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_dplus_normal, dim3(blocks_for(f->np, kBS)), dim3(kBS), 0, s, *f, phys, *ff,
+                     L, eig, bx, beig, bvol_n, ub ? *fb : none, grad, normal);
+  return launch_check("sph_dplus_normal");
+}
+
+int dplus_surface(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                  const float* normal, const float* eig, const float* bx, const SphCSR* fb,
+                  float* mask_tmp, float* amin, float* amax, float* surface_mask,
+                  hipStream_t s) {
+  SPH_REQUIRE(f && f->np >= 0 && f->x && f->pscene && phys && csr_ok(ff) && normal && eig &&
+                  mask_tmp && surface_mask,
+              "sph_dplus_surface: bad arguments");
+  const SphCSR none = {nullptr, nullptr, 0};
+  const bool ub = fb && fb->rowptr;
+  SPH_REQUIRE(!ub || (fb->col && bx), "sph_dplus_surface: boundary list without boundary points");
+  SPH_REQUIRE(mask_tmp != surface_mask, "sph_dplus_surface: mask_tmp must not alias the output");
+  if (f->np == 0) return SPH_OK;
+  const int nb = blocks_for(f->np, kBS);
+  hipLaunchKernelGGL(k_dplus_angle, dim3(nb), dim3(kBS), 0, s, *f, phys, *ff, normal, eig, bx,
+                     ub ? *fb : none, mask_tmp, amin, amax);
+  hipLaunchKernelGGL(k_dplus_classify, dim3(nb), dim3(kBS), 0, s, f->np, *ff, mask_tmp, eig,
+                     surface_mask);
+  return launch_check("sph_dplus_surface");
+}
+
+int dplus_shift(const SphFluid* f, const SphPhysics* phys, int nscenes, const float* packing,
+                const SphCSR* ff, const SphBoundary* b, const SphCSR* fb, const float* normal,
+                const float* surface_mask, unsigned* vmax_ws, float* shift, hipStream_t s) {
+  SPH_REQUIRE(fluid_ok(f) && f->v && phys && nscenes >= 1 && packing && csr_ok(ff) && normal &&
+                  surface_mask && vmax_ws && shift,
+              "sph_dplus_shift: bad arguments");
+  const SphCSR none = {nullptr, nullptr, 0};
+  const SphBoundary nob = {};
+  const bool ub = fb && fb->rowptr;
+  SPH_REQUIRE(!ub || (fb->col && b && b->x && b->vol && b->rho),
+              "sph_dplus_shift: boundary list without boundary state");
+  hipLaunchKernelGGL(k_clear_u32, dim3(blocks_for(nscenes, kBS)), dim3(kBS), 0, s, vmax_ws,
+                     nscenes);
+  if (f->np == 0) return launch_check("sph_dplus_shift");
+  const int nb = blocks_for(f->np, kBS);
+  hipLaunchKernelGGL(k_speed_max, dim3(nb), dim3(kBS), 0, s, *f, vmax_ws);
+  hipLaunchKernelGGL(k_dplus_shift, dim3(nb), dim3(kBS), 0, s, *f, phys, packing, *ff,
+                     ub ? *b : nob, ub ? *fb : none, normal, surface_mask, vmax_ws, shift);
+  return launch_check("sph_dplus_shift");
+}
+
+int dplus_pressure(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                   const SphBoundary* b, const SphCSR* fb, const float* surface_mask,
+                   unsigned terms, float* acc, hipStream_t s) {
+  SPH_REQUIRE(fluid_ok(f) && phys && surface_mask && acc, "sph_dplus_pressure: bad arguments");
+  const SphCSR none = {nullptr, nullptr, 0};
+  const SphBoundary nob = {};
+  const bool uf = (terms & SPH_T_PRESS) != 0, ub = (terms & SPH_T_PRESS_B) != 0;
+  SPH_REQUIRE(!uf || csr_ok(ff), "sph_dplus_pressure: SPH_T_PRESS needs the fluid list");
+  SPH_REQUIRE(!ub || (csr_ok(fb) && b && b->x && b->vol && b->rho && b->p),
+              "sph_dplus_pressure: SPH_T_PRESS_B needs the boundary list and state");
+  if (f->np == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_dplus_pressure, dim3(blocks_for(f->np, kBS)), dim3(kBS), 0, s, *f, phys,
+                     uf ? *ff : none, ub ? *b : nob, ub ? *fb : none, surface_mask, terms, acc);
+  return launch_check("sph_dplus_pressure");
+}
+
+}  // namespace sph
+
+extern "C" {
+int sph_normalization_rows(const float* xq, const int* qscene, int nq, const SphPhysics* phys,
+                           const SphCSR* a, const float* xa, const float* va, const SphCSR* b,
+                           const float* xb, const float* vb, float* M, float* L, float* eig,
+                           hipStream_t s) {
+  return sph::normalization_rows(xq, qscene, nq, phys, a, xa, va, b, xb, vb, M, L, eig, s);
+}
+int sph_dplus_normal(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, const float* L,
+                     const float* eig, const float* bx, const float* beig, const float* bvol_n,
+                     const SphCSR* fb, float* grad, float* normal, hipStream_t s) {
+  return sph::dplus_normal(f, phys, ff, L, eig, bx, beig, bvol_n, fb, grad, normal, s);
+}
+int sph_dplus_surface(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                      const float* normal, const float* eig, const float* bx, const SphCSR* fb,
+                      float* mask_tmp, float* angle_min, float* angle_max, float* surface_mask,
+                      hipStream_t s) {
+  return sph::dplus_surface(f, phys, ff, normal, eig, bx, fb, mask_tmp, angle_min, angle_max,
+                            surface_mask, s);
+}
+int sph_dplus_shift(const SphFluid* f, const SphPhysics* phys, int nscenes, const float* packing,
+                    const SphCSR* ff, const SphBoundary* b, const SphCSR* fb,
+                    const float* normal, const float* surface_mask, unsigned* vmax_ws,
+                    float* shift, hipStream_t s) {
+  return sph::dplus_shift(f, phys, nscenes, packing, ff, b, fb, normal, surface_mask, vmax_ws,
+                          shift, s);
+}
+int sph_dplus_pressure(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                       const SphBoundary* b, const SphCSR* fb, const float* surface_mask,
+                       unsigned terms, float* acc, hipStream_t s) {
+  return sph::dplus_pressure(f, phys, ff, b, fb, surface_mask, terms, acc, s);
+}
+}

@@ -5,6 +5,7 @@ from __future__ import annotations
 import torch
 
 from .modules.density import densityModule
+from .modules.deltaPlus import deltaPlusModule
 from .modules.densityDiffusion import densityDiffusionModule
 from .modules.diffusion import diffusionModule
 from .modules.gravity import gravityModule
@@ -35,6 +36,12 @@ class deltaSPHSimulation(SPHSimulation):
         self.modules.append(self.densityDiffusionModule)
         self.laminarViscosityModule = laminarViscosityModule()
         self.modules.append(self.laminarViscosityModule)
+        # deltaSPH.py:119-122
+        c = self.config
+        if c["simulation"]["scheme"] == "deltaPlus" or (
+                c["shifting"]["scheme"] == "deltaPlus" and c["shifting"]["enabled"]):
+            self.shiftingModule = deltaPlusModule()
+            self.modules.append(self.shiftingModule)
         if self.config["simulation"]["boundaryScheme"] != "solid":
             raise NotImplementedError("delta-SPH runs with the solid (mDBC) boundary")
         self.boundaryModule = solidBoundaryModule()
@@ -53,6 +60,8 @@ class deltaSPHSimulation(SPHSimulation):
         self.boundaryModule.boundaryNeighborhoodSearch(s, self)
         self.sphDensity.evaluate(s, self)
         self.boundaryModule.evalBoundaryDensity(s, self)
+        if hasattr(self, "shiftingModule"):
+            self.shiftingModule.detectSurface(s, self)
         s["fluidAcceleration"] = torch.zeros_like(s["fluidVelocity"])
         self.gravityModule.evaluate(s, self)
         self.momentumModule.computeDpDt(s, self)

@@ -1,12 +1,15 @@
 """pressureModule (src/modules/pressure.py:54-136): equation of state and
 pressure acceleration a_i = -(1/rho_i) sum_j (p_i + p_j) V_j gradW_ij plus
-the boundary module's term."""
+the boundary module's term; the delta-plus scheme switches to p_j - p_i for
+rows with negative pressure in the bulk class (pressure.py:40-52)."""
+import ctypes as C
+
 import torch
 
 from .. import _lib as L
 from ..module import Module
 from ..parameter import Parameter
-from ._ctx import gather
+from ._ctx import csr_c, fluid, gather
 from .momentum import speed_of_sound
 
 
@@ -49,8 +52,20 @@ class pressureModule(Module):
 
     def computePressureAcceleration(self, simulationState, simulation):
         if self.simulationScheme == "deltaPlus":
-            raise NotImplementedError("the deltaPlus pressure term is outside the hot path")
-        _, self.pressureAccel = gather(simulation, simulationState, L.T_PRESS)
+            # computePressureAccelDeltaPlus (pressure.py:40-52): p_j - p_i
+            # for a row with negative pressure in the bulk class
+            s = simulationState
+            n = s["numParticles"]
+            self.pressureAccel = torch.zeros((n, 2), dtype=torch.float32,
+                                             device=s["fluidPosition"].device)
+            F = fluid(simulation, s, s.get("fluidVolume"))
+            ns = simulation.neighborSearch
+            ff = csr_c(ns.csr.rowptr, ns.csr.col, n)
+            L.call("sph_dplus_pressure", C.byref(F), L.ptr(simulation.phys), C.byref(ff), None,
+                   None, L.ptr(s["fluidSurfaceMask"]), L.T_PRESS, L.ptr(self.pressureAccel),
+                   L.stream_handle())
+        else:
+            _, self.pressureAccel = gather(simulation, simulationState, L.T_PRESS)
         self.pressureAccel += simulation.boundaryModule.computePressureAcceleration(
             simulationState, simulation)
         simulationState["fluidAcceleration"] += self.pressureAccel

@@ -20,8 +20,8 @@ from .. import _lib as L
 from ..module import BoundaryModule
 from ..parameter import Parameter
 from ..scenes import _walls_solid
-from ..search import transpose
-from ._ctx import csr_c, fluid, gather
+from ..search import NeighborGrid, transpose
+from ._ctx import csr_c, f32, fluid, gather
 from .pressure import equation_of_state
 
 MLSscale = 1
@@ -51,6 +51,7 @@ class solidBoundaryModule(BoundaryModule):
         self.kappa = c["pressure"]["kappa"]
         self.c0 = c["fluid"]["c0"]
         self.simulationDensityScheme = c["simulation"]["densityScheme"]
+        self.simulationScheme = c["simulation"]["scheme"]
         dx = float(np.sqrt(c["particle"]["area"]))
         lo = np.array(c["domain"]["min"], dtype=np.float64)
         hi = np.array(c["domain"]["max"], dtype=np.float64)
@@ -69,6 +70,8 @@ class solidBoundaryModule(BoundaryModule):
         self.pressure = torch.zeros(nb, dtype=torch.float32, device=dev)
         self.bscene = torch.zeros(nb, dtype=torch.int32, device=dev)
         self.active = nb > 0
+        self.bb = None
+        self.eigVals = None
 
     def resetState(self, simulationState):
         self.boundaryPressure = None
@@ -132,6 +135,18 @@ class solidBoundaryModule(BoundaryModule):
                                           self.restDensity, self.c0, self.gamma, self.kappa)
 
     def computePressureAcceleration(self, simulationState, simulation):
+        if self.simulationScheme == "deltaPlus":
+            # computePressureAccelDeltaPlus over the boundary pairs
+            # (solidBoundary.py:882-887)
+            s = simulationState
+            acc = torch.zeros((s["numParticles"], 2), dtype=torch.float32, device=self.device)
+            if self.active:
+                F = fluid(simulation, s, s.get("fluidVolume"))
+                B, fb = self.boundary_struct(), self.fb_struct()
+                L.call("sph_dplus_pressure", C.byref(F), L.ptr(simulation.phys), None,
+                       C.byref(B), C.byref(fb), L.ptr(s["fluidSurfaceMask"]), L.T_PRESS_B,
+                       L.ptr(acc), L.stream_handle())
+            return acc
         _, acc = gather(simulation, simulationState, L.T_PRESS_B, boundary=self)
         return acc
 
@@ -139,10 +154,46 @@ class solidBoundaryModule(BoundaryModule):
         _, acc = gather(simulation, simulationState, L.T_VISC_B, boundary=self)
         return acc
 
+    def boundary_lists(self):
+        """Static boundary->boundary pairs (solidBoundary.py:180-190), found
+        once with the project's own search over the boundary points."""
+        if self.bb is None:
+            nb = self.numPtcls
+            bx = self.boundaryPositions
+            lo = bx.min(0).values.cpu().numpy()
+            hi = bx.max(0).values.cpu().numpy()
+            g = NeighborGrid([dict(vmin=lo, vmax=hi, hmax=self.support, p_begin=0, p_count=nb)],
+                             nb, self.device)
+            g.build(bx, self.bscene)
+            self.bb = g.search(bx, self.boundarySupport, self.bscene, 1, self_sorted=True,
+                               sort_rows=True)
+        return self.bb
+
     def computeNormalizationMatrices(self, simulationState, simulation):
-        # the fluid rows' boundary part is added inside sph_dsph_normalization
-        n = simulationState["numParticles"]
-        return torch.zeros((n, 2, 2), dtype=torch.float32, device=self.device)
+        """The boundary's own rows (solidBoundary.py:676-693): M_b over its
+        fluid and boundary neighbours, self.fluidL = pinv2x2(M_b) and
+        self.eigVals.  Returns the fluid rows' boundary part, which
+        sph_dsph_normalization already adds, as zeros."""
+        s = simulationState
+        n, nb = s["numParticles"], self.numPtcls
+        dev = self.device
+        self.normalizationMatrix = torch.empty((nb, 2, 2), dtype=torch.float32, device=dev)
+        self.fluidL = torch.empty((nb, 2, 2), dtype=torch.float32, device=dev)
+        self.eigVals = torch.empty((nb, 2), dtype=torch.float32, device=dev)
+        vf = s.get("fluidVolume")
+        if vf is None:
+            vf = s["fluidArea"] / s["fluidDensity"]
+        vf = f32(vf)
+        vb = f32(self.boundaryVolume / self.boundaryDensity)
+        self._keep_norm = (vf, vb)
+        bf = csr_c(self.bf.rowptr, self.bf.col, nb)
+        bb = self.boundary_lists().c()
+        L.call("sph_normalization_rows", L.ptr(self.boundaryPositions), L.ptr(self.bscene), nb,
+               L.ptr(simulation.phys), C.byref(bf), L.ptr(f32(s["fluidPosition"])), L.ptr(vf),
+               C.byref(bb), L.ptr(self.boundaryPositions), L.ptr(vb),
+               L.ptr(self.normalizationMatrix), L.ptr(self.fluidL), L.ptr(self.eigVals),
+               L.stream_handle())
+        return torch.zeros((n, 2, 2), dtype=torch.float32, device=dev)
 
     def computeRenormalizedDensityGradient(self, simulationState, simulation):
         n = simulationState["numParticles"]

@@ -80,8 +80,8 @@ class SPHSimulation:
         self.device = config["compute"]["device"]
         if config["periodicBC"]["periodicX"] or config["periodicBC"]["periodicY"]:
             raise NotImplementedError("periodic boundaries are outside the hot path")
-        if config["shifting"]["enabled"]:
-            raise NotImplementedError("particle shifting is outside the hot path")
+        if config["shifting"]["enabled"] and config["shifting"]["scheme"] != "deltaPlus":
+            raise NotImplementedError("only the deltaPlus shifting scheme is on the hot path")
         # simulationBase.py:848-855
         p = config["particle"]
         p["area"] = np.pi * p["radius"] ** 2
@@ -206,8 +206,24 @@ class SPHSimulation:
                                                     dt * dpdt / self.config["fluid"]["restDensity"])
         self.simulationState["fluidAcceleration"] = dudt
 
+    def shiftParticles(self):
+        """Step 14 of integrate (simulationBase.py:708-732): search and
+        densities at the integrated positions, then the shifting module's
+        shift.  The surface mask and normals are those of the last
+        right-hand side, which the state still holds."""
+        s = self.simulationState
+        if not hasattr(self, "shiftingModule"):
+            raise AttributeError("shifting is enabled but this scheme has no shifting module")
+        self.neighborSearch.search(s, self)
+        self.boundaryModule.boundaryFilterNeighborhoods(s, self)
+        self.boundaryModule.boundaryNeighborhoodSearch(s, self)
+        self.sphDensity.evaluate(s, self)
+        self.boundaryModule.evalBoundaryDensity(s, self)
+        self.shiftingModule.shift(s, self)
+        s["fluidPosition"] += s["fluidUpdate"]
+
     def integrate(self):
-        """One time step (simulationBase.py:632-782 without export/shifting)."""
+        """One time step (simulationBase.py:632-782 without export)."""
         self.perennialState = self.saveState()
         self.resetState()
         self.simulationState = self.setupSimulationState(self.perennialState)
@@ -246,6 +262,8 @@ class SPHSimulation:
             self.integrateValues(1 / 6 * dt, *comb)
         else:
             raise ValueError(f"unknown integration scheme {scheme}")
+        if self.config["shifting"]["enabled"]:
+            self.shiftParticles()
         self.perennialState["time"] += self.simulationState["dt"]
         self.perennialState["timestep"] += 1
         self.simulationState["time"] = self.perennialState["time"]
