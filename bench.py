@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shapes instead of autotuning: count,gather,solve[,fill] "
                          "lanes per row and the DFSPH grid cap, e.g. 2,4,2,8,16384")
+    ap.add_argument("--dplus-only", action="store_true",
+                    help="only the delta-plus surface detection and shift timings (profiling)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the kernel timings and the copy calibration (counter runs)")
     return ap.parse_args()
@@ -208,6 +210,8 @@ def plain_paths(steps: int = 3, warmup: int = 1):
       dsph_c1       the 5,000-particle delta-SPH dam break of the CPU
                     baseline, one graph-captured step per synchronise
                     (latency-bound at this size)
+      dplus_4m      delta-plus surface detection, shift and pressure switch
+                    on a 4,000,000-particle dam break (dplus_path)
     Algorithmic bytes (compulsory, each array once; K = mean neighbours,
     c = cells per point):
       search        x, h read (12) + sorted x, index written (12) + key (4)
@@ -310,6 +314,91 @@ def plain_paths(steps: int = 3, warmup: int = 1):
                           p99_step_ms=float(np.percentile(lat, 99) * 1e3), steps=len(lat))
     del b
     log(f"plain delta-SPH C1: {out['dsph_c1']}")
+    out["dplus_4m"] = dplus_path()
+    log(f"delta-plus 4M: {out['dplus_4m']}")
+    return out
+
+
+def dplus_path(reps: int = 10):
+    """delta-plus surface detection and shifting on a 2000 x 2000 dam break
+    (dx = 0.0005, 4,000,000 fluid particles, mDBC walls, velocities U(+-0.5)
+    and densities 1 +- 0.004 so that the shift and the pressure switch act),
+    through the module API: each verb timed with events over `reps` calls.
+    Algorithmic bytes per fluid row (each array once; K = mean fluid
+    neighbours, Kb = boundary pairs per row; columns 4 B, row pointers 4 B):
+      normalization  rows/cols (4 + 4K) + x, A, rho (16) + M, L, eig (40)
+      normal         rows/cols (4 + 4K) + x, V (12) + L, eig (24) + grad, n (16)
+      surface        two passes: (4 + 4K) + x, n, eig (24) + mask, angles (12);
+                     (4 + 4K) + mask, eig (12) + class (4)
+      shift          (4 + 4K) + v (8, speed pass) + x, A, rho, n, class (28)
+                     + shift (8)
+      pressure       (4 + 4K) + x, rho, V, class (20) + acc read and write (16)
+    plus, in every verb that reads the boundary, (4 + 4 Kb) per fluid row."""
+    import torch
+
+    from pytorchsph_amd.deltaSPH import deltaSPHSimulation
+
+    dx = 0.0005
+    cfg = {
+        "particle": {"radius": dx / math.sqrt(math.pi)},
+        "domain": {"min": [0.0, 0.0], "max": [2.0, 1.2]},
+        "emitter": {"damBreak": {"min": [0.0, 0.0], "max": [1.0, 1.0], "velocity": [0.0, 0.0]}},
+        "compute": {"device": "cuda"},
+        "fluid": {"restDensity": 1000.0},
+        "simulation": {"scheme": "deltaPlus", "boundaryScheme": "solid",
+                       "densityScheme": "continuum"},
+        "diffusion": {"densityScheme": "MOG", "kinematic": 0.001},
+        "integration": {"scheme": "RK4", "dt": 1e-5},
+        "shifting": {"enabled": True, "scheme": "deltaPlus"},
+    }
+    sim = deltaSPHSimulation(cfg)
+    sim.initializeSimulation()
+    s = sim.simulationState
+    n = s["numParticles"]
+    g = torch.Generator(device="cuda").manual_seed(7)
+    s["fluidVelocity"] += (torch.rand((n, 2), device="cuda", generator=g) - 0.5)
+    s["fluidDensity"] += (torch.rand(n, device="cuda", generator=g) - 0.5) * 0.008
+    sim.neighborSearch.search(s, sim)
+    bm = sim.boundaryModule
+    bm.boundaryFilterNeighborhoods(s, sim)
+    bm.boundaryNeighborhoodSearch(s, sim)
+    sim.sphDensity.evaluate(s, sim)
+    bm.evalBoundaryDensity(s, sim)
+    sm = sim.shiftingModule
+    sm.detectSurface(s, sim)
+    s["fluidAcceleration"] = torch.zeros_like(s["fluidVelocity"])
+    sim.pressureModule.computePressure(s, sim)
+    torch.cuda.synchronize()
+    K = int(sim.neighborSearch.csr.rowptr[n].item()) / n
+    Kb = int(bm.fb.rowptr[n].item()) / n
+    rb = 4 + 4 * Kb
+
+    def renorm():
+        s.pop("fluidL", None)
+        sm.computeNormalizationmatrix(s, sim)
+
+    verbs = {
+        "normalization": (renorm, 4 + 4 * K + 16 + 40 + rb),
+        "normal": (lambda: sm.computeFluidNormal(s, sim), 4 + 4 * K + 12 + 24 + 16 + rb),
+        "surface": (lambda: sm.detectFluidSurface(s, sim),
+                    (4 + 4 * K + 24 + 12 + rb) + (4 + 4 * K + 12 + 4)),
+        "shift": (lambda: sm.computeShiftAmount(s, sim), 4 + 4 * K + 8 + 28 + 8 + rb),
+        "pressure": (lambda: sim.pressureModule.computePressureAcceleration(s, sim),
+                     4 + 4 * K + 20 + 16 + rb),
+    }
+    peak = 8000.0
+    out = dict(particles=n, boundary=bm.numPtcls, mean_neighbours=K, boundary_pairs_per_row=Kb)
+    total = 0.0
+    for name, (fn, per_row) in verbs.items():
+        sec = _events_avg(fn, torch.cuda.current_stream(), reps=reps, warm=2)
+        gbs = n * per_row / sec / 1e9
+        out[name] = dict(us=sec * 1e6, bytes_per_row=per_row, achieved_gbs=gbs, frac=gbs / peak)
+        total += sec
+    out["detect_and_shift_us"] = total * 1e6 - out["pressure"]["us"]
+    surf = s["fluidSurfaceMask"]
+    out["classes"] = {str(k): int((surf == k).sum().item()) for k in (0.0, 1.0, 2.0)}
+    del sim, s
+    torch.cuda.empty_cache()
     return out
 
 
@@ -422,6 +511,9 @@ def main():
     from pytorchsph_amd.autotune import tune
     from pytorchsph_amd.engine import MixedBatch, SceneBatch
 
+    if args.dplus_only:
+        print(json.dumps({"dplus_4m": dplus_path()}), flush=True)
+        return
     t0 = time.perf_counter()
     scenes = W.c4_scenes(n_scenes=args.scenes)
     n_part = int(sum(s.n for s in scenes))

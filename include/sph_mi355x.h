@@ -444,8 +444,8 @@ int sph_permute_fields(const SphFields* f, const int* perm, int n, float* tmp, h
  *   M_q = sum_{j in a(q)} va_j (xa_j - xq_q) (x) gradW_qj
  *       + sum_{j in b(q)} vb_j (xb_j - xq_q) (x) gradW_qj,
  * L = pinv2x2(M), eig = its two singular values, larger first (util.py:
- * 386-433).  A list whose rowptr is NULL is skipped.  M and eig may be NULL.
- * qscene[q] selects the support. */
+ * 386-433).  A list whose rowptr is NULL is skipped.  M and eig may be NULL;
+ * M and L are 16-byte aligned.  qscene[q] selects the support. */
 int sph_normalization_rows(const float* xq, const int* qscene, int nq, const SphPhysics* phys,
                            const SphCSR* a, const float* xa, const float* va,
                            const SphCSR* b, const float* xb, const float* vb,
@@ -460,7 +460,8 @@ int sph_normalization_rows(const float* xq, const int* qscene, int nq, const Sph
  * with V_j = f->vol[j] (or A_j / rho_j) and eps = 0.1 h^2.  bvol_n is the
  * volume the reference uses for boundary neighbour b: fluidArea[b] /
  * fluidDensity[b], the fluid arrays indexed by the boundary index
- * (deltaPlus.py:151); the caller passes it.  grad may be NULL. */
+ * (deltaPlus.py:151); the caller passes it.  grad may be NULL; L ([np][2][2])
+ * is 16-byte aligned. */
 int sph_dplus_normal(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
                      const float* L, const float* eig, const float* bx, const float* beig,
                      const float* bvol_n, const SphCSR* fb, float* grad, float* normal,

@@ -31,18 +31,6 @@ __device__ __forceinline__ void st2(float* p, int i, float2 v) {
 }
 
 // ------------------------------------------------------------- density
-// Row owner of this lane: particle index t (before `order`) and the lane's
-// position inside its group.
-template <int KL>
-struct RowLane {
-  int t, sub;
-};
-template <int KL>
-__device__ __forceinline__ RowLane<KL> row_lane() {
-  const int64_t gt = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-  return {(int)(gt / KL), (int)(gt % KL)};
-}
-
 template <int KL>
 __global__ __launch_bounds__(kBS) void k_fluid_density(SphFluid f, const SphPhysics* __restrict__ phys,
                                                        SphCSR ff, SphBoundary b, SphCSR fb,
@@ -409,36 +397,53 @@ __global__ __launch_bounds__(kBS) void k_pinv2x2(const float* __restrict__ M, in
 }
 
 // M_i = sum_j V_j (x_j - x_i) (x) gradW_ij (densityDiffusion.py:30-43) with
-// the boundary part (solidBoundary.py:684-687); L_i = pinv2x2(M_i).
+// the boundary part (solidBoundary.py:684-687); L_i = pinv2x2(M_i).  A
+// group of KL lanes walks each row.
+struct XVol {
+  float x, y, v;
+};
+template <int KL>
 __global__ __launch_bounds__(kBS) void k_dsph_normalization(SphFluid f, const SphPhysics* __restrict__ phys,
                                                             SphCSR ff, SphBoundary b, SphCSR fb,
                                                             const int* __restrict__ order,
                                                             float* __restrict__ Mout,
                                                             float* __restrict__ Lout,
                                                             float* __restrict__ eig) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= f.np) return;
-  const int i = order ? order[t] : t;
+  const RowLane<KL> rl = row_lane<KL>();
+  if (rl.t >= f.np) return;
+  const int i = order ? order[rl.t] : rl.t;
   const KConst K = kconst(phys[f.pscene[i]].h);
   const float2 xi = ld2(f.x, i);
   float m00 = 0.f, m01 = 0.f, m10 = 0.f, m11 = 0.f;
-  for (int pass = 0; pass < 2; ++pass) {
-    const SphCSR& c = pass == 0 ? ff : fb;
-    if (!c.rowptr) continue;
-    for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
-      const int j = c.col[k];
-      const float2 xj = pass == 0 ? ld2(f.x, j) : ld2(b.x, j);
-      const float V = pass == 0 ? f.area[j] / f.rho[j] : b.vol[j] / b.rho[j];
-      const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, K.inv_h);
-      const float gw = -wendland_grad_k(g.q, K);
-      const float gx = gw * g.hx, gy = gw * g.hy;
-      const float rx = -g.rx, ry = -g.ry;
-      m00 += rx * gx * V;
-      m01 += rx * gy * V;
-      m10 += ry * gx * V;
-      m11 += ry * gy * V;
+  auto use = [&](int, const XVol& r) {
+    const PairGeo g = pair_geo(xi.x, xi.y, r.x, r.y, K.inv_h);
+    const float gw = -wendland_grad_k(g.q, K);
+    const float gx = gw * g.hx, gy = gw * g.hy;
+    const float rx = -g.rx, ry = -g.ry;
+    m00 += rx * gx * r.v;
+    m01 += rx * gy * r.v;
+    m10 += ry * gx * r.v;
+    m11 += ry * gy * r.v;
+  };
+  row_pairs<KL>(
+      ff.col, ff.rowptr[i], ff.rowptr[i + 1], rl.sub,
+      [&](int j) {
+        const float2 xj = ld2(f.x, j);
+        return XVol{xj.x, xj.y, f.vol ? f.vol[j] : f.area[j] / f.rho[j]};
+      },
+      use);
+  if (fb.rowptr) {
+    for (int k = fb.rowptr[i] + rl.sub; k < fb.rowptr[i + 1]; k += KL) {
+      const int j = fb.col[k];
+      const float2 xj = ld2(b.x, j);
+      use(j, XVol{xj.x, xj.y, b.vol[j] / b.rho[j]});
     }
   }
+  m00 = group_sum<KL>(m00);
+  m01 = group_sum<KL>(m01);
+  m10 = group_sum<KL>(m10);
+  m11 = group_sum<KL>(m11);
+  if (rl.sub != 0) return;
   if (Mout) {
     Mout[4 * i] = m00;
     Mout[4 * i + 1] = m01;
@@ -774,8 +779,12 @@ int dsph_normalization(const SphFluid* f, const SphPhysics* phys, const SphCSR* 
   SPH_REQUIRE(!fb || !fb->rowptr || (b && b->x && b->rho && b->vol),
               "sph_dsph_normalization: boundary list without boundary state");
   if (f->np == 0) return SPH_OK;
-  hipLaunchKernelGGL(k_dsph_normalization, dim3(blocks_for(f->np, kBS)), dim3(kBS), 0, s, *f, phys,
-                     *ff, bdy_or_null(b), csr_or_null(fb), order, M, L, eig);
+  with_lanes(tuning(SPH_TUNE_GATHER_LANES), [&](auto kl) {
+    constexpr int KL = decltype(kl)::value;
+    hipLaunchKernelGGL((k_dsph_normalization<KL>), dim3(blocks_for((int64_t)f->np * KL, kBS)),
+                       dim3(kBS), 0, s, *f, phys, *ff, bdy_or_null(b), csr_or_null(fb), order, M,
+                       L, eig);
+  });
   return launch_check("sph_dsph_normalization");
 }
 

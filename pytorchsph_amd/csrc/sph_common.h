@@ -181,6 +181,31 @@ __device__ __forceinline__ float eos(const SphPhysics& P, float rhohat) {
   return P.kappa * (rhohat * P.rho0 - P.rho0);
 }
 
+// eos() with its constant factor evaluated once per thread, for loops over
+// neighbours; the same arithmetic, so the same values.
+struct EosK {
+  float f, rho0, gamma;
+  int kind;
+};
+__device__ __forceinline__ EosK eos_k(const SphPhysics& P) {
+  EosK e;
+  e.kind = P.eos;
+  e.rho0 = P.rho0;
+  e.gamma = P.gamma;
+  e.f = P.eos == 0 ? P.rho0 * P.c0 * P.c0 / P.gamma : (P.eos == 1 ? P.c0 * P.c0 : P.kappa);
+  return e;
+}
+__device__ __forceinline__ float eos_eval(const EosK& e, float rhohat) {
+  if (e.kind == 0) {
+    float r2 = rhohat * rhohat;
+    float r4 = r2 * r2;
+    float r7 = r4 * r2 * rhohat;
+    float rg = (e.gamma == 7.f) ? r7 : powf(rhohat, e.gamma);
+    return e.f * (rg - 1.f);
+  }
+  return e.f * (rhohat * e.rho0 - e.rho0);
+}
+
 // ---------------------------------------------------- XCD-aware blocks
 // The dispatcher hands consecutive workgroups to the 8 XCDs in turn, and
 // each XCD has its own L2.  Rows in cell order share neighbours, so the
@@ -221,6 +246,56 @@ __device__ __forceinline__ unsigned long long group_mask() {
 __device__ __forceinline__ unsigned long long lanes_below() {
   const int lane = threadIdx.x & (kWave - 1);
   return (1ull << lane) - 1ull;
+}
+
+// Row owner of this lane: row t (before any row order) and the lane's
+// position inside its group of KL lanes (blocks remapped per XCD).
+template <int KL>
+struct RowLane {
+  int t, sub;
+};
+template <int KL>
+__device__ __forceinline__ RowLane<KL> row_lane() {
+  const int64_t gt = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  return {(int)(gt / KL), (int)(gt % KL)};
+}
+template <int KL>
+__device__ __forceinline__ float group_max(float v) {
+#pragma unroll
+  for (int o = KL / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+template <int KL>
+__device__ __forceinline__ float group_min(float v) {
+#pragma unroll
+  for (int o = KL / 2; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+// The pairs [k0, k1) of one row walked by the KL lanes of a group: lane
+// `sub` takes pairs sub, sub + KL, ...; per trip it issues the loads of NB
+// pairs (slots past the row re-read pair k0) before it uses any, so NB
+// neighbour loads are in flight per lane.  load(j) returns the neighbour
+// record, use(j, rec) consumes it, in the lane's pair order.
+template <int KL, int NB = 4, class Load, class Use>
+__device__ __forceinline__ void row_pairs(const int* __restrict__ col, int k0, int k1, int sub,
+                                          Load load, Use use) {
+  for (int kb = k0 + sub; kb < k1; kb += KL * NB) {
+    bool ok[NB];
+    int jj[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int k = kb + u * KL;
+      ok[u] = k < k1;
+      jj[u] = col[ok[u] ? k : k0];
+    }
+    decltype(load(0)) rec[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) rec[u] = load(jj[u]);
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+      if (ok[u]) use(jj[u], rec[u]);
+  }
 }
 
 // ------------------------------------------------ wave-staged column slices
