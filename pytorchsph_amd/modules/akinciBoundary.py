@@ -18,7 +18,7 @@ import torch
 from .. import _lib as L
 from ..module import BoundaryModule
 from ..parameter import Parameter
-from ..scenes import _walls_akinci, akinci_volumes
+from ..scenes import _walls_akinci, akinci_volumes, box_normals
 from ..search import transpose
 from ._ctx import csr_c, fluid
 
@@ -63,14 +63,35 @@ class akinciBoundaryModule(BoundaryModule):
         self.boundaryVelocity = torch.zeros((nb, 2), dtype=torch.float32, device=dev)
         self.boundaryPressure = torch.zeros(nb, dtype=torch.float32, device=dev)
         self.bscene = torch.zeros(nb, dtype=torch.int32, device=dev)
+        self.boundaryNormals = torch.from_numpy(box_normals(bx, lo, hi)).to(dev)
+        self.bodyAssociation = torch.zeros(nb, dtype=torch.int64, device=dev)
         self.active = nb > 0
 
     def saveState(self, perennialState, copy):
-        perennialState["boundaryPosition"] = self.boundaryPositions
-        perennialState["boundaryDensity"] = self.boundaryDensity
-        perennialState["boundaryArea"] = self.boundaryVolume
-        perennialState["boundaryPressure"] = self.boundaryPressure
+        # solidBoundary.py:266-278 / akinciBoundary.py:50-62 key set
+        cl = (lambda t: torch.clone(t)) if copy else (lambda t: t)
+        perennialState["boundaryPosition"] = cl(self.boundaryPositions)
+        perennialState["boundaryVelocity"] = cl(self.boundaryVelocity)
+        perennialState["boundaryDensity"] = cl(self.boundaryDensity)
+        perennialState["boundarySupport"] = cl(self.boundarySupport)
+        perennialState["boundaryRestDensity"] = cl(self.boundaryRestDensity)
+        perennialState["boundaryArea"] = cl(self.boundaryVolume)
+        perennialState["boundaryPressure"] = cl(self.boundaryPressure)
+        perennialState["boundaryNormals"] = cl(self.boundaryNormals)
+        perennialState["boundaryBodyAssociation"] = cl(self.bodyAssociation)
         perennialState["boundaryParticles"] = self.numPtcls
+
+    def exportState(self, simulationState, simulation, grp, mask):
+        # solidBoundary.py:253-261: per-step boundary only when it is not static
+        if not simulation.config["export"]["staticBoundary"]:
+            for key, t in (("boundaryPosition", self.boundaryPositions),
+                           ("boundaryVelocity", self.boundaryVelocity),
+                           ("boundarySupport", self.boundarySupport),
+                           ("boundaryRestDensity", self.boundaryRestDensity),
+                           ("boundaryArea", self.boundaryVolume),
+                           ("boundaryNormals", self.boundaryNormals),
+                           ("boundaryBodyAssociation", self.bodyAssociation)):
+                grp.create_dataset(key, data=t.detach().cpu().numpy())
 
     def boundaryNeighborhoodSearch(self, simulationState, simulation):
         if not self.active:

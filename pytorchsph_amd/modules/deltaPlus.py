@@ -34,6 +34,12 @@ class deltaPlusModule(Module):
     def __init__(self):
         super().__init__("densityInterpolation", "Evaluates density at the current timestep")
 
+    def exportState(self, simulationState, simulation, grp, mask):
+        # deltaPlus.py:45-47
+        if simulation.config["shifting"]["enabled"] and "fluidUpdate" in simulationState:
+            grp.create_dataset("fluidShifting",
+                               data=simulationState["fluidUpdate"].detach().cpu().numpy())
+
     def initialize(self, simulationConfig, simulationState):
         c = simulationConfig
         self.support = c["particle"]["support"]

@@ -19,7 +19,7 @@ import torch
 from .. import _lib as L
 from ..module import BoundaryModule
 from ..parameter import Parameter
-from ..scenes import _walls_solid
+from ..scenes import _walls_solid, box_normals
 from ..search import NeighborGrid, transpose
 from ._ctx import csr_c, f32, fluid, gather
 from .pressure import equation_of_state
@@ -69,6 +69,10 @@ class solidBoundaryModule(BoundaryModule):
         self.boundaryDensity = torch.ones(nb, dtype=torch.float32, device=dev)
         self.pressure = torch.zeros(nb, dtype=torch.float32, device=dev)
         self.bscene = torch.zeros(nb, dtype=torch.int32, device=dev)
+        # outward normals of the domain box (the polygon SDF gradient of
+        # solidBoundary.py:158) and one body (the domain)
+        self.boundaryNormals = torch.from_numpy(box_normals(bx + lo, lo, hi)).to(dev)
+        self.bodyAssociation = torch.zeros(nb, dtype=torch.int64, device=dev)
         self.active = nb > 0
         self.bb = None
         self.eigVals = None
@@ -77,9 +81,30 @@ class solidBoundaryModule(BoundaryModule):
         self.boundaryPressure = None
 
     def saveState(self, perennialState, copy):
-        perennialState["boundaryPosition"] = self.boundaryPositions
-        perennialState["boundaryDensity"] = self.boundaryDensity
+        # solidBoundary.py:266-278 / akinciBoundary.py:50-62 key set
+        cl = (lambda t: torch.clone(t)) if copy else (lambda t: t)
+        perennialState["boundaryPosition"] = cl(self.boundaryPositions)
+        perennialState["boundaryVelocity"] = cl(self.boundaryVelocity)
+        perennialState["boundaryDensity"] = cl(self.boundaryDensity)
+        perennialState["boundarySupport"] = cl(self.boundarySupport)
+        perennialState["boundaryRestDensity"] = cl(self.boundaryRestDensity)
+        perennialState["boundaryArea"] = cl(self.boundaryVolume)
+        perennialState["boundaryPressure"] = cl(self.pressure)
+        perennialState["boundaryNormals"] = cl(self.boundaryNormals)
+        perennialState["boundaryBodyAssociation"] = cl(self.bodyAssociation)
         perennialState["boundaryParticles"] = self.numPtcls
+
+    def exportState(self, simulationState, simulation, grp, mask):
+        # solidBoundary.py:253-261: per-step boundary only when it is not static
+        if not simulation.config["export"]["staticBoundary"]:
+            for key, t in (("boundaryPosition", self.boundaryPositions),
+                           ("boundaryVelocity", self.boundaryVelocity),
+                           ("boundarySupport", self.boundarySupport),
+                           ("boundaryRestDensity", self.boundaryRestDensity),
+                           ("boundaryArea", self.boundaryVolume),
+                           ("boundaryNormals", self.boundaryNormals),
+                           ("boundaryBodyAssociation", self.bodyAssociation)):
+                grp.create_dataset(key, data=t.detach().cpu().numpy())
 
     # ------------------------------------------------------------ search
     def boundaryNeighborhoodSearch(self, simulationState, simulation):

@@ -126,6 +126,16 @@ def _jitter(rng, pts, dx, amp=0.01):
     return pts + rng.uniform(-amp * dx, amp * dx, size=pts.shape)
 
 
+def box_normals(bx, lo, hi) -> np.ndarray:
+    """Outward unit normals of wall points outside the box [lo, hi]: the
+    gradient of the box's signed distance, (p - nearest box point) / |.|
+    (the polygon SDF gradient of solidBoundary.py:158 for a rectangle)."""
+    bx = np.asarray(bx, dtype=np.float64)
+    d = bx - np.clip(bx, np.asarray(lo, np.float64), np.asarray(hi, np.float64))
+    n = np.linalg.norm(d, axis=1, keepdims=True)
+    return (d / np.where(n > 0, n, 1.0)).astype(np.float32)
+
+
 def _walls_solid(W, H, dx, layers=3):
     nx, ny = int(round(W / dx)), int(round(H / dx))
     i = np.arange(-layers, nx + layers)

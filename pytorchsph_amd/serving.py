@@ -34,6 +34,7 @@ benchmark, caps of 1, 1.5, 2, 3, 6 and 12M particles gave 2.8, 3.1, 3.1,
 """
 from __future__ import annotations
 
+import os
 import time
 from collections import deque
 from dataclasses import dataclass
@@ -42,6 +43,7 @@ import numpy as np
 import torch
 
 from .engine import MixedBatch, SceneBatch
+from .export import write_job_result
 from .scenes import Scene
 
 
@@ -70,7 +72,8 @@ class Job:
 class Server:
     def __init__(self, device="cuda", budget_bytes: int | None = None,
                  max_particles: int = 2_000_000, min_repack_s: float = 0.5,
-                 waste_frac: float = 0.5, log=None, **batch_kw):
+                 waste_frac: float = 0.5, log=None, export_dir: str | None = None,
+                 **batch_kw):
         self.device = device
         if budget_bytes is None:
             free, _ = torch.cuda.mem_get_info()
@@ -81,6 +84,9 @@ class Server:
         self.waste_frac = float(waste_frac)
         self.batch_kw = batch_kw
         self.log = log
+        # finished jobs are written as export_dir/job_<id>.hdf5 in the
+        # reference's export layout (export.write_job_result)
+        self.export_dir = export_dir
         self.batch: MixedBatch | None = None
         self.live: list[Job] = []
         self.repacks = 0
@@ -101,6 +107,11 @@ class Server:
         st["status"] = b.scene_status(pos)
         job.result = st
         job.t_done = now
+        if self.export_dir is not None:
+            os.makedirs(self.export_dir, exist_ok=True)
+            path = os.path.join(self.export_dir, f"job_{job.id}.hdf5")
+            write_job_result(path, job.scene, st, job.steps_done)
+            st["export"] = path
 
     def _advance(self, pos_of: dict, seconds: float, now: float):
         """Book-keeping after one step of the whole batch."""

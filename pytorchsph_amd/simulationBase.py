@@ -13,6 +13,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+from .export import SimulationExport
 from .kernels import getKernelFunctions
 from .modules._ctx import phys_tensor
 from .parameter import Parameter
@@ -157,6 +158,19 @@ class SPHSimulation:
         self.phys, self.physDict = phys_tensor(self.config, self.device)
         self.perennialState = self.saveState(copy=False)
         self.simulationState = self.setupSimulationState(self.perennialState)
+        self.exporter = None
+        if c["export"]["active"]:
+            # simulationBase.py:442-492
+            self.exporter = SimulationExport(self, getattr(self, "exportPath", None))
+            self.exportPath = self.exporter.exportPath
+            self.outFile = self.exporter.outFile
+            self.dataGroup = self.exporter.dataGroup
+
+    def finalize(self):
+        """Closes the export file, if any."""
+        if getattr(self, "exporter", None) is not None:
+            self.exporter.close()
+            self.exporter = None
 
     def resetState(self):
         for module in self.modules:
@@ -223,7 +237,7 @@ class SPHSimulation:
         s["fluidPosition"] += s["fluidUpdate"]
 
     def integrate(self):
-        """One time step (simulationBase.py:632-782 without export)."""
+        """One time step (simulationBase.py:632-782)."""
         self.perennialState = self.saveState()
         self.resetState()
         self.simulationState = self.setupSimulationState(self.perennialState)
@@ -268,3 +282,5 @@ class SPHSimulation:
         self.perennialState["timestep"] += 1
         self.simulationState["time"] = self.perennialState["time"]
         self.simulationState["timestep"] = self.perennialState["timestep"]
+        if getattr(self, "exporter", None) is not None:
+            self.exporter.write_step(self)

@@ -198,3 +198,33 @@ def test_dplus_steps_match_oracle(cuda):
         assert ex.max() < 0.1 * dx, (step, ex.max())
         assert np.quantile(np.abs(gr - r), 0.999) < 1e-5, step
         x, v, r = gx, gv, gr            # continue both from the device state
+
+
+def test_export_of_a_dplus_run(cuda, tmp_path):
+    """The module API's HDF5 export (simulationBase.py:442-492, 742-780) of a
+    delta-plus run with shifting: one group per step, the end-of-step state,
+    the shift of the step (deltaPlus.py:45-47) and the static boundary."""
+    from pytorchsph_amd import h5
+
+    c = dplus_config()
+    c["export"] = {"active": True, "prefix": "dplus"}
+    sim = deltaSPHSimulation(c)
+    sim.exportPath = str(tmp_path / "run.hdf5")
+    sim.initializeSimulation()
+    for _ in range(2):
+        sim.integrate()
+    s = sim.simulationState
+    sim.finalize()
+    with h5.File(tmp_path / "run.hdf5", "r") as f:
+        steps = f["simulationExport"]
+        assert steps.keys() == ["00000", "00001"]
+        g = steps["00001"]
+        assert g.attrs["timestep"] == 2
+        assert np.array_equal(g["finalPosition"][()], s["fluidPosition"].cpu().numpy())
+        assert np.array_equal(g["fluidShifting"][()], s["fluidUpdate"].cpu().numpy())
+        assert np.abs(g["fluidShifting"][()]).max() > 0
+        bi = f["boundaryInformation"]
+        n = bi["boundaryNormals"][()]
+        assert np.allclose(np.linalg.norm(n, axis=1), 1, atol=1e-6)
+        assert f.attrs["shiftingEnabled"] == np.True_
+        assert f.attrs["simulationScheme"] == "deltaPlus"

@@ -67,6 +67,26 @@ def test_alone_equals_in_mix_with_repacks(cuda):
             assert np.max(np.abs(got["v"] - ref["v"])) <= 1e-4 * scale
 
 
+def test_served_results_exported_in_reference_layout(cuda, tmp_path):
+    """Server(export_dir=...) writes each finished job as an HDF5 file in
+    the reference's export layout holding the job's final state."""
+    from pytorchsph_amd import h5
+
+    jobs = make_jobs()[:3]
+    srv = Server(min_repack_s=0.0, waste_frac=0.2, export_dir=str(tmp_path))
+    out = srv.run(jobs, clock=FakeClock())
+    assert out["jobs_done"] == len(jobs)
+    for j in jobs:
+        with h5.File(j.result["export"], "r") as f:
+            g = f["simulationExport"]["00000"]
+            assert g.attrs["timestep"] == j.steps
+            assert np.array_equal(g["finalPosition"][()], j.result["x"])
+            assert np.array_equal(g["fluidDensity"][()], j.result["rho"])
+            assert f["boundaryInformation"]["boundaryPosition"][()].shape == (j.scene.nb, 2)
+            assert f.attrs["simulationScheme"] == ("deltaSPH" if j.scene.scheme == "dsph"
+                                                   else "dfsph")
+
+
 def test_footprint_matches_allocation(cuda):
     sc = S.scene_for_count("dambreak", "dfsph", 20000, 3)
     torch.cuda.synchronize()
