@@ -399,6 +399,18 @@ def dplus_path(reps: int = 10):
     out["classes"] = {str(k): int((surf == k).sum().item()) for k in (0.0, 1.0, 2.0)}
     del sim, s
     torch.cuda.empty_cache()
+    # the batched engine's delta-plus step (detection and the pressure switch
+    # in each RK4 stage, then search, mDBC and the shift) on the 4M dam break
+    from pytorchsph_amd import scenes as S
+    from pytorchsph_amd.engine import SceneBatch
+
+    sc = S.dplus_variant(S.dam_break(dx=0.0005, block=(1.0, 1.0), scheme="dsph", seed=5))
+    b = SceneBatch([sc])
+    sec = _timed_steps(b, 1, 3)
+    out["engine_step"] = dict(particles=b.NP, ms_per_step=sec * 1e3,
+                              particle_steps_per_s=b.NP / sec)
+    del b
+    torch.cuda.empty_cache()
     return out
 
 

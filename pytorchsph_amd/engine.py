@@ -143,10 +143,21 @@ class SceneBatch:
         self.phys_host = phys_records(scenes)
         self.phys = L.records_to_tensor(self.phys_host, dev)
         d0 = scenes[0].phys
+        # delta-plus: surface detection in every right-hand side with the
+        # delta-plus pressure switch (scheme 'deltaPlus'), and the shift
+        # after each step (shifting.enabled); one setting per batch
+        self.dplus = bool(d0.get("dplus", False))
+        self.shifting = bool(d0.get("shifting", False))
+        assert all(bool(s.phys.get("dplus", False)) == self.dplus and
+                   bool(s.phys.get("shifting", False)) == self.shifting for s in scenes), \
+            "one delta-plus setting per batch"
         if scheme == "dsph":
             self.terms = DSPH_TERMS_DD if d0.get("diffusion") == "deltaSPH" else DSPH_TERMS_MOG
             if not d0.get("boundary_diffusion", True):
                 self.terms &= ~L.T_VISC_B
+            if self.dplus:
+                # the pressure terms run in sph_dplus_pressure (switched)
+                self.terms &= ~(L.T_PRESS | L.T_PRESS_B)
         # ---- neighbour search
         grid_scenes = [dict(vmin=s.vmin, vmax=s.vmax, hmax=s.support, p_begin=int(pb),
                             p_count=int(s.n)) for s, pb in zip(scenes, self.p_begin)]
@@ -171,6 +182,10 @@ class SceneBatch:
             self.frec2 = torch.zeros_like(self.frec) if fuse_rk4 else None
             self.gradrho = torch.zeros((NP, 2), dtype=f32, device=dev)
             self.Lmat = torch.zeros((NP, 4), dtype=f32, device=dev)
+            if self.dplus or self.shifting:
+                assert self.solid and not fuse_rk4, \
+                    "delta-plus runs with mDBC walls and the separate RK4 kernels"
+                self._alloc_dplus()
         else:
             self.vol = torch.zeros(NP, dtype=f32, device=dev)
             self._alloc_dfsph()
@@ -239,6 +254,100 @@ class SceneBatch:
         return st
 
     # ----------------------------------------------------------- helpers
+    def _alloc_dplus(self):
+        """Buffers of the delta-plus passes, the static boundary->boundary
+        pair list (solidBoundary.py:180) and the map of quirk deltaPlus.py:151
+        (boundary neighbour b takes the volume of the scene's fluid particle
+        with original index b)."""
+        from .scenes import lattice_calibration
+
+        dev, NP, NB, f32 = self.dev, self.NP, self.NB, torch.float32
+        nb1 = max(NB, 1)
+        self.eig = torch.zeros((NP, 2), dtype=f32, device=dev)
+        self.normal = torch.zeros((NP, 2), dtype=f32, device=dev)
+        self.smask = torch.full((NP,), 2.0, dtype=f32, device=dev)
+        self.mask_tmp = torch.zeros(NP, dtype=f32, device=dev)
+        self.vf = torch.zeros(NP, dtype=f32, device=dev)
+        self.vb = torch.zeros(nb1, dtype=f32, device=dev)
+        self.bvol_n = torch.zeros(nb1, dtype=f32, device=dev)
+        self.beig = torch.zeros((nb1, 2), dtype=f32, device=dev)
+        self.bL = torch.zeros((nb1, 4), dtype=f32, device=dev)
+        self.shift = torch.zeros((NP, 2), dtype=f32, device=dev)
+        self.vmax_ws = torch.zeros(self.nscenes, dtype=torch.int32, device=dev)
+        self.slot_of = torch.zeros(NP, dtype=torch.int64, device=dev)
+        self.arange_np = torch.arange(NP, dtype=torch.int64, device=dev)
+        calib = lattice_calibration()[0]
+        self.packing = torch.tensor([math.sqrt(s.area / calib) / s.support for s in self.scenes],
+                                    dtype=f32, device=dev)
+        # original fluid index (global) of boundary point b's namesake, or -1
+        orig = np.full(nb1, -1, dtype=np.int64)
+        for k, s in enumerate(self.scenes):
+            m = min(int(s.n), int(s.nb))
+            b0, p0 = int(self.b_begin[k]), int(self.p_begin[k])
+            orig[b0:b0 + m] = p0 + np.arange(m)
+        self.b_orig = torch.from_numpy(orig).to(dev)
+        self.b_valid = self.b_orig >= 0
+        self.b_orig_c = self.b_orig.clamp(min=0)
+        if dev.type == "meta" or NB == 0:
+            cap = int(40 * nb1)
+            self.bb = CSR(torch.zeros(nb1 + 1, dtype=torch.int32, device=dev),
+                          torch.zeros(cap, dtype=torch.int32, device=dev), NB)
+            return
+        gs = [dict(vmin=s.vmin, vmax=s.vmax, hmax=s.support, p_begin=int(b0), p_count=int(s.nb))
+              for s, b0 in zip(self.scenes, self.b_begin)]
+        g = NeighborGrid(gs, NB, dev)
+        g.build(self.bx, self.bscene)
+        self.bb = g.search(self.bx, self.bh, self.bscene, 1, self_sorted=True, sort_rows=True)
+        del g
+
+    def _dplus_detect(self, F, ff, fbp, x, rho, sh):
+        """detectSurface of the stage state (deltaPlus.py:242-245): lambda of
+        fluid and boundary rows, normals, surface classes."""
+        B = self.boundary()
+        L.call("sph_dsph_normalization", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B),
+               fbp, None, None, L.ptr(self.Lmat), L.ptr(self.eig), sh)
+        torch.div(self.area, rho, out=self.vf)
+        bx, beig, bvol = None, None, None
+        if self.NB:
+            torch.div(self.bvol, self.brho[: self.NB], out=self.vb[: self.NB])
+            bf = self.bf.csr().c()
+            bb = self.bb.c()
+            L.call("sph_normalization_rows", L.ptr(self.bx), L.ptr(self.bscene), self.NB,
+                   L.ptr(self.phys), C.byref(bf), L.ptr(x), L.ptr(self.vf), C.byref(bb),
+                   L.ptr(self.bx), L.ptr(self.vb), None, L.ptr(self.bL), L.ptr(self.beig), sh)
+            # deltaPlus.py:151: fluid arrays indexed by the boundary index,
+            # through the particle ids of the cell-ordered state
+            self.slot_of.index_copy_(0, self.uid, self.arange_np)
+            v = self.vf[self.slot_of[self.b_orig_c]]
+            torch.where(self.b_valid, v, torch.zeros_like(v), out=self.bvol_n)
+            bx, beig, bvol = self.bx, self.beig, self.bvol_n
+        L.call("sph_dplus_normal", C.byref(F), L.ptr(self.phys), C.byref(ff), L.ptr(self.Lmat),
+               L.ptr(self.eig), L.ptr(bx), L.ptr(beig), L.ptr(bvol), fbp, None,
+               L.ptr(self.normal), sh)
+        L.call("sph_dplus_surface", C.byref(F), L.ptr(self.phys), C.byref(ff), L.ptr(self.normal),
+               L.ptr(self.eig), L.ptr(bx), fbp, L.ptr(self.mask_tmp), None, None,
+               L.ptr(self.smask), sh)
+
+    def _dplus_shift(self, s=None):
+        """Step 14 of integrate (simulationBase.py:708-732): search and mDBC
+        at the integrated positions, then the shift with the last stage's
+        normals and classes, added to the positions."""
+        sh = L.stream_handle(s)
+        self.search(self.x, s)
+        F = self.fluid(self.x, self.v, self.rho)
+        ff = self.ff.csr().c()
+        fb = self.fb.csr().c() if self.NB else None
+        if self.NB:
+            gf = self.gf.csr().c()
+            L.call("sph_mdbc_density", C.byref(F), L.ptr(self.phys), L.ptr(self.bx),
+                   L.ptr(self.bgx), L.ptr(self.bscene), self.NB, C.byref(gf), L.ptr(self.brho),
+                   L.ptr(self.bp), L.ptr(self.bcount), sh)
+        B = self.boundary()
+        L.call("sph_dplus_shift", C.byref(F), L.ptr(self.phys), self.nscenes, L.ptr(self.packing),
+               C.byref(ff), C.byref(B), C.byref(fb) if fb is not None else None,
+               L.ptr(self.normal), L.ptr(self.smask), L.ptr(self.vmax_ws), L.ptr(self.shift), sh)
+        self.x.add_(self.shift)
+
     def _alloc_dfsph(self):
         dev, NP, NB, S = self.dev, self.NP, self.NB, self.nscenes
         z = lambda *s, dt=torch.float32: torch.zeros(*s, dtype=dt, device=dev)  # noqa: E731
@@ -328,6 +437,8 @@ class SceneBatch:
                    L.ptr(self.bp), L.ptr(self.bcount), sh)
         B = self.boundary()
         fbp = C.byref(fb) if fb is not None else None
+        if self.dplus or self.shifting:
+            self._dplus_detect(F, ff, fbp, x, rho, sh)
         if self.terms & L.T_DDSPH:
             L.call("sph_dsph_normalization", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B),
                    fbp, self._order(), None, L.ptr(self.Lmat), None, sh)
@@ -344,6 +455,15 @@ class SceneBatch:
                self.NP, L.ptr(self.phys), C.byref(ff), fbp,
                L.ptr(self.gradrho) if self.terms & L.T_DDSPH else None, self.terms, self._order(),
                L.ptr(self.dpdt), L.ptr(self.acc), L.stream_handle(s))
+        if self.dplus:
+            # computePressureAccelDeltaPlus (pressure.py:40-52,
+            # solidBoundary.py:882-887), added to the gather's acceleration
+            F = self.fluid(x, v, rho)
+            B = self.boundary()
+            L.call("sph_dplus_pressure", C.byref(F), L.ptr(self.phys), C.byref(ff),
+                   C.byref(B) if self.NB else None, fbp, L.ptr(self.smask),
+                   L.T_PRESS | (L.T_PRESS_B if self.NB else 0) | L.T_ACCUM, L.ptr(self.acc),
+                   L.stream_handle(s))
 
     def _dsph_step(self, s=None):
         self.reorder_state(s)
@@ -373,6 +493,8 @@ class SceneBatch:
                    L.ptr(self.x), L.ptr(self.v), L.ptr(self.rho), L.ptr(cur[1]), L.ptr(self.acc),
                    L.ptr(self.dpdt), L.ptr(self.K), L.ptr(out[0]), L.ptr(out[1]), L.ptr(out[2]),
                    L.ptr(self.acc) if k == 4 else None, sh)
+        if self.shifting:
+            self._dplus_shift(s)
 
     # -------------------------------------------------------------- DFSPH
     def _dfsph_pre(self, s=None):
@@ -527,17 +649,21 @@ class SceneBatch:
 
 
 class MixedBatch:
-    """delta-SPH and DFSPH scenes advanced together: each scheme is one
-    SceneBatch; the two run on their own streams so their kernels overlap
-    on the device (default stream priority for both)."""
+    """delta-SPH and DFSPH scenes advanced together: each scheme (and each
+    delta-plus setting of delta-SPH scenes) is one SceneBatch; the parts run
+    on their own streams so their kernels overlap on the device (default
+    stream priority for all)."""
 
     def __init__(self, scenes: list[Scene], device="cuda", carry: list | None = None, **kw):
         self.scenes = scenes
         self.index = {}   # scene number -> (sub-batch, position)
         self.parts = []
         carry = carry if carry is not None else [None] * len(scenes)
-        for scheme in ("dsph", "dfsph"):
-            ids = [k for k, s in enumerate(scenes) if s.scheme == scheme]
+        def kind(s):
+            return (s.scheme, bool(s.phys.get("dplus", False)), bool(s.phys.get("shifting", False)))
+
+        for key in sorted({kind(s) for s in scenes}, key=lambda k: (k[0] != "dsph", k)):
+            ids = [k for k, s in enumerate(scenes) if kind(s) == key]
             if ids:
                 b = SceneBatch([scenes[k] for k in ids], device,
                                carry=[carry[k] for k in ids], **kw)

@@ -122,6 +122,18 @@ def default_phys(scheme: str, h: float, rho0: float = 1000.0) -> dict:
                 boundary_diffusion=False)
 
 
+def dplus_variant(scene: Scene, shifting: bool = True) -> Scene:
+    """The delta-SPH scene under the delta-plus scheme: surface detection
+    and the pressure switch in every right-hand side, and with `shifting`
+    the particle shift after each step (deltaSPH.py:119-122,
+    simulationBase.py:708-732)."""
+    import dataclasses
+
+    assert scene.scheme == "dsph" and scene.boundary == "solid"
+    phys = dict(scene.phys, dplus=True, shifting=bool(shifting))
+    return dataclasses.replace(scene, name=scene.name + "_dplus", phys=phys)
+
+
 def _jitter(rng, pts, dx, amp=0.01):
     return pts + rng.uniform(-amp * dx, amp * dx, size=pts.shape)
 

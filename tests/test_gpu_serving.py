@@ -67,6 +67,29 @@ def test_alone_equals_in_mix_with_repacks(cuda):
             assert np.max(np.abs(got["v"] - ref["v"])) <= 1e-4 * scale
 
 
+def test_dplus_jobs_alone_equal_in_mix(cuda):
+    """delta-plus jobs (surface detection, pressure switch, shifting) are
+    served in their own part of the mixed batch and come out bit-identical
+    to running alone."""
+    specs = [("dambreak", 2000, 31, 6, 0.0), ("drops", 2500, 32, 5, 0.0),
+             ("dambreak", 1500, 33, 7, 0.05)]
+    def mk():
+        out = [Job(id=k, scene=S.dplus_variant(S.scene_for_count(kind, "dsph", n, sd)),
+                   steps=st, t_arrive=ta) for k, (kind, n, sd, st, ta) in enumerate(specs)]
+        out.append(Job(id=3, scene=S.scene_for_count("dambreak", "dfsph", 2000, 34), steps=4,
+                       t_arrive=0.02))
+        return out
+    jobs = mk()
+    alone = {j.id: run_alone(j) for j in mk()}
+    srv = Server(min_repack_s=0.0, waste_frac=0.2)
+    out = srv.run(jobs, clock=FakeClock())
+    assert out["jobs_done"] == len(jobs)
+    for j in jobs[:3]:
+        ref, _ = alone[j.id]
+        for k in ("x", "v", "rho"):
+            assert np.array_equal(j.result[k], ref[k]), (j.id, k)
+
+
 def test_served_results_exported_in_reference_layout(cuda, tmp_path):
     """Server(export_dir=...) writes each finished job as an HDF5 file in
     the reference's export layout holding the job's final state."""
