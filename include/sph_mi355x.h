@@ -261,6 +261,17 @@ int sph_dsph_gather_packed(const float* frec, const float* brec, const int* psce
                            const float* gradrho, unsigned terms, const int* order,
                            float* dpdt, float* acc, hipStream_t s);
 
+/* sph_dsph_gather_packed under the delta-plus scheme: the pressure terms
+ * (SPH_T_PRESS, SPH_T_PRESS_B) take p_j - p_i for a row with p_i < 0 and
+ * surface_mask_i >= 1.5 (the bulk class of sph_dplus_surface), p_i + p_j
+ * otherwise (computePressureAccelDeltaPlus, pressure.py:40-52 and
+ * solidBoundary.py:882-887); every other term as sph_dsph_gather_packed. */
+int sph_dsph_gather_packed_dplus(const float* frec, const float* brec, const int* pscene,
+                                 int np, const SphPhysics* phys, const SphCSR* ff,
+                                 const SphCSR* fb, const float* gradrho, unsigned terms,
+                                 const int* order, const float* surface_mask, float* dpdt,
+                                 float* acc, hipStream_t s);
+
 /* One RK4 stage combined into the packed gather (simulationBase.py:679-705
  * applied to the rates of deltaSPH.py:146-200). */
 typedef struct SphRk4Stage {
@@ -480,6 +491,14 @@ int sph_dplus_surface(const SphFluid* f, const SphPhysics* phys, const SphCSR* f
                       const float* normal, const float* eig, const float* bx, const SphCSR* fb,
                       float* mask_tmp, float* angle_min, float* angle_max,
                       float* surface_mask, hipStream_t s);
+
+/* sph_dplus_normal followed by sph_dplus_surface in two launches: the
+ * normal pass also runs the angle test of its row (same results). */
+int sph_dplus_detect(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                     const float* L, const float* eig, const float* bx, const float* beig,
+                     const float* bvol_n, const SphCSR* fb, float* grad, float* normal,
+                     float* mask_tmp, float* angle_min, float* angle_max, float* surface_mask,
+                     hipStream_t s);
 
 /* Particle shift (deltaPlus.py:81-115 and 211-236):
  *   s_i = -1.5 Ma 4h^2 [ sum_{j in ff(i)} k_ij A_j / (rho_i + rho_j) gradW_ij

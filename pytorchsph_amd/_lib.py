@@ -156,6 +156,8 @@ _SIGS = {
     "sph_dsph_pack": ([_P(SphFluid), c_vp, _P(SphBoundary), c_vp, c_vp, c_vp], c_int),
     "sph_dsph_gather_packed": ([c_vp, c_vp, c_vp, c_int, c_vp, _P(SphCSR), _P(SphCSR), c_vp,
                                 C.c_uint, c_vp, c_vp, c_vp, c_vp], c_int),
+    "sph_dsph_gather_packed_dplus": ([c_vp, c_vp, c_vp, c_int, c_vp, _P(SphCSR), _P(SphCSR), c_vp,
+                                      C.c_uint, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "sph_dsph_gather_rk4": ([c_vp, c_vp, c_vp, c_int, c_vp, _P(SphCSR), _P(SphCSR), c_vp,
                              C.c_uint, c_vp, _P(SphRk4Stage), c_vp], c_int),
     "sph_integrate_rk4": ([c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
@@ -170,6 +172,8 @@ _SIGS = {
                           _P(SphCSR), c_vp, c_vp, c_vp], c_int),
     "sph_dplus_surface": ([_P(SphFluid), c_vp, _P(SphCSR), c_vp, c_vp, c_vp, _P(SphCSR), c_vp,
                            c_vp, c_vp, c_vp, c_vp], c_int),
+    "sph_dplus_detect": ([_P(SphFluid), c_vp, _P(SphCSR), c_vp, c_vp, c_vp, c_vp, c_vp,
+                          _P(SphCSR), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "sph_dplus_shift": ([_P(SphFluid), c_vp, c_int, c_vp, _P(SphCSR), _P(SphBoundary), _P(SphCSR),
                          c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "sph_dplus_pressure": ([_P(SphFluid), c_vp, _P(SphCSR), _P(SphBoundary), _P(SphCSR), c_vp,

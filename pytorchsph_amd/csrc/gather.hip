@@ -306,7 +306,8 @@ __global__ __launch_bounds__(kBS) void k_dsph_gather_packed(
     const float4* __restrict__ frec, const float4* __restrict__ brec,
     const int* __restrict__ pscene, int np, const SphPhysics* __restrict__ phys, SphCSR ff,
     SphCSR fb, const float* __restrict__ gradrho, unsigned rterms, const int* __restrict__ order,
-    float* __restrict__ dpdt, float* __restrict__ acc, SphRk4Stage rk) {
+    float* __restrict__ dpdt, float* __restrict__ acc, SphRk4Stage rk,
+    const float* __restrict__ smask) {
   const unsigned terms = T ? T : rterms;
   const RowLane<KL> rl = row_lane<KL>();
   if (rl.t >= np) return;
@@ -319,6 +320,10 @@ __global__ __launch_bounds__(kBS) void k_dsph_gather_packed(
   o.x = a0.x; o.y = a0.y; o.vx = a0.z; o.vy = a0.w;
   o.rho = a1.x;
   o.p = a1.z;
+  // delta-plus switch (pressure.py:40-52): p_j - p_i for a row with negative
+  // pressure in the bulk class; the pair term adds o.p + p_j, so the row's
+  // own pressure enters with its sign flipped
+  if (smask && !(o.p >= 0.f || smask[i] < 1.5f)) o.p = -o.p;
   const float2 gri = dd ? ld2(gradrho, i) : make_float2(0.f, 0.f);
   o.gx = gri.x; o.gy = gri.y;
   Acc a = {};
@@ -717,19 +722,20 @@ template <bool FUSE>
 static void launch_gather(const float4* fr, const float4* br, const int* pscene, int np,
                           const SphPhysics* phys, const SphCSR& ff, const SphCSR& fb,
                           const float* gradrho, unsigned terms, const int* order, float* dpdt,
-                          float* acc, const SphRk4Stage& rk, hipStream_t s) {
+                          float* acc, const SphRk4Stage& rk, hipStream_t s,
+                          const float* smask = nullptr) {
   with_lanes(tuning(SPH_TUNE_GATHER_LANES), [&](auto kl) {
     constexpr int KL = decltype(kl)::value;
     const dim3 grid(blocks_for((int64_t)np * KL, kBS)), block(kBS);
     if (terms == kMaskMOG)
       hipLaunchKernelGGL((k_dsph_gather_packed<kMaskMOG, KL, FUSE>), grid, block, 0, s, fr, br,
-                         pscene, np, phys, ff, fb, gradrho, terms, order, dpdt, acc, rk);
+                         pscene, np, phys, ff, fb, gradrho, terms, order, dpdt, acc, rk, smask);
     else if (!FUSE && terms == kMaskVL)
       hipLaunchKernelGGL((k_dsph_gather_packed<kMaskVL, KL, false>), grid, block, 0, s, fr, br,
-                         pscene, np, phys, ff, fb, gradrho, terms, order, dpdt, acc, rk);
+                         pscene, np, phys, ff, fb, gradrho, terms, order, dpdt, acc, rk, smask);
     else
       hipLaunchKernelGGL((k_dsph_gather_packed<0, KL, FUSE>), grid, block, 0, s, fr, br, pscene,
-                         np, phys, ff, fb, gradrho, terms, order, dpdt, acc, rk);
+                         np, phys, ff, fb, gradrho, terms, order, dpdt, acc, rk, smask);
   });
 }
 
@@ -748,6 +754,24 @@ int dsph_gather_packed(const float* frec, const float* brec, const int* pscene, 
                        pscene, np, phys, *ff, csr_or_null(fb), gradrho, terms, order, dpdt, acc,
                        none, s);
   return launch_check("sph_dsph_gather_packed");
+}
+
+int dsph_gather_packed_dplus(const float* frec, const float* brec, const int* pscene, int np,
+                             const SphPhysics* phys, const SphCSR* ff, const SphCSR* fb,
+                             const float* gradrho, unsigned terms, const int* order,
+                             const float* surface_mask, float* dpdt, float* acc, hipStream_t s) {
+  SPH_REQUIRE(frec && pscene && phys && ff && ff->rowptr && np >= 0 && surface_mask,
+              "sph_dsph_gather_packed_dplus: bad arguments");
+  SPH_REQUIRE(!(terms & SPH_T_DDSPH) || gradrho,
+              "sph_dsph_gather_packed_dplus: DDSPH needs gradrho");
+  SPH_REQUIRE(!(terms & kBdyPairTerms) || !fb || !fb->rowptr || brec,
+              "sph_dsph_gather_packed_dplus: boundary terms without boundary records");
+  if (np == 0) return SPH_OK;
+  const SphRk4Stage none = {};
+  launch_gather<false>(reinterpret_cast<const float4*>(frec), reinterpret_cast<const float4*>(brec),
+                       pscene, np, phys, *ff, csr_or_null(fb), gradrho, terms, order, dpdt, acc,
+                       none, s, surface_mask);
+  return launch_check("sph_dsph_gather_packed_dplus");
 }
 
 int dsph_gather_rk4(const float* frec, const float* brec, const int* pscene, int np,
@@ -861,6 +885,14 @@ int sph_dsph_gather_packed(const float* frec, const float* brec, const int* psce
                            float* acc, hipStream_t s) {
   return sph::dsph_gather_packed(frec, brec, pscene, np, phys, ff, fb, gradrho, terms, order, dpdt,
                                  acc, s);
+}
+int sph_dsph_gather_packed_dplus(const float* frec, const float* brec, const int* pscene,
+                                 int np, const SphPhysics* phys, const SphCSR* ff,
+                                 const SphCSR* fb, const float* gradrho, unsigned terms,
+                                 const int* order, const float* surface_mask, float* dpdt,
+                                 float* acc, hipStream_t s) {
+  return sph::dsph_gather_packed_dplus(frec, brec, pscene, np, phys, ff, fb, gradrho, terms,
+                                       order, surface_mask, dpdt, acc, s);
 }
 int sph_dsph_gather_rk4(const float* frec, const float* brec, const int* pscene, int np,
                         const SphPhysics* phys, const SphCSR* ff, const SphCSR* fb,

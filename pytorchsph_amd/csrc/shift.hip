@@ -94,8 +94,99 @@ __global__ __launch_bounds__(kBS) void k_norm_rows(const float* __restrict__ xq,
   if (eig) st2(eig, q, make_float2(e[0], e[1]));
 }
 
-// g_i = -sum V_j (lam_j - lam_i) L_i gradW_ij; n_i = g_i / (|g_i| + eps)
-// (deltaPlus.py:132-163).
+// g_i = -sum V_j (lam_j - lam_i) L_i gradW_ij (deltaPlus.py:132-161), summed
+// by the KL lanes of row i's group; every lane of the group gets the sum.
+template <int KL>
+__device__ __forceinline__ float2 lambda_grad(const SphFluid& f, const KConst& K, const SphCSR& ff,
+                                              const float* __restrict__ Lm,
+                                              const float* __restrict__ eig,
+                                              const float* __restrict__ bx,
+                                              const float* __restrict__ beig,
+                                              const float* __restrict__ bvol_n, const SphCSR& fb,
+                                              int i, int sub, float2 xi) {
+  const float lam_i = eig[2 * i + 1];
+  const float4 L4 = *reinterpret_cast<const float4*>(Lm + 4 * i);
+  auto pair = [&](float& sx, float& sy, const XVL& r) {
+    const PairGeo pg = pair_geo(xi.x, xi.y, r.x, r.y, K.inv_h);
+    const float gw = -wendland_grad_k(pg.q, K);
+    const float gx = gw * pg.hx, gy = gw * pg.hy;
+    const float cx = L4.x * gx + L4.y * gy, cy = L4.z * gx + L4.w * gy;
+    const float w = -(r.v * (r.l - lam_i));
+    sx += w * cx;
+    sy += w * cy;
+  };
+  float fx = 0.f, fy = 0.f;
+  row_pairs<KL>(
+      ff.col, ff.rowptr[i], ff.rowptr[i + 1], sub,
+      [&](int j) {
+        const float2 xj = ld2(f.x, j);
+        return XVL{xj.x, xj.y, fluid_vol(f, j), eig[2 * j + 1]};
+      },
+      [&](int, const XVL& r) { pair(fx, fy, r); });
+  fx = group_sum<KL>(fx);
+  fy = group_sum<KL>(fy);
+  float bxs = 0.f, bys = 0.f;
+  if (fb.rowptr) {
+    for (int k = fb.rowptr[i] + sub; k < fb.rowptr[i + 1]; k += KL) {
+      const int j = fb.col[k];
+      const float2 xj = ld2(bx, j);
+      pair(bxs, bys, XVL{xj.x, xj.y, bvol_n[j], beig[2 * j + 1]});
+    }
+    bxs = group_sum<KL>(bxs);
+    bys = group_sum<KL>(bys);
+  }
+  return make_float2(fx + bxs, fy + bys);
+}
+
+// The angle test of detectFluidSurface (deltaPlus.py:165-195) for row i with
+// normal n: min / max of (-rhat_ij) . n over the non-self fluid pairs, max
+// of rhat_bi . n over the boundary pairs, an empty max or min being 0
+// (torch_scatter's fill).  Lane 0 of the group writes the mask.
+template <int KL>
+__device__ __forceinline__ void angle_test(const SphFluid& f, const KConst& K, const SphCSR& ff,
+                                           const float* __restrict__ bx, const SphCSR& fb, int i,
+                                           int sub, float2 xi, float2 n, float lam_i,
+                                           float* __restrict__ mask, float* __restrict__ amin,
+                                           float* __restrict__ amax) {
+  float dmin = INFINITY, dmax = -INFINITY;
+  row_pairs<KL>(
+      ff.col, ff.rowptr[i], ff.rowptr[i + 1], sub, [&](int j) { return ld2(f.x, j); },
+      [&](int j, const float2& xj) {
+        if (j == i) return;
+        const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, K.inv_h);
+        const float d = -g.hx * n.x + -g.hy * n.y;   // (-rhat_ij) . n_i
+        dmin = fminf(dmin, d);
+        dmax = fmaxf(dmax, d);
+      });
+  dmin = group_min<KL>(dmin);
+  dmax = group_max<KL>(dmax);
+  float bmax = -INFINITY;
+  if (fb.rowptr) {
+    for (int k = fb.rowptr[i] + sub; k < fb.rowptr[i + 1]; k += KL) {
+      const float2 xb = ld2(bx, fb.col[k]);
+      const PairGeo g = pair_geo(xi.x, xi.y, xb.x, xb.y, K.inv_h);
+      bmax = fmaxf(bmax, -g.hx * n.x + -g.hy * n.y);   // rhat_bi . n_i
+    }
+    bmax = group_max<KL>(bmax);
+  }
+  if (sub != 0) return;
+  if (dmax == -INFINITY) {   // no non-self pair
+    dmin = 0.f;
+    dmax = 0.f;
+  }
+  if (bmax == -INFINITY) bmax = 0.f;
+  const float ang = acosf(fmaxf(dmax, bmax));
+  mask[i] = (ang > kPi6 && lam_i < kLamSurface) ? 1.f : 0.f;
+  if (amin) amin[i] = acosf(dmin);
+  if (amax) amax[i] = acosf(dmax);
+}
+
+// n_i = g_i / (|g_i| + eps) (deltaPlus.py:163).
+__device__ __forceinline__ float2 normal_of(float2 g, float eps) {
+  const float inv = 1.f / (sqrtf(g.x * g.x + g.y * g.y) + eps);
+  return make_float2(g.x * inv, g.y * inv);
+}
+
 template <int KL>
 __global__ __launch_bounds__(kBS) void k_dplus_normal(SphFluid f, const SphPhysics* __restrict__ phys,
                                                       SphCSR ff, const float* __restrict__ Lm,
@@ -111,46 +202,12 @@ __global__ __launch_bounds__(kBS) void k_dplus_normal(SphFluid f, const SphPhysi
   const SphPhysics& P = phys[f.pscene[i]];
   const KConst K = kconst(P.h);
   const float2 xi = ld2(f.x, i);
-  const float lam_i = eig[2 * i + 1];
-  const float4 L4 = *reinterpret_cast<const float4*>(Lm + 4 * i);
-  auto pair = [&](float& sx, float& sy, const XVL& r) {
-    const PairGeo pg = pair_geo(xi.x, xi.y, r.x, r.y, K.inv_h);
-    const float gw = -wendland_grad_k(pg.q, K);
-    const float gx = gw * pg.hx, gy = gw * pg.hy;
-    const float cx = L4.x * gx + L4.y * gy, cy = L4.z * gx + L4.w * gy;
-    const float w = -(r.v * (r.l - lam_i));
-    sx += w * cx;
-    sy += w * cy;
-  };
-  float fx = 0.f, fy = 0.f;
-  row_pairs<KL>(
-      ff.col, ff.rowptr[i], ff.rowptr[i + 1], rl.sub,
-      [&](int j) {
-        const float2 xj = ld2(f.x, j);
-        return XVL{xj.x, xj.y, fluid_vol(f, j), eig[2 * j + 1]};
-      },
-      [&](int, const XVL& r) { pair(fx, fy, r); });
-  fx = group_sum<KL>(fx);
-  fy = group_sum<KL>(fy);
-  float bxs = 0.f, bys = 0.f;
-  if (fb.rowptr) {
-    for (int k = fb.rowptr[i] + rl.sub; k < fb.rowptr[i + 1]; k += KL) {
-      const int j = fb.col[k];
-      const float2 xj = ld2(bx, j);
-      pair(bxs, bys, XVL{xj.x, xj.y, bvol_n[j], beig[2 * j + 1]});
-    }
-    bxs = group_sum<KL>(bxs);
-    bys = group_sum<KL>(bys);
-  }
+  const float2 g = lambda_grad<KL>(f, K, ff, Lm, eig, bx, beig, bvol_n, fb, i, rl.sub, xi);
   if (rl.sub != 0) return;
-  const float gx = fx + bxs, gy = fy + bys;
-  if (grad) st2(grad, i, make_float2(gx, gy));
-  const float inv = 1.f / (sqrtf(gx * gx + gy * gy) + P.eps);
-  st2(normal, i, make_float2(gx * inv, gy * inv));
+  if (grad) st2(grad, i, g);
+  st2(normal, i, normal_of(g, P.eps));
 }
 
-// First pass of detectFluidSurface (deltaPlus.py:165-195): the angle test.
-// Maxima and minima over no pair are 0, torch_scatter's fill.
 template <int KL>
 __global__ __launch_bounds__(kBS) void k_dplus_angle(SphFluid f, const SphPhysics* __restrict__ phys,
                                                      SphCSR ff, const float* __restrict__ normal,
@@ -163,39 +220,33 @@ __global__ __launch_bounds__(kBS) void k_dplus_angle(SphFluid f, const SphPhysic
   if (rl.t >= f.np) return;
   const int i = rl.t;
   const KConst K = kconst(phys[f.pscene[i]].h);
+  angle_test<KL>(f, K, ff, bx, fb, i, rl.sub, ld2(f.x, i), ld2(normal, i), eig[2 * i + 1], mask,
+                 amin, amax);
+}
+
+// The normal and the angle test in one pass over the row: the group holds
+// n_i after the lambda-gradient sum and walks the same pairs again, their
+// positions now cached.
+template <int KL>
+__global__ __launch_bounds__(kBS) void k_dplus_normal_angle(
+    SphFluid f, const SphPhysics* __restrict__ phys, SphCSR ff, const float* __restrict__ Lm,
+    const float* __restrict__ eig, const float* __restrict__ bx, const float* __restrict__ beig,
+    const float* __restrict__ bvol_n, SphCSR fb, float* __restrict__ grad,
+    float* __restrict__ normal, float* __restrict__ mask, float* __restrict__ amin,
+    float* __restrict__ amax) {
+  const RowLane<KL> rl = row_lane<KL>();
+  if (rl.t >= f.np) return;
+  const int i = rl.t;
+  const SphPhysics& P = phys[f.pscene[i]];
+  const KConst K = kconst(P.h);
   const float2 xi = ld2(f.x, i);
-  const float2 n = ld2(normal, i);
-  float dmin = INFINITY, dmax = -INFINITY;
-  row_pairs<KL>(
-      ff.col, ff.rowptr[i], ff.rowptr[i + 1], rl.sub, [&](int j) { return ld2(f.x, j); },
-      [&](int j, const float2& xj) {
-        if (j == i) return;
-        const PairGeo g = pair_geo(xi.x, xi.y, xj.x, xj.y, K.inv_h);
-        const float d = -g.hx * n.x + -g.hy * n.y;   // (-rhat_ij) . n_i
-        dmin = fminf(dmin, d);
-        dmax = fmaxf(dmax, d);
-      });
-  dmin = group_min<KL>(dmin);
-  dmax = group_max<KL>(dmax);
-  float bmax = -INFINITY;
-  if (fb.rowptr) {
-    for (int k = fb.rowptr[i] + rl.sub; k < fb.rowptr[i + 1]; k += KL) {
-      const float2 xb = ld2(bx, fb.col[k]);
-      const PairGeo g = pair_geo(xi.x, xi.y, xb.x, xb.y, K.inv_h);
-      bmax = fmaxf(bmax, -g.hx * n.x + -g.hy * n.y);   // rhat_bi . n_i
-    }
-    bmax = group_max<KL>(bmax);
+  const float2 g = lambda_grad<KL>(f, K, ff, Lm, eig, bx, beig, bvol_n, fb, i, rl.sub, xi);
+  const float2 n = normal_of(g, P.eps);
+  if (rl.sub == 0) {
+    if (grad) st2(grad, i, g);
+    st2(normal, i, n);
   }
-  if (rl.sub != 0) return;
-  if (dmax == -INFINITY) {   // no non-self pair
-    dmin = 0.f;
-    dmax = 0.f;
-  }
-  if (bmax == -INFINITY) bmax = 0.f;
-  const float ang = acosf(fmaxf(dmax, bmax));
-  mask[i] = (ang > kPi6 && eig[2 * i + 1] < kLamSurface) ? 1.f : 0.f;
-  if (amin) amin[i] = acosf(dmin);
-  if (amax) amax[i] = acosf(dmax);
+  angle_test<KL>(f, K, ff, bx, fb, i, rl.sub, xi, n, eig[2 * i + 1], mask, amin, amax);
 }
 
 // Second pass (deltaPlus.py:197-209): one-ring spread of the mask and the
@@ -475,6 +526,32 @@ int dplus_surface(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
   return launch_check("sph_dplus_surface");
 }
 
+int dplus_detect(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, const float* L,
+                 const float* eig, const float* bx, const float* beig, const float* bvol_n,
+                 const SphCSR* fb, float* grad, float* normal, float* mask_tmp, float* amin,
+                 float* amax, float* surface_mask, hipStream_t s) {
+  SPH_REQUIRE(fluid_ok(f) && phys && csr_ok(ff) && L && eig && normal && mask_tmp &&
+                  surface_mask,
+              "sph_dplus_detect: bad arguments");
+  SPH_REQUIRE(aligned16(L), "sph_dplus_detect: L needs 16-byte alignment");
+  SPH_REQUIRE(mask_tmp != surface_mask, "sph_dplus_detect: mask_tmp must not alias the output");
+  const SphCSR none = {nullptr, nullptr, 0};
+  const bool ub = fb && fb->rowptr;
+  SPH_REQUIRE(!ub || (fb->col && bx && beig && bvol_n),
+              "sph_dplus_detect: boundary list without boundary points, lambda or volumes");
+  if (f->np == 0) return SPH_OK;
+  with_lanes(tuning(SPH_TUNE_GATHER_LANES), [&](auto kl) {
+    constexpr int KL = decltype(kl)::value;
+    const int nb = blocks_for((int64_t)f->np * KL, kBS);
+    hipLaunchKernelGGL((k_dplus_normal_angle<KL>), dim3(nb), dim3(kBS), 0, s, *f, phys, *ff, L,
+                       eig, bx, beig, bvol_n, ub ? *fb : none, grad, normal, mask_tmp, amin,
+                       amax);
+    hipLaunchKernelGGL((k_dplus_classify<KL>), dim3(nb), dim3(kBS), 0, s, f->np, *ff, mask_tmp,
+                       eig, surface_mask);
+  });
+  return launch_check("sph_dplus_detect");
+}
+
 int dplus_shift(const SphFluid* f, const SphPhysics* phys, int nscenes, const float* packing,
                 const SphCSR* ff, const SphBoundary* b, const SphCSR* fb, const float* normal,
                 const float* surface_mask, unsigned* vmax_ws, float* shift, hipStream_t s) {
@@ -540,6 +617,13 @@ int sph_dplus_surface(const SphFluid* f, const SphPhysics* phys, const SphCSR* f
                       hipStream_t s) {
   return sph::dplus_surface(f, phys, ff, normal, eig, bx, fb, mask_tmp, angle_min, angle_max,
                             surface_mask, s);
+}
+int sph_dplus_detect(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, const float* L,
+                     const float* eig, const float* bx, const float* beig, const float* bvol_n,
+                     const SphCSR* fb, float* grad, float* normal, float* mask_tmp,
+                     float* angle_min, float* angle_max, float* surface_mask, hipStream_t s) {
+  return sph::dplus_detect(f, phys, ff, L, eig, bx, beig, bvol_n, fb, grad, normal, mask_tmp,
+                           angle_min, angle_max, surface_mask, s);
 }
 int sph_dplus_shift(const SphFluid* f, const SphPhysics* phys, int nscenes, const float* packing,
                     const SphCSR* ff, const SphBoundary* b, const SphCSR* fb,

@@ -155,9 +155,7 @@ class SceneBatch:
             self.terms = DSPH_TERMS_DD if d0.get("diffusion") == "deltaSPH" else DSPH_TERMS_MOG
             if not d0.get("boundary_diffusion", True):
                 self.terms &= ~L.T_VISC_B
-            if self.dplus:
-                # the pressure terms run in sph_dplus_pressure (switched)
-                self.terms &= ~(L.T_PRESS | L.T_PRESS_B)
+
         # ---- neighbour search
         grid_scenes = [dict(vmin=s.vmin, vmax=s.vmax, hmax=s.support, p_begin=int(pb),
                             p_count=int(s.n)) for s, pb in zip(scenes, self.p_begin)]
@@ -321,12 +319,9 @@ class SceneBatch:
             v = self.vf[self.slot_of[self.b_orig_c]]
             torch.where(self.b_valid, v, torch.zeros_like(v), out=self.bvol_n)
             bx, beig, bvol = self.bx, self.beig, self.bvol_n
-        L.call("sph_dplus_normal", C.byref(F), L.ptr(self.phys), C.byref(ff), L.ptr(self.Lmat),
+        L.call("sph_dplus_detect", C.byref(F), L.ptr(self.phys), C.byref(ff), L.ptr(self.Lmat),
                L.ptr(self.eig), L.ptr(bx), L.ptr(beig), L.ptr(bvol), fbp, None,
-               L.ptr(self.normal), sh)
-        L.call("sph_dplus_surface", C.byref(F), L.ptr(self.phys), C.byref(ff), L.ptr(self.normal),
-               L.ptr(self.eig), L.ptr(bx), fbp, L.ptr(self.mask_tmp), None, None,
-               L.ptr(self.smask), sh)
+               L.ptr(self.normal), L.ptr(self.mask_tmp), None, None, L.ptr(self.smask), sh)
 
     def _dplus_shift(self, s=None):
         """Step 14 of integrate (simulationBase.py:708-732): search and mDBC
@@ -451,19 +446,18 @@ class SceneBatch:
     def dsph_rhs(self, x, v, rho, s=None):
         """dpdt / acc of the state (x, v, rho) into self.dpdt / self.acc."""
         ff, fbp = self._dsph_prepare(x, v, rho, self.frec, s)
-        L.call("sph_dsph_gather_packed", L.ptr(self.frec), L.ptr(self.brec), L.ptr(self.pscene),
-               self.NP, L.ptr(self.phys), C.byref(ff), fbp,
-               L.ptr(self.gradrho) if self.terms & L.T_DDSPH else None, self.terms, self._order(),
-               L.ptr(self.dpdt), L.ptr(self.acc), L.stream_handle(s))
+        grr = L.ptr(self.gradrho) if self.terms & L.T_DDSPH else None
         if self.dplus:
-            # computePressureAccelDeltaPlus (pressure.py:40-52,
-            # solidBoundary.py:882-887), added to the gather's acceleration
-            F = self.fluid(x, v, rho)
-            B = self.boundary()
-            L.call("sph_dplus_pressure", C.byref(F), L.ptr(self.phys), C.byref(ff),
-                   C.byref(B) if self.NB else None, fbp, L.ptr(self.smask),
-                   L.T_PRESS | (L.T_PRESS_B if self.NB else 0) | L.T_ACCUM, L.ptr(self.acc),
-                   L.stream_handle(s))
+            # the pressure terms with the delta-plus switch
+            # (computePressureAccelDeltaPlus, pressure.py:40-52)
+            L.call("sph_dsph_gather_packed_dplus", L.ptr(self.frec), L.ptr(self.brec),
+                   L.ptr(self.pscene), self.NP, L.ptr(self.phys), C.byref(ff), fbp, grr,
+                   self.terms, self._order(), L.ptr(self.smask), L.ptr(self.dpdt),
+                   L.ptr(self.acc), L.stream_handle(s))
+            return
+        L.call("sph_dsph_gather_packed", L.ptr(self.frec), L.ptr(self.brec), L.ptr(self.pscene),
+               self.NP, L.ptr(self.phys), C.byref(ff), fbp, grr, self.terms, self._order(),
+               L.ptr(self.dpdt), L.ptr(self.acc), L.stream_handle(s))
 
     def _dsph_step(self, s=None):
         self.reorder_state(s)

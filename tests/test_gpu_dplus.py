@@ -298,3 +298,68 @@ def test_engine_dplus_alone_equals_in_mix_and_graph(cuda):
         for got in (mix.scene_state(k), graph.scene_state(k)):
             for key in ("x", "v", "rho"):
                 assert np.array_equal(got[key], ref[key]), (k, key)
+
+
+def test_switched_packed_gather_equals_dplus_pressure(stage):
+    """sph_dsph_gather_packed_dplus with the pressure terms alone gives the
+    acceleration of sph_dplus_pressure (the same switch, summed in another
+    order)."""
+    import ctypes as C
+
+    from pytorchsph_amd import _lib as L
+    from pytorchsph_amd.modules._ctx import csr_c, fluid
+
+    sim, s, bm, P, f, b, lists = stage
+    sim.pressureModule.computePressure(s, sim)
+    n = s["numParticles"]
+    dev = s["fluidPosition"].device
+    F = fluid(sim, s, s.get("fluidVolume"))
+    B = bm.boundary_struct()
+    frec = torch.zeros((n, 8), dtype=torch.float32, device=dev)
+    brec = torch.zeros((bm.numPtcls, 8), dtype=torch.float32, device=dev)
+    L.call("sph_dsph_pack", C.byref(F), L.ptr(sim.phys), C.byref(B), L.ptr(frec), L.ptr(brec),
+           L.stream_handle())
+    ns = sim.neighborSearch
+    ff = csr_c(ns.csr.rowptr, ns.csr.col, n)
+    fb = bm.fb_struct()
+    a1 = torch.zeros((n, 2), dtype=torch.float32, device=dev)
+    L.call("sph_dsph_gather_packed_dplus", L.ptr(frec), L.ptr(brec), L.ptr(sim.pscene(n)), n,
+           L.ptr(sim.phys), C.byref(ff), C.byref(fb), None, L.T_PRESS | L.T_PRESS_B, None,
+           L.ptr(s["fluidSurfaceMask"]), None, L.ptr(a1), L.stream_handle())
+    a2 = torch.zeros((n, 2), dtype=torch.float32, device=dev)
+    L.call("sph_dplus_pressure", C.byref(F), L.ptr(sim.phys), C.byref(ff), C.byref(B),
+           C.byref(fb), L.ptr(s["fluidSurfaceMask"]), L.T_PRESS | L.T_PRESS_B, L.ptr(a2),
+           L.stream_handle())
+    torch.cuda.synchronize()
+    assert rel_err(a1.cpu().numpy(), a2.cpu().numpy()) < 1e-5
+
+
+def test_fused_detect_equals_normal_then_surface(stage):
+    """sph_dplus_detect (normal and angle test in one pass) gives the normal,
+    angles and classes of sph_dplus_normal + sph_dplus_surface, bit for bit."""
+    import ctypes as C
+
+    from pytorchsph_amd import _lib as L
+    from pytorchsph_amd.modules._ctx import csr_c, fluid
+
+    sim, s, bm, P, f, b, lists = stage
+    n = s["numParticles"]
+    dev = s["fluidPosition"].device
+    F = fluid(sim, s, s.get("fluidVolume"))
+    ns = sim.neighborSearch
+    ff = csr_c(ns.csr.rowptr, ns.csr.col, n)
+    fb = bm.fb_struct()
+    bvol = sim.shiftingModule._boundary_volumes_quirk(s, bm)
+    out = {k: torch.zeros(n if k != "normal" else (n, 2), dtype=torch.float32, device=dev)
+           for k in ("normal", "tmp", "amin", "amax", "surf")}
+    L.call("sph_dplus_detect", C.byref(F), L.ptr(sim.phys), C.byref(ff), L.ptr(s["fluidL"]),
+           L.ptr(s["eigVals"]), L.ptr(bm.boundaryPositions), L.ptr(bm.eigVals), L.ptr(bvol),
+           C.byref(fb), None, L.ptr(out["normal"]), L.ptr(out["tmp"]), L.ptr(out["amin"]),
+           L.ptr(out["amax"]), L.ptr(out["surf"]), L.stream_handle())
+    sm = sim.shiftingModule
+    sm.computeFluidNormal(s, sim)
+    sm.detectFluidSurface(s, sim)
+    torch.cuda.synchronize()
+    assert torch.equal(out["normal"], s["fluidNormal"])
+    assert torch.equal(out["surf"], s["fluidSurfaceMask"])
+    assert torch.equal(out["amin"], s["angleMin"]) and torch.equal(out["amax"], s["angleMax"])
