@@ -90,29 +90,43 @@ __device__ bool outside_slot(const SphGrid& g, int c) {
   return (int64_t)c == (int64_t)d.cell_begin + (int64_t)d.cells_x * d.cells_y;
 }
 
+// Insertion network over N registers (unused entries hold INT_MAX).
+template <int N>
+__device__ __forceinline__ void sort_cell_regs(int* __restrict__ idx, int a, int b) {
+  int v[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) v[u] = a + u < b ? idx[a + u] : 0x7fffffff;
+#pragma unroll
+  for (int i = 1; i < N; ++i) {
+#pragma unroll
+    for (int j = i; j > 0; --j) {
+      const int lo = min(v[j - 1], v[j]), hi = max(v[j - 1], v[j]);
+      v[j - 1] = lo;
+      v[j] = hi;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < N; ++u)
+    if (a + u < b) idx[a + u] = v[u];
+}
+
 __global__ __launch_bounds__(kBS) void k_grid_cell_sort(SphGrid g) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.ncells) return;
   const int a = g.cell_start[c];
   const int b = g.cell_start[c + 1];
+  const float2* x = reinterpret_cast<const float2*>(g.x);
+  float2* sx = reinterpret_cast<float2*>(g.sorted_x);
   int* idx = g.sorted_idx;
-  if (b - a <= kRegSort) {
-    // sorting network over a register array (unused entries hold INT_MAX)
-    int v[kRegSort];
-#pragma unroll
-    for (int u = 0; u < kRegSort; ++u) v[u] = a + u < b ? idx[a + u] : 0x7fffffff;
-#pragma unroll
-    for (int i = 1; i < kRegSort; ++i) {
-#pragma unroll
-      for (int j = i; j > 0; --j) {
-        const int lo = min(v[j - 1], v[j]), hi = max(v[j - 1], v[j]);
-        v[j - 1] = lo;
-        v[j] = hi;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kRegSort; ++u)
-      if (a + u < b) idx[a + u] = v[u];
+  // most cells of a tank hold no point or one: nothing to order
+  if (b - a <= 1) {
+    if (b > a) sx[a] = x[idx[a]];
+    return;
+  }
+  if (b - a <= 8) {
+    sort_cell_regs<8>(idx, a, b);
+  } else if (b - a <= kRegSort) {
+    sort_cell_regs<kRegSort>(idx, a, b);
   } else if (!outside_slot(g, c)) {
     for (int i = a + 1; i < b; ++i) {
       const int v = idx[i];
@@ -124,8 +138,6 @@ __global__ __launch_bounds__(kBS) void k_grid_cell_sort(SphGrid g) {
       idx[j + 1] = v;
     }
   }
-  const float2* x = reinterpret_cast<const float2*>(g.x);
-  float2* sx = reinterpret_cast<float2*>(g.sorted_x);
   for (int i = a; i < b; ++i) sx[i] = x[idx[i]];
 }
 
