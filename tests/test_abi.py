@@ -56,3 +56,36 @@ def test_scan_workspace_sizes():
     lib = L.load()
     assert lib.sph_scan_workspace_bytes(1) >= 8
     assert lib.sph_scan_workspace_bytes(10_000_000) >= 4 * (10_000_000 // 2048)
+
+
+def test_bad_arguments_rejected_on_host():
+    """Entry points validate their arguments before any launch: a null or
+    inconsistent argument returns SPH_E_ARG with the function named in the
+    message (no GPU is touched)."""
+    import ctypes as C
+
+    lib = L.load()
+    cases = {
+        "sph_neighbor_count": lambda: lib.sph_neighbor_count(None, None, None, None),
+        "sph_dsph_gather_packed": lambda: lib.sph_dsph_gather_packed(
+            None, None, None, 0, None, None, None, None, 0, None, None, None, None),
+        "sph_dsph_gather_packed_dplus": lambda: lib.sph_dsph_gather_packed_dplus(
+            None, None, None, 0, None, None, None, None, 0, None, None, None, None, None),
+        "sph_normalization_rows": lambda: lib.sph_normalization_rows(
+            None, None, 4, None, None, None, None, None, None, None, None, None, None, None),
+        "sph_dplus_normal": lambda: lib.sph_dplus_normal(None, None, None, None, None, None, None,
+                                                         None, None, None, None, None),
+        "sph_dplus_surface": lambda: lib.sph_dplus_surface(None, None, None, None, None, None,
+                                                           None, None, None, None, None, None),
+        "sph_dplus_detect": lambda: lib.sph_dplus_detect(None, None, None, None, None, None, None,
+                                                         None, None, None, None, None, None,
+                                                         None, None, None),
+        "sph_dplus_shift": lambda: lib.sph_dplus_shift(None, None, 1, None, None, None, None,
+                                                       None, None, None, None, None),
+        "sph_dplus_pressure": lambda: lib.sph_dplus_pressure(None, None, None, None, None, None,
+                                                             0, None, None),
+        "sph_permute_fields": lambda: lib.sph_permute_fields(None, None, 0, None, None),
+    }
+    for name, call in cases.items():
+        assert call() == 1, name
+        assert name.encode() in lib.sph_last_error(), (name, lib.sph_last_error())
