@@ -436,13 +436,21 @@ def serving_run(window_s: float):
     return out
 
 
+def host_threads() -> int:
+    """Threads for the CPU legs: the OpenMP thread count the host grants
+    (OMP_NUM_THREADS), at most the cores Python sees."""
+    n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(n, int(env))) if env and env.isdigit() else n
+
+
 def cpu_baseline(seconds: float):
     """The oracle's delta-SPH RK4 step (fp32 neighbour search, FP64 physics,
     OpenMP) on the 5,000-particle dam break, on this host's cores."""
     from oracle import oracle as O
     from pytorchsph_amd import workloads as W
 
-    threads = os.cpu_count() or 1
+    threads = host_threads()
     O.set_threads(threads)
     sc = W.c1_scene()
     so = O.SceneOracle(sc)
@@ -468,12 +476,18 @@ def cpu_paths():
                         20 steps after 3 warm-up)
       search_c2         the 4,000,000-point search, all threads, one search
       dfsph_c3          one DFSPH step of the 1,000,000-particle dam break,
-                        all threads (the search included)"""
+                        all threads (the search included)
+      search_c2_reference   the same 4,000,000-point search by the
+                        reference's own CPU code (buildNeighborList of
+                        cppSrc/neighSearch.cpp, compiled from its source into
+                        oracle/_ref) after its sortPositions / constructHashMap
+                        steps restated in torch; all threads"""
     from oracle import oracle as O
+    from oracle import ref_driver as R
     from pytorchsph_amd import workloads as W
 
     out = {}
-    threads = os.cpu_count() or 1
+    threads = host_threads()
     sc = W.c1_scene()
     so = O.SceneOracle(sc)
     O.set_threads(1)
@@ -497,6 +511,23 @@ def cpu_paths():
     sec = time.perf_counter() - t0
     out["search_c2"] = dict(seconds=sec, searches_per_s=1.0 / sec, pairs=int(csr.nnz),
                             threads=threads)
+    del csr
+    if R.available():
+        import torch
+
+        torch.set_num_threads(threads)
+        m = R.load()
+        t0 = time.perf_counter()
+        st = R.hash_structures(x, hq, (0.0, 0.0), (1.0, 1.0))
+        t1 = time.perf_counter()
+        rows, _ = m.buildNeighborList(st["sx"], st["sh"], st["table"], st["cell_keys"],
+                                      st["counts"], st["starts"], st["sort"], st["cells_x"],
+                                      st["P"], 1)
+        t2 = time.perf_counter()
+        out["search_c2_reference"] = dict(seconds=t2 - t0, build_list_seconds=t2 - t1,
+                                          pairs=int(rows.numel()), threads=threads,
+                                          kind="reference")
+        del rows, st
     sc = W.c3_scene()
     so = O.SceneOracle(sc)
     t0 = time.perf_counter()
