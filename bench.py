@@ -1,7 +1,7 @@
 """Benchmark: particle-steps/s on one MI355X, batched vs plain, with step
 latency percentiles, the roofline of the dominant kernel and a CPU baseline.
 
-Workload: 256 independent dam-break / colliding-drop
+Workload (the batched benchmark): 256 independent dam-break / colliding-drop
 scenes with particle counts log-uniform in [1e3, 2e5] (default_rng(3)),
 delta-SPH (RK4, mDBC walls) and DFSPH (semi-implicit, Akinci walls) 50/50,
 fp32 state, one hipGraph-captured step per time step per scheme.
@@ -542,10 +542,10 @@ def cpu_paths():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
-        log("multi-GPU runs are not part of this build: scenes shard across GPUs with no "
-            "exchange (see DESIGN.md); run with --gpus 1")
+        log("runs on several GPUs are not implemented: scenes would shard across GPUs with "
+            "no exchange; run with --gpus 1")
         sys.exit(2)
     import torch
 
