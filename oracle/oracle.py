@@ -12,6 +12,8 @@ always as the checker.  It restates the reference's algorithms:
   dfsph           modules/dfsph.py:25-311 + akinciBoundary.py:166-389
   rk4 / semi-implicit steps   simulationBase.py:564-569, 632-705
 Physics is FP64; the neighbour predicate is the reference's fp32 one.
+Parity unpinned: the near-singular branches of pinv2x2, the pinv cutoff of
+mDBC and the fp32 stopping sequence of densitySolve (see sph_oracle.c).
 """
 from __future__ import annotations
 

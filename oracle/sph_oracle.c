@@ -18,6 +18,17 @@
  *    passed as double; pair lists are CSR with int64 row pointers.
  *
  * Build: see oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
+ *
+ * Parity unpinned (pinned only by reading the reference source, which
+ * cannot be imported here):
+ *  - the near-singular branches of pinv2x2 (util.py:412-426): orc_pinv2x2
+ *    for singular values at or below 1e-5;
+ *  - the cutoff of torch.linalg.pinv in mDBC (solidBoundary.py:603),
+ *    restated in orc_pinv3x3 as rtol = 3 eps_f32 (used for > 10 neighbours,
+ *    where the system is well conditioned);
+ *  - the fp32 .item() iteration sequence of densitySolve (dfsph.py:205-231):
+ *    orc_dfsph sums the residual in FP64 and may stop one iteration earlier
+ *    or later near the threshold.
  */
 #include <math.h>
 #include <stdint.h>
