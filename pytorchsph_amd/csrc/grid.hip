@@ -362,8 +362,9 @@ int grid_build(const SphGrid* gp, void* ws, size_t ws_bytes, hipStream_t s) {
   SPH_REQUIRE(gp, "sph_grid_build: null grid");
   const SphGrid& g = *gp;
   SPH_REQUIRE(g.np >= 0 && g.ncells > 0 && g.nscenes > 0, "sph_grid_build: bad sizes");
-  SPH_REQUIRE(g.scenes && g.pscene && g.x && g.cell_count && g.cell_start && g.key && g.slot &&
-                  g.sorted_idx && g.sorted_x && g.status,
+  // point arrays may be NULL for an empty point set
+  SPH_REQUIRE(g.scenes && g.cell_count && g.cell_start && g.status &&
+                  (g.np == 0 || (g.pscene && g.x && g.key && g.slot && g.sorted_idx && g.sorted_x)),
               "sph_grid_build: null buffer");
   int rc;
   if ((rc = fill_i32(g.cell_count, g.ncells, 0, s))) return rc;
@@ -378,7 +379,7 @@ int grid_build(const SphGrid* gp, void* ws, size_t ws_bytes, hipStream_t s) {
 }
 
 static int check_query(const SphGrid* g, const SphQuery* q, const char* who) {
-  if (!g || !q || !q->x || !q->h || !q->qscene || q->nq < 0 || q->radius < 0) {
+  if (!g || !q || q->nq < 0 || q->radius < 0 || (q->nq > 0 && (!q->x || !q->h || !q->qscene))) {
     set_error(std::string(who) + ": bad query");
     return SPH_E_ARG;
   }

@@ -153,6 +153,18 @@ def test_pair_geometry_matches_reference_layout(cuda):
     assert np.all(rhat.cpu().numpy()[rows == cols] == 0.0)
 
 
+@pytest.mark.parametrize("n", [0, 1])
+def test_empty_and_single_point(cuda, n):
+    """An empty point set gives an empty list; one point pairs with itself
+    only (the reference's self pair)."""
+    x = np.full((n, 2), 0.5, np.float32)
+    hmax = np.float32(0.05)
+    rp, c, st, _ = gpu_search(x, np.full(n, hmax, np.float32), (0, 0), (1, 1), hmax)
+    assert st == 0
+    assert rp.tolist() == list(range(n + 1))
+    assert c.tolist() == list(range(n))
+
+
 def test_overflow_is_flagged_and_bounded(cuda):
     rng = np.random.default_rng(8)
     x = rng.uniform(0, 1, (4000, 2)).astype(np.float32)
