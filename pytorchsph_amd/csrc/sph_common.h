@@ -274,10 +274,12 @@ __device__ __forceinline__ float group_min(float v) {
 
 // The pairs [k0, k1) of one row walked by the KL lanes of a group: lane
 // `sub` takes pairs sub, sub + KL, ...; per trip it issues the loads of NB
-// pairs (slots past the row re-read pair k0) before it uses any, so NB
-// neighbour loads are in flight per lane.  load(j) returns the neighbour
-// record, use(j, rec) consumes it, in the lane's pair order.
-template <int KL, int NB = 4, class Load, class Use>
+// pairs before it uses any, so NB neighbour loads are in flight per lane.
+// Slots past the row load nothing: these per-lane gathers are bound by the
+// texture units' data return, which would serve re-reads as well.  load(j)
+// returns the neighbour record, use(j, rec) consumes it, in the lane's pair
+// order.
+template <int KL, int NB = 3, class Load, class Use>
 __device__ __forceinline__ void row_pairs(const int* __restrict__ col, int k0, int k1, int sub,
                                           Load load, Use use) {
   for (int kb = k0 + sub; kb < k1; kb += KL * NB) {
@@ -291,7 +293,8 @@ __device__ __forceinline__ void row_pairs(const int* __restrict__ col, int k0, i
     }
     decltype(load(0)) rec[NB];
 #pragma unroll
-    for (int u = 0; u < NB; ++u) rec[u] = load(jj[u]);
+    for (int u = 0; u < NB; ++u)
+      if (ok[u]) rec[u] = load(jj[u]);
 #pragma unroll
     for (int u = 0; u < NB; ++u)
       if (ok[u]) use(jj[u], rec[u]);
