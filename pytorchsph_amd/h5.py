@@ -206,19 +206,11 @@ class _Attrs:
         if L.H5Aexists(self._id, name) > 0:
             _check(L.H5Adelete(self._id, name), "H5Adelete")
         if isinstance(value, str):
-            t = _vstr_type()
-            sp = _space(())
-            a = _check(L.H5Acreate2(self._id, name, t, sp, H5P_DEFAULT, H5P_DEFAULT), "H5Acreate2")
             buf = C.c_char_p(value.encode())
-            _check(L.H5Awrite(a, t, C.byref(buf)), "H5Awrite")
-            L.H5Aclose(a)
-            L.H5Sclose(sp)
-            L.H5Tclose(t)
-            return
-        if isinstance(value, (bool, np.bool_)):
+            mem, own, shape, data = _vstr_type(), True, (), C.byref(buf)
+        elif isinstance(value, (bool, np.bool_)):
             arr = np.asarray(value, dtype=np.int8)
-            t = _bool_type()
-            mem, own = t, True
+            mem, own, shape, data = _bool_type(), True, (), arr.ctypes.data
         else:
             if isinstance(value, (int, np.integer)) and not isinstance(value, np.ndarray):
                 arr = np.asarray(value, dtype=np.int64 if isinstance(value, int) else value.dtype)
@@ -227,14 +219,20 @@ class _Attrs:
             else:
                 arr = np.asarray(value)
                 arr = np.ascontiguousarray(arr) if arr.ndim else arr.copy()
-            mem, own = _mem_type(arr.dtype), False
-        sp = _space(arr.shape)
-        a = _check(L.H5Acreate2(self._id, name, mem, sp, H5P_DEFAULT, H5P_DEFAULT), "H5Acreate2")
-        _check(L.H5Awrite(a, mem, arr.ctypes.data), "H5Awrite")
-        L.H5Aclose(a)
-        L.H5Sclose(sp)
-        if own:
-            L.H5Tclose(mem)
+            mem, own, shape, data = _mem_type(arr.dtype), False, arr.shape, arr.ctypes.data
+        sp = a = -1
+        try:
+            sp = _space(shape)
+            a = _check(L.H5Acreate2(self._id, name, mem, sp, H5P_DEFAULT, H5P_DEFAULT),
+                       "H5Acreate2")
+            _check(L.H5Awrite(a, mem, data), "H5Awrite")
+        finally:
+            if a >= 0:
+                L.H5Aclose(a)
+            if sp >= 0:
+                L.H5Sclose(sp)
+            if own:
+                L.H5Tclose(mem)
 
     def __getitem__(self, key: str):
         L = lib()
@@ -249,9 +247,11 @@ class _Attrs:
             if dt == "vstr":
                 out = C.c_char_p()
                 mt = _vstr_type()
-                _check(L.H5Aread(a, mt, C.byref(out)), "H5Aread")
-                L.H5Tclose(mt)
-                val = out.value.decode()
+                try:
+                    _check(L.H5Aread(a, mt, C.byref(out)), "H5Aread")
+                finally:
+                    L.H5Tclose(mt)
+                val = out.value.decode() if out.value is not None else ""
                 L.H5free_memory(C.cast(out, C.c_void_p))
                 return val
             if isinstance(dt, str):
@@ -259,8 +259,10 @@ class _Attrs:
             if dt == np.dtype(bool):
                 arr = np.zeros(shape, dtype=np.int8)
                 mt = _bool_type()
-                _check(L.H5Aread(a, mt, arr.ctypes.data), "H5Aread")
-                L.H5Tclose(mt)
+                try:
+                    _check(L.H5Aread(a, mt, arr.ctypes.data), "H5Aread")
+                finally:
+                    L.H5Tclose(mt)
                 arr = arr.astype(bool)
             else:
                 arr = np.zeros(shape, dtype=dt)
@@ -319,16 +321,19 @@ class Dataset:
         if dt == np.dtype(bool):
             arr = np.zeros(self.shape, dtype=np.int8)
             mt = _bool_type()
-            _check(L.H5Dread(self._id, mt, H5S_ALL, H5S_ALL, H5P_DEFAULT, arr.ctypes.data),
-                   "H5Dread")
-            L.H5Tclose(mt)
+            try:
+                if arr.size:
+                    _check(L.H5Dread(self._id, mt, H5S_ALL, H5S_ALL, H5P_DEFAULT,
+                                     arr.ctypes.data), "H5Dread")
+            finally:
+                L.H5Tclose(mt)
             arr = arr.astype(bool)
         else:
             arr = np.zeros(self.shape, dtype=dt)
             if arr.size:
                 _check(L.H5Dread(self._id, _mem_type(dt), H5S_ALL, H5S_ALL, H5P_DEFAULT,
                                  arr.ctypes.data), "H5Dread")
-        return arr[idx] if idx != () else arr
+        return arr if isinstance(idx, tuple) and idx == () else arr[idx]
 
     def close(self) -> None:
         if self._id >= 0:
@@ -357,16 +362,21 @@ class Group:
         else:
             arr = np.ascontiguousarray(arr)
             mem, own = _mem_type(arr.dtype), False
-        sp = _space(arr.shape)
-        d = _check(L.H5Dcreate2(self._id, name.encode(), mem, sp, H5P_DEFAULT, H5P_DEFAULT,
-                                H5P_DEFAULT), f"H5Dcreate2 {name}")
-        if arr.size:
-            _check(L.H5Dwrite(d, mem, H5S_ALL, H5S_ALL, H5P_DEFAULT, arr.ctypes.data),
-                   f"H5Dwrite {name}")
-        L.H5Sclose(sp)
-        if own:
-            L.H5Tclose(mem)
-        L.H5Dclose(d)
+        sp = d = -1
+        try:
+            sp = _space(arr.shape)
+            d = _check(L.H5Dcreate2(self._id, name.encode(), mem, sp, H5P_DEFAULT, H5P_DEFAULT,
+                                    H5P_DEFAULT), f"H5Dcreate2 {name}")
+            if arr.size:
+                _check(L.H5Dwrite(d, mem, H5S_ALL, H5S_ALL, H5P_DEFAULT, arr.ctypes.data),
+                       f"H5Dwrite {name}")
+        finally:
+            if d >= 0:
+                L.H5Dclose(d)
+            if sp >= 0:
+                L.H5Sclose(sp)
+            if own:
+                L.H5Tclose(mem)
 
     def keys(self) -> list[str]:
         L = lib()
