@@ -71,8 +71,9 @@ class SceneBatch:
         """fuse_rk4: delta-SPH stages run the RK4 combine and the next
         stage's records inside the gather (sph_dsph_gather_rk4) instead of
         as separate kernels; the results are the same, bit for bit.  Off by
-        default: the fused gather needs 70 registers instead of 63 (seven
-        wavefronts per SIMD instead of eight) and the step takes as long.
+        default: the fused gather needs 67 registers instead of 60 (seven
+        wavefronts per SIMD instead of eight) and the step takes about as
+        long.
 
         carry[k], when given, continues scene k from another batch: a dict
         from `export` with device tensors of its positions, velocities and
