@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shapes instead of autotuning: count,gather,solve[,fill] "
                          "lanes per row and the DFSPH grid cap, e.g. 2,4,2,8,16384")
+    ap.add_argument("--path", choices=PATHS, default=None,
+                    help="run one single-scene workload of the plain paths and print it")
     ap.add_argument("--dplus-only", action="store_true",
                     help="only the delta-plus surface detection and shift timings (profiling)")
     ap.add_argument("--roofline-only", action="store_true",
@@ -197,7 +199,10 @@ def _timed_steps(b, warmup: int, steps: int):
     return (time.perf_counter() - t0) / steps
 
 
-def plain_paths(steps: int = 3, warmup: int = 1):
+PATHS = ("search_c2", "dsph_4m", "dfsph_c3", "dsph_c1", "dplus_4m")
+
+
+def plain_paths(steps: int = 3, warmup: int = 1, which=None):
     """The single-scene workloads, each on its own:
       search_c2     neighbour search of 4,000,000 points (half uniform, half
                     8 Gaussian clusters, ~20 mean neighbours, default_rng(1)):
@@ -212,6 +217,7 @@ def plain_paths(steps: int = 3, warmup: int = 1):
                     (latency-bound at this size)
       dplus_4m      delta-plus surface detection, shift and pressure switch
                     on a 4,000,000-particle dam break (dplus_path)
+    `which` names the ones to run (default: all of PATHS).
     Algorithmic bytes (compulsory, each array once; K = mean neighbours,
     c = cells per point):
       search        x, h read (12) + sorted x, index written (12) + key (4)
@@ -227,95 +233,101 @@ def plain_paths(steps: int = 3, warmup: int = 1):
     from pytorchsph_amd.engine import SceneBatch
     from pytorchsph_amd.search import NeighborGrid, grid_params, scan
 
+    which = set(which or PATHS)
     out = {}
     dev = torch.device("cuda")
     peak = 8000.0
-    # ---- C2: search alone
-    x, h = W.c2_points()
-    n = len(x)
-    g = NeighborGrid([dict(vmin=(0.0, 0.0), vmax=(1.0, 1.0), hmax=np.float32(h), p_begin=0,
-                           p_count=n)], n, dev)
-    xt = torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(dev)
-    ht = torch.full((n,), float(h), dtype=torch.float32, device=dev)
-    ps = torch.zeros(n, dtype=torch.int32, device=dev)
-    counts = torch.empty(n, dtype=torch.int32, device=dev)
-    rowptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
-    g.build(xt, ps)
-    q = g.query_struct(xt, ht, ps, 1, True)
-    g.count(q, counts)
-    scan(counts, n, rowptr)
-    nnz = int(rowptr[n].item())
-    col = torch.empty(nnz, dtype=torch.int32, device=dev)
-
-    def one_search():
+    if "search_c2" in which:
+        # ---- C2: search alone
+        x, h = W.c2_points()
+        n = len(x)
+        g = NeighborGrid([dict(vmin=(0.0, 0.0), vmax=(1.0, 1.0), hmax=np.float32(h), p_begin=0,
+                               p_count=n)], n, dev)
+        xt = torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(dev)
+        ht = torch.full((n,), float(h), dtype=torch.float32, device=dev)
+        ps = torch.zeros(n, dtype=torch.int32, device=dev)
+        counts = torch.empty(n, dtype=torch.int32, device=dev)
+        rowptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
         g.build(xt, ps)
+        q = g.query_struct(xt, ht, ps, 1, True)
         g.count(q, counts)
         scan(counts, n, rowptr)
-        g.fill(q, rowptr, col)
+        nnz = int(rowptr[n].item())
+        col = torch.empty(nnz, dtype=torch.int32, device=dev)
 
-    sec = _events_avg(one_search, torch.cuda.current_stream(), reps=10, warm=2)
-    cells = g.ncells / n
-    k_mean = nnz / n
-    nbytes = n * (12 + 12 + 4 + 8 * cells + 4 + 4 * k_mean)
-    out["search_c2"] = dict(points=n, pairs=nnz, ms=sec * 1e3, searches_per_s=1 / sec,
-                            achieved_gbs=nbytes / sec / 1e9, frac=nbytes / sec / 1e9 / peak,
-                            bytes=int(nbytes))
-    del g, xt, ht, ps, counts, rowptr, col
-    torch.cuda.empty_cache()
-    log(f"search C2: {out['search_c2']}")
-    # ---- plain delta-SPH at 4M
-    sc = S.dam_break(dx=0.0005, block=(1.0, 1.0), scheme="dsph", seed=5)
-    b = SceneBatch([sc])
-    sec = _timed_steps(b, warmup, steps)
-    nnz = int(b.ff.rowptr[b.NP].item())
-    k_mean = nnz / b.NP
-    cells = b.grid.ncells / b.NP
-    rhs = 12 + 12 + 4 + 8 * cells + 4 + 4 * k_mean + 4 + 4 * k_mean + 24 + 12
-    nbytes = b.NP * 4 * (rhs + 52)
-    out["dsph_4m"] = dict(particles=b.NP, ms_per_step=sec * 1e3,
-                          particle_steps_per_s=b.NP / sec, achieved_gbs=nbytes / sec / 1e9,
-                          frac=nbytes / sec / 1e9 / peak, bytes_per_step=int(nbytes),
-                          mean_neighbours=k_mean)
-    del b
-    torch.cuda.empty_cache()
-    log(f"plain delta-SPH 4M: {out['dsph_4m']}")
-    # ---- C3: plain DFSPH at 1M
-    sc = W.c3_scene()
-    b = SceneBatch([sc])
-    sec = _timed_steps(b, warmup, steps)
-    out["dfsph_c3"] = dict(particles=b.NP, ms_per_step=sec * 1e3,
-                           particle_steps_per_s=b.NP / sec,
-                           iterations_last_step=int(b.dfsph_iterations()[0]))
-    rl = pressure_roofline(b)
-    out["dfsph_c3"]["k_pressure"] = dict(avg_launch_us=rl["seconds"] * 1e6,
-                                         bytes_per_launch=rl["bytes"],
-                                         achieved_gbs=rl["achieved_gbs"],
-                                         frac=rl["achieved_gbs"] / peak)
-    del b
-    torch.cuda.empty_cache()
-    log(f"plain DFSPH C3: {out['dfsph_c3']}")
-    # ---- C1: the 5,000-particle delta-SPH dam break, one scene on the GPU
-    sc = W.c1_scene()
-    b = SceneBatch([sc])
-    b.capture()
-    for _ in range(warmup):
-        b.step_graph()
-    torch.cuda.synchronize()
-    lat = []
-    for _ in range(50):
-        t0 = time.perf_counter()
-        b.step_graph()
+        def one_search():
+            g.build(xt, ps)
+            g.count(q, counts)
+            scan(counts, n, rowptr)
+            g.fill(q, rowptr, col)
+
+        sec = _events_avg(one_search, torch.cuda.current_stream(), reps=10, warm=2)
+        cells = g.ncells / n
+        k_mean = nnz / n
+        nbytes = n * (12 + 12 + 4 + 8 * cells + 4 + 4 * k_mean)
+        out["search_c2"] = dict(points=n, pairs=nnz, ms=sec * 1e3, searches_per_s=1 / sec,
+                                achieved_gbs=nbytes / sec / 1e9, frac=nbytes / sec / 1e9 / peak,
+                                bytes=int(nbytes))
+        del g, xt, ht, ps, counts, rowptr, col
+        torch.cuda.empty_cache()
+        log(f"search C2: {out['search_c2']}")
+    if "dsph_4m" in which:
+        # ---- plain delta-SPH at 4M
+        sc = S.dam_break(dx=0.0005, block=(1.0, 1.0), scheme="dsph", seed=5)
+        b = SceneBatch([sc])
+        sec = _timed_steps(b, warmup, steps)
+        nnz = int(b.ff.rowptr[b.NP].item())
+        k_mean = nnz / b.NP
+        cells = b.grid.ncells / b.NP
+        rhs = 12 + 12 + 4 + 8 * cells + 4 + 4 * k_mean + 4 + 4 * k_mean + 24 + 12
+        nbytes = b.NP * 4 * (rhs + 52)
+        out["dsph_4m"] = dict(particles=b.NP, ms_per_step=sec * 1e3,
+                              particle_steps_per_s=b.NP / sec, achieved_gbs=nbytes / sec / 1e9,
+                              frac=nbytes / sec / 1e9 / peak, bytes_per_step=int(nbytes),
+                              mean_neighbours=k_mean)
+        del b
+        torch.cuda.empty_cache()
+        log(f"plain delta-SPH 4M: {out['dsph_4m']}")
+    if "dfsph_c3" in which:
+        # ---- C3: plain DFSPH at 1M
+        sc = W.c3_scene()
+        b = SceneBatch([sc])
+        sec = _timed_steps(b, warmup, steps)
+        out["dfsph_c3"] = dict(particles=b.NP, ms_per_step=sec * 1e3,
+                               particle_steps_per_s=b.NP / sec,
+                               iterations_last_step=int(b.dfsph_iterations()[0]))
+        rl = pressure_roofline(b)
+        out["dfsph_c3"]["k_pressure"] = dict(avg_launch_us=rl["seconds"] * 1e6,
+                                             bytes_per_launch=rl["bytes"],
+                                             achieved_gbs=rl["achieved_gbs"],
+                                             frac=rl["achieved_gbs"] / peak)
+        del b
+        torch.cuda.empty_cache()
+        log(f"plain DFSPH C3: {out['dfsph_c3']}")
+    if "dsph_c1" in which:
+        # ---- C1: the 5,000-particle delta-SPH dam break, one scene on the GPU
+        sc = W.c1_scene()
+        b = SceneBatch([sc])
+        b.capture()
+        for _ in range(warmup):
+            b.step_graph()
         torch.cuda.synchronize()
-        lat.append(time.perf_counter() - t0)
-    lat = np.array(lat)
-    out["dsph_c1"] = dict(particles=b.NP, ms_per_step=float(lat.mean() * 1e3),
-                          particle_steps_per_s=float(b.NP / lat.mean()),
-                          p50_step_ms=float(np.percentile(lat, 50) * 1e3),
-                          p99_step_ms=float(np.percentile(lat, 99) * 1e3), steps=len(lat))
-    del b
-    log(f"plain delta-SPH C1: {out['dsph_c1']}")
-    out["dplus_4m"] = dplus_path()
-    log(f"delta-plus 4M: {out['dplus_4m']}")
+        lat = []
+        for _ in range(50):
+            t0 = time.perf_counter()
+            b.step_graph()
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t0)
+        lat = np.array(lat)
+        out["dsph_c1"] = dict(particles=b.NP, ms_per_step=float(lat.mean() * 1e3),
+                              particle_steps_per_s=float(b.NP / lat.mean()),
+                              p50_step_ms=float(np.percentile(lat, 50) * 1e3),
+                              p99_step_ms=float(np.percentile(lat, 99) * 1e3), steps=len(lat))
+        del b
+        log(f"plain delta-SPH C1: {out['dsph_c1']}")
+    if "dplus_4m" in which:
+        out["dplus_4m"] = dplus_path()
+        log(f"delta-plus 4M: {out['dplus_4m']}")
     return out
 
 
@@ -554,6 +566,10 @@ def main():
     from pytorchsph_amd.autotune import tune
     from pytorchsph_amd.engine import MixedBatch, SceneBatch
 
+    if args.path:
+        print(json.dumps({args.path: plain_paths(args.steps, args.warmup, {args.path})[args.path]}),
+              flush=True)
+        return
     if args.dplus_only:
         print(json.dumps({"dplus_4m": dplus_path()}), flush=True)
         return
