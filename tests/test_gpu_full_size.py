@@ -1,6 +1,7 @@
 """The benchmark configurations at their full sizes against the oracle:
 C1 (the 5,000-particle delta-SPH dam break, ten RK4 steps), C3 (the
-1,000,000-particle DFSPH dam break, one solved step) and the 4,000,000-
+1,000,000-particle DFSPH dam break: one solved step, and eight steps
+through the onset of its instability) and the 4,000,000-
 particle delta-SPH dam break of the plain-step measurement (one RK4 step).
 The oracle runs on the host cores (OpenMP), FP64 physics on the fp32
 search, as in the smaller parity tests."""
@@ -47,6 +48,31 @@ def test_c3_one_solved_step(cuda):
     got = b.scene_state(0)
     assert np.max(np.abs(got["x"] - x1)) < 1e-6
     assert rel_err(got["v"], v1) < 1e-3
+
+
+def test_c3_eight_steps_track_the_oracle(cuda):
+    # C3 as configured (fixed dt = 0.04 h, no divergence solve, mean
+    # residual 1e-3) turns unstable from step 5 on: max|v| grows from 0.3
+    # to 2.8 m/s by step 8 and the iteration count from 2 to 21.  The
+    # oracle does the same; engine and oracle take the same number of
+    # Jacobi iterations in every step and stay within fp32 rounding of
+    # each other through the onset.
+    sc = W.c3_scene()
+    b = SceneBatch([sc])
+    so = O.SceneOracle(sc)
+    x, v = sc.x.astype(np.float64), sc.v.astype(np.float64)
+    for k in range(8):
+        b.step()
+        x, v, _, out = so.dfsph_step(x, v)
+        torch.cuda.synchronize()
+        assert abs(int(b.dfsph_iterations()[0]) - out["iters"]) <= 1, k
+    st, _ = b.check()
+    assert st[0] & 7 == 0
+    got = b.scene_state(0)
+    assert np.max(np.abs(got["x"] - x)) < 1e-5
+    assert rel_err(got["v"], v) < 1e-2
+    sp = np.linalg.norm(v, axis=1).max()
+    assert 1.0 < sp < 10.0   # the onset, not yet the blow-up
 
 
 def test_dsph_four_million_one_step(cuda):
