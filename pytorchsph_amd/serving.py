@@ -20,8 +20,10 @@ read only its own scene's data.  Admission is by device footprint
 `budget_bytes` (default 85 % of the free device memory at start).
 
 A job's result is read back the tick it finishes; a finished job keeps
-being stepped (and ignored) until the next repack drops it.  Per-step
-latency is the wall time of the tick in which a job advanced.
+being stepped (and ignored) until the next repack drops it.  A job whose
+scene goes NaN or leaves its cell table finishes at once, failed, with
+that status (`fail_fast`).  Per-step latency is the wall time of the tick
+in which a job advanced.
 
 The particle cap (`max_particles`, default 2M) bounds the tick.  A tick
 lasts as long as its slowest scene: the DFSPH iterations of a tick are the
@@ -42,6 +44,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
+from ._lib import ST_NAN, ST_OUTSIDE
 from .engine import MixedBatch, SceneBatch
 from .export import write_job_result
 from .scenes import Scene
@@ -59,6 +62,7 @@ class Job:
     t_done: float | None = None
     result: dict | None = None     # x, v, rho in the original particle order, status
     carry: dict | None = None      # exported state between repacks
+    failed: bool = False           # stopped early: its scene went NaN or left its table
 
     @property
     def n(self) -> int:
@@ -66,15 +70,18 @@ class Job:
 
     @property
     def finished(self) -> bool:
-        return self.steps_done >= self.steps
+        return self.failed or self.steps_done >= self.steps
 
 
 class Server:
     def __init__(self, device="cuda", budget_bytes: int | None = None,
                  max_particles: int = 2_000_000, min_repack_s: float = 0.5,
                  waste_frac: float = 0.5, log=None, export_dir: str | None = None,
-                 **batch_kw):
+                 fail_fast: bool = True, **batch_kw):
         self.device = device
+        # finish a job as failed on SPH_ST_NAN / SPH_ST_OUTSIDE (one status
+        # read per tick)
+        self.fail_fast = bool(fail_fast)
         if budget_bytes is None:
             free, _ = torch.cuda.mem_get_info()
             budget_bytes = int(0.85 * free)
@@ -114,14 +121,23 @@ class Server:
             st["export"] = path
 
     def _advance(self, pos_of: dict, seconds: float, now: float):
-        """Book-keeping after one step of the whole batch."""
+        """Book-keeping after one step of the whole batch.  A job whose
+        scene raised SPH_ST_NAN (the DFSPH pressure went NaN / Inf, where the
+        reference raises, dfsph.py:179-180) or SPH_ST_OUTSIDE (a particle
+        left the scene's cell table, so the pair lists no longer hold all
+        its pairs) is finished at once with that status, failed."""
         moved = 0
+        status = {id(b): b.status() for b in self.batch.parts} if self.fail_fast else {}
         for j in self.live:
             if j.finished:
                 continue
             j.steps_done += 1
             moved += j.n
             self.latency.append(seconds)
+            if self.fail_fast:
+                b, pos = self.batch.index[pos_of[j.id]]
+                if int(status[id(b)][pos]) & (ST_NAN | ST_OUTSIDE):
+                    j.failed = True
             if j.finished:
                 self._finish(j, pos_of[j.id], now)
         self.ticks.append((seconds, moved))
@@ -239,6 +255,11 @@ class Server:
             jobs_waiting=len(queue) + len(pending),
             repacks=self.repacks,
             jobs_rejected=len(self.rejected),
+            # finished jobs whose scene raised a status bit: 1 outside the
+            # cell table, 2 pair list truncated, 4 NaN pressure, 8 iteration cap
+            jobs_flagged={str(bit): sum(1 for j in done if j.result.get("status", 0) & bit)
+                          for bit in (1, 2, 4, 8)},
+            jobs_failed=sum(1 for j in done if j.failed),
             mean_queue_s=float(np.mean([j.t_admit - j.t_arrive for j in jobs
                                         if j.t_admit is not None])) if any(
                 j.t_admit is not None for j in jobs) else None,

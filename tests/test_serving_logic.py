@@ -41,8 +41,20 @@ class CountingBatch:
     def index(self):
         return {k: (self, k) for k in range(len(self.scenes))}
 
+    # scene k's status bits from step `at` on: {scene name: (at, bits)}
+    fail_at: dict = {}
+
+    @property
+    def parts(self):
+        return [self]
+
+    def status(self):
+        return np.array([bits if s.name in self.fail_at and c >= self.fail_at[s.name][0] else 0
+                         for s, c in zip(self.scenes, self.count)
+                         for bits in [self.fail_at.get(s.name, (0, 0))[1]]], dtype=np.int64)
+
     def scene_status(self, k):
-        return 0
+        return int(self.status()[k])
 
     def export(self, k):
         return dataclasses.replace(self.scenes[k]), dict(steps=self.count[k], status=0)
@@ -60,6 +72,7 @@ class Clock:
 @pytest.fixture
 def fake(monkeypatch):
     CountingBatch.built = []
+    CountingBatch.fail_at = {}
     monkeypatch.setattr(serving, "MixedBatch", CountingBatch)
     monkeypatch.setattr(serving.torch.cuda, "synchronize", lambda *a: None)
     return CountingBatch
@@ -127,3 +140,28 @@ def test_footprint_on_meta_device():
         large = SceneBatch.footprint([S.scene_for_count("dambreak", scheme, 8000, 1)])
         assert 0 < small < large
         assert 200 * 8000 < large < 4000 * 8000
+
+
+def test_job_whose_scene_fails_is_finished_failed(fake):
+    # scene "bad" raises NaN (bit 4) at its third step: it stops there, as
+    # the reference raises, and the other jobs run their full counts
+    jobs = jobs_of([(10, 20, 0.0), (12, 20, 0.0), (8, 5, 0.0)])
+    jobs[1].scene = dataclasses.replace(jobs[1].scene, name="bad")
+    fake.fail_at = {"bad": (3, 4)}
+    srv = serving.Server(budget_bytes=10**9, min_repack_s=0.0, waste_frac=0.9)
+    out = srv.run(jobs, clock=Clock())
+    assert jobs[1].failed and jobs[1].steps_done == 3
+    assert jobs[1].result["status"] & 4
+    assert not jobs[0].failed and jobs[0].result["steps"] == 20
+    assert jobs[2].result["steps"] == 5
+    assert out["jobs_failed"] == 1 and out["jobs_flagged"]["4"] == 1
+
+
+def test_fail_fast_off_runs_failed_scenes_to_the_end(fake):
+    jobs = jobs_of([(10, 6, 0.0)])
+    jobs[0].scene = dataclasses.replace(jobs[0].scene, name="bad")
+    fake.fail_at = {"bad": (2, 1)}
+    srv = serving.Server(budget_bytes=10**9, min_repack_s=0.0, fail_fast=False)
+    out = srv.run(jobs, clock=Clock())
+    assert not jobs[0].failed and jobs[0].steps_done == 6
+    assert out["jobs_failed"] == 0 and out["jobs_flagged"]["1"] == 1
