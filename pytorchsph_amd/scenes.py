@@ -116,8 +116,12 @@ def default_phys(scheme: str, h: float, rho0: float = 1000.0) -> dict:
                     tau=1e-3, fluid_omega=0.3, min_iters=2, max_iters=256, eos=0,
                     diffusion="MOG", boundary_diffusion=True)
     c0 = 100.0
+    # configs/dambreak_dfsph.toml:1-4,25: fixed dt = 0.0004 s at radius 0.01,
+    # support 0.01 sqrt(20); the same dt / h at every resolution.  (At four
+    # times this step the 1M-particle dam break diverges within 16 steps.)
+    dt = 0.0004 * h / (0.01 * math.sqrt(20.0))
     return dict(h=h, rho0=rho0, c0=c0, gamma=7.0, kappa=1.3, alpha=0.01, delta=0.1, nu=0.01,
-                eps=0.1 * h * h, gx=0.0, gy=-G, dt=0.04 * h, beta=0.125, omega=0.5, tau=1e-3,
+                eps=0.1 * h * h, gx=0.0, gy=-G, dt=dt, beta=0.125, omega=0.5, tau=1e-3,
                 fluid_omega=0.3, min_iters=2, max_iters=256, eos=0, diffusion="none",
                 boundary_diffusion=False)
 

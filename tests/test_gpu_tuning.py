@@ -65,6 +65,8 @@ def test_dsph_rhs_lanes(cuda, kl):
 @pytest.mark.parametrize("kl", [1, 4, 16, 32])
 def test_dfsph_lanes(cuda, kl):
     sc = S.dam_break(dx=0.01, block=(1.0, 0.6), scheme="dfsph", seed=4)
+    # a step of 0.04 h, large enough that the solve runs all five iterations
+    sc.phys["dt"] = 0.04 * float(sc.support)
     rng = np.random.default_rng(4)
     sc.v = rng.normal(0, 0.5, sc.v.shape).astype(np.float32)
     with tuned(solve=kl, gather=kl):
