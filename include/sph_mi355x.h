@@ -77,7 +77,8 @@ extern "C" {
 #define SPH_E_SPACE 3    /* a caller buffer is smaller than required        */
 
 /* per-scene device status bits */
-#define SPH_ST_OUTSIDE   1u  /* a point's cell key fell outside the cell table */
+#define SPH_ST_OUTSIDE   1u  /* a point lies outside the cell table: its key is out
+                                of range or a coordinate is past a side edge */
 #define SPH_ST_OVERFLOW  2u  /* CSR capacity exceeded: pair list truncated     */
 #define SPH_ST_NAN       4u  /* DFSPH pressure became NaN / Inf                */
 #define SPH_ST_ITERCAP   8u  /* DFSPH solve stopped at the iteration cap       */
@@ -369,6 +370,14 @@ typedef struct SphDfsph {
   int* act_n;       /* [3] active fluid items, boundary items, scenes  */
   int nscenes, max_hist;
   int grid_blocks;  /* grid of the iteration kernels (work is strided) */
+  int divergence;   /* 0: densitySolve (dfsph.py:195-234); 1: divergenceSolve
+                       (dfsph.py:236-260): source = divergence only (no
+                       1 - rho term), pressures not clamped, the residual
+                       mean over fluid rows only; boundary arguments must be
+                       NULL (the reference's Akinci hooks return zeros when
+                       density = False, akinciBoundary.py:167,190,255,319,356).
+                       phys carries the divergence threshold and iteration
+                       bounds for that call                                */
 } SphDfsph;
 
 #define SPH_DFSPH_ERR_SLOTS 64

@@ -279,7 +279,16 @@ def eos(phys, rho):
     return p
 
 
-def dfsph(f, phys, ff: CSR, b, fb: CSR, bf: CSR, acc, fixed_iters=0, max_hist=512):
+def divergence_phys(phys: dict) -> dict:
+    """The limits divergenceSolve runs with (dfsph.py:240): divergence
+    threshold and iteration bounds in place of the density solve's."""
+    return dict(phys, tau=phys.get("div_tau", 1e-2), min_iters=phys.get("div_min_iters", 2),
+                max_iters=phys.get("div_max_iters", 8))
+
+
+def dfsph(f, phys, ff: CSR, b, fb: CSR, bf: CSR, acc, fixed_iters=0, max_hist=512, density=True):
+    """densitySolve (density=True) or divergenceSolve (density=False, run
+    with divergence_phys(phys)) of modules/dfsph.py:195-311."""
     keep = _Keep()
     F, B = fluid_struct(f, keep), bdy_struct(b, keep)
     n, nb = F.n, (B.n if B is not None else 0)
@@ -291,9 +300,10 @@ def dfsph(f, phys, ff: CSR, b, fb: CSR, bf: CSR, acc, fixed_iters=0, max_hist=51
                                             "b_src", "b_p", "err_hist")], max_hist, 0)
     acc = keep(_f64(acc))
     P = phys_struct(phys)
-    lib().orc_dfsph(C.byref(F), C.byref(P), C.byref(ff.c()), _byref(B),
-                    _byref(fb.c() if fb is not None else None),
-                    _byref(bf.c() if bf is not None else None), acc, int(fixed_iters), C.byref(O))
+    lib().orc_dfsph_solve(C.byref(F), C.byref(P), C.byref(ff.c()), _byref(B),
+                          _byref(fb.c() if fb is not None else None),
+                          _byref(bf.c() if bf is not None else None), acc, int(fixed_iters),
+                          1 if density else 0, C.byref(O))
     out["iters"] = O.iters
     out["err_hist"] = out["err_hist"][:min(O.iters, max_hist)]
     for k in ("b_alpha", "b_src", "b_p"):
@@ -525,7 +535,15 @@ class SceneOracle:
         f["rho"] = rho
         b["rho"] = brho
         acc = np.tile([P["gx"], P["gy"]], (len(x0), 1)).astype(np.float64)
+        div = None
+        if P.get("divergence", False):
+            # dfsph.py:175-179: the divergence solve first, its pressure
+            # acceleration added to the acceleration the density solve
+            # predicts velocities with
+            div = dfsph(f, divergence_phys(P), ff, b, fb, bf, acc, density=False)
+            acc = acc + div["apred"]
         out = dfsph(f, P, ff, b, fb, bf, acc, fixed_iters=fixed_iters)
+        out["divergence"] = div
         acc = acc + out["apred"]
         fv = dict(f, vol=vol)
         _, visc, _ = dsph_rhs(fv, P, ff, None, None, T_VISC | T_LAM)

@@ -728,14 +728,28 @@ static void dfsph_accel(const OrcFluid* F, const OrcPhys* P, const OrcCSR* ff, c
 }
 
 /*
- * DFSPH density solve (modules/dfsph.py:263-311) with Akinci PBSPH boundary
+ * DFSPH solve (modules/dfsph.py:263-311) with Akinci PBSPH boundary
  * (akinciBoundary.py:166-389), FP64.  acc is the acceleration before the
  * solve (gravity); on return out->apred is the final pressure
  * acceleration.  fixed_iters > 0 runs exactly that many iterations.
+ *  density = 1: densitySolve (dfsph.py:195-234): source 1 - rho + divergence
+ *    (:161), pressure clamped at 0 (:177), error over fluid and boundary
+ *    (:219-222);
+ *  density = 0: divergenceSolve (dfsph.py:236-260): every Akinci hook
+ *    returns zeros (akinciBoundary.py:167,190,255,319,356), so the solve is
+ *    fluid-only; source = divergence only, no clamp, error = mean of the
+ *    clamped fluid residuals (:253).
+ * P->tau, P->min_iters and P->max_iters are the limits of the solve run
+ * (densityThreshold / divergenceThreshold and their iteration bounds).
  */
-void orc_dfsph(const OrcFluid* F, const OrcPhys* P, const OrcCSR* ff, const OrcBdy* B,
-               const OrcCSR* fb, const OrcCSR* bf, const double* acc, int fixed_iters,
-               OrcDfsphOut* out) {
+void orc_dfsph_solve(const OrcFluid* F, const OrcPhys* P, const OrcCSR* ff, const OrcBdy* B,
+                     const OrcCSR* fb, const OrcCSR* bf, const double* acc, int fixed_iters,
+                     int density, OrcDfsphOut* out) {
+  if (!density) {
+    B = NULL;
+    fb = NULL;
+    bf = NULL;
+  }
   const int n = F->n, nb = B ? B->n : 0;
   const double dt = P->dt;
   double* aact = (double*)malloc(sizeof(double) * (size_t)(n + 1));
@@ -786,7 +800,7 @@ void orc_dfsph(const OrcFluid* F, const OrcPhys* P, const OrcCSR* ff, const OrcB
     if (al < -1) al = -1;
     if (al > -1e-7) al = -1e-7;
     out->alpha[i] = al;
-    out->src[i] = 1.0 - F->rho[i] + src;
+    out->src[i] = density ? 1.0 - F->rho[i] + src : src;
   }
 #pragma omp parallel for schedule(static)
   for (int b = 0; b < nb; ++b) {
@@ -833,7 +847,7 @@ void orc_dfsph(const OrcFluid* F, const OrcPhys* P, const OrcCSR* ff, const OrcB
       }
       double r = ks - out->src[i];
       double pn = p[i] - P->fluid_omega * r / out->alpha[i];
-      if (pn < 0) pn = 0;
+      if (density && pn < 0) pn = 0;
       p2[i] = pn;
       esum += r < -P->tau ? -P->tau : r;
     }
@@ -867,6 +881,13 @@ void orc_dfsph(const OrcFluid* F, const OrcPhys* P, const OrcCSR* ff, const OrcB
   free(aact);
   free(baact);
   free(p);
+}
+
+/* densitySolve (density = 1) of orc_dfsph_solve. */
+void orc_dfsph(const OrcFluid* F, const OrcPhys* P, const OrcCSR* ff, const OrcBdy* B,
+               const OrcCSR* fb, const OrcCSR* bf, const double* acc, int fixed_iters,
+               OrcDfsphOut* out) {
+  orc_dfsph_solve(F, P, ff, B, fb, bf, acc, fixed_iters, 1, out);
 }
 
 /* ====================================================== delta-plus shifting */

@@ -76,7 +76,7 @@ DFSPH_FIELDS = ("vpred", "aact", "alpha", "src", "p", "p2", "apred", "resid", "x
 
 class SphDfsph(C.Structure):
     _fields_ = [(n, c_vp) for n in DFSPH_FIELDS] + [
-        ("nscenes", c_int), ("max_hist", c_int), ("grid_blocks", c_int)]
+        ("nscenes", c_int), ("max_hist", c_int), ("grid_blocks", c_int), ("divergence", c_int)]
 
 
 # device-array record types (numpy layouts identical to the C structs)

@@ -154,7 +154,7 @@ __device__ void build_active_lists(SphDfsph& d) {
     const int s = base + threadIdx.x;
     const int act = (s < d.nscenes) ? d.active[s] : 0;
     const int nf = act ? d.nfluid[s] : 0;
-    const int nb = act ? d.nbdy[s] : 0;
+    const int nb = (act && !d.divergence) ? d.nbdy[s] : 0;
     int tot;
     const int rank = block_exclusive_scan(act, lds, &tot) + carry_s;
     const int tot_s = tot;
@@ -233,7 +233,8 @@ __global__ __launch_bounds__(kBS) void k_alpha_source(SphFluid f, const SphPhysi
     float al = fac / mass * (k1x * k1x + k1y * k1y) + fac * k2;
     al = al < -1.f ? -1.f : (al > -1e-7f ? -1e-7f : al);  // dfsph.py:82
     d.alpha[i] = al;
-    d.src[i] = 1.f - f.rho[i] + src;
+    // divergenceSolve: the source is the divergence term only (dfsph.py:161)
+    d.src[i] = d.divergence ? src : 1.f - f.rho[i] + src;
     // W_i = dt^2 k1_i: the part of the pressure-update sum that does not
     // change during the solve
     st2(d.wvec, i, make_float2(P.dt * P.dt * k1x, P.dt * P.dt * k1y));
@@ -427,7 +428,9 @@ __device__ __forceinline__ double pressure_fluid_row(const SphFluid& f, const Sp
   const float pold = d.p2[i];
   d.p[i] = pold;
   float pn = pold - omega * rr / d.alpha[i];
-  pn = pn < 0.f ? 0.f : pn;  // clamp(min = 0) keeps NaN
+  // densitySolve clamps at 0 (clamp(min = 0) keeps NaN); divergenceSolve
+  // does not (dfsph.py:177)
+  if (!d.divergence) pn = pn < 0.f ? 0.f : pn;
   if (isnan(pn) || isinf(pn)) atomicOr(&d.status[s], SPH_ST_NAN);
   d.p2[i] = pn;
   d.resid[i] = rr;
@@ -532,7 +535,9 @@ __global__ __launch_bounds__(kBS) void k_control(const SphPhysics* __restrict__ 
   for (int s = threadIdx.x; s < d.nscenes; s += blockDim.x) {
     if (!d.active[s]) continue;
     const SphPhysics P = phys[s];
-    const double n = (double)(d.nfluid[s] + d.nbdy[s]);
+    // densitySolve with the PBSPH boundary averages over fluid and boundary
+    // rows (dfsph.py:219-222); divergenceSolve over fluid rows (:253)
+    const double n = (double)(d.nfluid[s] + (d.divergence ? 0 : d.nbdy[s]));
     double sum = 0.0;
     const size_t stride = SPH_DFSPH_ERR_STRIDE(d.nscenes);
 #pragma unroll 8
@@ -548,7 +553,8 @@ __global__ __launch_bounds__(kBS) void k_control(const SphPhysics* __restrict__ 
     const bool go = (i < P.min_iters || err > (double)P.tau) && i <= P.max_iters;
     if (!go) {
       d.active[s] = 0;
-      if (err > (double)P.tau) atomicOr(&d.status[s], SPH_ST_ITERCAP);
+      // the divergence solve stopping at its small cap is routine
+      if (err > (double)P.tau && !d.divergence) atomicOr(&d.status[s], SPH_ST_ITERCAP);
     }
   }
   __syncthreads();
@@ -607,6 +613,8 @@ int dfsph_prepare(const SphFluid* f, const float* acc, const SphPhysics* phys, c
               "sph_dfsph_prepare: bad arguments");
   const SphBoundary bb = bdy_or_null(b);
   SPH_REQUIRE(bdy_ok(bb, d), "sph_dfsph_prepare: bad boundary");
+  SPH_REQUIRE(!d->divergence || bb.nb == 0,
+              "sph_dfsph_prepare: the divergence solve takes no boundary");
   int n = f->np + bb.nb;
   const int nacc = SPH_DFSPH_ERR_SLOTS * SPH_DFSPH_ERR_STRIDE(d->nscenes);
   if (n < nacc) n = nacc;
@@ -623,6 +631,8 @@ int dfsph_alpha_source(const SphFluid* f, const SphPhysics* phys, const SphCSR* 
   const SphBoundary bb = bdy_or_null(b);
   SPH_REQUIRE(bdy_ok(bb, d) && (bb.nb == 0 || (bf && bf->rowptr)),
               "sph_dfsph_alpha_source: boundary needs its pair list");
+  SPH_REQUIRE(!d->divergence || (bb.nb == 0 && (!fb || !fb->rowptr)),
+              "sph_dfsph_alpha_source: the divergence solve takes no boundary");
   const int n = f->np + bb.nb;
   if (n == 0) return SPH_OK;
   with_lanes(tuning(SPH_TUNE_SOLVE_LANES), [&](auto kl) {
@@ -637,6 +647,8 @@ int dfsph_accel(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, con
                 SphDfsph* d, hipStream_t s) {
   SPH_REQUIRE(f && phys && ff && ff->rowptr && dfsph_ok(d), "sph_dfsph_accel: bad arguments");
   SPH_REQUIRE(!fb || !fb->rowptr || d->bxrec, "sph_dfsph_accel: boundary list without records");
+  SPH_REQUIRE(!d->divergence || !fb || !fb->rowptr,
+              "sph_dfsph_accel: the divergence solve takes no boundary");
   with_lanes(tuning(SPH_TUNE_SOLVE_LANES), [&](auto kl) {
     constexpr int KL = decltype(kl)::value;
     hipLaunchKernelGGL((k_accel<KL>), dim3(d->grid_blocks), dim3(kBS), 0, s, *f, phys, *ff,
@@ -653,6 +665,8 @@ int dfsph_pressure(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
   const SphBoundary bb = bdy_or_null(b);
   SPH_REQUIRE(bdy_ok(bb, d) && (bb.nb == 0 || (bf && bf->rowptr)),
               "sph_dfsph_pressure: bad boundary");
+  SPH_REQUIRE(!d->divergence || (bb.nb == 0 && (!fb || !fb->rowptr)),
+              "sph_dfsph_pressure: the divergence solve takes no boundary");
   with_lanes(tuning(SPH_TUNE_SOLVE_LANES), [&](auto kl) {
     constexpr int KL = decltype(kl)::value;
     hipLaunchKernelGGL((k_pressure<KL>), dim3(d->grid_blocks), dim3(kBS), 0, s, *f, phys, *ff,
@@ -679,6 +693,8 @@ int dfsph_finish(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, co
                  SphDfsph* d, float* acc, hipStream_t s) {
   SPH_REQUIRE(f && f->pscene && phys && ff && ff->rowptr && dfsph_ok(d),
               "sph_dfsph_finish: bad arguments");
+  SPH_REQUIRE(!d->divergence || !fb || !fb->rowptr,
+              "sph_dfsph_finish: the divergence solve takes no boundary");
   if (f->np == 0) return SPH_OK;
   with_lanes(tuning(SPH_TUNE_SOLVE_LANES), [&](auto kl) {
     constexpr int KL = decltype(kl)::value;

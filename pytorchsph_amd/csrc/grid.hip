@@ -42,6 +42,12 @@ __global__ __launch_bounds__(kBS) void k_grid_keys(SphGrid g) {
     const bool finite = isfinite(x) && isfinite(y);
     if (finite && key >= 0 && key < ncell) {
       k = (int)key;
+      // a point past a side edge keeps the reference's aliased key (the
+      // first cells of the next row, or the last of the previous one) so
+      // that pairs stay those of the reference, and is flagged: its own
+      // stencil no longer covers its true neighbourhood
+      if (ix < 0 || ix >= d.cells_x || iy < 0 || iy >= d.cells_y)
+        atomicOr(&g.status[s], SPH_ST_OUTSIDE);
     } else {
       // the scene's last slot collects points outside its table; stencils
       // never reach it, and the scene stays contiguous in sorted order

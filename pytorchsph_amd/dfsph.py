@@ -36,8 +36,6 @@ class dfsphSimulation(SPHSimulation):
             if params is not None:
                 for p in params:
                     p.parseConfig(self.config)
-        if self.config["dfsph"]["divergenceSolver"]:
-            raise NotImplementedError("the divergence solve is off in every shipped config")
 
     def timestep(self):
         """dfsph.py:134-202."""
@@ -49,6 +47,9 @@ class dfsphSimulation(SPHSimulation):
         self.boundaryModule.evalBoundaryDensity(s, self)
         s["fluidAcceleration"] = torch.zeros_like(s["fluidVelocity"])
         self.gravityModule.evaluate(s, self)
+        if self.config["dfsph"]["divergenceSolver"]:
+            s["divergenceIterations"] = self.DFSPH.divergenceSolver(s, self)
+            s["fluidAcceleration"] += s["fluidPredAccel"]
         s["densityIterations"] = self.DFSPH.incompressibleSolver(s, self)
         s["fluidAcceleration"] += s["fluidPredAccel"]
         self.velocityDiffusionModule.evaluate(s, self)

@@ -34,6 +34,20 @@ from .search import CSR, NeighborGrid, scan_ws_bytes
 DSPH_TERMS_MOG = (L.T_CONT | L.T_CONT_B | L.T_MOG | L.T_PRESS | L.T_PRESS_B | L.T_VISC |
                   L.T_VISC_B | L.T_LAM | L.T_GRAV)
 DSPH_TERMS_DD = (DSPH_TERMS_MOG & ~L.T_MOG) | L.T_DDSPH
+DIV_MAX_HIST = 16
+
+
+def eos_host(phys: dict, rho) -> np.ndarray:
+    """Equation of state on the host (pressure.py:99-111): Tait (eos 0),
+    linear delta-SPH (1) or compressible (2) of the normalised density."""
+    r = np.asarray(rho, dtype=np.float64)
+    rho0, c0 = phys["rho0"], phys["c0"]
+    e = int(phys.get("eos", 0))
+    if e == 0:
+        return rho0 * c0 * c0 / phys["gamma"] * (r ** phys["gamma"] - 1.0)
+    if e == 1:
+        return c0 * c0 * (r * rho0 - rho0)
+    return phys["kappa"] * (r * rho0 - rho0)
 
 
 def phys_records(scenes) -> np.ndarray:
@@ -152,6 +166,11 @@ class SceneBatch:
         assert all(bool(s.phys.get("dplus", False)) == self.dplus and
                    bool(s.phys.get("shifting", False)) == self.shifting for s in scenes), \
             "one delta-plus setting per batch"
+        # DFSPH divergence solve before the density solve (divergenceSolver,
+        # dfsph.py:175-179); one setting per batch
+        self.divergence = scheme == "dfsph" and bool(d0.get("divergence", False))
+        assert all(bool(s.phys.get("divergence", False)) == self.divergence
+                   for s in scenes if s.scheme == "dfsph"), "one divergence setting per batch"
         if scheme == "dsph":
             self.terms = DSPH_TERMS_DD if d0.get("diffusion") == "deltaSPH" else DSPH_TERMS_MOG
             if not d0.get("boundary_diffusion", True):
@@ -194,10 +213,12 @@ class SceneBatch:
             self._apply_carry()
 
     def _apply_carry(self):
-        """Density, particle ids and pending cell order of carried scenes;
-        fresh scenes get the identity order, so the first reorder leaves
-        them as they are."""
+        """Density, particle ids, pending cell order and status bits of
+        carried scenes; fresh scenes get the identity order, so the first
+        reorder leaves them as they are.  The status bits a scene raised in
+        earlier batches stay raised (device status words are sticky)."""
         order = torch.arange(self.NP, dtype=torch.int32, device=self.dev)
+        grid_bits = L.ST_OUTSIDE | L.ST_OVERFLOW
         for k, c in enumerate(self.carry):
             if c is None:
                 continue
@@ -205,6 +226,11 @@ class SceneBatch:
             self.rho[a:a + n].copy_(c["rho"])
             self.uid[a:a + n].copy_(c["uid"] + a)
             order[a:a + n].copy_(c["order"] + a)
+            st = int(c.get("status", 0))
+            if st & grid_bits:
+                self.grid.status[k] |= st & grid_bits
+            if self.scheme == "dfsph" and st & ~grid_bits:
+                self.df["status"][k] |= st & ~grid_bits
         self.grid.sorted_idx[: self.NP].copy_(order)
         self._sorted_valid = True
 
@@ -362,10 +388,29 @@ class SceneBatch:
             act_pre=z(2 * (S + 1), dt=torch.int32), act_n=z(3, dt=torch.int32))
         self.any_active_host = torch.zeros(1, dtype=torch.int32,
                                            pin_memory=self.dev.type == "cuda")
+        if self.divergence:
+            # the divergence solve's own iteration counters and residual
+            # history, and the limits it runs with (divergenceThreshold,
+            # min/maxDivergenceSolverIterations, dfsph.py:240)
+            self.div_iters = z(S, dt=torch.int32)
+            self.div_err_hist = z(S, DIV_MAX_HIST, dt=torch.float64)
+            ph = self.phys_host.copy()
+            for k, sc in enumerate(self.scenes):
+                ph["tau"][k] = sc.phys.get("div_tau", 1e-2)
+                ph["min_iters"][k] = sc.phys.get("div_min_iters", 2)
+                ph["max_iters"][k] = sc.phys.get("div_max_iters", 8)
+            self.phys_div = L.records_to_tensor(ph, dev)
+            self.div_chunk = int(max(ph["max_iters"])) + 1
 
-    def _dfsph_struct(self) -> L.SphDfsph:
-        return L.SphDfsph(*[L.ptr(self.df[k]) for k in L.DFSPH_FIELDS], self.nscenes,
-                          self.max_hist, L.dfsph_grid_blocks(self.NP + self.NB))
+    def _dfsph_struct(self, divergence: bool = False) -> L.SphDfsph:
+        ptrs = {k: L.ptr(self.df[k]) for k in L.DFSPH_FIELDS}
+        hist = self.max_hist
+        if divergence:
+            ptrs["iters"] = L.ptr(self.div_iters)
+            ptrs["err_hist"] = L.ptr(self.div_err_hist)
+            hist = DIV_MAX_HIST
+        return L.SphDfsph(*[ptrs[k] for k in L.DFSPH_FIELDS], self.nscenes, hist,
+                          L.dfsph_grid_blocks(self.NP + self.NB), 1 if divergence else 0)
 
     def fluid(self, x, v, rho, vol=None) -> L.SphFluid:
         return L.SphFluid(L.ptr(x), L.ptr(v), L.ptr(rho), L.ptr(self.area), L.ptr(self.rest),
@@ -509,12 +554,31 @@ class SceneBatch:
                    L.ptr(self.bscene), self.NB, L.ptr(self.bterm), C.byref(bf), L.ptr(self.brho), sh)
         L.call("sph_dsph_gather", C.byref(F), L.ptr(self.phys), C.byref(ff), None, None, None,
                L.T_GRAV, None, None, L.ptr(self.acc), sh)
+        if self.divergence:
+            self._divergence_solve(F, ff, sh)
         D = self._dfsph_struct()
         L.call("sph_dfsph_prepare", C.byref(F), L.ptr(self.acc), L.ptr(self.phys), C.byref(B),
                C.byref(D), sh)
         bfc = self.bf.csr().c()
         L.call("sph_dfsph_alpha_source", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B),
                C.byref(fb) if fb is not None else None, C.byref(bfc), C.byref(D), sh)
+
+    def _divergence_solve(self, F, ff, sh):
+        """divergenceSolver (dfsph.py:175-179, 236-260): fluid-only solve on
+        v* = v + dt a, then acc += its pressure acceleration.  At most
+        maxDivergenceSolverIterations + 1 iterations, so one fixed chunk
+        covers it and no host poll is needed; scenes that stop early are
+        idle for the rest of the chunk."""
+        D = self._dfsph_struct(divergence=True)
+        pd = L.ptr(self.phys_div)
+        L.call("sph_dfsph_prepare", C.byref(F), L.ptr(self.acc), pd, None, C.byref(D), sh)
+        L.call("sph_dfsph_alpha_source", C.byref(F), pd, C.byref(ff), None, None, None,
+               C.byref(D), sh)
+        for _ in range(self.div_chunk):
+            L.call("sph_dfsph_iterate", C.byref(F), pd, C.byref(ff), None, None, None,
+                   C.byref(D), sh)
+        L.call("sph_dfsph_finish", C.byref(F), pd, C.byref(ff), None, C.byref(D),
+               L.ptr(self.acc), sh)
 
     def _dfsph_iters(self, k, s=None):
         sh = L.stream_handle(s)
@@ -631,16 +695,38 @@ class SceneBatch:
         nnz = int(self.ff.rowptr[self.NP].item())
         return st, nnz
 
-    def scene_state(self, k: int) -> dict:
-        """State of scene k in its original particle order."""
+    def scene_state(self, k: int, extra: bool = False) -> dict:
+        """State of scene k in its original particle order.  With `extra`,
+        also the keys the reference leaves in simulationState after
+        integrate(): `acc` (fluidAcceleration: the RK4 sum k1 + 2 k2 + 2 k3
+        + k4, simulationBase.py:701, or the DFSPH step's total
+        acceleration) and `p` (fluidPressure: the density solve's final
+        pressure for DFSPH, dfsph.py:307; for delta-SPH the EOS of the last
+        right-hand side's density, pressure.py:99-111)."""
         a, b = int(self.p_begin[k]), int(self.p_begin[k] + self.n[k])
         inv = np.empty(b - a, dtype=np.int64)
         inv[self.uid[a:b].cpu().numpy() - a] = np.arange(b - a)
-        return dict(x=self.x[a:b].cpu().numpy()[inv], v=self.v[a:b].cpu().numpy()[inv],
-                    rho=self.rho[a:b].cpu().numpy()[inv])
+        out = dict(x=self.x[a:b].cpu().numpy()[inv], v=self.v[a:b].cpu().numpy()[inv],
+                   rho=self.rho[a:b].cpu().numpy()[inv])
+        if extra and self.steps_done:
+            out["acc"] = self.acc[a:b].cpu().numpy()[inv]
+            if self.scheme == "dfsph":
+                out["p"] = self.df["p"][a:b].cpu().numpy()[inv]
+            else:
+                # stage 4 read S[0] (the RK4 stage order of _dsph_step)
+                out["p"] = eos_host(self.scenes[k].phys,
+                                    self.S[0][2][a:b].cpu().numpy()[inv]).astype(np.float32)
+        return out
 
     def dfsph_iterations(self) -> np.ndarray:
         return self.df["iters"].cpu().numpy()
+
+    def divergence_iterations(self) -> np.ndarray:
+        """Iterations of the last divergence solve per scene (zeros when the
+        batch runs none)."""
+        if not self.divergence:
+            return np.zeros(self.nscenes, dtype=np.int32)
+        return self.div_iters.cpu().numpy()
 
 
 class MixedBatch:
@@ -655,7 +741,8 @@ class MixedBatch:
         self.parts = []
         carry = carry if carry is not None else [None] * len(scenes)
         def kind(s):
-            return (s.scheme, bool(s.phys.get("dplus", False)), bool(s.phys.get("shifting", False)))
+            return (s.scheme, bool(s.phys.get("dplus", False)), bool(s.phys.get("shifting", False)),
+                    bool(s.phys.get("divergence", False)))
 
         for key in sorted({kind(s) for s in scenes}, key=lambda k: (k[0] != "dsph", k)):
             ids = [k for k, s in enumerate(scenes) if kind(s) == key]
@@ -705,9 +792,9 @@ class MixedBatch:
         for s in self.streams:
             cur.wait_stream(s)
 
-    def scene_state(self, k):
+    def scene_state(self, k, extra: bool = False):
         b, pos = self.index[k]
-        return b.scene_state(pos)
+        return b.scene_state(pos, extra)
 
     def export(self, k):
         b, pos = self.index[k]

@@ -139,9 +139,16 @@ def write_job_result(path: str, scene, result: dict, steps: int) -> None:
     scene's constants give, boundaryInformation, and one group
     simulationExport/00000 with the job's final state (fluidPosition /
     fluidVelocity and finalPosition / finalVelocity both hold it; a served
-    job keeps no start-of-step copy) and its density, area, support and
-    EOS pressure.  `result` is `Server`'s job.result (x, v, rho in the
-    original particle order, status)."""
+    job keeps no start-of-step copy), its density, area, support, pressure
+    and acceleration.  `result` is `Server`'s job.result (x, v, rho in the
+    original particle order, status, and from SceneBatch.scene_state(extra)
+    `p` and `acc`): fluidPressure is the DFSPH density solve's pressure
+    (dfsph.py:307) or, for delta-SPH, the EOS of the last right-hand side's
+    density; fluidAcceleration is the reference's last fluidAcceleration
+    (simulationBase.py:569,701).  Without `p` the pressure is the EOS of the
+    final density; without `acc` no fluidAcceleration is written.  Not
+    written: fluidDpdt (delta-SPH continuity rate), which served jobs do not
+    keep."""
     from .scenes import box_normals
 
     P = scene.phys
@@ -198,8 +205,11 @@ def write_job_result(path: str, scene, result: dict, steps: int) -> None:
         v = np.asarray(result["v"], np.float32)
         rho = np.asarray(result["rho"], np.float32)
         n = len(x)
-        r = rho.astype(np.float64)
-        p = scene.rho0 * P["c0"] ** 2 / P["gamma"] * (r ** P["gamma"] - 1.0)
+        if "p" in result:
+            p = np.asarray(result["p"], np.float64)
+        else:
+            r = rho.astype(np.float64)
+            p = scene.rho0 * P["c0"] ** 2 / P["gamma"] * (r ** P["gamma"] - 1.0)
         grp.create_dataset("UID", data=np.arange(n, dtype=np.int64))
         for key, val in (("fluidPosition", x), ("fluidVelocity", v), ("finalPosition", x),
                          ("finalVelocity", v)):
@@ -208,5 +218,7 @@ def write_job_result(path: str, scene, result: dict, steps: int) -> None:
         grp.create_dataset("fluidDensity", data=rho)
         grp.create_dataset("fluidSupport", data=np.full(n, scene.support, np.float32))
         grp.create_dataset("fluidPressure", data=p.astype(np.float32))
+        if "acc" in result:
+            grp.create_dataset("fluidAcceleration", data=np.asarray(result["acc"], np.float32))
         grp.close()
         d.close()

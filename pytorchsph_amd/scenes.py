@@ -105,9 +105,29 @@ def lattice_calibration(target_neighbors: int = 20, gamma: float = AKINCI_GAMMA)
     return area / (dx * dx), 0.5 * (lo + hi) / dx
 
 
-def default_phys(scheme: str, h: float, rho0: float = 1000.0) -> dict:
-    """Solver constants of the shipped dam-break configs
-    (configs/dambreak_deltasph.toml, configs/dambreak_dfsph.toml)."""
+# DFSPH time steps as dt / h (s per m of support), the invariant that carries a
+# config's step to another resolution at the same geometry and speeds.
+#  - colliding drops: configs/collision_dfsph.toml steps its fixed
+#    integration.dt = 0.0025 s ([timestep] fixed = true keeps
+#    simulationState['dt'] = integration.dt, simulationBase.py:433,
+#    adaptiveDT.py:61-62) at radius 0.025 and 30 target neighbours, support
+#    0.025 sqrt(30) (simulationBase.py:854-855);
+#  - dam break: configs/dambreak_dfsph.toml steps integration.dt = 0.005 s at
+#    radius 0.01, support 0.01 sqrt(20), i.e. 0.112 s/m.  That step holds the
+#    reference's solver (no warm start, mean-residual stop) only on columns
+#    of about 50 particle rows, the size of the shipped scene: with the FP64
+#    oracle, 0.112 s/m diverges at 140 and more rows, 0.03 s/m at 316 rows.
+#    The dam breaks therefore step the config's [timestep] bound instead,
+#    0.0004 s at the same support (min = max = 0.0004), a chosen value that
+#    is not the step the reference takes with fixed = true.
+DFSPH_DT_PER_H_DROPS = 0.0025 / float(np.float32(0.025 * math.sqrt(30.0)))
+DFSPH_DT_PER_H_DAMBREAK = 0.0004 / float(np.float32(0.01 * math.sqrt(20.0)))
+
+
+def default_phys(scheme: str, h: float, rho0: float = 1000.0, kind: str = "dambreak") -> dict:
+    """Solver constants of the shipped configs (configs/dambreak_deltasph.toml,
+    configs/dambreak_dfsph.toml, configs/collision_dfsph.toml); `kind` picks
+    the DFSPH time step of the scene family."""
     if scheme == "dsph":
         c0 = 10.0 * math.sqrt(2.0 * G * 0.3)   # momentum.py:62-66
         dt = 0.4 * h / c0                       # CFL 0.4, fixed
@@ -116,14 +136,14 @@ def default_phys(scheme: str, h: float, rho0: float = 1000.0) -> dict:
                     tau=1e-3, fluid_omega=0.3, min_iters=2, max_iters=256, eos=0,
                     diffusion="MOG", boundary_diffusion=True)
     c0 = 100.0
-    # configs/dambreak_dfsph.toml:1-4,25: fixed dt = 0.0004 s at radius 0.01,
-    # support 0.01 sqrt(20); the same dt / h at every resolution.  (At four
-    # times this step the 1M-particle dam break diverges within 16 steps.)
-    dt = 0.0004 * h / (0.01 * math.sqrt(20.0))
+    ratio = DFSPH_DT_PER_H_DROPS if kind == "drops" else DFSPH_DT_PER_H_DAMBREAK
+    dt = ratio * h
+    # [dfsph] of both shipped DFSPH configs; divergenceSolver = false there
     return dict(h=h, rho0=rho0, c0=c0, gamma=7.0, kappa=1.3, alpha=0.01, delta=0.1, nu=0.01,
                 eps=0.1 * h * h, gx=0.0, gy=-G, dt=dt, beta=0.125, omega=0.5, tau=1e-3,
                 fluid_omega=0.3, min_iters=2, max_iters=256, eos=0, diffusion="none",
-                boundary_diffusion=False)
+                boundary_diffusion=False, divergence=False, div_tau=1e-2, div_min_iters=2,
+                div_max_iters=8)
 
 
 def dplus_variant(scene: Scene, shifting: bool = True) -> Scene:
@@ -209,7 +229,7 @@ def akinci_volumes(bx: np.ndarray, h: float, gamma: float = 0.7):
     return vol.astype(np.float32), term.astype(np.float32)
 
 
-def _finish(name, scheme, boundary, fluid, vel, dx, W, H, seed, h=None):
+def _finish(name, scheme, boundary, fluid, vel, dx, W, H, seed, h=None, kind="dambreak"):
     h = support_for(dx) if h is None else np.float32(h)
     area = np.float32(dx * dx * lattice_calibration()[0])
     if boundary == "solid":
@@ -220,7 +240,7 @@ def _finish(name, scheme, boundary, fluid, vel, dx, W, H, seed, h=None):
         bx = _walls_akinci(W, H, dx)
         bgx = None
         bvol, bterm = akinci_volumes(bx, h, AKINCI_GAMMA)
-    phys = default_phys(scheme, float(h))
+    phys = default_phys(scheme, float(h), kind=kind)
     buf = 2.0 * float(h)
     return Scene(name=name, scheme=scheme, boundary=boundary, x=fluid.astype(np.float32),
                  v=vel.astype(np.float32), area=float(area), support=float(h),
@@ -259,7 +279,8 @@ def colliding_drops(dx=0.01, radius=0.25, speed=2.0, tank=(4.0, 2.0), scheme="ds
         vel.append(np.tile([u, 0.0], (len(p), 1)))
     pts = _jitter(rng, np.concatenate(pts), dx)
     vel = np.concatenate(vel)
-    return _finish(f"drops_{scheme}_{len(pts)}", scheme, boundary, pts, vel, dx, W, H, seed)
+    return _finish(f"drops_{scheme}_{len(pts)}", scheme, boundary, pts, vel, dx, W, H, seed,
+                   kind="drops")
 
 
 def scene_for_count(kind: str, scheme: str, n_target: int, seed: int) -> Scene:

@@ -21,9 +21,10 @@ read only its own scene's data.  Admission is by device footprint
 
 A job's result is read back the tick it finishes; a finished job keeps
 being stepped (and ignored) until the next repack drops it.  A job whose
-scene goes NaN or leaves its cell table finishes at once, failed, with
-that status (`fail_fast`).  Per-step latency is the wall time of the tick
-in which a job advanced.
+scene goes NaN, leaves its cell table or has its pair list truncated
+finishes at once, failed, with that status (`fail_fast`).  A job's status
+is the OR of every batch it ran in.  Per-step latency is the wall time of
+the tick in which a job advanced.
 
 The particle cap (`max_particles`, default 2M) bounds the tick.  A tick
 lasts as long as its slowest scene: the DFSPH iterations of a tick are the
@@ -44,7 +45,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from ._lib import ST_NAN, ST_OUTSIDE
+from ._lib import ST_NAN, ST_OUTSIDE, ST_OVERFLOW
 from .engine import MixedBatch, SceneBatch
 from .export import write_job_result
 from .scenes import Scene
@@ -62,7 +63,8 @@ class Job:
     t_done: float | None = None
     result: dict | None = None     # x, v, rho in the original particle order, status
     carry: dict | None = None      # exported state between repacks
-    failed: bool = False           # stopped early: its scene went NaN or left its table
+    failed: bool = False           # stopped early: NaN, outside its table or truncated pairs
+    status: int = 0                # status bits raised in the batches it has left
 
     @property
     def n(self) -> int:
@@ -79,8 +81,8 @@ class Server:
                  waste_frac: float = 0.5, log=None, export_dir: str | None = None,
                  fail_fast: bool = True, **batch_kw):
         self.device = device
-        # finish a job as failed on SPH_ST_NAN / SPH_ST_OUTSIDE (one status
-        # read per tick)
+        # finish a job as failed on SPH_ST_NAN / SPH_ST_OUTSIDE /
+        # SPH_ST_OVERFLOW (one status read per tick)
         self.fail_fast = bool(fail_fast)
         if budget_bytes is None:
             free, _ = torch.cuda.mem_get_info()
@@ -109,9 +111,9 @@ class Server:
         return job.bytes
 
     def _finish(self, job: Job, k: int, now: float):
-        st = self.batch.scene_state(k)
+        st = self.batch.scene_state(k, extra=self.export_dir is not None)
         b, pos = self.batch.index[k]
-        st["status"] = b.scene_status(pos)
+        st["status"] = int(b.scene_status(pos)) | job.status
         job.result = st
         job.t_done = now
         if self.export_dir is not None:
@@ -123,9 +125,11 @@ class Server:
     def _advance(self, pos_of: dict, seconds: float, now: float):
         """Book-keeping after one step of the whole batch.  A job whose
         scene raised SPH_ST_NAN (the DFSPH pressure went NaN / Inf, where the
-        reference raises, dfsph.py:179-180) or SPH_ST_OUTSIDE (a particle
+        reference raises, dfsph.py:179-180), SPH_ST_OUTSIDE (a particle
         left the scene's cell table, so the pair lists no longer hold all
-        its pairs) is finished at once with that status, failed."""
+        its pairs) or SPH_ST_OVERFLOW (the batch's shared pair capacity was
+        exceeded and this scene's rows were truncated) is finished at once
+        with that status, failed."""
         moved = 0
         status = {id(b): b.status() for b in self.batch.parts} if self.fail_fast else {}
         for j in self.live:
@@ -136,7 +140,7 @@ class Server:
             self.latency.append(seconds)
             if self.fail_fast:
                 b, pos = self.batch.index[pos_of[j.id]]
-                if int(status[id(b)][pos]) & (ST_NAN | ST_OUTSIDE):
+                if (int(status[id(b)][pos]) | j.status) & (ST_NAN | ST_OUTSIDE | ST_OVERFLOW):
                     j.failed = True
             if j.finished:
                 self._finish(j, pos_of[j.id], now)
@@ -148,6 +152,7 @@ class Server:
         for k, j in enumerate(self.live):
             if not j.finished:
                 j.scene, j.carry = self.batch.export(k)
+                j.status |= int(j.carry.get("status", 0))
         self.batch = None
         t_export = clock()
         used = sum(self.footprint(j) for j in keep)

@@ -183,7 +183,28 @@ def test_served_job_result_layout(tmp_path):
         r = float(np.float32(1.001))      # the stored fp32 density
         ref = 1000.0 * sc.phys["c0"] ** 2 / 7.0 * (r ** 7 - 1)
         assert np.allclose(p, ref, rtol=1e-5)
+        assert "fluidAcceleration" not in g     # none in the result: not written
         nrm = f["boundaryInformation"]["boundaryNormals"][()]
         bx = f["boundaryInformation"]["boundaryPosition"][()]
         # outward: walls below the tank point down, walls left of it point left
         assert np.all(nrm[bx[:, 1] < -0.05, 1] < 0) and np.all(nrm[bx[:, 0] < -0.05, 0] < 0)
+
+
+def test_served_job_result_writes_carried_pressure_and_acceleration(tmp_path):
+    """A DFSPH job's result carries the solver pressure and the last
+    acceleration (SceneBatch.scene_state(extra=True)); both are written as
+    they are, not recomputed from the density."""
+    from pytorchsph_amd import scenes as S
+    from pytorchsph_amd.export import write_job_result
+
+    sc = S.dam_break(dx=0.05, block=(0.5, 0.5), tank=(1.0, 1.0), scheme="dfsph", seed=1)
+    rng = np.random.default_rng(1)
+    res = dict(x=sc.x, v=sc.v, rho=np.ones(sc.n, np.float32), status=0,
+               p=rng.uniform(0, 50, sc.n).astype(np.float32),
+               acc=rng.standard_normal((sc.n, 2)).astype(np.float32))
+    path = tmp_path / "job.hdf5"
+    write_job_result(str(path), sc, res, steps=3)
+    with h5.File(path, "r") as f:
+        g = f["simulationExport"]["00000"]
+        assert np.array_equal(g["fluidPressure"][()], res["p"])
+        assert np.array_equal(g["fluidAcceleration"][()], res["acc"])

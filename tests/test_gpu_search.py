@@ -201,6 +201,25 @@ def test_points_outside_table_are_flagged(cuda):
     assert np.array_equal(sorted_rows(rp[:3], c[: rp[2]]), orc.col.astype(np.int64))
 
 
+def test_point_past_a_side_edge_is_flagged_and_keeps_reference_pairs(cuda):
+    """A point past the right edge of the table aliases into a cell of the
+    next row, as the reference's linear key does (neighSearch.cpp:96): its
+    pairs stay the reference's, and the scene is flagged SPH_ST_OUTSIDE."""
+    hmax = np.float32(0.05)
+    qx, qy, cx, cy, _ = grid_params((0, 0), (1, 1), hmax)
+    rng = np.random.default_rng(5)
+    inside = rng.uniform(0.0, 1.0, (400, 2)).astype(np.float32)
+    past = np.array([[qx + (cx + 0.5) * hmax, 0.5]], np.float32)   # ix == cells_x
+    x = np.concatenate([inside, past])
+    h = np.full(len(x), hmax, np.float32)
+    rp, c, st, _ = gpu_search(x, h, (0, 0), (1, 1), hmax)
+    assert st & L.ST_OUTSIDE
+    assert_same_pairs(rp, c, oracle_self(x, h, (0, 0), (1, 1), hmax))
+    # the inside points alone raise nothing
+    _, _, st0, _ = gpu_search(inside, h[:-1], (0, 0), (1, 1), hmax)
+    assert st0 == 0
+
+
 @pytest.mark.slow
 def test_c2_four_million_points(cuda):
     """Neighbour search alone at 4M points (half uniform, half 8 Gaussian

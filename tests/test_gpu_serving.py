@@ -108,6 +108,11 @@ def test_served_results_exported_in_reference_layout(cuda, tmp_path):
             assert f["boundaryInformation"]["boundaryPosition"][()].shape == (j.scene.nb, 2)
             assert f.attrs["simulationScheme"] == ("deltaSPH" if j.scene.scheme == "dsph"
                                                    else "dfsph")
+            assert np.array_equal(g["fluidPressure"][()], j.result["p"])
+            assert np.array_equal(g["fluidAcceleration"][()], j.result["acc"])
+            assert np.all(np.isfinite(j.result["acc"]))
+            if j.scene.scheme == "dfsph":
+                assert j.result["p"].min() >= 0.0   # the density solve's clamped pressure
 
 
 def test_footprint_matches_allocation(cuda):

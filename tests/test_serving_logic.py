@@ -34,7 +34,7 @@ class CountingBatch:
     def step(self):
         self.count = [c + 1 for c in self.count]
 
-    def scene_state(self, k):
+    def scene_state(self, k, extra=False):
         return dict(steps=self.count[k])
 
     @property
@@ -57,7 +57,10 @@ class CountingBatch:
         return int(self.status()[k])
 
     def export(self, k):
-        return dataclasses.replace(self.scenes[k]), dict(steps=self.count[k], status=0)
+        # a new batch starts with clear status words, as a device batch
+        # would if the carry's status were dropped
+        return (dataclasses.replace(self.scenes[k]),
+                dict(steps=self.count[k], status=self.scene_status(k)))
 
 
 class Clock:
@@ -165,3 +168,48 @@ def test_fail_fast_off_runs_failed_scenes_to_the_end(fake):
     out = srv.run(jobs, clock=Clock())
     assert not jobs[0].failed and jobs[0].steps_done == 6
     assert out["jobs_failed"] == 0 and out["jobs_flagged"]["1"] == 1
+
+
+def test_status_raised_before_a_repack_reaches_the_result(fake):
+    """A status bit (here the iteration cap, 8) raised in one batch stays in
+    the job's result after repacks into batches whose status words start
+    clear."""
+    jobs = jobs_of([(10, 40, 0.0), (10, 5, 0.0), (10, 5, 0.2), (10, 5, 0.4)])
+    jobs[0].scene = dataclasses.replace(jobs[0].scene, name="capped")
+    # raised at the job's second step; sticky in the batch that raised it,
+    # clear in every batch built after it
+    fake.fail_at = {}
+
+    class Once(CountingBatch):
+        def step(self):
+            super().step()
+            self.raised = getattr(self, "raised", set()) | {
+                k for k, (s, c) in enumerate(zip(self.scenes, self.count))
+                if s.name == "capped" and c == 2}
+
+        def status(self):
+            r = getattr(self, "raised", set())
+            return np.array([8 if k in r else 0 for k in range(len(self.scenes))],
+                            dtype=np.int64)
+
+    serving.MixedBatch = Once
+    srv = serving.Server(budget_bytes=10**9, min_repack_s=0.0, waste_frac=0.5)
+    out = srv.run(jobs, clock=Clock())
+    assert srv.repacks >= 2
+    assert jobs[0].result["steps"] == 40 and not jobs[0].failed
+    assert jobs[0].result["status"] & 8
+    assert out["jobs_flagged"]["8"] == 1
+
+
+def test_truncated_pair_list_fails_the_job(fake):
+    """SPH_ST_OVERFLOW (2): the job's rows were cut at the batch's pair
+    capacity, so it is finished failed like a NaN scene."""
+    jobs = jobs_of([(10, 20, 0.0), (12, 20, 0.0)])
+    jobs[1].scene = dataclasses.replace(jobs[1].scene, name="bad")
+    fake.fail_at = {"bad": (4, 2)}
+    srv = serving.Server(budget_bytes=10**9, min_repack_s=0.0)
+    out = srv.run(jobs, clock=Clock())
+    assert jobs[1].failed and jobs[1].steps_done == 4
+    assert jobs[1].result["status"] & 2
+    assert jobs[0].result["steps"] == 20
+    assert out["jobs_failed"] == 1 and out["jobs_flagged"]["2"] == 1
