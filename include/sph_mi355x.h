@@ -188,6 +188,17 @@ size_t sph_scan_workspace_bytes(int n);
 /* out[0] = 0, out[k+1] = in[0] + ... + in[k]; out has n+1 entries. */
 int sph_scan_exclusive(const int* in, int n, int* out, void* ws, size_t ws_bytes,
                        hipStream_t s);
+/* sph_scan_exclusive without a workspace and without any allocation: the
+ * tile partials live in a static device array of SPH_SCAN_NOALLOC_PARTIALS
+ * ints per GPU, so n may be at most 2048 * (SPH_SCAN_NOALLOC_PARTIALS - 1)
+ * = 134,215,680 items (larger n: SPH_E_SPACE, nothing launched).  The array
+ * is shared by every call on that GPU: calls that overlap on one GPU (on
+ * different streams, or from different host threads) race and must be
+ * ordered by the caller.  No counterpart in the reference; it is for
+ * callers that cannot allocate device memory (code under hipGraph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). */
+#define SPH_SCAN_NOALLOC_PARTIALS 65536
+int sph_scan_exclusive_noalloc(const int* in, int n, int* out, hipStream_t s);
 int sph_fill_i32(int* p, int n, int value, hipStream_t s);
 int sph_fill_f32(float* p, int n, float value, hipStream_t s);
 

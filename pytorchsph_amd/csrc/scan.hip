@@ -6,6 +6,7 @@
 #include <stdio.h>
 
 #include <atomic>
+#include <mutex>
 
 namespace sph {
 
@@ -139,7 +140,59 @@ __global__ void k_fill(T* p, int n, T v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+
+// Tile partials of the no-allocation scan: one int per tile of kTile items
+// plus the total, so n <= kTile * (SPH_SCAN_NOALLOC_PARTIALS - 1).  Each GPU
+// the library runs on holds its own copy until the process ends.
+__device__ __align__(256) int scratch[SPH_SCAN_NOALLOC_PARTIALS];
 }  // namespace
+
+// Device address of `scratch` on each device, looked up once per device with
+// that device current.  The array's name is a host-side stand-in; only these
+// addresses may be handed to kernels.
+constexpr int kMaxDevices = 64;
+static std::atomic<int*> g_scratch[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_ptr(int** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return hip_fail("sph_scan_exclusive_noalloc(device)");
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("sph_scan_exclusive_noalloc: device index beyond the supported 64");
+    return SPH_E_ARG;
+  }
+  int* p = g_scratch[dev].load(std::memory_order_acquire);
+  if (!p) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    p = g_scratch[dev].load(std::memory_order_relaxed);
+    if (!p) {
+      void* addr = nullptr;
+      if (hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)) != hipSuccess || !addr)
+        return hip_fail("sph_scan_exclusive_noalloc(hipGetSymbolAddress)");
+      p = static_cast<int*>(addr);
+      g_scratch[dev].store(p, std::memory_order_release);
+    }
+  }
+  *out = p;
+  return SPH_OK;
+}
+
+int scan_exclusive_noalloc(const int* in, int n, int* out, hipStream_t s) {
+  SPH_REQUIRE(n >= 0 && out && (n == 0 || in), "sph_scan_exclusive_noalloc: bad size or buffers");
+  const int64_t max_n = (int64_t)kTile * (SPH_SCAN_NOALLOC_PARTIALS - 1);
+  if ((int64_t)n > max_n) {
+    set_error("sph_scan_exclusive_noalloc: n exceeds the static partials (at most " +
+              std::to_string(max_n) + " items); use sph_scan_exclusive with a workspace");
+    return SPH_E_SPACE;
+  }
+  int* partial = nullptr;
+  int rc = scratch_ptr(&partial);
+  if (rc) return rc;
+  // every partial read is written earlier in the same call (k_scan_reduce
+  // writes one per tile, k_scan_partials rescans them), so nothing left by
+  // a previous call is read and nothing needs clearing
+  return scan_exclusive(in, n, out, partial, sizeof(int) * (size_t)SPH_SCAN_NOALLOC_PARTIALS, s);
+}
 
 int scan_exclusive(const int* in, int n, int* out, void* ws, size_t ws_bytes, hipStream_t s) {
   SPH_REQUIRE(n >= 0 && out, "sph_scan_exclusive: bad size or output");
@@ -188,6 +241,9 @@ size_t sph_scan_workspace_bytes(int n) { return sph::scan_ws_bytes(n); }
 int sph_scan_exclusive(const int* in, int n, int* out, void* ws, size_t ws_bytes,
                        hipStream_t s) {
   return sph::scan_exclusive(in, n, out, ws, ws_bytes, s);
+}
+int sph_scan_exclusive_noalloc(const int* in, int n, int* out, hipStream_t s) {
+  return sph::scan_exclusive_noalloc(in, n, out, s);
 }
 int sph_fill_i32(int* p, int n, int value, hipStream_t s) { return sph::fill_i32(p, n, value, s); }
 int sph_fill_f32(float* p, int n, float value, hipStream_t s) {
