@@ -12,8 +12,8 @@ always as the checker.  It restates the reference's algorithms:
   dfsph           modules/dfsph.py:25-311 + akinciBoundary.py:166-389
   rk4 / semi-implicit steps   simulationBase.py:564-569, 632-705
 Physics is FP64; the neighbour predicate is the reference's fp32 one.
-Parity unpinned: the near-singular branches of pinv2x2, the pinv cutoff of
-mDBC and the fp32 stopping sequence of densitySolve (see sph_oracle.c).
+Parity unpinned: the fp32 stopping sequence of densitySolve (see
+sph_oracle.c); everything else is pinned by the tests named there.
 """
 from __future__ import annotations
 
@@ -405,6 +405,13 @@ def dsph_terms(phys: dict) -> int:
     return t
 
 
+def semi_implicit_step(x0, v0, acc, dt):
+    """semiImplicitEuler (simulationBase.py:647-649): integrateValues with
+    dudt = acc and dxdt = v0 + dt acc, i.e. v1 = v0 + dt a,
+    x1 = x0 + dt (v0 + dt a)."""
+    return x0 + dt * (v0 + dt * acc), v0 + dt * acc
+
+
 class SceneOracle:
     """FP64 restatement of one scene's step (search in fp32 on the rounded
     positions, as the reference searches its fp32 state)."""
@@ -548,7 +555,6 @@ class SceneOracle:
         fv = dict(f, vol=vol)
         _, visc, _ = dsph_rhs(fv, P, ff, None, None, T_VISC | T_LAM)
         acc = acc + visc
-        v1 = v0 + P["dt"] * acc
-        x1 = x0 + P["dt"] * (v0 + P["dt"] * acc)
+        x1, v1 = semi_implicit_step(x0, v0, acc, P["dt"])
         out["rho"], out["vol"], out["brho"], out["acc"] = rho, vol, brho, acc
         return x1, v1, rho, out

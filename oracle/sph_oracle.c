@@ -19,16 +19,21 @@
  *
  * Build: see oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
  *
- * Parity unpinned (pinned only by reading the reference source, which
- * cannot be imported here):
- *  - the near-singular branches of pinv2x2 (util.py:412-426): orc_pinv2x2
- *    for singular values at or below 1e-5;
- *  - the cutoff of torch.linalg.pinv in mDBC (solidBoundary.py:603),
- *    restated in orc_pinv3x3 as rtol = 3 eps_f32 (used for > 10 neighbours,
- *    where the system is well conditioned);
- *  - the fp32 .item() iteration sequence of densitySolve (dfsph.py:205-231):
- *    orc_dfsph sums the residual in FP64 and may stop one iteration earlier
- *    or later near the threshold.
+ * How each part is pinned (tests/test_oracle.py, test_oracle_pins.py,
+ * test_oracle_dplus.py, test_oracle_ref.py): the search by the reference's
+ * own compiled search and brute force; every physics term by closed forms,
+ * symmetries and independent computations that a dropped factor, a flipped
+ * sign or a wrong index fails (Laplacians of quadratic densities, gradients
+ * of linear pressures, exact mDBC extrapolation of linear fields, dry-wall
+ * Akinci densities, two-particle viscosity and alpha, momentum
+ * conservation, the predicted-density error a converged DFSPH solve must
+ * meet, the RK4 and symplectic-Euler step maps); pinv2x2 and pinv3x3 by
+ * textbook pseudo-inverses on their special cases and across the cutoff.
+ *
+ * Parity unpinned: the fp32 .item() iteration sequence of densitySolve
+ * (dfsph.py:205-231).  orc_dfsph sums the residual in FP64 and may stop one
+ * iteration earlier or later than an fp32 mean near the threshold; only an
+ * import of the reference could pin which.
  */
 #include <math.h>
 #include <stdint.h>

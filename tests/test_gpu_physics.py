@@ -230,3 +230,31 @@ def test_pinv2x2_device_matches_oracle(cuda):
     de = np.abs(np.sort(et.cpu().numpy(), 1) - np.sort(eo, 1))
     tol = 50 * np.finfo(np.float32).eps * sv[:, 0] ** 2 / sv[:, 1]
     assert np.all(de <= tol[:, None])
+
+
+def test_pinv2x2_device_special_cases_are_textbook(cuda):
+    """The zero, rank-1 and diag(a, < 1e-5) matrices (util.py:386-433 drops
+    singular values at or below 1e-5 and zeroes a singular vector with
+    sign(0) = 0 only where its value is dropped) give the textbook
+    pseudo-inverse on the device too, as on the oracle
+    (tests/test_oracle_pins.py)."""
+    th = 0.7
+    R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    cases = [np.zeros((2, 2)), np.array([[1.0, 2.0], [2.0, 4.0]]),
+             np.array([[1.0, 2.0], [0.0, 0.0]]), np.array([[0.0, 0.0], [3.0, -1.0]]),
+             np.diag([2.0, 5e-6]), np.diag([3e-6, 2.0]), R @ np.diag([2.0, 4e-6]) @ R.T]
+    M = np.stack(cases).astype(np.float32)
+    n = len(cases)
+    Mt = torch.from_numpy(M.reshape(-1, 4)).cuda()
+    Lt = torch.empty_like(Mt)
+    et = torch.empty((n, 2), dtype=torch.float32, device="cuda")
+    L.call("sph_pinv2x2", L.ptr(Mt), n, L.ptr(Lt), L.ptr(et), L.stream_handle())
+    torch.cuda.synchronize()
+    Lg = Lt.cpu().numpy().reshape(-1, 2, 2).astype(np.float64)
+    eg = et.cpu().numpy()
+    for k, A in enumerate(M.astype(np.float64)):
+        U, sv, Vt = np.linalg.svd(A)
+        inv = np.where(sv > 1e-5, 1.0 / np.where(sv > 1e-5, sv, 1.0), 0.0)
+        want = Vt.T @ np.diag(inv) @ U.T
+        assert np.allclose(Lg[k], want, atol=2e-6 * max(1.0, np.abs(want).max())), k
+        assert abs(abs(eg[k, 0]) - sv[0]) <= 1e-6 * max(sv[0], 1.0), k
