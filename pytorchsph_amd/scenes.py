@@ -9,9 +9,10 @@ reference's geometric contract rather than its init code:
     at (k + 1/2) dx with ghosts mirrored across the wall
     (solidBoundary.py:139-161, ghostParticles.py:128-133), boundary volume
     pi r^2 (solidBoundary.py:196-197);
-  - Akinci walls: one row at dx / 2 outside, volume gamma / sum_b' W_bb' and
-    density term sum_b' W_bb' V_b' (akinciBoundary.py:137-157), computed with
-    this package's own predicate |x_b - x_b'| < h instead of torch_cluster;
+  - Akinci walls: one row outside the fluid, volume gamma / sum_b' W_bb' and
+    density term sum_b' W_bb' V_b' (akinciBoundary.py:137-157), over the wall
+    pairs of this package's own host search (pairs_within, the fp32
+    predicate |x_b - x_b'| < h) instead of torch_cluster's radius();
   - search domain = tank +/- 2 h (simulationBase.py:968-969).
 """
 from __future__ import annotations
@@ -209,19 +210,50 @@ def _wendland_np(q, h):
     return a ** 4 * (1.0 + 4.0 * q) * (7.0 / math.pi) / (h * h)
 
 
+def pairs_within(x: np.ndarray, h: float) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """All ordered pairs (i, j), self pairs included, with fp32
+    |x_i - x_j| < h (the neighbour predicate of the search,
+    neighSearch.cpp:93-107), by a uniform grid of cell size h on the host:
+    points are sorted by cell, and each point's 3x3 cell stencil is a set of
+    contiguous runs of the sorted order.  The init-time counterpart of the
+    reference's torch_cluster radius() call (akinciBoundary.py:137,
+    solidBoundary.py:180).  Returns (i, j, fp32 distance)."""
+    xf = np.ascontiguousarray(x, dtype=np.float32)
+    n = len(xf)
+    if n == 0:
+        z = np.zeros(0, np.int64)
+        return z, z, np.zeros(0, np.float32)
+    h32 = np.float32(h)
+    c = np.floor((xf - xf.min(0)) / h32).astype(np.int64)
+    ny = int(c[:, 1].max()) + 3
+    key = (c[:, 0] + 1) * ny + (c[:, 1] + 1)
+    order = np.argsort(key, kind="stable")
+    skey = key[order]
+    ii, jj = [], []
+    for ox in (-1, 0, 1):
+        for oy in (-1, 0, 1):
+            t = key + ox * ny + oy
+            lo = np.searchsorted(skey, t, side="left")
+            hi = np.searchsorted(skey, t, side="right")
+            cnt = hi - lo
+            tot = int(cnt.sum())
+            if tot == 0:
+                continue
+            start = np.repeat(np.cumsum(cnt) - cnt, cnt)
+            ii.append(np.repeat(np.arange(n), cnt))
+            jj.append(order[np.arange(tot) - start + np.repeat(lo, cnt)])
+    i, j = np.concatenate(ii), np.concatenate(jj)
+    d = np.sqrt(((xf[i] - xf[j]) ** 2).sum(1, dtype=np.float32)).astype(np.float32)
+    keep = d < h32
+    return i[keep], j[keep], d[keep]
+
+
 def akinci_volumes(bx: np.ndarray, h: float, gamma: float = 0.7):
     """V_b = gamma / sum_b' W_bb' and term_b = sum_b' W_bb' V_b'
-    (akinciBoundary.py:152-157), boundary pairs |x_b - x_b'| < h."""
-    from scipy.spatial import cKDTree
-
-    tree = cKDTree(bx.astype(np.float64))
-    pairs = tree.query_pairs(r=float(h) * 1.0001, output_type="ndarray")
-    i = np.concatenate([pairs[:, 0], pairs[:, 1], np.arange(len(bx))])
-    j = np.concatenate([pairs[:, 1], pairs[:, 0], np.arange(len(bx))])
-    bxf = bx.astype(np.float32)
-    d = np.sqrt(((bxf[i] - bxf[j]) ** 2).sum(1).astype(np.float32))
-    keep = d < np.float32(h)
-    i, j, d = i[keep], j[keep], d[keep].astype(np.float64)
+    (akinciBoundary.py:152-157), boundary pairs |x_b - x_b'| < h from
+    pairs_within."""
+    i, j, d = pairs_within(bx, h)
+    d = d.astype(np.float64)
     w = _wendland_np(d / h, h)
     wsum = np.bincount(i, weights=w, minlength=len(bx))
     vol = gamma / wsum
