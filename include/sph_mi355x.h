@@ -122,6 +122,11 @@ typedef struct SphQuery {
   int radius;         /* stencil half width in cells (searchRadius)        */
   int self_sorted;    /* 1: queries are the grid's own points; rows are
                          visited in the grid's sorted order for locality  */
+  const int* qcell;   /* optional [nq] linear cell key of each query: its
+                         cell is (qcell % cells_x, qcell / cells_x) in C
+                         integer arithmetic, as the reference's CPU search
+                         derives it from a cell's key (neighSearch.cpp:207-
+                         209); NULL: the cell is computed from x            */
 } SphQuery;
 
 /* Physical constants of one scene (simulationBase.py:22-111 defaults). */
@@ -224,6 +229,14 @@ int sph_get_tuning(int knob);
  * qx, qy >= 0 matches exactly the points whose key is qx + cells_x * qy. */
 size_t sph_grid_workspace_bytes(int ncells);
 int sph_grid_build(const SphGrid* g, void* ws, size_t ws_bytes, hipStream_t s);
+/* sph_grid_build with the cell key of every point given (keys[np]) rather
+ * than computed from its position: keys outside [0, cells_x * cells_y) go
+ * to the scene's outside slot.  For callers that hold the reference's cell
+ * structures (the linear keys of sortPositions, neighborSearch.py:44-48)
+ * but not the grid origin they came from (buildNeighborList,
+ * neighSearch.cpp:172-177, takes no qMin). */
+int sph_grid_build_keyed(const SphGrid* g, const int* keys, void* ws, size_t ws_bytes,
+                         hipStream_t s);
 /* Number of pairs of each query (rows indexed by query number). */
 int sph_neighbor_count(const SphGrid* g, const SphQuery* q, int* counts, hipStream_t s);
 /* Writes col[rowptr[r] + k] = grid point index; cap = capacity of col.

@@ -46,7 +46,7 @@ class SphGrid(C.Structure):
 class SphQuery(C.Structure):
     _fields_ = [
         ("x", c_vp), ("h", c_vp), ("qscene", c_vp),
-        ("nq", c_int), ("radius", c_int), ("self_sorted", c_int),
+        ("nq", c_int), ("radius", c_int), ("self_sorted", c_int), ("qcell", c_vp),
     ]
 
 
@@ -118,6 +118,7 @@ _SIGS = {
     "sph_grid_workspace_bytes": ([c_int], C.c_size_t),
     "sph_scan_exclusive": ([c_vp, c_int, c_vp, c_vp, C.c_size_t, c_vp], c_int),
     "sph_scan_exclusive_noalloc": ([c_vp, c_int, c_vp, c_vp], c_int),
+    "sph_grid_build_keyed": ([_P(SphGrid), c_vp, c_vp, C.c_size_t, c_vp], c_int),
     "sph_fill_i32": ([c_vp, c_int, c_int, c_vp], c_int),
     "sph_fill_f32": ([c_vp, c_int, C.c_float, c_vp], c_int),
     "sph_set_tuning": ([c_int, c_int], c_int),

@@ -20,7 +20,9 @@ constexpr int kBS = 256;
 // and its lanes take consecutive slots.  (Which points fill which slots of
 // a cell still depends on the order the atomics land; k_grid_cell_sort
 // makes it deterministic.)
-__global__ __launch_bounds__(kBS) void k_grid_keys(SphGrid g) {
+// keys_in == nullptr: the key is computed from the position; else it is
+// keys_in[p] (sph_grid_build_keyed).
+__global__ __launch_bounds__(kBS) void k_grid_keys(SphGrid g, const int* __restrict__ keys_in) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & (kWave - 1);
   const bool valid = p < g.np;
@@ -30,10 +32,16 @@ __global__ __launch_bounds__(kBS) void k_grid_keys(SphGrid g) {
     s = g.pscene[p];
     const SphGridDesc d = g.scenes[s];
     const float x = g.x[2 * p], y = g.x[2 * p + 1];
-    const int ix = cell_coord(x, d.qmin_x, d.hmax);
-    const int iy = cell_coord(y, d.qmin_y, d.hmax);
-    // reference key: ix + cellsX * iy in int32 (neighborSearch.py:47)
-    const int64_t key = (int64_t)ix + (int64_t)d.cells_x * (int64_t)iy;
+    int ix = 0, iy = 0;
+    int64_t key;
+    if (keys_in) {
+      key = keys_in[p];
+    } else {
+      ix = cell_coord(x, d.qmin_x, d.hmax);
+      iy = cell_coord(y, d.qmin_y, d.hmax);
+      // reference key: ix + cellsX * iy in int32 (neighborSearch.py:47)
+      key = (int64_t)ix + (int64_t)d.cells_x * (int64_t)iy;
+    }
     const int64_t ncell = (int64_t)d.cells_x * d.cells_y;
     // a NaN / Inf position has no neighbour under the fp32 predicate; it
     // goes to the outside slot rather than to whatever cell the float to
@@ -46,7 +54,7 @@ __global__ __launch_bounds__(kBS) void k_grid_keys(SphGrid g) {
       // first cells of the next row, or the last of the previous one) so
       // that pairs stay those of the reference, and is flagged: its own
       // stencil no longer covers its true neighbourhood
-      if (ix < 0 || ix >= d.cells_x || iy < 0 || iy >= d.cells_y)
+      if (!keys_in && (ix < 0 || ix >= d.cells_x || iy < 0 || iy >= d.cells_y))
         atomicOr(&g.status[s], SPH_ST_OUTSIDE);
     } else {
       // the scene's last slot collects points outside its table; stencils
@@ -223,8 +231,17 @@ __global__ __launch_bounds__(kBS) void k_neighbors(SphGrid g, SphQuery q,
   const SphGridDesc d = g.scenes[s];
   const float* __restrict__ xp = q.self_sorted ? g.sorted_x + 2 * (int64_t)t : q.x + 2 * (int64_t)qi;
   const float xq = xp[0], yq = xp[1];
-  const int ix = cell_coord(xq, d.qmin_x, d.hmax);
-  const int iy = cell_coord(yq, d.qmin_y, d.hmax);
+  int ix, iy;
+  if (q.qcell) {
+    // the cell of a given key, in C integer arithmetic as
+    // neighSearch.cpp:207-209 derives it
+    const int key = q.qcell[qi];
+    ix = key % d.cells_x;
+    iy = key / d.cells_x;
+  } else {
+    ix = cell_coord(xq, d.qmin_x, d.hmax);
+    iy = cell_coord(yq, d.qmin_y, d.hmax);
+  }
   const int R = q.radius;
   const int* __restrict__ cs = g.cell_start + d.cell_begin;
   const float2* __restrict__ sx = reinterpret_cast<const float2*>(g.sorted_x);
@@ -364,7 +381,8 @@ __global__ __launch_bounds__(kBS) void k_pair_geometry(const int* __restrict__ r
 
 }  // namespace
 
-int grid_build(const SphGrid* gp, void* ws, size_t ws_bytes, hipStream_t s) {
+static int grid_build_impl(const SphGrid* gp, const int* keys, void* ws, size_t ws_bytes,
+                           hipStream_t s) {
   SPH_REQUIRE(gp, "sph_grid_build: null grid");
   const SphGrid& g = *gp;
   SPH_REQUIRE(g.np >= 0 && g.ncells > 0 && g.nscenes > 0, "sph_grid_build: bad sizes");
@@ -375,13 +393,23 @@ int grid_build(const SphGrid* gp, void* ws, size_t ws_bytes, hipStream_t s) {
   int rc;
   if ((rc = fill_i32(g.cell_count, g.ncells, 0, s))) return rc;
   if (g.np > 0)
-    hipLaunchKernelGGL(k_grid_keys, dim3(blocks_for(g.np, kBS)), dim3(kBS), 0, s, g);
+    hipLaunchKernelGGL(k_grid_keys, dim3(blocks_for(g.np, kBS)), dim3(kBS), 0, s, g, keys);
   if ((rc = launch_check("sph_grid_build(keys)"))) return rc;
   if ((rc = scan_exclusive(g.cell_count, g.ncells, g.cell_start, ws, ws_bytes, s))) return rc;
   if (g.np > 0)
     hipLaunchKernelGGL(k_grid_scatter, dim3(blocks_for(g.np, kBS)), dim3(kBS), 0, s, g);
   hipLaunchKernelGGL(k_grid_cell_sort, dim3(blocks_for(g.ncells, kBS)), dim3(kBS), 0, s, g);
   return launch_check("sph_grid_build");
+}
+
+int grid_build(const SphGrid* gp, void* ws, size_t ws_bytes, hipStream_t s) {
+  return grid_build_impl(gp, nullptr, ws, ws_bytes, s);
+}
+
+int grid_build_keyed(const SphGrid* gp, const int* keys, void* ws, size_t ws_bytes,
+                     hipStream_t s) {
+  SPH_REQUIRE(gp && (gp->np == 0 || keys), "sph_grid_build_keyed: null keys");
+  return grid_build_impl(gp, keys, ws, ws_bytes, s);
 }
 
 static int check_query(const SphGrid* g, const SphQuery* q, const char* who) {
@@ -471,6 +499,10 @@ extern "C" {
 size_t sph_grid_workspace_bytes(int ncells) { return sph::scan_ws_bytes(ncells); }
 int sph_grid_build(const SphGrid* g, void* ws, size_t ws_bytes, hipStream_t s) {
   return sph::grid_build(g, ws, ws_bytes, s);
+}
+int sph_grid_build_keyed(const SphGrid* g, const int* keys, void* ws, size_t ws_bytes,
+                         hipStream_t s) {
+  return sph::grid_build_keyed(g, keys, ws, ws_bytes, s);
 }
 int sph_neighbor_count(const SphGrid* g, const SphQuery* q, int* counts, hipStream_t s) {
   return sph::neighbor_count(g, q, counts, s);

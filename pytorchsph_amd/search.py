@@ -97,9 +97,9 @@ class NeighborGrid:
         g = self._grid()
         L.call("sph_grid_build", C.byref(g), L.ptr(self.ws), self.ws_bytes, L.stream_handle(stream))
 
-    def query_struct(self, xq, hq, qscene, radius=1, self_sorted=False) -> L.SphQuery:
+    def query_struct(self, xq, hq, qscene, radius=1, self_sorted=False, qcell=None) -> L.SphQuery:
         return L.SphQuery(L.ptr(xq), L.ptr(hq), L.ptr(qscene), int(xq.shape[0]), int(radius),
-                          1 if self_sorted else 0)
+                          1 if self_sorted else 0, L.ptr(qcell))
 
     def count(self, q: L.SphQuery, counts: torch.Tensor, stream=None) -> None:
         g = self._grid()
