@@ -47,7 +47,9 @@ def test_record_layouts():
     assert L.GRID_DESC_DTYPE.itemsize == 32
     assert L.PHYS_DTYPE.itemsize == 80
     assert C.sizeof(L.SphGrid) == 8 * 3 + 4 * 3 + 4 + 8 * 7
-    assert C.sizeof(L.SphQuery) == 8 * 3 + 4 * 3 + 4
+    # x, h, qscene; nq, radius, self_sorted; padding; qcell
+    assert C.sizeof(L.SphQuery) == 8 * 3 + 4 * 3 + 4 + 8
+    assert L.SphQuery.qcell.offset == 40
     assert C.sizeof(L.SphFields) == 8 * 8 + 4 * 8 + 4 + 4
     assert C.sizeof(L.SphRk4Stage) == 8 + 8 * 10
 
