@@ -123,6 +123,14 @@ def lattice_calibration(target_neighbors: int = 20, gamma: float = AKINCI_GAMMA)
 #    is not the step the reference takes with fixed = true.
 DFSPH_DT_PER_H_DROPS = 0.0025 / float(np.float32(0.025 * math.sqrt(30.0)))
 DFSPH_DT_PER_H_DAMBREAK = 0.0004 / float(np.float32(0.01 * math.sqrt(20.0)))
+# delta-SPH colliding drops: configs/collision_deltasph.toml sets fluid.c0 =
+# 100, kinematic viscosity 0 and a fixed integration.dt = 0.001 s at radius
+# 0.025 and 50 target neighbours (acoustic CFL c0 dt / h = 0.57).  The drops
+# meet at 4 m/s and throw jets of 10-25 m/s: with the dam-break sound speed
+# (10 sqrt(2 g 0.3) = 24 m/s, momentum.py:62-66) those jets reach Mach 1 and
+# go through the floor's mDBC layers.
+DSPH_C0_DROPS = 100.0
+DSPH_DT_PER_H_DROPS = 0.001 / float(np.float32(0.025 * math.sqrt(50.0)))
 
 
 def default_phys(scheme: str, h: float, rho0: float = 1000.0, kind: str = "dambreak") -> dict:
@@ -130,10 +138,14 @@ def default_phys(scheme: str, h: float, rho0: float = 1000.0, kind: str = "dambr
     configs/dambreak_dfsph.toml, configs/collision_dfsph.toml); `kind` picks
     the DFSPH time step of the scene family."""
     if scheme == "dsph":
-        c0 = 10.0 * math.sqrt(2.0 * G * 0.3)   # momentum.py:62-66
-        dt = 0.4 * h / c0                       # CFL 0.4, fixed
+        if kind == "drops":
+            c0, dt, nu = DSPH_C0_DROPS, DSPH_DT_PER_H_DROPS * h, 0.0
+        else:
+            c0 = 10.0 * math.sqrt(2.0 * G * 0.3)   # momentum.py:62-66
+            dt = 0.4 * h / c0                       # CFL 0.4, fixed
+            nu = 0.001
         return dict(h=h, rho0=rho0, c0=c0, gamma=7.0, kappa=1.3, alpha=0.01, delta=0.1,
-                    nu=0.001, eps=0.1 * h * h, gx=0.0, gy=-G, dt=dt, beta=0.125, omega=0.5,
+                    nu=nu, eps=0.1 * h * h, gx=0.0, gy=-G, dt=dt, beta=0.125, omega=0.5,
                     tau=1e-3, fluid_omega=0.3, min_iters=2, max_iters=256, eos=0,
                     diffusion="MOG", boundary_diffusion=True)
     c0 = 100.0
