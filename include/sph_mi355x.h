@@ -237,6 +237,12 @@ int sph_grid_build(const SphGrid* g, void* ws, size_t ws_bytes, hipStream_t s);
  * neighSearch.cpp:172-177, takes no qMin). */
 int sph_grid_build_keyed(const SphGrid* g, const int* keys, void* ws, size_t ws_bytes,
                          hipStream_t s);
+/* The squared-distance threshold the search tests each support with:
+ * out[i] = the largest float T with fl(sqrt(T)) < h[i], so that for every
+ * float s >= 0, fl(sqrt(s)) < h[i] (the reference's predicate with a
+ * correctly rounded sqrt, neighSearch.cpp:222-223) iff s <= out[i].
+ * Computed in integer arithmetic; exact for 2^-62 <= h <= 2^62. */
+int sph_pair_thresholds(const float* h, int n, float* out, hipStream_t s);
 /* Number of pairs of each query (rows indexed by query number). */
 int sph_neighbor_count(const SphGrid* g, const SphQuery* q, int* counts, hipStream_t s);
 /* Writes col[rowptr[r] + k] = grid point index; cap = capacity of col.

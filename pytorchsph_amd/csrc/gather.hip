@@ -513,52 +513,53 @@ __global__ __launch_bounds__(kBS) void k_dsph_density_gradient(SphFluid f,
 }
 
 // ----------------------------------------------------------------- mDBC
-// 3x3 Moore-Penrose inverse via one-sided Jacobi SVD in FP64, singular
-// values below rtol * sigma_max dropped (torch.linalg.pinv default rtol =
-// max(m, n) * eps of float32).
-__device__ void pinv3x3(const double A[9], double X[9]) {
-  double U[9], V[9];
+// 3x3 Moore-Penrose inverse via one-sided Jacobi SVD in fp32, singular
+// values at or below rtol * sigma_max dropped with torch.linalg.pinv's
+// default rtol = max(m, n) * eps(float32) (solidBoundary.py:603).  Column
+// pairs are rotated until they are orthogonal to fp32 precision.
+__device__ void pinv3x3f(const float A[9], float X[9]) {
+  float U[9], V[9];
   for (int k = 0; k < 9; ++k) {
     U[k] = A[k];
-    V[k] = (k % 4 == 0) ? 1.0 : 0.0;
+    V[k] = (k % 4 == 0) ? 1.f : 0.f;
   }
-  for (int sweep = 0; sweep < 30; ++sweep) {
-    double off = 0.0;
+  for (int sweep = 0; sweep < 20; ++sweep) {
+    bool rotated = false;
     for (int p = 0; p < 2; ++p) {
       for (int q = p + 1; q < 3; ++q) {
-        double al = 0.0, be = 0.0, ga = 0.0;
+        float al = 0.f, be = 0.f, ga = 0.f;
         for (int r = 0; r < 3; ++r) {
           al += U[3 * r + p] * U[3 * r + p];
           be += U[3 * r + q] * U[3 * r + q];
           ga += U[3 * r + p] * U[3 * r + q];
         }
-        if (fabs(ga) <= 1e-300 || fabs(ga) <= 1e-15 * sqrt(al * be)) continue;
-        off = fmax(off, fabs(ga) / sqrt(al * be));
-        const double zeta = (be - al) / (2.0 * ga);
-        const double tt = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        const double c = 1.0 / sqrt(1.0 + tt * tt), s = c * tt;
+        if (!(fabsf(ga) > 1.2e-7f * sqrtf(al * be))) continue;
+        rotated = true;
+        const float zeta = (be - al) / (2.f * ga);
+        const float tt = (zeta >= 0.f ? 1.f : -1.f) / (fabsf(zeta) + sqrtf(1.f + zeta * zeta));
+        const float c = 1.f / sqrtf(1.f + tt * tt), sn = c * tt;
         for (int r = 0; r < 3; ++r) {
-          const double up = U[3 * r + p], uq = U[3 * r + q];
-          U[3 * r + p] = c * up - s * uq;
-          U[3 * r + q] = s * up + c * uq;
-          const double vp = V[3 * r + p], vq = V[3 * r + q];
-          V[3 * r + p] = c * vp - s * vq;
-          V[3 * r + q] = s * vp + c * vq;
+          const float up = U[3 * r + p], uq = U[3 * r + q];
+          U[3 * r + p] = c * up - sn * uq;
+          U[3 * r + q] = sn * up + c * uq;
+          const float vp = V[3 * r + p], vq = V[3 * r + q];
+          V[3 * r + p] = c * vp - sn * vq;
+          V[3 * r + q] = sn * vp + c * vq;
         }
       }
     }
-    if (off < 1e-15) break;
+    if (!rotated) break;
   }
-  double sig[3], smax = 0.0;
+  float sig[3], smax = 0.f;
   for (int k = 0; k < 3; ++k) {
-    sig[k] = sqrt(U[k] * U[k] + U[3 + k] * U[3 + k] + U[6 + k] * U[6 + k]);
-    smax = fmax(smax, sig[k]);
+    sig[k] = sqrtf(U[k] * U[k] + U[3 + k] * U[3 + k] + U[6 + k] * U[6 + k]);
+    smax = fmaxf(smax, sig[k]);
   }
-  const double cut = 3.0 * 1.1920928955078125e-07 * smax;
-  for (int k = 0; k < 9; ++k) X[k] = 0.0;
+  const float cut = 3.f * 1.1920928955078125e-07f * smax;
+  for (int k = 0; k < 9; ++k) X[k] = 0.f;
   for (int k = 0; k < 3; ++k) {
-    if (!(sig[k] > cut) || sig[k] == 0.0) continue;
-    const double inv = 1.0 / (sig[k] * sig[k]);  // u_k = U[:,k] / sig, times 1/sig
+    if (!(sig[k] > cut) || sig[k] == 0.f) continue;
+    const float inv = 1.f / (sig[k] * sig[k]);  // u_k = U[:,k] / sig, times 1/sig
     for (int r = 0; r < 3; ++r)
       for (int c = 0; c < 3; ++c) X[3 * r + c] += V[3 * r + k] * inv * U[3 * c + k];
   }
@@ -606,15 +607,33 @@ __global__ __launch_bounds__(kBS) void k_mdbc_density(SphFluid f, const SphPhysi
   const int cnt = a1 - a0;
   float rho = 1.f;
   if (cnt > 10) {
-    const double A[9] = {den, psx, psy, vgx, m00, m01, vgy, m10, m11};
-    double X[9];
-    pinv3x3(A, X);
-    const double rhs[3] = {num, gsx, gsy};
-    double res[3];
-    for (int r = 0; r < 3; ++r) res[r] = X[3 * r] * rhs[0] + X[3 * r + 1] * rhs[1] + X[3 * r + 2] * rhs[2];
+    // The MLS system A [rho, grad rho] = [num, sum m grad W] in units of the
+    // support: unknowns z = [rho, h grad rho] and the gradient rows times h,
+    //   [ den      psx / h  psy / h ] [ rho            ]   [ num   ]
+    //   [ h vgx    m00      m01     ] [ h d rho / dx   ] = [ h gsx ]
+    //   [ h vgy    m10      m11     ] [ h d rho / dy   ]   [ h gsy ]
+    // whose entries are all O(1), so the fp32 solve loses no accuracy to
+    // the 1/h^2 spread of the raw system.  For every system whose singular
+    // values stay above the cutoff (a full set of more than 10 neighbours)
+    // the result is pinv(A) rhs of solidBoundary.py:603-612.  The solve is
+    // for the correction dz to the Shepard value z0 = [num / den, 0, 0]:
+    // A dz = rhs - A z0, z = z0 + dz.  The same solution in exact
+    // arithmetic; in fp32 the pseudo-inverse's rounding then scales with dz
+    // (the density's variation over a support) instead of with rho itself,
+    // which matters because the Tait EOS turns rho_b - 1 of a few ulp at
+    // rest into the wall pressure.
+    const float h = P.h, ih = K.inv_h;
+    const float A[9] = {den, psx * ih, psy * ih, vgx * h, m00, m01, vgy * h, m10, m11};
+    float X[9];
+    pinv3x3f(A, X);
+    const float rs = num / den;
+    const float rhs[3] = {num - den * rs, (gsx - vgx * rs) * h, (gsy - vgy * rs) * h};
+    float dz[3];
+    for (int r = 0; r < 3; ++r)
+      dz[r] = X[3 * r] * rhs[0] + X[3 * r + 1] * rhs[1] + X[3 * r + 2] * rhs[2];
     const float2 xb = ld2(bx, bI);
-    const double dx = (double)xb.x - g.x, dy = (double)xb.y - g.y;
-    rho = (float)((res[0] + dx * res[1] + dy * res[2]) / P.rho0);
+    const float dx = xb.x - g.x, dy = xb.y - g.y;
+    rho = (rs + dz[0] + (dx * dz[1] + dy * dz[2]) * ih) / P.rho0;
   } else if (cnt > 0) {
     rho = num / den / P.rho0;
   }

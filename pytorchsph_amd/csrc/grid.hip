@@ -379,7 +379,20 @@ __global__ __launch_bounds__(kBS) void k_pair_geometry(const int* __restrict__ r
   }
 }
 
+__global__ __launch_bounds__(kBS) void k_pair_thresholds(const float* __restrict__ h, int n,
+                                                         float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = sq_threshold(h[i]);
+}
+
 }  // namespace
+
+int pair_thresholds(const float* h, int n, float* out, hipStream_t s) {
+  SPH_REQUIRE(n >= 0 && (n == 0 || (h && out)), "sph_pair_thresholds: bad arguments");
+  if (n == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_pair_thresholds, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, h, n, out);
+  return launch_check("sph_pair_thresholds");
+}
 
 static int grid_build_impl(const SphGrid* gp, const int* keys, void* ws, size_t ws_bytes,
                            hipStream_t s) {
@@ -503,6 +516,9 @@ int sph_grid_build(const SphGrid* g, void* ws, size_t ws_bytes, hipStream_t s) {
 int sph_grid_build_keyed(const SphGrid* g, const int* keys, void* ws, size_t ws_bytes,
                          hipStream_t s) {
   return sph::grid_build_keyed(g, keys, ws, ws_bytes, s);
+}
+int sph_pair_thresholds(const float* h, int n, float* out, hipStream_t s) {
+  return sph::pair_thresholds(h, n, out, s);
 }
 int sph_neighbor_count(const SphGrid* g, const SphQuery* q, int* counts, hipStream_t s) {
   return sph::neighbor_count(g, q, counts, s);

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 #include <string>
 #include <type_traits>
 #include "sph_mi355x.h"
@@ -68,21 +69,36 @@ __device__ __forceinline__ float dist2_nofma(float xq, float yq, float xj, float
 // Canonical neighbour predicate of the reference, fl(sqrt(s)) < h with a
 // correctly rounded sqrt (neighSearch.cpp:222-223), evaluated without any
 // sqrt: for every float s >= 0,  fl(sqrt(s)) < h  <=>  s <= sq_threshold(h).
-// Rounding is monotone, so fl(sqrt(s)) < h iff sqrt(s) lies below the
-// midpoint m of h and its predecessor hp (or on it, when round-half-even
-// picks hp).  m has at most 25 significant bits, so m*m is exact in FP64 and
-// the comparison against the float s is exact.
-__device__ __forceinline__ float sq_threshold(float h) {
+// Rounding is monotone, so fl(sqrt(s)) < h iff sqrt(s) < m, the midpoint of
+// h and its predecessor hp = mp 2^ep (mp its 24-bit significand):
+// m = (2 mp + 1) 2^(ep - 1).  m^2 = (2 mp + 1)^2 2^(2 ep - 2) has an odd
+// significand of 49 or 50 bits, so it is no float: sqrt(s) never ties at m,
+// and the largest float below m^2 is m^2 truncated to 24 bits.  Integer
+// arithmetic only (64-bit products); exact for 2^-62 <= h <= 2^62, where
+// the threshold is a normal float.  Outside that range the threshold
+// saturates to 0 or the largest float.
+__host__ __device__ __forceinline__ float sq_threshold(float h) {
   if (!(h > 0.f)) return -1.f;
-  if (isinf(h)) return INFINITY;
-  const float hp = __uint_as_float(__float_as_uint(h) - 1u);
-  const double m = ((double)hp + (double)h) * 0.5;
-  const double m2 = m * m;
-  float f = (float)m2;
-  if ((double)f > m2) f = __uint_as_float(__float_as_uint(f) - 1u);
-  // s == m*m: sqrt(s) is the midpoint; it rounds to hp (a pair) only when
-  // hp's last significand bit is even.
-  if ((double)f == m2 && (__float_as_uint(hp) & 1u)) f = __uint_as_float(__float_as_uint(f) - 1u);
+  uint32_t hb;
+  memcpy(&hb, &h, 4);
+  if (hb == 0x7f800000u) return INFINITY;
+  hb -= 1u;                                             // hp, the predecessor
+  const uint32_t be = hb >> 23;                         // biased exponent of hp
+  if (be == 0) return 0.f;                              // hp subnormal
+  const uint64_t mp = (hb & 0x7fffffu) | 0x800000u;     // 24-bit significand
+  const uint64_t odd = 2u * mp + 1u;                    // m = odd 2^(ep - 1)
+  const uint64_t M = odd * odd;                         // < 2^50
+  const int bits = 64 - __builtin_clzll(M);             // 49 or 50
+  const int k = bits - 24;
+  const uint32_t t = (uint32_t)(M >> k);                // in [2^23, 2^24)
+  // value t 2^(2 (ep - 1) + k) with ep = be - 150; biased exponent of the
+  // float t 2^E is E + 23 + 127
+  const int eb = 2 * ((int)be - 151) + k + 150;
+  if (eb <= 0) return 0.f;
+  if (eb >= 255) return 3.4028234663852886e38f;
+  const uint32_t out = ((uint32_t)eb << 23) | (t & 0x7fffffu);
+  float f;
+  memcpy(&f, &out, 4);
   return f;
 }
 
@@ -167,14 +183,23 @@ __device__ __forceinline__ PairGeo pair_geo(float xi, float yi, float xj, float 
   return g;
 }
 
+// x^y for x > 0 in fp32 arithmetic only (a few ulp); 0 for x == 0 and
+// y > 0, as pow gives.
+__device__ __forceinline__ float pow_f32(float x, float y) {
+  if (x == 0.f) return y > 0.f ? 0.f : INFINITY;
+  return exp2f(y * log2f(x));
+}
+
 // Equation of state (pressure.py:99-111).
 __device__ __forceinline__ float eos(const SphPhysics& P, float rhohat) {
   if (P.eos == 0) {
     float r2 = rhohat * rhohat;
     float r4 = r2 * r2;
     float r7 = r4 * r2 * rhohat;
-    // gamma is 7 in every shipped configuration; other exponents use powf.
-    float rg = (P.gamma == 7.f) ? r7 : powf(rhohat, P.gamma);
+    // gamma is 7 in every shipped configuration; other exponents go through
+    // exp2(gamma log2(rho)) in fp32 (the device's fp32 powf reduces its
+    // argument in FP64, which the path keeps for the DFSPH residual alone)
+    float rg = (P.gamma == 7.f) ? r7 : pow_f32(rhohat, P.gamma);
     return P.rho0 * P.c0 * P.c0 / P.gamma * (rg - 1.f);
   }
   if (P.eos == 1) return P.c0 * P.c0 * (rhohat * P.rho0 - P.rho0);
@@ -200,7 +225,7 @@ __device__ __forceinline__ float eos_eval(const EosK& e, float rhohat) {
     float r2 = rhohat * rhohat;
     float r4 = r2 * r2;
     float r7 = r4 * r2 * rhohat;
-    float rg = (e.gamma == 7.f) ? r7 : powf(rhohat, e.gamma);
+    float rg = (e.gamma == 7.f) ? r7 : pow_f32(rhohat, e.gamma);
     return e.f * (rg - 1.f);
   }
   return e.f * (rhohat * e.rho0 - e.rho0);

@@ -306,3 +306,36 @@ def test_non_finite_points_have_no_neighbours(cuda):
     for k, i in enumerate(good[:2000]):
         # good is ascending, so mapping the reduced indices keeps rows sorted
         assert np.array_equal(s1[rp[i]:rp[i + 1]], good[s2[rp2[k]:rp2[k + 1]]])
+
+
+def exact_threshold(h):
+    """Largest float32 T with fl(sqrt(T)) < h, from the FP64 midpoint:
+    m = (pred(h) + h) / 2 has 25 significant bits, so m^2 is exact in FP64
+    and is never a float32 (odd 49-50 bit significand)."""
+    h = np.asarray(h, np.float32)
+    hp = np.nextafter(h, np.float32(0))
+    m = (hp.astype(np.float64) + h.astype(np.float64)) / 2
+    m2 = m * m
+    f = m2.astype(np.float32)
+    over = f.astype(np.float64) >= m2
+    f[over] = np.nextafter(f[over], np.float32(0))
+    return f
+
+
+def test_device_pair_thresholds_are_exact(cuda):
+    """sph_pair_thresholds (the integer-arithmetic threshold every search
+    query uses) against the FP64 definition, bit for bit: random supports,
+    powers of two and their neighbours over 2^-60..2^60, and the supports
+    of the benchmark scenes."""
+    rng = np.random.default_rng(11)
+    pow2 = np.ldexp(np.float32(1), np.arange(-60, 61)).astype(np.float32)
+    bits = pow2.view(np.uint32)[:, None] + np.arange(-3, 4, dtype=np.int64)[None, :]
+    h = np.concatenate([rng.uniform(1e-6, 1e3, 1_000_000).astype(np.float32),
+                        bits.astype(np.uint32).ravel().view(np.float32),
+                        np.array([S.support_for(d) for d in (0.0005, 0.001, 0.01, 0.02)],
+                                 np.float32)])
+    ht = torch.from_numpy(h).cuda()
+    out = torch.empty_like(ht)
+    L.call("sph_pair_thresholds", L.ptr(ht), len(h), L.ptr(out), L.stream_handle())
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), exact_threshold(h).view(np.uint32))
