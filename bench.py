@@ -42,8 +42,8 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="arrival window of the serving run (0 = skip)")
     ap.add_argument("--tuning", default=None,
-                    help="fixed launch shapes instead of autotuning: count,gather,solve[,fill] "
-                         "lanes per row and the DFSPH grid cap, e.g. 2,4,2,8,16384")
+                    help="fixed launch shapes instead of autotuning: count,gather,solve[,fill"
+                         "[,walk]] lanes per row and the DFSPH grid cap, e.g. 2,4,2,8,1,16384")
     ap.add_argument("--path", choices=PATHS, default=None,
                     help="run one single-scene workload of the plain paths and print it")
     ap.add_argument("--dplus-only", action="store_true",
@@ -134,7 +134,8 @@ def gather_roofline(b, reps=20):
 
     from pytorchsph_amd import _lib as L
 
-    b.dsph_rhs(b.x, b.v, b.rho)      # pair lists, mDBC and records of the current state
+    b.search(b.x)                    # every pair list of the current state
+    b.dsph_rhs(b.x, b.v, b.rho)      # mDBC and records of the current state
     torch.cuda.synchronize()
     nnz = int(b.ff.rowptr[b.NP].item())
     nnz_b = int(b.fb.rowptr[b.NP].item())
@@ -582,9 +583,9 @@ def main():
     tuned = None
     if args.tuning:
         vals = [int(t) for t in args.tuning.split(",")]
-        if len(vals) not in (4, 5):
-            raise SystemExit("--tuning takes count,gather,solve[,fill],cap")
-        L.set_tuning(**dict(zip(("search", "gather", "solve", "fill"), vals[:-1])))
+        if len(vals) not in (4, 5, 6):
+            raise SystemExit("--tuning takes count,gather,solve[,fill[,walk]],cap")
+        L.set_tuning(**dict(zip(("search", "gather", "solve", "fill", "walk"), vals[:-1])))
         cap = vals[-1]
         L.DFSPH_GRID_CAP = cap
         tuned = {"choice": dict(L.get_tuning(), grid_cap=cap)}

@@ -218,7 +218,8 @@ int sph_fill_f32(float* p, int n, float value, hipStream_t s);
 #define SPH_TUNE_GATHER_LANES 1   /* density and delta-SPH gathers            */
 #define SPH_TUNE_SOLVE_LANES  2   /* DFSPH alpha/source, iterate and finish   */
 #define SPH_TUNE_FILL_LANES   3   /* sph_neighbor_fill                        */
-#define SPH_TUNE_COUNT        4
+#define SPH_TUNE_WALK_LANES   4   /* sph_dsph_cellwalk's list walk            */
+#define SPH_TUNE_COUNT        5
 int sph_set_tuning(int knob, int value);
 int sph_get_tuning(int knob);
 
@@ -330,6 +331,28 @@ int sph_dsph_gather_rk4(const float* frec, const float* brec, const int* pscene,
                         const SphPhysics* phys, const SphCSR* ff, const SphCSR* fb,
                         const float* gradrho, unsigned terms, const int* order,
                         const SphRk4Stage* rk, hipStream_t s);
+/* sph_dsph_gather_packed with the fluid pairs found in the kernel from the
+ * grid g (sph_grid_build of the records' positions) instead of read from a
+ * list: the same pairs as sph_neighbor_fill of g's own points (radius 1,
+ * pair iff fl(|x_i - x_j|) < phys[scene].h, which must not exceed the
+ * scene's cell size), walked in the same order on the same lanes, so
+ * dp/dt and a are bit for bit those of sph_dsph_gather_packed over that
+ * list.  Replaces, for the delta-SPH right-hand side, the fluid pair list
+ * of neighborSearch.py:126-157 (neighSearch_cuda.cu:214-307) together with
+ * the gathers of momentum.py:31-38, densityDiffusion.py:74-82,
+ * pressure.py:31-38, diffusion.py:30-44 and laminar.py:30-46.
+ * segs[nseg][2] = {first, end} partitions the sorted slots [0, g->np) into
+ * runs of one scene each, one workgroup per run; any partition gives the
+ * same results, runs of at most 256 slots keep the stencil records in LDS.
+ * Rows of frec are in point order (g->x's); fb rows too.  SPH_T_DDSPH not
+ * allowed (its gradient needs the list). */
+int sph_dsph_cellwalk(const SphGrid* g, const int* segs, int nseg, const float* frec,
+                      const float* brec, const SphPhysics* phys, const SphCSR* fb,
+                      unsigned terms, float* dpdt, float* acc, hipStream_t s);
+/* sph_dsph_cellwalk followed by the RK4 stage, as sph_dsph_gather_rk4. */
+int sph_dsph_cellwalk_rk4(const SphGrid* g, const int* segs, int nseg, const float* frec,
+                          const float* brec, const SphPhysics* phys, const SphCSR* fb,
+                          unsigned terms, const SphRk4Stage* rk, hipStream_t s);
 /* Renormalisation matrix M_i (fluid + boundary neighbours) and L_i =
  * pinv2x2(M_i); M, L are [np][2][2]. */
 int sph_dsph_normalization(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,

@@ -99,7 +99,8 @@ ST_OUTSIDE, ST_OVERFLOW, ST_NAN, ST_ITERCAP = 1, 2, 4, 8
 T_CONT, T_CONT_B, T_MOG, T_DDSPH = 1 << 0, 1 << 1, 1 << 2, 1 << 3
 T_PRESS, T_PRESS_B, T_VISC, T_VISC_B = 1 << 4, 1 << 5, 1 << 6, 1 << 7
 T_LAM, T_GRAV, T_ACCUM = 1 << 8, 1 << 9, 1 << 16
-TUNE_SEARCH_LANES, TUNE_GATHER_LANES, TUNE_SOLVE_LANES, TUNE_FILL_LANES = 0, 1, 2, 3
+TUNE_SEARCH_LANES, TUNE_GATHER_LANES, TUNE_SOLVE_LANES, TUNE_FILL_LANES, TUNE_WALK_LANES = \
+    0, 1, 2, 3, 4
 DFSPH_ERR_SLOTS = 64
 
 
@@ -107,7 +108,7 @@ def dfsph_err_stride(nscenes: int) -> int:
     """Row stride (doubles) of SphDfsph.err_acc: SPH_DFSPH_ERR_STRIDE."""
     return (nscenes + 15) // 16 * 16
 TUNE_KNOBS = {"search": TUNE_SEARCH_LANES, "gather": TUNE_GATHER_LANES, "solve": TUNE_SOLVE_LANES,
-              "fill": TUNE_FILL_LANES}
+              "fill": TUNE_FILL_LANES, "walk": TUNE_WALK_LANES}
 
 # exported entry points: name -> argtypes (restype is int unless noted)
 _P = C.POINTER
@@ -163,6 +164,10 @@ _SIGS = {
                                       C.c_uint, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "sph_dsph_gather_rk4": ([c_vp, c_vp, c_vp, c_int, c_vp, _P(SphCSR), _P(SphCSR), c_vp,
                              C.c_uint, c_vp, _P(SphRk4Stage), c_vp], c_int),
+    "sph_dsph_cellwalk": ([_P(SphGrid), c_vp, c_int, c_vp, c_vp, c_vp, _P(SphCSR), C.c_uint, c_vp,
+                           c_vp, c_vp], c_int),
+    "sph_dsph_cellwalk_rk4": ([_P(SphGrid), c_vp, c_int, c_vp, c_vp, c_vp, _P(SphCSR), C.c_uint,
+                               _P(SphRk4Stage), c_vp], c_int),
     "sph_integrate_rk4": ([c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                            c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "sph_integrate_semi_implicit": ([c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
@@ -217,7 +222,7 @@ def call(name: str, *args) -> None:
 
 
 def set_tuning(**lanes: int) -> dict:
-    """Sets lanes-per-row knobs by name (search=, gather=, solve=, fill=); returns
+    """Sets lanes-per-row knobs by name (search=, gather=, solve=, fill=, walk=); returns
     the previous values.  Host-side only: no device is touched."""
     old = get_tuning()
     for name, v in lanes.items():

@@ -1,8 +1,8 @@
 """Launch-shape autotuner of the batched step.
 
 The knobs are the lanes per row of the neighbour count and fill, of the
-kernel-sum gathers and of the DFSPH solve (sph_set_tuning), and the workgroup cap of
-the DFSPH iteration kernels.  Each knob is timed on the batch it will run
+kernel-sum gathers over pair lists, of the delta-SPH cell walk and of the DFSPH
+solve (sph_set_tuning), and the workgroup cap of the DFSPH iteration kernels.  Each knob is timed on the batch it will run
 (same scenes, same pair lists), one knob at a time with the others held,
 and the fastest value is kept.  Pair sets do not depend on any knob; sums
 are re-associated, so results move within fp32 rounding.
@@ -84,7 +84,10 @@ def tune(parts, reps: int = 5, log=None) -> dict:
     plans = {
         "search": [_search_fn(b) for b in parts],
         "fill": [_search_fn(b) for b in parts],
-        "gather": [_gather_fn(b) for b in parts],
+        # the delta-SPH right-hand side of a cell-walk batch runs on the walk
+        # knob, every other gather on the gather knob
+        "gather": [_gather_fn(b) for b in parts if not getattr(b, "cellwalk", False)],
+        "walk": [_gather_fn(b) for b in dsph if b.cellwalk],
         "solve": [_solve_fn(b) for b in dfsph],
     }
     times: dict = {}

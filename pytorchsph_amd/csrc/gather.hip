@@ -12,6 +12,8 @@
 //    per particle {x, y, vx, vy, rho, V, p, rho_hat} so that a neighbour is
 //    two 16-byte loads from one cache line, with V = A / rho_hat and the EOS
 //    evaluated once per particle instead of once per pair.
+#include <climits>
+
 #include "sph_common.h"
 
 namespace sph {
@@ -303,6 +305,51 @@ __global__ __launch_bounds__(kBS) void k_dsph_pack(SphFluid f, const SphPhysics*
   }
 }
 
+// Boundary pairs of fluid row i from the fluid -> boundary list, lane
+// `sub` of KL taking pairs sub, sub + KL, ...
+template <int KL>
+__device__ __forceinline__ void packed_bdy_pairs(const SphPhysics& P, const KConst& K,
+                                                 const Own& o, const float4* __restrict__ brec,
+                                                 const SphCSR& fb, int i, int sub,
+                                                 unsigned terms, Acc& a) {
+  if (!fb.rowptr || !(terms & kBdyPairTerms)) return;
+  const int k1 = fb.rowptr[i + 1];
+  for (int k = fb.rowptr[i] + sub; k < k1; k += KL) {
+    const int j = fb.col[k];
+    const float4 r0 = brec[2 * j], r1 = brec[2 * j + 1];
+    const Nbr n = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z};
+    dsph_pair(P, K, o, n, 0.f, 0.f, false, terms, a);
+  }
+}
+
+// Output of row i from its reduced sums.  FUSE: the row's RK4 stage
+// combine (rk4_particle) and the next stage's record instead of dp/dt and
+// a (see sph_dsph_gather_rk4).
+template <bool FUSE>
+__device__ __forceinline__ void packed_row_out(const SphPhysics& P, const Own& o, const Acc& a,
+                                               unsigned terms, int i, float* __restrict__ dpdt,
+                                               float* __restrict__ acc, const SphRk4Stage& rk) {
+  if (!FUSE) {
+    dsph_store(P, o, a, terms, i, dpdt, acc);
+    return;
+  }
+  float dp, ax, ay;
+  dsph_rates(P, o, a, terms, dp, ax, ay);
+  const Rk4State s0 = {rk.x0[2 * i], rk.x0[2 * i + 1], rk.v0[2 * i], rk.v0[2 * i + 1], rk.r0[i]};
+  float2 asum;
+  const Rk4State n = rk4_particle(rk.stage, P, ax, ay, o.vx, o.vy, dp, true,
+                                  rk.K + 5 * (size_t)i, s0, &asum);
+  st2(rk.x_out, i, make_float2(n.x, n.y));
+  st2(rk.v_out, i, make_float2(n.vx, n.vy));
+  rk.rho_out[i] = n.r;
+  if (rk.stage == 4) {
+    if (rk.acc_out) st2(rk.acc_out, i, asum);
+  } else if (rk.frec_next) {
+    pack_fluid(reinterpret_cast<float4*>(rk.frec_next), i, P, n.x, n.y, n.vx, n.vy, n.r,
+               rk.area[i] / n.r);
+  }
+}
+
 // FUSE: the row's RK4 stage combine (rk4_particle) and the next stage's
 // record run in the epilogue instead of storing dp/dt and a (see
 // sph_dsph_gather_rk4); the record buffers of successive stages alternate,
@@ -366,35 +413,271 @@ __global__ __launch_bounds__(kBS) void k_dsph_gather_packed(
       }
     }
   }
-  if (fb.rowptr && (terms & kBdyPairTerms)) {
-    const int k1 = fb.rowptr[i + 1];
-    for (int k = fb.rowptr[i] + rl.sub; k < k1; k += KL) {
-      const int j = fb.col[k];
-      const float4 r0 = brec[2 * j], r1 = brec[2 * j + 1];
-      const Nbr n = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z};
-      dsph_pair(P, K, o, n, 0.f, 0.f, false, terms, a);
+  packed_bdy_pairs<KL>(P, K, o, brec, fb, i, rl.sub, terms, a);
+  reduce_acc<KL>(a, terms);
+  if (rl.sub == 0) packed_row_out<FUSE>(P, o, a, terms, i, dpdt, acc, rk);
+}
+
+// ------------------------------------------- cell-walk delta-SPH gather
+// The packed gather with its fluid pairs found in-kernel instead of read
+// from a pair list (the default MOG delta-SPH right-hand side needs no list
+// for anything else).  A workgroup takes chunks of kWalkRows consecutive
+// sorted points of one scene.  The three stencil rows of a run of sorted
+// points are three contiguous runs of sorted points (each about the run
+// plus a cell at either end), so the chunk works in three phases:
+//  1. stage: the positions and point indices of those runs are copied into
+//     LDS, coalesced;
+//  2. test: one lane per row walks its stencil in LDS in the order the
+//     pair-list fill visits it (Stencil3, the same fp32 predicate) and
+//     writes the hits into the row's list in LDS, up to kWalkCap;
+//  3. gather: KL lanes per row walk that list as the packed gather walks a
+//     CSR row -- the same pairs, in the same order, on the same lanes -- so
+//     dp/dt and a are bit for bit those of sph_dsph_gather_packed over the
+//     list of sph_neighbor_fill.
+// A row with more hits than its list holds walks its stencil again from
+// global memory, the lanes of its group testing KL candidates at a time and
+// each lane taking the unlisted hit of its rank; a chunk whose stencil does
+// not fit in LDS (crowded cells, points outside the cell table) does all
+// its rows that way.
+constexpr int kWalkRows = kBS;                  // rows per chunk
+// staged stencil points per chunk: three stencil rows of a chunk's kWalkRows
+// points, with room for neighbouring cell rows half again as full as the
+// chunk's own (cells of one row hold 2 or 3 lattice rows at the support of
+// ~20 neighbours) and a cell at either end
+constexpr int kWalkRecs = 3 * kWalkRows * 3 / 2 + 256;
+constexpr int kWalkCap = 27;                    // listed hits per row
+constexpr int kWalkBatch = 4;                   // candidates tested per trip
+
+struct WalkShared {
+  // sorted positions of the staged runs; the test loads up to
+  // kWalkBatch - 1 slots past a run's end (no hit), hence the padding
+  float2 pos[kWalkRecs + kWalkBatch];
+  int ridx[kWalkRecs];             // their point indices (record rows)
+  unsigned short list[kWalkRows][kWalkCap + 1];   // slot kWalkCap: scratch
+  unsigned short cnt[kWalkRows];   // listed hits, kWalkCap + 1 = more
+  int wlo[kBS / kWave][3], whi[kBS / kWave][3];   // stencil runs per wavefront
+};
+
+// Wavefront min / max over the active lanes (the device library's DPP
+// reductions; the LDS atomics they replace are serialised lane by lane).
+extern "C" __device__ int __ockl_wfred_min_i32(int);
+extern "C" __device__ int __ockl_wfred_max_i32(int);
+
+// Listed fluid pairs of a row: entry k (rank k) on lane k mod KL, in
+// batches of kPackedBatch as the packed gather takes them.
+template <int KL>
+__device__ __forceinline__ void walk_list(const SphPhysics& P, const KConst& K, const Own& o,
+                                          unsigned terms, const unsigned short* lst, int n,
+                                          int sub, const int* ridx,
+                                          const float4* __restrict__ frec, Acc& a) {
+  for (int kb = sub; kb < n; kb += KL * kPackedBatch) {
+    bool ok[kPackedBatch];
+    float4 r0[kPackedBatch], r1[kPackedBatch];
+#pragma unroll
+    for (int u = 0; u < kPackedBatch; ++u) {
+      const int k = kb + u * KL;
+      ok[u] = k < n;
+      if (ok[u]) {
+        const int j = ridx[lst[k]];
+        r0[u] = frec[2 * j];
+        r1[u] = frec[2 * j + 1];
+      } else {
+        r0[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        r1[u] = r0[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kPackedBatch; ++u) {
+      if (ok[u]) {
+        const Nbr nb = {r0[u].x, r0[u].y, r0[u].z, r0[u].w, r1[u].x, r1[u].y, r1[u].z};
+        dsph_pair(P, K, o, nb, 0.f, 0.f, true, terms, a);
+      }
     }
   }
-  reduce_acc<KL>(a, terms);
-  if (rl.sub != 0) return;
-  if (!FUSE) {
-    dsph_store(P, o, a, terms, i, dpdt, acc);
-    return;
+}
+
+// Fluid pairs of rank >= skip of the rows with `slow` set, from global
+// memory: the group tests KL candidates per trip and lane L takes the hit of
+// rank congruent to L mod KL.  Every lane of the wavefront runs the loop
+// (ballots and shuffles), so call it converged.
+template <int KL>
+__device__ __forceinline__ void walk_slow(bool slow, const SphGrid& g, int sc, int sp, float thr,
+                                          int skip, int sub, const SphPhysics& P,
+                                          const KConst& K, const Own& o, unsigned terms,
+                                          const float4* __restrict__ frec, Acc& a) {
+  const float2* __restrict__ sx = reinterpret_cast<const float2*>(g.sorted_x);
+  Stencil3 st;
+  int ncand = 0;
+  float xq = 0.f, yq = 0.f;
+  if (slow) {
+    const SphGridDesc d = g.scenes[sc];
+    const float2 q = sx[sp];
+    xq = q.x;
+    yq = q.y;
+    st.init(d, g.cell_start + d.cell_begin, cell_coord(xq, d.qmin_x, d.hmax),
+            cell_coord(yq, d.qmin_y, d.hmax));
+    ncand = st.ncand;
   }
-  float dp, ax, ay;
-  dsph_rates(P, o, a, terms, dp, ax, ay);
-  const Rk4State s0 = {rk.x0[2 * i], rk.x0[2 * i + 1], rk.v0[2 * i], rk.v0[2 * i + 1], rk.r0[i]};
-  float2 asum;
-  const Rk4State n = rk4_particle(rk.stage, P, ax, ay, o.vx, o.vy, dp, true,
-                                  rk.K + 5 * (size_t)i, s0, &asum);
-  st2(rk.x_out, i, make_float2(n.x, n.y));
-  st2(rk.v_out, i, make_float2(n.vx, n.vy));
-  rk.rho_out[i] = n.r;
-  if (rk.stage == 4) {
-    if (rk.acc_out) st2(rk.acc_out, i, asum);
-  } else if (rk.frec_next) {
-    pack_fluid(reinterpret_cast<float4*>(rk.frec_next), i, P, n.x, n.y, n.vx, n.vy, n.r,
-               rk.area[i] / n.r);
+  const unsigned long long gmask = group_mask<KL>();
+  const int lane = threadIdx.x & (kWave - 1);
+  int base = 0;   // rank of the trip's first hit
+  for (int p0 = 0; __ballot(slow && p0 < ncand) != 0; p0 += KL) {
+    const int p = p0 + sub;
+    bool hit = false;
+    int slot = 0;
+    if (slow && p < ncand) {
+      slot = st.slot(p);
+      const float2 xy = sx[slot];
+      hit = dist2_nofma(xq, yq, xy.x, xy.y) <= thr;
+    }
+    const unsigned long long m = __ballot(hit) & gmask;
+    const int nh = __popcll(m);
+    const int want = (sub - base) & (KL - 1);   // rank of this lane's hit in the trip
+    unsigned long long mm = m;
+    for (int i = 0; i < want && mm; ++i) mm &= mm - 1;
+    const int src = mm ? __ffsll((long long)mm) - 1 : lane;
+    const int js = __shfl(slot, src, kWave);
+    if (slow && want < nh && base + want >= skip) {
+      const int j = g.sorted_idx[js];
+      const float4 r0 = frec[2 * j], r1 = frec[2 * j + 1];
+      const Nbr nb = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z};
+      dsph_pair(P, K, o, nb, 0.f, 0.f, true, terms, a);
+    }
+    base += nh;
+  }
+}
+
+// segs[b] = {first, end} sorted slots of workgroup b (one scene each).
+template <unsigned T, int KL, bool FUSE>
+__global__ __launch_bounds__(kBS) void k_dsph_cellwalk(
+    SphGrid g, const int2* __restrict__ segs, const float4* __restrict__ frec,
+    const float4* __restrict__ brec, const SphPhysics* __restrict__ phys, SphCSR fb,
+    unsigned rterms, float* __restrict__ dpdt, float* __restrict__ acc, SphRk4Stage rk) {
+  const unsigned terms = T ? T : rterms;
+  __shared__ WalkShared S;
+  const int2 seg = segs[xcd_block()];
+  const int s0 = seg.x > 0 ? seg.x : 0, s1 = seg.y < g.np ? seg.y : g.np;
+  const int t = threadIdx.x;
+  const float2* __restrict__ sx = reinterpret_cast<const float2*>(g.sorted_x);
+  for (int c0 = s0; c0 < s1; c0 += kWalkRows) {
+    // ---- the stencil of row t (one lane per row)
+    const int sp = c0 + t;
+    const bool valid = sp < s1;
+    Stencil3 st;
+    float xq = 0.f, yq = 0.f, thr = 0.f;
+    int rlo[3] = {INT_MAX, INT_MAX, INT_MAX}, rhi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    if (valid) {
+      const int qi = g.sorted_idx[sp];
+      const int sc = g.nscenes == 1 ? 0 : g.pscene[qi];
+      const SphGridDesc d = g.scenes[sc];
+      const float2 q = sx[sp];
+      xq = q.x;
+      yq = q.y;
+      st.init(d, g.cell_start + d.cell_begin, cell_coord(xq, d.qmin_x, d.hmax),
+              cell_coord(yq, d.qmin_y, d.hmax));
+      thr = sq_threshold(phys[sc].h);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        if (st.ra[r] < st.rb[r]) {
+          rlo[r] = st.ra[r];
+          rhi[r] = st.rb[r];
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      rlo[r] = __ockl_wfred_min_i32(rlo[r]);
+      rhi[r] = __ockl_wfred_max_i32(rhi[r]);
+    }
+    if ((t & (kWave - 1)) == 0) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        S.wlo[t / kWave][r] = rlo[r];
+        S.whi[t / kWave][r] = rhi[r];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kBS / kWave; ++w) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        rlo[r] = S.wlo[w][r] < rlo[r] ? S.wlo[w][r] : rlo[r];
+        rhi[r] = S.whi[w][r] > rhi[r] ? S.whi[w][r] : rhi[r];
+      }
+    }
+    // ---- stage: stencil row r's run [lo_r, hi_r) at LDS offset off_r (a
+    // row that no point of the chunk has is empty at lo_r = 0)
+    const bool e0 = rhi[0] > rlo[0], e1 = rhi[1] > rlo[1], e2 = rhi[2] > rlo[2];
+    const int lo0 = e0 ? rlo[0] : 0, lo1 = e1 ? rlo[1] : 0, lo2 = e2 ? rlo[2] : 0;
+    const int off1 = e0 ? rhi[0] - lo0 : 0;
+    const int off2 = off1 + (e1 ? rhi[1] - lo1 : 0);
+    const int tot = off2 + (e2 ? rhi[2] - lo2 : 0);
+    const bool staged = tot <= kWalkRecs;
+    if (staged) {
+      for (int k = t; k < tot; k += kBS) {
+        const int p = k < off1 ? lo0 + k : (k < off2 ? lo1 + (k - off1) : lo2 + (k - off2));
+        S.pos[k] = sx[p];
+        S.ridx[k] = g.sorted_idx[p];
+      }
+    }
+    __syncthreads();
+    // ---- test: row t's hits, in candidate order
+    if (valid && staged) {
+      unsigned short* lst = S.list[t];
+      int cnt = 0;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const int lo = r == 0 ? lo0 : (r == 1 ? lo1 : lo2);
+        const int off = r == 0 ? 0 : (r == 1 ? off1 : off2);
+        const int b = off + (st.ra[r] - lo), e = b + (st.rb[r] - st.ra[r]);
+        // kWalkBatch LDS loads in flight per trip, no branch per candidate
+        for (int k = b; k < e; k += kWalkBatch) {
+          float2 xy[kWalkBatch];
+#pragma unroll
+          for (int u = 0; u < kWalkBatch; ++u) xy[u] = S.pos[k + u];
+#pragma unroll
+          for (int u = 0; u < kWalkBatch; ++u) {
+            const float d2 = dist2_nofma(xq, yq, xy[u].x, xy[u].y);
+            const bool hit = (d2 <= thr) & (k + u < e);
+            lst[cnt < kWalkCap ? cnt : kWalkCap] = (unsigned short)(k + u);
+            cnt += hit ? 1 : 0;
+          }
+        }
+      }
+      S.cnt[t] = (unsigned short)(cnt > kWalkCap ? kWalkCap + 1 : cnt);
+    }
+    __syncthreads();
+    // ---- gather: KL lanes per row
+    const int nrow = s1 - c0 < kWalkRows ? s1 - c0 : kWalkRows;
+    const int sub = t % KL;
+    for (int rr = t / KL; rr < nrow; rr += kBS / KL) {
+      const int spr = c0 + rr;
+      const int qi = g.sorted_idx[spr];
+      const int sc = g.nscenes == 1 ? 0 : g.pscene[qi];
+      const SphPhysics P = phys[sc];
+      const KConst K = kconst(P.h);
+      const float4 a0 = frec[2 * qi], a1 = frec[2 * qi + 1];
+      Own o;
+      o.x = a0.x; o.y = a0.y; o.vx = a0.z; o.vy = a0.w;
+      o.rho = a1.x;
+      o.p = a1.z;
+      o.gx = 0.f;
+      o.gy = 0.f;
+      Acc a = {};
+      if (terms & kFluidPairTerms) {
+        const int n = staged ? S.cnt[rr] : 0;
+        walk_list<KL>(P, K, o, terms, S.list[rr], n < kWalkCap ? n : kWalkCap, sub, S.ridx,
+                      frec, a);
+        const bool slow = !staged || n > kWalkCap;
+        if (__ballot(slow) != 0)
+          walk_slow<KL>(slow, g, sc, spr, sq_threshold(P.h), staged ? kWalkCap : 0, sub, P, K,
+                        o, terms, frec, a);
+      }
+      packed_bdy_pairs<KL>(P, K, o, brec, fb, qi, sub, terms, a);
+      reduce_acc<KL>(a, terms);
+      if (sub == 0) packed_row_out<FUSE>(P, o, a, terms, qi, dpdt, acc, rk);
+    }
+    __syncthreads();
   }
 }
 
@@ -804,18 +1087,26 @@ int dsph_gather_packed_dplus(const float* frec, const float* brec, const int* ps
   return launch_check("sph_dsph_gather_packed_dplus");
 }
 
+static int check_rk4(const SphRk4Stage* rk, const float* frec, unsigned terms,
+                     const std::string& who) {
+  SPH_REQUIRE(rk, who + ": bad arguments");
+  SPH_REQUIRE(rk->stage >= 1 && rk->stage <= 4, who + ": stage must be 1..4");
+  SPH_REQUIRE(rk->x0 && rk->v0 && rk->r0 && rk->K && rk->x_out && rk->v_out && rk->rho_out,
+              who + ": missing state arrays");
+  SPH_REQUIRE(!rk->frec_next || (rk->area && rk->frec_next != frec),
+              who + ": next records need the areas and a buffer of their own");
+  SPH_REQUIRE(!(terms & SPH_T_ACCUM), who + ": SPH_T_ACCUM has no meaning here");
+  return SPH_OK;
+}
+
 int dsph_gather_rk4(const float* frec, const float* brec, const int* pscene, int np,
                     const SphPhysics* phys, const SphCSR* ff, const SphCSR* fb,
                     const float* gradrho, unsigned terms, const int* order, const SphRk4Stage* rk,
                     hipStream_t s) {
   SPH_REQUIRE(frec && pscene && phys && ff && ff->rowptr && np >= 0 && rk,
               "sph_dsph_gather_rk4: bad arguments");
-  SPH_REQUIRE(rk->stage >= 1 && rk->stage <= 4, "sph_dsph_gather_rk4: stage must be 1..4");
-  SPH_REQUIRE(rk->x0 && rk->v0 && rk->r0 && rk->K && rk->x_out && rk->v_out && rk->rho_out,
-              "sph_dsph_gather_rk4: missing state arrays");
-  SPH_REQUIRE(!rk->frec_next || (rk->area && rk->frec_next != frec),
-              "sph_dsph_gather_rk4: next records need the areas and a buffer of their own");
-  SPH_REQUIRE(!(terms & SPH_T_ACCUM), "sph_dsph_gather_rk4: SPH_T_ACCUM has no meaning here");
+  int rc = check_rk4(rk, frec, terms, "sph_dsph_gather_rk4");
+  if (rc) return rc;
   SPH_REQUIRE(!(terms & SPH_T_DDSPH) || gradrho, "sph_dsph_gather_rk4: DDSPH needs gradrho");
   SPH_REQUIRE(!(terms & kBdyPairTerms) || !fb || !fb->rowptr || brec,
               "sph_dsph_gather_rk4: boundary terms without boundary records");
@@ -824,6 +1115,63 @@ int dsph_gather_rk4(const float* frec, const float* brec, const int* pscene, int
                       pscene, np, phys, *ff, csr_or_null(fb), gradrho, terms, order, nullptr,
                       nullptr, *rk, s);
   return launch_check("sph_dsph_gather_rk4");
+}
+
+template <bool FUSE>
+static void launch_cellwalk(const SphGrid& g, const int* segs, int nseg, const float* frec,
+                            const float* brec, const SphPhysics* phys, const SphCSR& fb,
+                            unsigned terms, float* dpdt, float* acc, const SphRk4Stage& rk,
+                            hipStream_t s) {
+  const int2* sg = reinterpret_cast<const int2*>(segs);
+  const float4* fr = reinterpret_cast<const float4*>(frec);
+  const float4* br = reinterpret_cast<const float4*>(brec);
+  with_lanes(tuning(SPH_TUNE_WALK_LANES), [&](auto kl) {
+    constexpr int KL = decltype(kl)::value;
+    if (terms == kMaskMOG)
+      hipLaunchKernelGGL((k_dsph_cellwalk<kMaskMOG, KL, FUSE>), dim3(nseg), dim3(kBS), 0, s, g,
+                         sg, fr, br, phys, fb, terms, dpdt, acc, rk);
+    else
+      hipLaunchKernelGGL((k_dsph_cellwalk<0, KL, FUSE>), dim3(nseg), dim3(kBS), 0, s, g, sg, fr,
+                         br, phys, fb, terms, dpdt, acc, rk);
+  });
+}
+
+static int check_cellwalk(const SphGrid* g, const int* segs, int nseg, const float* frec,
+                          const float* brec, const SphPhysics* phys, const SphCSR* fb,
+                          unsigned terms, const std::string& who) {
+  SPH_REQUIRE(g && g->scenes && g->sorted_idx && g->sorted_x && g->cell_start && g->np >= 0 &&
+                  (g->nscenes == 1 || g->pscene),
+              who + ": the grid is not built");
+  SPH_REQUIRE(nseg >= 0 && (nseg == 0 || segs) && frec && phys, who + ": bad arguments");
+  SPH_REQUIRE(!(terms & SPH_T_DDSPH),
+              who + ": the DDSPH term needs its pair list (sph_dsph_gather_packed)");
+  SPH_REQUIRE(!(terms & kBdyPairTerms) || !fb || !fb->rowptr || brec,
+              who + ": boundary terms without boundary records");
+  return SPH_OK;
+}
+
+int dsph_cellwalk(const SphGrid* g, const int* segs, int nseg, const float* frec,
+                  const float* brec, const SphPhysics* phys, const SphCSR* fb, unsigned terms,
+                  float* dpdt, float* acc, hipStream_t s) {
+  int rc = check_cellwalk(g, segs, nseg, frec, brec, phys, fb, terms, "sph_dsph_cellwalk");
+  if (rc) return rc;
+  if (nseg == 0 || g->np == 0) return SPH_OK;
+  const SphRk4Stage none = {};
+  launch_cellwalk<false>(*g, segs, nseg, frec, brec, phys, csr_or_null(fb), terms, dpdt, acc,
+                         none, s);
+  return launch_check("sph_dsph_cellwalk");
+}
+
+int dsph_cellwalk_rk4(const SphGrid* g, const int* segs, int nseg, const float* frec,
+                      const float* brec, const SphPhysics* phys, const SphCSR* fb,
+                      unsigned terms, const SphRk4Stage* rk, hipStream_t s) {
+  int rc = check_cellwalk(g, segs, nseg, frec, brec, phys, fb, terms, "sph_dsph_cellwalk_rk4");
+  if (rc) return rc;
+  if ((rc = check_rk4(rk, frec, terms, "sph_dsph_cellwalk_rk4"))) return rc;
+  if (nseg == 0 || g->np == 0) return SPH_OK;
+  launch_cellwalk<true>(*g, segs, nseg, frec, brec, phys, csr_or_null(fb), terms, nullptr,
+                        nullptr, *rk, s);
+  return launch_check("sph_dsph_cellwalk_rk4");
 }
 
 int dsph_normalization(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
@@ -929,6 +1277,16 @@ int sph_dsph_gather_rk4(const float* frec, const float* brec, const int* pscene,
                         const float* gradrho, unsigned terms, const int* order,
                         const SphRk4Stage* rk, hipStream_t s) {
   return sph::dsph_gather_rk4(frec, brec, pscene, np, phys, ff, fb, gradrho, terms, order, rk, s);
+}
+int sph_dsph_cellwalk(const SphGrid* g, const int* segs, int nseg, const float* frec,
+                      const float* brec, const SphPhysics* phys, const SphCSR* fb, unsigned terms,
+                      float* dpdt, float* acc, hipStream_t s) {
+  return sph::dsph_cellwalk(g, segs, nseg, frec, brec, phys, fb, terms, dpdt, acc, s);
+}
+int sph_dsph_cellwalk_rk4(const SphGrid* g, const int* segs, int nseg, const float* frec,
+                          const float* brec, const SphPhysics* phys, const SphCSR* fb,
+                          unsigned terms, const SphRk4Stage* rk, hipStream_t s) {
+  return sph::dsph_cellwalk_rk4(g, segs, nseg, frec, brec, phys, fb, terms, rk, s);
 }
 int sph_dsph_normalization(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
                            const SphBoundary* b, const SphCSR* fb, const int* order, float* M,

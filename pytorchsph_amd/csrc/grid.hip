@@ -155,55 +155,6 @@ __global__ __launch_bounds__(kBS) void k_grid_cell_sort(SphGrid g) {
   for (int i = a; i < b; ++i) sx[i] = x[idx[i]];
 }
 
-// Candidate range [a, b) of sorted points in stencil row dy of a query in
-// cell (ix, iy): cells qx in [max(ix - R, 0), ix + R] of row qy = iy + dy,
-// skipped when qy < 0 (queryHashMap's negative-index rule,
-// neighSearch.cpp:95), keys past the scene's table clipped.  Returns false
-// when the row, and every later one, lies past the table.
-__device__ __forceinline__ bool stencil_row(const SphGridDesc& d, const int* __restrict__ cs,
-                                            int ix, int iy, int dy, int R, int& a, int& b) {
-  a = b = 0;
-  const int qy = iy + dy;
-  if (qy < 0) return true;
-  const int qx0 = ix - R < 0 ? 0 : ix - R;
-  const int qx1 = ix + R;
-  if (qx1 < qx0) return true;
-  const int64_t ncell = (int64_t)d.cells_x * d.cells_y;
-  const int64_t k0 = qx0 + (int64_t)d.cells_x * qy;
-  int64_t k1 = qx1 + (int64_t)d.cells_x * qy;
-  if (k0 >= ncell) return false;
-  if (k1 >= ncell) k1 = ncell - 1;
-  a = cs[k0];
-  b = cs[k1 + 1];
-  return true;
-}
-
-// The radius-1 stencil of a query as one sequence of candidates: rows
-// dy = -1, 0, 1 are three contiguous runs of sorted points, and candidate p
-// is the p-th point of their concatenation.
-struct Stencil3 {
-  int ra[3], rb[3];
-  int n0, n01, ncand;
-  __device__ __forceinline__ void init(const SphGridDesc& d, const int* __restrict__ cs, int ix,
-                                       int iy) {
-    bool more = true;
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      if (more) more = stencil_row(d, cs, ix, iy, r - 1, 1, ra[r], rb[r]);
-      if (!more) ra[r] = rb[r] = 0;
-    }
-    n0 = rb[0] - ra[0];
-    n01 = n0 + rb[1] - ra[1];
-    ncand = n01 + rb[2] - ra[2];
-  }
-  // Sorted slot of candidate p, p clamped into [0, ncand) (ncand > 0) so a
-  // batch of loads needs no guard.
-  __device__ __forceinline__ int slot(int p) const {
-    p = p < ncand ? p : ncand - 1;
-    return p < n0 ? ra[0] + p : (p < n01 ? ra[1] + (p - n0) : ra[2] + (p - n01));
-  }
-};
-
 // Count (FILL = false) or emit (FILL = true) the pairs of every query.
 // Stencil rows are walked bottom to top and each row is one contiguous range
 // of sorted points, so the columns of a row come out in sorted-point order.

@@ -31,7 +31,7 @@ int launch_check(const char* where) {
 // Defaults (the shapes the autotuner picks on a 256-scene mixed batch):
 // lanes per row of the neighbour count, the gathers, the DFSPH solve and the
 // neighbour fill.
-static std::atomic<int> g_tuning[SPH_TUNE_COUNT] = {{2}, {4}, {2}, {8}};
+static std::atomic<int> g_tuning[SPH_TUNE_COUNT] = {{2}, {4}, {2}, {8}, {1}};
 
 int tuning(int knob) { return g_tuning[knob].load(std::memory_order_relaxed); }
 

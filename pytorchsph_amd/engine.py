@@ -50,6 +50,19 @@ def eos_host(phys: dict, rho) -> np.ndarray:
     return phys["kappa"] * (r * rho0 - rho0)
 
 
+def walk_segments(p_begin, n, run: int = 256) -> np.ndarray:
+    """Runs of at most `run` sorted slots of one scene each, covering every
+    scene's slots [p_begin, p_begin + n) in order: the workgroups of
+    sph_dsph_cellwalk.  [nseg, 2] int32 {first, end}."""
+    parts = []
+    for a, c in zip(p_begin, n):
+        st = np.arange(int(a), int(a) + int(c), run, dtype=np.int64)
+        parts.append(np.stack([st, np.minimum(st + run, int(a) + int(c))], 1))
+    if not parts:
+        return np.zeros((0, 2), np.int32)
+    return np.ascontiguousarray(np.concatenate(parts).astype(np.int32))
+
+
 def phys_records(scenes) -> np.ndarray:
     rec = np.zeros(len(scenes), dtype=L.PHYS_DTYPE)
     for s, sc in enumerate(scenes):
@@ -81,13 +94,19 @@ class SceneBatch:
 
     def __init__(self, scenes: list[Scene], device="cuda", pair_cap: float = 40.0,
                  bdy_pair_cap: float = 40.0, dfsph_chunk: int = 8, reorder: bool = True,
-                 carry: list | None = None, fuse_rk4: bool = False):
+                 carry: list | None = None, fuse_rk4: bool = False, cellwalk: bool = True):
         """fuse_rk4: delta-SPH stages run the RK4 combine and the next
         stage's records inside the gather (sph_dsph_gather_rk4) instead of
         as separate kernels; the results are the same, bit for bit.  Off by
         default: the fused gather needs 67 registers instead of 60 (seven
         wavefronts per SIMD instead of eight) and the step takes about as
         long.
+
+        cellwalk: delta-SPH stages with the default terms (MOG diffusion, no
+        delta-plus) find their fluid pairs inside the gather
+        (sph_dsph_cellwalk) and build no fluid pair list; the results are
+        those of the list gather, bit for bit.  The boundary lists are built
+        either way.
 
         carry[k], when given, continues scene k from another batch: a dict
         from `export` with device tensors of its positions, velocities and
@@ -176,6 +195,14 @@ class SceneBatch:
             if not d0.get("boundary_diffusion", True):
                 self.terms &= ~L.T_VISC_B
 
+        self.cellwalk = (cellwalk and scheme == "dsph" and not (self.dplus or self.shifting)
+                         and not self.terms & L.T_DDSPH)
+        if self.cellwalk:
+            # the cell walk's pair predicate takes the support from the
+            # physics records; it must be the support the grid was sized for
+            assert all(float(np.float32(ph["h"])) == float(np.float32(s.support))
+                       for ph, s in zip(self.phys_host, scenes))
+
         # ---- neighbour search
         grid_scenes = [dict(vmin=s.vmin, vmax=s.vmax, hmax=s.support, p_begin=int(pb),
                             p_count=int(s.n)) for s, pb in zip(scenes, self.p_begin)]
@@ -186,6 +213,9 @@ class SceneBatch:
         self.gf = _PairBuf(NB, bcap, dev) if self.solid else None
         self.fb = _PairBuf(NP, bcap, dev)
         self.t_slot = torch.zeros(bcap, dtype=torch.int32, device=dev)
+        segs = walk_segments(self.p_begin, self.n)
+        self.nseg = len(segs)
+        self.segs = torch.from_numpy(segs).to(dev) if self.cellwalk and self.nseg else None
         self.scan_ws = torch.zeros(scan_ws_bytes(max(NP, NB, 1)), dtype=torch.uint8, device=dev)
         # ---- step buffers
         self.acc = torch.zeros((NP, 2), dtype=f32, device=dev)
@@ -427,11 +457,13 @@ class SceneBatch:
                L.ptr(self.scan_ws), int(self.scan_ws.numel()), L.stream_handle(s))
         g.fill(q, buf.rowptr, buf.col, s)
 
-    def search(self, x, s=None):
-        """Grid + every pair list of the current positions."""
+    def search(self, x, s=None, fluid_pairs: bool = True):
+        """Grid + every pair list of the current positions (without the
+        fluid list when not fluid_pairs)."""
         g = self.grid
         g.build(x, self.pscene, s)
-        self._pairs(self.ff, g.query_struct(x, self.h, self.pscene, 1, True), s)
+        if fluid_pairs:
+            self._pairs(self.ff, g.query_struct(x, self.h, self.pscene, 1, True), s)
         if self.NB:
             if self.solid:
                 self._pairs(self.gf, g.query_struct(self.bgx, self.bh, self.bscene, 1, False), s)
@@ -466,7 +498,7 @@ class SceneBatch:
         """Search, mDBC boundary density, delta-SPH density gradient and the
         records (fluid records into frec unless it is None) of the state
         (x, v, rho).  Returns the gather's arguments."""
-        self.search(x, s)
+        self.search(x, s, fluid_pairs=not self.cellwalk)
         F = self.fluid(x, v, rho)
         sh = L.stream_handle(s)
         ff = self.ff.csr().c()
@@ -501,6 +533,12 @@ class SceneBatch:
                    self.terms, self._order(), L.ptr(self.smask), L.ptr(self.dpdt),
                    L.ptr(self.acc), L.stream_handle(s))
             return
+        if self.cellwalk:
+            g = self.grid._grid()
+            L.call("sph_dsph_cellwalk", C.byref(g), L.ptr(self.segs), self.nseg,
+                   L.ptr(self.frec), L.ptr(self.brec), L.ptr(self.phys), fbp, self.terms,
+                   L.ptr(self.dpdt), L.ptr(self.acc), L.stream_handle(s))
+            return
         L.call("sph_dsph_gather_packed", L.ptr(self.frec), L.ptr(self.brec), L.ptr(self.pscene),
                self.NP, L.ptr(self.phys), C.byref(ff), fbp, grr, self.terms, self._order(),
                L.ptr(self.dpdt), L.ptr(self.acc), L.stream_handle(s))
@@ -522,6 +560,12 @@ class SceneBatch:
                                    L.ptr(self.area), L.ptr(self.K), L.ptr(out[0]), L.ptr(out[1]),
                                    L.ptr(out[2]), L.ptr(self.acc) if k == 4 else None,
                                    L.ptr(nxt) if k < 4 else None)
+                if self.cellwalk:
+                    g = self.grid._grid()
+                    L.call("sph_dsph_cellwalk_rk4", C.byref(g), L.ptr(self.segs), self.nseg,
+                           L.ptr(frec), L.ptr(self.brec), L.ptr(self.phys), fbp, self.terms,
+                           C.byref(rk), sh)
+                    continue
                 L.call("sph_dsph_gather_rk4", L.ptr(frec), L.ptr(self.brec), L.ptr(self.pscene),
                        self.NP, L.ptr(self.phys), C.byref(ff), fbp,
                        L.ptr(self.gradrho) if self.terms & L.T_DDSPH else None, self.terms,
@@ -692,7 +736,8 @@ class SceneBatch:
         st = self.status()
         if (st & L.ST_OVERFLOW).any():
             raise RuntimeError("pair-list capacity exceeded; rebuild the batch with a larger cap")
-        nnz = int(self.ff.rowptr[self.NP].item())
+        # cell-walk steps build no fluid list
+        nnz = None if self.cellwalk else int(self.ff.rowptr[self.NP].item())
         return st, nnz
 
     def scene_state(self, k: int, extra: bool = False) -> dict:

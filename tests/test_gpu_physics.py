@@ -30,8 +30,11 @@ def perturbed_dsph_scene(dx=0.01, seed=0, diffusion="MOG"):
 
 
 def run_rhs(sc, rho):
+    """The engine's right-hand side (the cell walk for MOG scenes), with every
+    pair list of the state built as well."""
     b = SceneBatch([sc])
     b.rho.copy_(torch.from_numpy(rho.astype(np.float32)))
+    b.search(b.x)
     b.dsph_rhs(b.x, b.v, b.rho)
     torch.cuda.synchronize()
     return b

@@ -47,7 +47,7 @@ def test_dsph_rhs_lanes(cuda, kl):
     sc = perturbed_dsph_scene(dx=0.01, seed=2)
     rng = np.random.default_rng(5)
     rho = (1.0 + rng.uniform(-2e-3, 4e-3, sc.n)).astype(np.float32)
-    with tuned(gather=kl, search=kl):
+    with tuned(gather=kl, search=kl, walk=kl):
         b = SceneBatch([sc])
         b.rho.copy_(torch.from_numpy(rho))
         b.dsph_rhs(b.x, b.v, b.rho)
