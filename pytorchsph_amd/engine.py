@@ -94,13 +94,16 @@ class SceneBatch:
 
     def __init__(self, scenes: list[Scene], device="cuda", pair_cap: float = 40.0,
                  bdy_pair_cap: float = 40.0, dfsph_chunk: int = 8, reorder: bool = True,
-                 carry: list | None = None, fuse_rk4: bool = False, cellwalk: bool = True):
+                 carry: list | None = None, fuse_rk4: bool | None = None,
+                 cellwalk: bool = True):
         """fuse_rk4: delta-SPH stages run the RK4 combine and the next
-        stage's records inside the gather (sph_dsph_gather_rk4) instead of
-        as separate kernels; the results are the same, bit for bit.  Off by
-        default: the fused gather needs 67 registers instead of 60 (seven
-        wavefronts per SIMD instead of eight) and the step takes about as
-        long.
+        stage's records inside the gather (sph_dsph_gather_rk4 or
+        sph_dsph_cellwalk_rk4) instead of as separate kernels; the results
+        are the same, bit for bit.  None (the default): on for cell-walk
+        batches, where it saves the pack and combine launches of every stage
+        (0.21 ms of the 4M-particle dam-break step), off for list batches,
+        where the fused gather's extra registers (67 instead of 60, seven
+        wavefronts per SIMD instead of eight) cost what the launches save.
 
         cellwalk: delta-SPH stages with the default terms (MOG diffusion, no
         delta-plus) find their fluid pairs inside the gather
@@ -129,7 +132,6 @@ class SceneBatch:
         self.solid = scenes[0].boundary == "solid"
         self.dfsph_chunk = dfsph_chunk
         self.reorder = reorder
-        self.fuse_rk4 = fuse_rk4
         self._sorted_valid = False
         f32 = torch.float32
 
@@ -197,6 +199,7 @@ class SceneBatch:
 
         self.cellwalk = (cellwalk and scheme == "dsph" and not (self.dplus or self.shifting)
                          and not self.terms & L.T_DDSPH)
+        self.fuse_rk4 = self.cellwalk if fuse_rk4 is None else bool(fuse_rk4)
         if self.cellwalk:
             # the cell walk's pair predicate takes the support from the
             # physics records; it must be the support the grid was sized for
@@ -227,11 +230,11 @@ class SceneBatch:
                       for _ in range(2)]
             self.K = torch.zeros((NP, 5), dtype=f32, device=dev)
             # records of the next RK4 stage while the gather reads this one's
-            self.frec2 = torch.zeros_like(self.frec) if fuse_rk4 else None
+            self.frec2 = torch.zeros_like(self.frec) if self.fuse_rk4 else None
             self.gradrho = torch.zeros((NP, 2), dtype=f32, device=dev)
             self.Lmat = torch.zeros((NP, 4), dtype=f32, device=dev)
             if self.dplus or self.shifting:
-                assert self.solid and not fuse_rk4, \
+                assert self.solid and not self.fuse_rk4, \
                     "delta-plus runs with mDBC walls and the separate RK4 kernels"
                 self._alloc_dplus()
         else:
