@@ -27,8 +27,9 @@ import numpy as np
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=200,
+                    help="untimed steps before the timed window (the scenes flow first)")
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -49,7 +50,7 @@ def parse():
     ap.add_argument("--dplus-only", action="store_true",
                     help="only the delta-plus surface detection and shift timings (profiling)")
     ap.add_argument("--roofline-only", action="store_true",
-                    help="only the kernel timings and the copy calibration (counter runs)")
+                    help="only the kernel timings and the calibration kernels (counter runs)")
     return ap.parse_args()
 
 
@@ -123,11 +124,13 @@ def pressure_roofline(b, reps=20):
 
 
 def gather_roofline(b, reps=20):
-    """sph_dsph_gather_packed (k_dsph_gather_packed, all delta-SPH terms),
-    the gather of every RK4 stage, and its algorithmic HBM bytes: per fluid
-    particle its two 16-byte records (32 B), scene id (4 B), fluid and
-    boundary row pointers (8 B), dpdt and acc written (12 B); per boundary
-    particle its records (32 B); 4 B per pair (column index)."""
+    """sph_dsph_gather_packed (k_dsph_gather_packed, all delta-SPH terms)
+    over the fluid pair list, the gather the cell walk replaces, and its
+    algorithmic HBM bytes (each array once): per fluid row x, v, rho, A (24),
+    dpdt and a written (12) and its fluid and boundary row pointers (8); per
+    pair its column (4); per boundary point x, rho_b, V_b, p_b (20).  Its
+    packed records (32 B per fluid row, 32 per boundary point) and the scene
+    ids (4 per row) are reported apart as `record_bytes`."""
     import ctypes as C
 
     import torch
@@ -150,9 +153,73 @@ def gather_roofline(b, reps=20):
                L.ptr(b.dpdt), L.ptr(b.acc), sh)
 
     sec = _events_avg(launch, stream, reps)
-    nbytes = 56 * b.NP + 32 * b.NB + 4 * (nnz + nnz_b)
+    nbytes = 44 * b.NP + 20 * b.NB + 4 * (nnz + nnz_b)
     return dict(kernel="k_dsph_gather_packed (sph_dsph_gather_packed)", seconds=sec,
-                bytes=nbytes, rows=b.NP, pairs=nnz + nnz_b, achieved_gbs=nbytes / sec / 1e9)
+                bytes=nbytes, record_bytes=36 * b.NP + 32 * b.NB, rows=b.NP,
+                pairs=nnz + nnz_b, achieved_gbs=nbytes / sec / 1e9)
+
+
+def cellwalk_roofline(b, reps=20):
+    """sph_dsph_cellwalk_rk4 (k_dsph_cellwalk with the MOG delta-SPH terms
+    and the RK4 stage fused), the dominant kernel of the delta-SPH step:
+    stage 1 of the current state, with its algorithmic HBM bytes (each array
+    once):
+      per fluid row     x, v, rho, A (24) + the stencil's sorted positions
+                        and point indices (12) + the RK4 stage combine (52:
+                        x, v, rho of the step start 20, running sums 12,
+                        new state 20) + the fluid -> boundary row pointer (4)
+      per cell          its start in the cell table (4)
+      per boundary pair its column (4); per boundary point its record (32)
+    The kernel reads each fluid neighbour through a 32-byte packed record
+    {x, y, vx, vy, rho, V, p, rho_hat}; those bytes (32 per fluid row) are
+    reported apart as `record_bytes`."""
+    import ctypes as C
+
+    import torch
+
+    from pytorchsph_amd import _lib as L
+
+    assert b.cellwalk and b.fuse_rk4, "a cell-walk batch with the fused stage"
+    _, fbp = b._dsph_prepare(b.x, b.v, b.rho, b.frec)
+    torch.cuda.synchronize()
+    nnz_b = int(b.fb.rowptr[b.NP].item()) if b.NB else 0
+    stream = torch.cuda.current_stream()
+    sh = L.stream_handle(stream)
+    s0 = b.S[0]
+    rk = L.SphRk4Stage(1, L.ptr(b.x), L.ptr(b.v), L.ptr(b.rho), L.ptr(b.area), L.ptr(b.K),
+                       L.ptr(s0[0]), L.ptr(s0[1]), L.ptr(s0[2]), None, L.ptr(b.frec2))
+    g = b.grid._grid()
+
+    def launch():
+        L.call("sph_dsph_cellwalk_rk4", C.byref(g), L.ptr(b.segs), b.nseg, L.ptr(b.frec),
+               L.ptr(b.brec), L.ptr(b.phys), fbp, b.terms, C.byref(rk), sh)
+
+    sec = _events_avg(launch, stream, reps)
+    nbytes = (24 + 12 + 52 + 4) * b.NP + 4 * b.grid.ncells + 4 * nnz_b + 32 * b.NB
+    return dict(kernel="k_dsph_cellwalk (sph_dsph_cellwalk_rk4)", seconds=sec, bytes=nbytes,
+                record_bytes=32 * b.NP, rows=b.NP, boundary_pairs=nnz_b,
+                achieved_gbs=nbytes / sec / 1e9)
+
+
+def read_write_calibration(n_ints=1 << 26):
+    """Two more kernels of known traffic for the counter scale factors: a
+    read-only pass (k_scan_partials of sph_scan_exclusive reads 256 MiB of
+    counts and writes 128 KiB of partial sums) and a write-only pass
+    (torch's fill of 256 MiB)."""
+    import torch
+
+    from pytorchsph_amd import _lib as L
+    from pytorchsph_amd.search import scan
+
+    cnt = torch.ones(n_ints, dtype=torch.int32, device="cuda")
+    out = torch.empty(n_ints + 1, dtype=torch.int32, device="cuda")
+    for _ in range(3):
+        scan(cnt, n_ints, out)
+    fill = torch.empty(n_ints, dtype=torch.float32, device="cuda")
+    for _ in range(3):
+        fill.fill_(2.0)
+    torch.cuda.synchronize()
+    return dict(read_only_bytes=4 * n_ints, write_only_bytes=4 * n_ints)
 
 
 def copy_calibration(n_floats=1 << 26):
@@ -171,20 +238,38 @@ def copy_calibration(n_floats=1 << 26):
     return 4 * n_floats
 
 
-def traffic_from_profile(kernel: str):
-    """HBM bytes per launch of `kernel` as measured with rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration), read from the
-    committed summary of that measurement; None when there is none."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "k_dsph_gather_packed_and_k_pressure_traffic_c4.md")
+TRAFFIC_PROFILE = "profiles/roofline_kernels_traffic_c4.md"
+ISSUE_PROFILE = "profiles/roofline_kernels_counters_c4.md"
+
+
+def _profile_value(rel: str, key: str, kernel: str):
+    """`key=<number> kernel=<name>` from a committed counter summary (the
+    rocprofv3 runs of `bench.py --roofline-only` it records); None when the
+    file or the line is absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), rel)
     if not os.path.exists(path):
         return None
     want = kernel.split()[0]
     with open(path) as fh:
         for line in fh:
-            if line.startswith("traffic_bytes_per_launch") and f"kernel={want}" in line:
-                return int(line.split("=")[1].split()[0])
+            if line.startswith(key + "=") and f"kernel={want}" in line:
+                return float(line.split("=")[1].split()[0])
     return None
+
+
+def traffic_from_profile(kernel: str):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE of rocprofv3
+    --pmc, scaled by the calibration kernels), from the committed summary of
+    that measurement: counters cannot be read from inside this process."""
+    v = _profile_value(TRAFFIC_PROFILE, "traffic_bytes_per_launch", kernel)
+    return int(v) if v is not None else None
+
+
+def valu_issue_from_profile(kernel: str):
+    """Fraction of `kernel`'s cycles its VALU instructions need at one wave
+    instruction per SIMD every 4 cycles (SQ_INSTS_VALU against
+    GRBM_GUI_ACTIVE), from the committed counter summary."""
+    return _profile_value(ISSUE_PROFILE, "valu_issue_frac", kernel)
 
 
 def _timed_steps(b, warmup: int, steps: int):
@@ -225,7 +310,10 @@ def plain_paths(steps: int = 3, warmup: int = 1, which=None):
                     + cell tables (8 c) + rowptr (4) + columns (4 K)
       delta-SPH     per right-hand side: search + the gather's row pointers
                     (4) + columns (4 K) + x, v, rho, A (24) + dpdt, a (12);
-                    per RK stage combine 52; four of each per step"""
+                    per RK stage combine 52; four of each per step: the
+                    compulsory bytes of the pair-list algorithm with one fused
+                    gather per right-hand side (the engine's cell walk moves
+                    fewer: it writes and reads no fluid list)"""
     import torch
 
     from pytorchsph_amd import _lib as L
@@ -277,6 +365,8 @@ def plain_paths(steps: int = 3, warmup: int = 1, which=None):
         sc = S.dam_break(dx=0.0005, block=(1.0, 1.0), scheme="dsph", seed=5)
         b = SceneBatch([sc])
         sec = _timed_steps(b, warmup, steps)
+        # the step builds no fluid pair list (the cell walk); K from one
+        b.search(b.x)
         nnz = int(b.ff.rowptr[b.NP].item())
         k_mean = nnz / b.NP
         cells = b.grid.ncells / b.NP
@@ -595,18 +685,28 @@ def main():
         tuned = tune(batch.parts, log=log)
         log(f"autotuned in {time.perf_counter() - t0:.1f}s: {tuned['choice']}")
     if args.roofline_only:
+        dsph_part = next(b for b in batch.parts if b.scheme == "dsph")
+        rl_w = cellwalk_roofline(dsph_part)
+        rl_g = gather_roofline(dsph_part)
         rl = pressure_roofline(next(b for b in batch.parts if b.scheme == "dfsph"))
-        rl_g = gather_roofline(next(b for b in batch.parts if b.scheme == "dsph"))
         copied = copy_calibration()
-        print(json.dumps({"roofline": rl, "dsph_gather": rl_g, "copy_bytes": copied}), flush=True)
+        rw = read_write_calibration()
+        print(json.dumps({"cellwalk": rl_w, "dsph_gather": rl_g, "pressure": rl,
+                          "copy_bytes": copied, **rw}), flush=True)
         return
     batch.capture()
     torch.cuda.synchronize()
     log("captured")
-    for _ in range(args.warmup):
+    # the scenes flow before the timed window (dam breaks collapse, drops
+    # approach): the timed steps are not those of the fluid at rest
+    for k in range(args.warmup):
         batch.step(serial=args.serial)
+        if (k + 1) % 50 == 0:
+            torch.cuda.synchronize()
+            log(f"flow step {k + 1}")
     torch.cuda.synchronize()
-    lat = []
+    dfsph_parts = [b for b in batch.parts if b.scheme == "dfsph"]
+    lat, iters = [], []
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for k in range(args.steps):
@@ -614,17 +714,22 @@ def main():
         batch.step(serial=args.serial)
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t0)
+        # the device already finished the step: reading its per-scene
+        # iteration counts does not hold the GPU
+        iters += [int(i) for b in dfsph_parts for i in b.dfsph_iterations()]
         log(f"step {k}: {lat[-1] * 1e3:.1f} ms")
     total = time.perf_counter() - t_start
     ms_per_step = total / args.steps * 1e3
     value = n_part * args.steps / total
     status = batch.status()
-    iters = {b.scheme: b.dfsph_iterations().tolist() for b in batch.parts if b.scheme == "dfsph"}
 
-    rl = pressure_roofline(next(b for b in batch.parts if b.scheme == "dfsph"))
-    log(f"roofline: {rl}")
-    rl_g = gather_roofline(next(b for b in batch.parts if b.scheme == "dsph"))
-    log(f"delta-SPH gather: {rl_g}")
+    dsph_part = next(b for b in batch.parts if b.scheme == "dsph")
+    rl_w = cellwalk_roofline(dsph_part)
+    log(f"delta-SPH cell walk: {rl_w}")
+    rl_g = gather_roofline(dsph_part)
+    log(f"delta-SPH list gather: {rl_g}")
+    rl = pressure_roofline(dfsph_parts[0])
+    log(f"DFSPH pressure: {rl}")
     del batch
     torch.cuda.empty_cache()
 
@@ -688,24 +793,42 @@ def main():
         "batched_over_plain": (value / plain) if plain else None,
         "p50_step_ms": float(np.percentile(lat, 50) * 1e3),
         "p99_step_ms": float(np.percentile(lat, 99) * 1e3),
-        "dfsph_iterations_mean": float(np.mean(iters["dfsph"])) if iters else None,
+        "flow_steps_before_timing": args.warmup,
+        "dfsph_iterations_mean": float(np.mean(iters)) if iters else None,
+        # Jacobi iterations of every DFSPH scene in every timed step
+        "dfsph_iterations_histogram": {str(k): int(c) for k, c in
+                                       enumerate(np.bincount(iters)) if c} if iters else None,
         "status_flags": int(np.bitwise_or.reduce(status)),
-        # the delta-SPH gather is the step's largest kernel (four launches per
-        # step); the DFSPH pressure update, whose count per step follows the
-        # solver's iterations, is reported beside it
-        "roofline": {"bound": "hbm", "achieved": rl_g["achieved_gbs"], "peak": peak,
-                     "unit": "GB/s", "frac": rl_g["achieved_gbs"] / peak,
-                     "traffic": traffic_from_profile(rl_g["kernel"]),
-                     "kernel": rl_g["kernel"], "avg_launch_us": rl_g["seconds"] * 1e6,
-                     "bytes_per_launch": rl_g["bytes"], "rows": rl_g["rows"],
-                     "pairs": rl_g["pairs"]},
+        # the delta-SPH cell walk is the step's largest kernel (four launches
+        # per step, each with its RK4 stage); it is bound by VALU issue, not
+        # by HBM (valu_issue_frac); the list gather it replaces and the DFSPH
+        # pressure update, whose count per step follows the solver's
+        # iterations, are reported beside it
+        "roofline": {"bound": "hbm", "achieved": rl_w["achieved_gbs"], "peak": peak,
+                     "unit": "GB/s", "frac": rl_w["achieved_gbs"] / peak,
+                     "traffic": traffic_from_profile(rl_w["kernel"]),
+                     "traffic_source": TRAFFIC_PROFILE,
+                     "kernel": rl_w["kernel"], "avg_launch_us": rl_w["seconds"] * 1e6,
+                     "bytes_per_launch": rl_w["bytes"], "record_bytes": rl_w["record_bytes"],
+                     "rows": rl_w["rows"], "boundary_pairs": rl_w["boundary_pairs"],
+                     "valu_issue_frac": valu_issue_from_profile(rl_w["kernel"]),
+                     "valu_issue_source": ISSUE_PROFILE},
+        "roofline_dsph_list_gather": {"achieved": rl_g["achieved_gbs"],
+                                      "frac": rl_g["achieved_gbs"] / peak,
+                                      "traffic": traffic_from_profile(rl_g["kernel"]),
+                                      "kernel": rl_g["kernel"],
+                                      "avg_launch_us": rl_g["seconds"] * 1e6,
+                                      "bytes_per_launch": rl_g["bytes"], "rows": rl_g["rows"],
+                                      "pairs": rl_g["pairs"],
+                                      "valu_issue_frac": valu_issue_from_profile(rl_g["kernel"])},
         "roofline_dfsph_pressure": {"achieved": rl["achieved_gbs"],
                                     "frac": rl["achieved_gbs"] / peak,
                                     "traffic": traffic_from_profile(rl["kernel"]),
                                     "kernel": rl["kernel"],
                                     "avg_launch_us": rl["seconds"] * 1e6,
                                     "bytes_per_launch": rl["bytes"], "rows": rl["rows"],
-                                    "pairs": rl["pairs"]},
+                                    "pairs": rl["pairs"],
+                                    "valu_issue_frac": valu_issue_from_profile(rl["kernel"])},
         "tuning": tuned["choice"] if tuned else None,
         "serving": serve,
         "plain_paths": paths,
