@@ -455,13 +455,8 @@ struct WalkShared {
   int ridx[kWalkRecs];             // their point indices (record rows)
   unsigned short list[kWalkRows][kWalkCap + 1];   // slot kWalkCap: scratch
   unsigned short cnt[kWalkRows];   // listed hits, kWalkCap + 1 = more
-  int wlo[kBS / kWave][3], whi[kBS / kWave][3];   // stencil runs per wavefront
+  int wlo[kBS / kWave][3], whi[kBS / kWave][3];   // chunk_runs scratch
 };
-
-// Wavefront min / max over the active lanes (the device library's DPP
-// reductions; the LDS atomics they replace are serialised lane by lane).
-extern "C" __device__ int __ockl_wfred_min_i32(int);
-extern "C" __device__ int __ockl_wfred_max_i32(int);
 
 // Listed fluid pairs of a row: entry k (rank k) on lane k mod KL, in
 // batches of kPackedBatch as the packed gather takes them.
@@ -565,7 +560,6 @@ __global__ __launch_bounds__(kBS) void k_dsph_cellwalk(
     const bool valid = sp < s1;
     Stencil3 st;
     float xq = 0.f, yq = 0.f, thr = 0.f;
-    int rlo[3] = {INT_MAX, INT_MAX, INT_MAX}, rhi[3] = {INT_MIN, INT_MIN, INT_MIN};
     if (valid) {
       const int qi = g.sorted_idx[sp];
       const int sc = g.nscenes == 1 ? 0 : g.pscene[qi];
@@ -576,46 +570,14 @@ __global__ __launch_bounds__(kBS) void k_dsph_cellwalk(
       st.init(d, g.cell_start + d.cell_begin, cell_coord(xq, d.qmin_x, d.hmax),
               cell_coord(yq, d.qmin_y, d.hmax));
       thr = sq_threshold(phys[sc].h);
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        if (st.ra[r] < st.rb[r]) {
-          rlo[r] = st.ra[r];
-          rhi[r] = st.rb[r];
-        }
-      }
     }
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      rlo[r] = __ockl_wfred_min_i32(rlo[r]);
-      rhi[r] = __ockl_wfred_max_i32(rhi[r]);
-    }
-    if ((t & (kWave - 1)) == 0) {
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        S.wlo[t / kWave][r] = rlo[r];
-        S.whi[t / kWave][r] = rhi[r];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int w = 0; w < kBS / kWave; ++w) {
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        rlo[r] = S.wlo[w][r] < rlo[r] ? S.wlo[w][r] : rlo[r];
-        rhi[r] = S.whi[w][r] > rhi[r] ? S.whi[w][r] : rhi[r];
-      }
-    }
-    // ---- stage: stencil row r's run [lo_r, hi_r) at LDS offset off_r (a
-    // row that no point of the chunk has is empty at lo_r = 0)
-    const bool e0 = rhi[0] > rlo[0], e1 = rhi[1] > rlo[1], e2 = rhi[2] > rlo[2];
-    const int lo0 = e0 ? rlo[0] : 0, lo1 = e1 ? rlo[1] : 0, lo2 = e2 ? rlo[2] : 0;
-    const int off1 = e0 ? rhi[0] - lo0 : 0;
-    const int off2 = off1 + (e1 ? rhi[1] - lo1 : 0);
-    const int tot = off2 + (e2 ? rhi[2] - lo2 : 0);
+    // ---- stage: the chunk's three stencil runs, positions and indices
+    const ChunkRuns cr = chunk_runs(st, valid, S.wlo, S.whi);
+    const int tot = cr.tot;
     const bool staged = tot <= kWalkRecs;
     if (staged) {
       for (int k = t; k < tot; k += kBS) {
-        const int p = k < off1 ? lo0 + k : (k < off2 ? lo1 + (k - off1) : lo2 + (k - off2));
+        const int p = cr.slot(k);
         S.pos[k] = sx[p];
         S.ridx[k] = g.sorted_idx[p];
       }
@@ -627,9 +589,7 @@ __global__ __launch_bounds__(kBS) void k_dsph_cellwalk(
       int cnt = 0;
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
-        const int lo = r == 0 ? lo0 : (r == 1 ? lo1 : lo2);
-        const int off = r == 0 ? 0 : (r == 1 ? off1 : off2);
-        const int b = off + (st.ra[r] - lo), e = b + (st.rb[r] - st.ra[r]);
+        const int b = cr.first(r, st), e = b + (st.rb[r] - st.ra[r]);
         // kWalkBatch LDS loads in flight per trip, no branch per candidate
         for (int k = b; k < e; k += kWalkBatch) {
           float2 xy[kWalkBatch];

@@ -256,6 +256,84 @@ __global__ __launch_bounds__(kBS) void k_neighbors(SphGrid g, SphQuery q,
   }
 }
 
+// Count the pairs of self-sorted, radius-1 queries with the stencil staged
+// in LDS.  A workgroup takes 256 consecutive sorted points; the three
+// stencil runs of such a chunk are three contiguous runs of sorted points
+// (chunk_runs), copied into LDS with coalesced loads, and one lane per row
+// tests its candidates from there with the fill's predicate, four loads in
+// flight and no ballot per candidate (C2: 139 us against 223 for
+// k_neighbors' count).  Chunks whose stencil does not fit in LDS count from
+// global memory, one lane per row.  (The fill keeps k_neighbors: the same
+// walk storing each hit as found, or listing a chunk's hits in LDS and
+// writing them after the walk, wrote C2's randomly ordered rows back from
+// the L2 in more pieces and took 15-60% longer.)
+constexpr int kSWalkRecs = 3 * kBS * 3 / 2 + 256;   // staged stencil points per chunk
+constexpr int kSWalkBatch = 4;                      // candidates tested per trip
+
+__global__ __launch_bounds__(kBS) void k_walk_count(SphGrid g, SphQuery q,
+                                                    int* __restrict__ counts) {
+  __shared__ float2 s_pos[kSWalkRecs + kSWalkBatch];   // padded: a trip reads past a run's end
+  __shared__ int s_wlo[kBS / kWave][3], s_whi[kBS / kWave][3];
+  const int t = threadIdx.x;
+  const int sp = xcd_block() * kBS + t;
+  const bool valid = sp < q.nq;
+  const float2* __restrict__ sx = reinterpret_cast<const float2*>(g.sorted_x);
+  int qi = 0;
+  float xq = 0.f, yq = 0.f, thr = 0.f;
+  Stencil3 st;
+  if (valid) {
+    qi = g.sorted_idx[sp];
+    const int sc = g.nscenes == 1 ? 0 : q.qscene[qi];
+    const SphGridDesc d = g.scenes[sc];
+    const float2 xy = sx[sp];
+    xq = xy.x;
+    yq = xy.y;
+    int ix, iy;
+    if (q.qcell) {
+      // the cell of a given key, in C integer arithmetic as
+      // neighSearch.cpp:207-209 derives it
+      const int key = q.qcell[qi];
+      ix = key % d.cells_x;
+      iy = key / d.cells_x;
+    } else {
+      ix = cell_coord(xq, d.qmin_x, d.hmax);
+      iy = cell_coord(yq, d.qmin_y, d.hmax);
+    }
+    st.init(d, g.cell_start + d.cell_begin, ix, iy);
+    thr = sq_threshold(q.h[qi]);
+  }
+  const ChunkRuns cr = chunk_runs(st, valid, s_wlo, s_whi);
+  const bool staged = cr.tot <= kSWalkRecs;
+  if (staged) {
+    for (int k = t; k < cr.tot; k += kBS) s_pos[k] = sx[cr.slot(k)];
+  }
+  __syncthreads();
+  if (!valid) return;
+  int cnt = 0;
+  if (staged) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int b = cr.first(r, st), e = b + (st.rb[r] - st.ra[r]);
+      for (int k = b; k < e; k += kSWalkBatch) {
+        float2 xy[kSWalkBatch];
+#pragma unroll
+        for (int u = 0; u < kSWalkBatch; ++u) xy[u] = s_pos[k + u];
+#pragma unroll
+        for (int u = 0; u < kSWalkBatch; ++u) {
+          const float d2 = dist2_nofma(xq, yq, xy[u].x, xy[u].y);
+          cnt += ((d2 <= thr) & (k + u < e)) ? 1 : 0;
+        }
+      }
+    }
+  } else {
+    for (int p = 0; p < st.ncand; ++p) {
+      const float2 xy = sx[st.slot(p)];
+      cnt += dist2_nofma(xq, yq, xy.x, xy.y) <= thr ? 1 : 0;
+    }
+  }
+  counts[qi] = cnt;
+}
+
 __global__ __launch_bounds__(kBS) void k_clamp_rowptr(int* __restrict__ rowptr, int n, int64_t cap) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -388,11 +466,20 @@ static int check_query(const SphGrid* g, const SphQuery* q, const char* who) {
   return SPH_OK;
 }
 
+// Self-sorted radius-1 queries (a scene's own points) are counted with the
+// LDS-staged walk; foreign and wider queries by the per-query kernel.
+static bool walks(const SphQuery* q) { return q->self_sorted && q->radius == 1; }
+
 int neighbor_count(const SphGrid* g, const SphQuery* q, int* counts, hipStream_t s) {
   int rc = check_query(g, q, "sph_neighbor_count");
   if (rc) return rc;
   SPH_REQUIRE(counts, "sph_neighbor_count: null counts");
   if (q->nq == 0) return SPH_OK;
+  if (walks(q)) {
+    hipLaunchKernelGGL(k_walk_count, dim3(blocks_for(q->nq, kBS)), dim3(kBS), 0, s, *g, *q,
+                       counts);
+    return launch_check("sph_neighbor_count");
+  }
   with_lanes(tuning(SPH_TUNE_SEARCH_LANES), [&](auto kl) {
     constexpr int KL = decltype(kl)::value;
     hipLaunchKernelGGL((k_neighbors<false, KL>), dim3(blocks_for((int64_t)q->nq * KL, kBS)),

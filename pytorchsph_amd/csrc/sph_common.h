@@ -2,6 +2,7 @@
 // scans, the canonical fp32 neighbour predicate and the SPH kernel functions.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <climits>
 #include <stdint.h>
 #include <string.h>
 #include <string>
@@ -106,6 +107,75 @@ struct Stencil3 {
     return p < n0 ? ra[0] + p : (p < n01 ? ra[1] + (p - n0) : ra[2] + (p - n01));
   }
 };
+
+// Wavefront min / max over the active lanes (the device library's DPP
+// reductions; LDS atomics in their place are serialised lane by lane).
+extern "C" __device__ int __ockl_wfred_min_i32(int);
+extern "C" __device__ int __ockl_wfred_max_i32(int);
+
+// The stencil runs of a chunk of rows, one row per thread of the block: run
+// r (stencil row dy = r - 1) is the union [lo_r, hi_r) of the rows' runs, in
+// sorted slots, staged at offset off_r of a chunk-wide array of `tot` slots
+// (a run no row of the chunk has is empty at lo_r = 0).
+struct ChunkRuns {
+  int lo0, lo1, lo2, off1, off2, tot;
+  // sorted slot of staged slot k < tot
+  __device__ __forceinline__ int slot(int k) const {
+    return k < off1 ? lo0 + k : (k < off2 ? lo1 + (k - off1) : lo2 + (k - off2));
+  }
+  // staged slot of the first candidate of run r of a row's stencil
+  __device__ __forceinline__ int first(int r, const Stencil3& st) const {
+    return r == 0 ? st.ra[0] - lo0 : (r == 1 ? off1 + (st.ra[1] - lo1) : off2 + (st.ra[2] - lo2));
+  }
+};
+
+// Every thread of the block calls this with its row's stencil (`valid`
+// false: no row); wlo / whi are LDS scratch of 3 ints per wavefront.  Two
+// wavefront reductions per run and one barrier.
+__device__ __forceinline__ ChunkRuns chunk_runs(const Stencil3& st, bool valid, int (*wlo)[3],
+                                                int (*whi)[3]) {
+  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+  if (valid) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      if (st.ra[r] < st.rb[r]) {
+        lo[r] = st.ra[r];
+        hi[r] = st.rb[r];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    lo[r] = __ockl_wfred_min_i32(lo[r]);
+    hi[r] = __ockl_wfred_max_i32(hi[r]);
+  }
+  const int w = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      wlo[w][r] = lo[r];
+      whi[w][r] = hi[r];
+    }
+  }
+  __syncthreads();
+  const int nw = blockDim.x / kWave;
+  for (int v = 0; v < nw; ++v) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      lo[r] = wlo[v][r] < lo[r] ? wlo[v][r] : lo[r];
+      hi[r] = whi[v][r] > hi[r] ? whi[v][r] : hi[r];
+    }
+  }
+  ChunkRuns c;
+  const bool e0 = hi[0] > lo[0], e1 = hi[1] > lo[1], e2 = hi[2] > lo[2];
+  c.lo0 = e0 ? lo[0] : 0;
+  c.lo1 = e1 ? lo[1] : 0;
+  c.lo2 = e2 ? lo[2] : 0;
+  c.off1 = e0 ? hi[0] - c.lo0 : 0;
+  c.off2 = c.off1 + (e1 ? hi[1] - c.lo1 : 0);
+  c.tot = c.off2 + (e2 ? hi[2] - c.lo2 : 0);
+  return c;
+}
 
 // Squared distance fl(fl(dx*dx) + fl(dy*dy)) with no FMA contraction.
 __device__ __forceinline__ float dist2_nofma(float xq, float yq, float xj, float yj) {

@@ -165,6 +165,27 @@ def test_empty_and_single_point(cuda, n):
     assert c.tolist() == list(range(n))
 
 
+def test_walk_count_crowded_chunks_and_long_rows(cuda):
+    """The LDS-staged count where it cannot stage and where rows are long:
+    3,000 points in a tenth of a cell (every one a neighbour of all; their
+    chunks' stencils are far larger than the LDS stage and are counted from
+    global memory) and 600 points in a patch of 1.5 x 1.5 supports (rows of
+    ~300 hits, staged) beside 20,000 uniform points.  Pairs bit-exact
+    against the oracle (the fill writes exactly the counted rows)."""
+    rng = np.random.default_rng(12)
+    n_u = 20000
+    hmax = np.float32(np.sqrt(20 / (np.pi * n_u)))
+    x = np.concatenate([rng.uniform(0, 1, (n_u, 2)),
+                        0.5 + rng.uniform(0, 0.1 * hmax, (3000, 2)),
+                        0.25 + rng.uniform(0, 1.5 * hmax, (600, 2))]).astype(np.float32)
+    h = np.full(len(x), hmax, np.float32)
+    rp, c, st, _ = gpu_search(x, h, (0, 0), (1, 1), hmax)
+    assert st == 0
+    rows = np.diff(rp)
+    assert rows.max() >= 3000 and (rows > 200).sum() >= 500
+    assert_same_pairs(rp, c, oracle_self(x, h, (0, 0), (1, 1), hmax))
+
+
 def test_overflow_is_flagged_and_bounded(cuda):
     rng = np.random.default_rng(8)
     x = rng.uniform(0, 1, (4000, 2)).astype(np.float32)
