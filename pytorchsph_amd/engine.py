@@ -421,6 +421,12 @@ class SceneBatch:
             act_pre=z(2 * (S + 1), dt=torch.int32), act_n=z(3, dt=torch.int32))
         self.any_active_host = torch.zeros(1, dtype=torch.int32,
                                            pin_memory=self.dev.type == "cuda")
+        self.iters_host = torch.zeros(S, dtype=torch.int32, pin_memory=self.dev.type == "cuda")
+        # iterations the next step's first chunk runs: the last step's most
+        # iterated scene (at least the stopping rule's minimum); the loop
+        # continues in chunks of dfsph_chunk when a scene needs more
+        self.min_iters = int(max(1, min(self.phys_host["min_iters"])))
+        self.pred_iters = self.min_iters
         if self.divergence:
             # the divergence solve's own iteration counters and residual
             # history, and the limits it runs with (divergenceThreshold,
@@ -657,18 +663,35 @@ class SceneBatch:
                L.ptr(self.x), L.ptr(self.v), None, L.ptr(self.v), L.ptr(self.acc), None,
                L.ptr(self.x), L.ptr(self.v), None, sh)
 
+    # iteration counts with a captured chunk graph: a step's first chunk runs
+    # the smallest of these that covers the previous step's iterations
+    CHUNK_SIZES = (1, 2, 3, 4, 6, 8, 12, 16)
+
+    def _first_chunk(self) -> int:
+        want = max(self.pred_iters, self.min_iters)
+        fits = [k for k in self.CHUNK_SIZES if k >= want and k <= self.dfsph_chunk]
+        return fits[0] if fits else self.dfsph_chunk
+
     def _dfsph_loop(self, s=None, graphs=None):
-        """Iterates in chunks until every scene has converged."""
+        """Iterates in chunks until every scene has converged: first the
+        chunk that covers the last step's iterations, then chunks of
+        dfsph_chunk (converged scenes cost nothing in either; a chunk longer
+        than the solve costs its empty launches, a shorter one a host poll).
+        The per-scene iteration counts come back with the poll."""
         stream = s if s is not None else torch.cuda.current_stream()
+        k = self._first_chunk()
         while True:
             if graphs is not None:
-                graphs.replay()
+                graphs[k].replay()
             else:
-                self._dfsph_iters(self.dfsph_chunk, s)
+                self._dfsph_iters(k, s)
             self.any_active_host.copy_(self.df["any_active"], non_blocking=True)
+            self.iters_host.copy_(self.df["iters"], non_blocking=True)
             stream.synchronize()
             if int(self.any_active_host[0]) == 0:
                 break
+            k = self.dfsph_chunk
+        self.pred_iters = int(self.iters_host.max())
 
     # ----------------------------------------------------------- stepping
     def step(self, s=None):
@@ -707,14 +730,18 @@ class SceneBatch:
                 self._dsph_step()
             self.graph = (g,)
         else:
-            g1, g2, g3 = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+            g1, g3 = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
             with torch.cuda.graph(g1):
                 self._dfsph_pre()
-            with torch.cuda.graph(g2):
-                self._dfsph_iters(self.dfsph_chunk)
+            chunks = {}
+            for k in sorted({*[c for c in self.CHUNK_SIZES if c <= self.dfsph_chunk],
+                             self.dfsph_chunk}):
+                chunks[k] = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(chunks[k]):
+                    self._dfsph_iters(k)
             with torch.cuda.graph(g3):
                 self._dfsph_post()
-            self.graph = (g1, g2, g3)
+            self.graph = (g1, chunks, g3)
         torch.cuda.synchronize()
 
     def step_graph(self):
