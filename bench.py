@@ -44,7 +44,8 @@ def parse():
                     help="arrival window of the serving run (0 = skip)")
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shapes instead of autotuning: count,gather,solve[,fill"
-                         "[,walk]] lanes per row and the DFSPH grid cap, e.g. 2,4,2,8,1,16384")
+                         "[,walk[,block]]] lanes per row (block: workgroup size) and the DFSPH "
+                         "grid cap, e.g. 2,4,2,8,1,256,16384")
     ap.add_argument("--path", choices=PATHS, default=None,
                     help="run one single-scene workload of the plain paths and print it")
     ap.add_argument("--dplus-only", action="store_true",
@@ -669,13 +670,23 @@ def main():
     n_part = int(sum(s.n for s in scenes))
     log(f"{len(scenes)} scenes, {n_part} fluid particles, generated in "
         f"{time.perf_counter() - t0:.1f}s")
-    batch = MixedBatch(scenes)
+    scheme_kw = {}
+    cell = None
+    if not args.no_tune and not args.tuning:
+        # the DFSPH part searches a list every step: try half cells
+        from pytorchsph_amd.autotune import tune_cell_size
+        t0 = time.perf_counter()
+        cell = tune_cell_size([s for s in scenes if s.scheme == "dfsph"], log=log)
+        scheme_kw["dfsph"] = {"cell_div": cell["cell_div"]}
+        log(f"cell size tuned in {time.perf_counter() - t0:.1f}s")
+    batch = MixedBatch(scenes, scheme_kw=scheme_kw)
     tuned = None
     if args.tuning:
         vals = [int(t) for t in args.tuning.split(",")]
-        if len(vals) not in (4, 5, 6):
-            raise SystemExit("--tuning takes count,gather,solve[,fill[,walk]],cap")
-        L.set_tuning(**dict(zip(("search", "gather", "solve", "fill", "walk"), vals[:-1])))
+        if len(vals) not in (4, 5, 6, 7):
+            raise SystemExit("--tuning takes count,gather,solve[,fill[,walk[,block]]],cap")
+        L.set_tuning(**dict(zip(("search", "gather", "solve", "fill", "walk", "block"),
+                                vals[:-1])))
         cap = vals[-1]
         L.DFSPH_GRID_CAP = cap
         tuned = {"choice": dict(L.get_tuning(), grid_cap=cap)}
@@ -829,7 +840,9 @@ def main():
                                     "bytes_per_launch": rl["bytes"], "rows": rl["rows"],
                                     "pairs": rl["pairs"],
                                     "valu_issue_frac": valu_issue_from_profile(rl["kernel"])},
-        "tuning": tuned["choice"] if tuned else None,
+        "tuning": dict(tuned["choice"], dfsph_cell_div=cell["cell_div"] if cell else 1)
+                  if tuned else None,
+        "cell_size_times": cell["times"] if cell else None,
         "serving": serve,
         "plain_paths": paths,
         "cpu_baseline": cpu,

@@ -219,7 +219,11 @@ int sph_fill_f32(float* p, int n, float value, hipStream_t s);
 #define SPH_TUNE_SOLVE_LANES  2   /* DFSPH alpha/source, iterate and finish   */
 #define SPH_TUNE_FILL_LANES   3   /* sph_neighbor_fill                        */
 #define SPH_TUNE_WALK_LANES   4   /* sph_dsph_cellwalk's list walk            */
-#define SPH_TUNE_COUNT        5
+#define SPH_TUNE_BLOCK        5   /* threads per workgroup (64, 128 or 256) of
+                                     the list gathers (sph_fluid_density,
+                                     sph_dsph_gather*, sph_dsph_gather_packed*)
+                                     and the per-query search kernels        */
+#define SPH_TUNE_COUNT        6
 int sph_set_tuning(int knob, int value);
 int sph_get_tuning(int knob);
 

@@ -101,6 +101,7 @@ T_PRESS, T_PRESS_B, T_VISC, T_VISC_B = 1 << 4, 1 << 5, 1 << 6, 1 << 7
 T_LAM, T_GRAV, T_ACCUM = 1 << 8, 1 << 9, 1 << 16
 TUNE_SEARCH_LANES, TUNE_GATHER_LANES, TUNE_SOLVE_LANES, TUNE_FILL_LANES, TUNE_WALK_LANES = \
     0, 1, 2, 3, 4
+TUNE_BLOCK = 5
 DFSPH_ERR_SLOTS = 64
 
 
@@ -108,7 +109,7 @@ def dfsph_err_stride(nscenes: int) -> int:
     """Row stride (doubles) of SphDfsph.err_acc: SPH_DFSPH_ERR_STRIDE."""
     return (nscenes + 15) // 16 * 16
 TUNE_KNOBS = {"search": TUNE_SEARCH_LANES, "gather": TUNE_GATHER_LANES, "solve": TUNE_SOLVE_LANES,
-              "fill": TUNE_FILL_LANES, "walk": TUNE_WALK_LANES}
+              "fill": TUNE_FILL_LANES, "walk": TUNE_WALK_LANES, "block": TUNE_BLOCK}
 
 # exported entry points: name -> argtypes (restype is int unless noted)
 _P = C.POINTER
@@ -222,7 +223,9 @@ def call(name: str, *args) -> None:
 
 
 def set_tuning(**lanes: int) -> dict:
-    """Sets lanes-per-row knobs by name (search=, gather=, solve=, fill=, walk=); returns
+    """Sets launch-shape knobs by name: lanes per row (search=, gather=, solve=,
+    fill=, walk=) and the workgroup size of the list gathers and per-query
+    search (block= 64, 128 or 256); returns
     the previous values.  Host-side only: no device is touched."""
     old = get_tuning()
     for name, v in lanes.items():

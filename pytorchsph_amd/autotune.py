@@ -2,7 +2,10 @@
 
 The knobs are the lanes per row of the neighbour count and fill, of the
 kernel-sum gathers over pair lists, of the delta-SPH cell walk and of the DFSPH
-solve (sph_set_tuning), and the workgroup cap of the DFSPH iteration kernels.  Each knob is timed on the batch it will run
+solve, the workgroup size of the list gathers and the per-query search
+(sph_set_tuning), the workgroup cap of the DFSPH iteration kernels, and
+(tune_cell_size, which builds the batch once per candidate) the cell size of
+the grid of list batches.  Each knob is timed on the batch it will run
 (same scenes, same pair lists), one knob at a time with the others held,
 and the fastest value is kept.  Pair sets do not depend on any knob; sums
 are re-associated, so results move within fp32 rounding.
@@ -16,6 +19,7 @@ import torch
 from . import _lib as L
 
 LANE_VALUES = (1, 2, 4, 8, 16, 32)
+BLOCK_VALUES = (64, 128, 256)
 GRID_CAPS = (2048, 4096, 8192, 16384)
 
 
@@ -90,13 +94,16 @@ def tune(parts, reps: int = 5, log=None) -> dict:
         "walk": [_gather_fn(b) for b in dsph if b.cellwalk],
         "solve": [_solve_fn(b) for b in dfsph],
     }
+    # the workgroup size of the list gathers and the per-query search, timed
+    # on both once their lanes are chosen
+    plans["block"] = plans["search"] + plans["gather"]
     times: dict = {}
     choice = L.get_tuning()
     for knob, fns in plans.items():
         if not fns:
             continue
         res = {}
-        for v in LANE_VALUES:
+        for v in (BLOCK_VALUES if knob == "block" else LANE_VALUES):
             L.set_tuning(**{knob: v})
             res[v] = sum(_time(fn, reps) for fn in fns)
         best = min(res, key=res.get)
@@ -125,3 +132,33 @@ def tune(parts, reps: int = 5, log=None) -> dict:
     torch.cuda.synchronize()
     return {"choice": choice, "times": {k: {str(a): c for a, c in v.items()}
                                         for k, v in times.items()}}
+
+
+def tune_cell_size(scenes, steps: int = 3, divs=(1, 2), log=None, **kw) -> dict:
+    """Cell size of a list batch of `scenes` (one scheme): each candidate
+    cell (support / d, stencil radius d) gets its own batch, captured and
+    timed over `steps` graph steps after one warm-up step; returns
+    {"cell_div": best, "times": {d: seconds per step}}.  The pair sets are
+    the same for every d; the order inside a row follows the stencil."""
+    import time
+
+    from .engine import SceneBatch
+
+    times = {}
+    for d in divs:
+        b = SceneBatch(scenes, cell_div=d, **kw)
+        b.capture()
+        b.step_graph()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.step_graph()
+        torch.cuda.synchronize()
+        times[d] = (time.perf_counter() - t0) / steps
+        del b
+        torch.cuda.empty_cache()
+    best = min(times, key=times.get)
+    if log:
+        log("autotune cell_div: " + ", ".join(f"{d}:{s * 1e3:.2f}ms" for d, s in times.items())
+            + f" -> {best}")
+    return {"cell_div": best, "times": {str(d): s for d, s in times.items()}}

@@ -939,8 +939,9 @@ int fluid_density(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, c
   if (f->np == 0) return SPH_OK;
   with_lanes(tuning(SPH_TUNE_GATHER_LANES), [&](auto kl) {
     constexpr int KL = decltype(kl)::value;
-    hipLaunchKernelGGL((k_fluid_density<KL>), dim3(blocks_for((int64_t)f->np * KL, kBS)),
-                       dim3(kBS), 0, s, *f, phys, *ff, bdy_or_null(b), csr_or_null(fb), order,
+    const int bs = tuning(SPH_TUNE_BLOCK);
+    hipLaunchKernelGGL((k_fluid_density<KL>), dim3(blocks_for((int64_t)f->np * KL, bs)),
+                       dim3(bs), 0, s, *f, phys, *ff, bdy_or_null(b), csr_or_null(fb), order,
                        rho_out, vol_out);
   });
   return launch_check("sph_fluid_density");
@@ -965,8 +966,9 @@ int dsph_gather(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, con
   if (f->np == 0) return SPH_OK;
   with_lanes(tuning(SPH_TUNE_GATHER_LANES), [&](auto kl) {
     constexpr int KL = decltype(kl)::value;
-    hipLaunchKernelGGL((k_dsph_gather<KL>), dim3(blocks_for((int64_t)f->np * KL, kBS)),
-                       dim3(kBS), 0, s, *f, phys, *ff, bdy_or_null(b), csr_or_null(fb), gradrho,
+    const int bs = tuning(SPH_TUNE_BLOCK);
+    hipLaunchKernelGGL((k_dsph_gather<KL>), dim3(blocks_for((int64_t)f->np * KL, bs)),
+                       dim3(bs), 0, s, *f, phys, *ff, bdy_or_null(b), csr_or_null(fb), gradrho,
                        terms, order, dpdt, acc);
   });
   return launch_check("sph_dsph_gather");
@@ -999,7 +1001,8 @@ static void launch_gather(const float4* fr, const float4* br, const int* pscene,
                           const float* smask = nullptr) {
   with_lanes(tuning(SPH_TUNE_GATHER_LANES), [&](auto kl) {
     constexpr int KL = decltype(kl)::value;
-    const dim3 grid(blocks_for((int64_t)np * KL, kBS)), block(kBS);
+    const int bs = tuning(SPH_TUNE_BLOCK);
+    const dim3 grid(blocks_for((int64_t)np * KL, bs)), block(bs);
     if (terms == kMaskMOG)
       hipLaunchKernelGGL((k_dsph_gather_packed<kMaskMOG, KL, FUSE>), grid, block, 0, s, fr, br,
                          pscene, np, phys, ff, fb, gradrho, terms, order, dpdt, acc, rk, smask);

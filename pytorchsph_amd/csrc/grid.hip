@@ -482,8 +482,9 @@ int neighbor_count(const SphGrid* g, const SphQuery* q, int* counts, hipStream_t
   }
   with_lanes(tuning(SPH_TUNE_SEARCH_LANES), [&](auto kl) {
     constexpr int KL = decltype(kl)::value;
-    hipLaunchKernelGGL((k_neighbors<false, KL>), dim3(blocks_for((int64_t)q->nq * KL, kBS)),
-                       dim3(kBS), 0, s, *g, *q, (const int*)nullptr, counts, (int64_t)0);
+    const int bs = tuning(SPH_TUNE_BLOCK);
+    hipLaunchKernelGGL((k_neighbors<false, KL>), dim3(blocks_for((int64_t)q->nq * KL, bs)),
+                       dim3(bs), 0, s, *g, *q, (const int*)nullptr, counts, (int64_t)0);
   });
   return launch_check("sph_neighbor_count");
 }
@@ -496,8 +497,9 @@ int neighbor_fill(const SphGrid* g, const SphQuery* q, int* rowptr, int* col, in
   if (q->nq == 0) return SPH_OK;
   with_lanes(tuning(SPH_TUNE_FILL_LANES), [&](auto kl) {
     constexpr int KL = decltype(kl)::value;
-    hipLaunchKernelGGL((k_neighbors<true, KL>), dim3(blocks_for((int64_t)q->nq * KL, kBS)),
-                       dim3(kBS), 0, s, *g, *q, rowptr, col, cap);
+    const int bs = tuning(SPH_TUNE_BLOCK);
+    hipLaunchKernelGGL((k_neighbors<true, KL>), dim3(blocks_for((int64_t)q->nq * KL, bs)),
+                       dim3(bs), 0, s, *g, *q, rowptr, col, cap);
   });
   hipLaunchKernelGGL(k_clamp_rowptr, dim3(blocks_for(q->nq + 1, kBS)), dim3(kBS), 0, s, rowptr,
                      q->nq + 1, cap);

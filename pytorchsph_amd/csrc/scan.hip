@@ -29,9 +29,10 @@ int launch_check(const char* where) {
 }
 
 // Defaults (the shapes the autotuner picks on a 256-scene mixed batch):
-// lanes per row of the neighbour count, the gathers, the DFSPH solve and the
-// neighbour fill.
-static std::atomic<int> g_tuning[SPH_TUNE_COUNT] = {{2}, {4}, {2}, {8}, {1}};
+// lanes per row of the neighbour count, the gathers, the DFSPH solve, the
+// neighbour fill and the cell walk, and the workgroup size of the list
+// gathers and the per-query search.
+static std::atomic<int> g_tuning[SPH_TUNE_COUNT] = {{2}, {4}, {2}, {8}, {1}, {256}};
 
 int tuning(int knob) { return g_tuning[knob].load(std::memory_order_relaxed); }
 
@@ -40,7 +41,12 @@ static int set_tuning(int knob, int value) {
     set_error("sph_set_tuning: unknown knob");
     return SPH_E_ARG;
   }
-  if (value < 1 || value > 32 || (value & (value - 1))) {
+  if (knob == SPH_TUNE_BLOCK) {
+    if (value != 64 && value != 128 && value != 256) {
+      set_error("sph_set_tuning: workgroup size must be 64, 128 or 256");
+      return SPH_E_ARG;
+    }
+  } else if (value < 1 || value > 32 || (value & (value - 1))) {
     set_error("sph_set_tuning: lanes per row must be 1, 2, 4, 8, 16 or 32");
     return SPH_E_ARG;
   }

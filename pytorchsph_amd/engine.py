@@ -95,7 +95,7 @@ class SceneBatch:
     def __init__(self, scenes: list[Scene], device="cuda", pair_cap: float = 40.0,
                  bdy_pair_cap: float = 40.0, dfsph_chunk: int = 8, reorder: bool = True,
                  carry: list | None = None, fuse_rk4: bool | None = None,
-                 cellwalk: bool = True):
+                 cellwalk: bool = True, cell_div: int = 1):
         """fuse_rk4: delta-SPH stages run the RK4 combine and the next
         stage's records inside the gather (sph_dsph_gather_rk4 or
         sph_dsph_cellwalk_rk4) instead of as separate kernels; the results
@@ -110,6 +110,11 @@ class SceneBatch:
         (sph_dsph_cellwalk) and build no fluid pair list; the results are
         those of the list gather, bit for bit.  The boundary lists are built
         either way.
+
+        cell_div: grid cells of support / cell_div searched with a stencil
+        of radius cell_div (1 or 2).  The pair sets are the same; the order
+        inside a row, and so the rounding of the sums, follows the stencil.
+        The cell walk needs cell_div 1.
 
         carry[k], when given, continues scene k from another batch: a dict
         from `export` with device tensors of its positions, velocities and
@@ -197,8 +202,10 @@ class SceneBatch:
             if not d0.get("boundary_diffusion", True):
                 self.terms &= ~L.T_VISC_B
 
-        self.cellwalk = (cellwalk and scheme == "dsph" and not (self.dplus or self.shifting)
-                         and not self.terms & L.T_DDSPH)
+        assert cell_div in (1, 2), "cell_div is 1 or 2"
+        self.cell_div = int(cell_div)
+        self.cellwalk = (cellwalk and cell_div == 1 and scheme == "dsph"
+                         and not (self.dplus or self.shifting) and not self.terms & L.T_DDSPH)
         self.fuse_rk4 = self.cellwalk if fuse_rk4 is None else bool(fuse_rk4)
         if self.cellwalk:
             # the cell walk's pair predicate takes the support from the
@@ -207,8 +214,13 @@ class SceneBatch:
                        for ph, s in zip(self.phys_host, scenes))
 
         # ---- neighbour search
-        grid_scenes = [dict(vmin=s.vmin, vmax=s.vmax, hmax=s.support, p_begin=int(pb),
-                            p_count=int(s.n)) for s, pb in zip(scenes, self.p_begin)]
+        # cells of support / d, the table keeping the margins of the
+        # reference's grid (qmin = min - h, qmax = max + 2 h)
+        d = self.cell_div
+        grid_scenes = [dict(vmin=np.asarray(s.vmin, np.float64) - s.support * (1 - 1 / d),
+                            vmax=np.asarray(s.vmax, np.float64) + 2 * s.support * (1 - 1 / d),
+                            hmax=s.support / d, p_begin=int(pb), p_count=int(s.n))
+                       for s, pb in zip(scenes, self.p_begin)]
         self.grid = NeighborGrid(grid_scenes, NP, dev)
         self.ff = _PairBuf(NP, int(math.ceil(pair_cap * NP)), dev)
         bcap = int(math.ceil(bdy_pair_cap * max(NB, 1)))
@@ -472,11 +484,13 @@ class SceneBatch:
         g = self.grid
         g.build(x, self.pscene, s)
         if fluid_pairs:
-            self._pairs(self.ff, g.query_struct(x, self.h, self.pscene, 1, True), s)
+            self._pairs(self.ff, g.query_struct(x, self.h, self.pscene, self.cell_div, True), s)
         if self.NB:
             if self.solid:
-                self._pairs(self.gf, g.query_struct(self.bgx, self.bh, self.bscene, 1, False), s)
-            self._pairs(self.bf, g.query_struct(self.bx, self.bh, self.bscene, 1, False), s)
+                self._pairs(self.gf, g.query_struct(self.bgx, self.bh, self.bscene, self.cell_div,
+                                                    False), s)
+            self._pairs(self.bf, g.query_struct(self.bx, self.bh, self.bscene, self.cell_div,
+                                                False), s)
             c = self.bf.csr().c()
             L.call("sph_csr_transpose", C.byref(c), self.NP, L.ptr(self.fb.counts),
                    L.ptr(self.t_slot), L.ptr(self.fb.rowptr), L.ptr(self.fb.col),
@@ -810,7 +824,10 @@ class MixedBatch:
     on their own streams so their kernels overlap on the device (default
     stream priority for all)."""
 
-    def __init__(self, scenes: list[Scene], device="cuda", carry: list | None = None, **kw):
+    def __init__(self, scenes: list[Scene], device="cuda", carry: list | None = None,
+                 scheme_kw: dict | None = None, **kw):
+        """kw goes to every part; scheme_kw[scheme] to that scheme's parts
+        (e.g. {"dfsph": {"cell_div": 2}})."""
         self.scenes = scenes
         self.index = {}   # scene number -> (sub-batch, position)
         self.parts = []
@@ -823,7 +840,8 @@ class MixedBatch:
             ids = [k for k, s in enumerate(scenes) if kind(s) == key]
             if ids:
                 b = SceneBatch([scenes[k] for k in ids], device,
-                               carry=[carry[k] for k in ids], **kw)
+                               carry=[carry[k] for k in ids],
+                               **{**kw, **(scheme_kw or {}).get(key[0], {})})
                 for pos, k in enumerate(ids):
                     self.index[k] = (b, pos)
                 self.parts.append(b)

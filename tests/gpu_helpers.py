@@ -62,7 +62,7 @@ def grid_of(scene):
 
 
 class tuned:
-    """Sets lanes-per-row knobs for the duration of a with-block."""
+    """Sets launch-shape knobs for the duration of a with-block."""
 
     def __init__(self, **lanes):
         self.lanes = lanes

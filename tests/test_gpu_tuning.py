@@ -9,7 +9,8 @@ from oracle import oracle as O
 from pytorchsph_amd import _lib as L
 from pytorchsph_amd import scenes as S
 from pytorchsph_amd.engine import SceneBatch
-from tests.gpu_helpers import assert_same_pairs, gpu_search, rel_err, tuned
+from tests.gpu_helpers import (assert_same_pairs, csr_numpy, gpu_search, rel_err, sorted_rows,
+                                tuned)
 from tests.test_gpu_physics import perturbed_dsph_scene
 from tests.test_gpu_search import oracle_self
 
@@ -91,3 +92,75 @@ def test_dfsph_lanes(cuda, kl):
     hist = d["err_hist"][0, :5].cpu().numpy()
     assert np.allclose(hist, out["err_hist"][:5], rtol=1e-3, atol=1e-7)
     assert rel_err(d["p2"].cpu().numpy(), out["p2"]) < 1e-3
+
+
+def test_knob_validation_block(cuda):
+    old = L.get_tuning()
+    for bad in (32, 96, 512, 0):
+        with pytest.raises(RuntimeError):
+            L.set_tuning(block=bad)
+    assert L.get_tuning() == old
+
+
+def test_workgroup_size_changes_nothing(cuda):
+    """The workgroup size of the list gathers and the per-query search only
+    regroups rows into workgroups: every pair list and every gathered value
+    is the same bit for bit at 64, 128 and 256 threads."""
+    import ctypes as C
+
+    sc = S.dam_break(dx=0.01, scheme="dsph", seed=3)
+    ref = None
+    for bs in (256, 128, 64):
+        with tuned(block=bs):
+            b = SceneBatch([sc], cellwalk=False)
+            b.search(b.x)
+            ff, fbp = b._dsph_prepare(b.x, b.v, b.rho, b.frec)
+            L.call("sph_dsph_gather_packed", L.ptr(b.frec), L.ptr(b.brec), L.ptr(b.pscene), b.NP,
+                   L.ptr(b.phys), C.byref(ff), fbp, None, b.terms, None, L.ptr(b.dpdt),
+                   L.ptr(b.acc), L.stream_handle())
+            F, B = b.fluid(b.x, b.v, b.rho), b.boundary()
+            fb = b.fb.csr().c()
+            rho = torch.empty_like(b.rho)
+            L.call("sph_fluid_density", C.byref(F), L.ptr(b.phys), C.byref(ff), C.byref(B),
+                   C.byref(fb), None, L.ptr(rho), None, L.stream_handle())
+            torch.cuda.synchronize()
+            got = [t.cpu().numpy().copy() for t in (b.ff.rowptr, b.ff.col, b.gf.rowptr, b.gf.col,
+                                                     b.fb.rowptr, b.fb.col, b.dpdt, b.acc, rho)]
+        if ref is None:
+            ref = got
+        else:
+            for k, (a, r) in enumerate(zip(got, ref)):
+                assert np.array_equal(a.view(np.int32), r.view(np.int32)), (bs, k)
+
+
+@pytest.mark.parametrize("scheme", ["dfsph", "dsph"])
+def test_half_cells_same_pairs_and_trajectory(cuda, scheme):
+    """Cells of half the support with a radius-2 stencil (the autotuner's
+    cell-size knob): the same pair sets as the default grid (rows sorted),
+    and five steps within the oracle tolerances (the order of the sums
+    inside a row follows the stencil)."""
+    sc = S.dam_break(dx=0.02, block=(0.8, 0.6), scheme=scheme, seed=8)
+    b1, b2 = SceneBatch([sc]), SceneBatch([sc], cell_div=2)
+    assert not b2.cellwalk
+    for b in (b1, b2):
+        b.search(b.x)
+    torch.cuda.synchronize()
+    for name in ("ff", "bf", "fb") + (("gf",) if scheme == "dsph" else ()):
+        n = b1.NP if name in ("ff", "fb") else b1.NB
+        p1, p2 = getattr(b1, name), getattr(b2, name)
+        rp1, c1 = csr_numpy(p1.rowptr, p1.col, n)
+        rp2, c2 = csr_numpy(p2.rowptr, p2.col, n)
+        assert np.array_equal(rp1, rp2), name
+        assert np.array_equal(sorted_rows(rp1, c1), sorted_rows(rp2, c2)), name
+    so = O.SceneOracle(sc)
+    x, v, r = sc.x.astype(np.float64), sc.v.astype(np.float64), np.ones(sc.n)
+    for _ in range(5):
+        b2.step()
+        if scheme == "dsph":
+            x, v, r = so.dsph_step(x, v, r)
+        else:
+            x, v, r, _ = so.dfsph_step(x, v)
+    torch.cuda.synchronize()
+    got = b2.scene_state(0)
+    assert np.max(np.abs(got["x"] - x)) < 1e-5 * 4.0
+    assert rel_err(got["v"], v) < 2e-3
