@@ -165,15 +165,17 @@ def cellwalk_roofline(b, reps=20):
     and the RK4 stage fused), the dominant kernel of the delta-SPH step:
     stage 1 of the current state, with its algorithmic HBM bytes (each array
     once):
-      per fluid row     x, v, rho, A (24) + the stencil's sorted positions
-                        and point indices (12) + the RK4 stage combine (52:
-                        x, v, rho of the step start 20, running sums 12,
-                        new state 20) + the fluid -> boundary row pointer (4)
-      per cell          its start in the cell table (4)
-      per boundary pair its column (4); per boundary point its record (32)
+      per fluid row      x, v, rho, A (24) + the stencil's sorted positions
+                         and point indices (12) + the RK4 stage combine (52:
+                         x, v, rho of the step start 20, running sums 12,
+                         new state 20)
+      per cell           its start in the fluid and in the boundary cell
+                         table (4 + 4)
+      per boundary point its record (32) + its sorted position and index
+                         in the static boundary grid (12)
     The kernel reads each fluid neighbour through a 32-byte packed record
     {x, y, vx, vy, rho, V, p, rho_hat}; those bytes (32 per fluid row) are
-    reported apart as `record_bytes`."""
+    reported apart as `record_bytes`.  It reads no pair list."""
     import ctypes as C
 
     import torch
@@ -181,22 +183,27 @@ def cellwalk_roofline(b, reps=20):
     from pytorchsph_amd import _lib as L
 
     assert b.cellwalk and b.fuse_rk4, "a cell-walk batch with the fused stage"
-    _, fbp = b._dsph_prepare(b.x, b.v, b.rho, b.frec)
+    b._dsph_prepare(b.x, b.v, b.rho, b.frec)
     torch.cuda.synchronize()
-    nnz_b = int(b.fb.rowptr[b.NP].item()) if b.NB else 0
     stream = torch.cuda.current_stream()
     sh = L.stream_handle(stream)
     s0 = b.S[0]
     rk = L.SphRk4Stage(1, L.ptr(b.x), L.ptr(b.v), L.ptr(b.rho), L.ptr(b.area), L.ptr(b.K),
                        L.ptr(s0[0]), L.ptr(s0[1]), L.ptr(s0[2]), None, L.ptr(b.frec2))
     g = b.grid._grid()
+    bg = b._bgrid_arg()
 
     def launch():
         L.call("sph_dsph_cellwalk_rk4", C.byref(g), L.ptr(b.segs), b.nseg, L.ptr(b.frec),
-               L.ptr(b.brec), L.ptr(b.phys), fbp, b.terms, C.byref(rk), sh)
+               L.ptr(b.brec), L.ptr(b.phys), None, bg, b.terms, C.byref(rk), sh)
 
     sec = _events_avg(launch, stream, reps)
-    nbytes = (24 + 12 + 52 + 4) * b.NP + 4 * b.grid.ncells + 4 * nnz_b + 32 * b.NB
+    bcells = b.bgrid.ncells if b.bgrid is not None else 0
+    nbytes = (24 + 12 + 52) * b.NP + 4 * (b.grid.ncells + bcells) + 44 * b.NB
+    # the boundary pairs the launch found, for the record (one search)
+    b.search(b.x)
+    torch.cuda.synchronize()
+    nnz_b = int(b.fb.rowptr[b.NP].item()) if b.NB else 0
     return dict(kernel="k_dsph_cellwalk (sph_dsph_cellwalk_rk4)", seconds=sec, bytes=nbytes,
                 record_bytes=32 * b.NP, rows=b.NP, boundary_pairs=nnz_b,
                 achieved_gbs=nbytes / sec / 1e9)

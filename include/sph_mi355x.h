@@ -348,15 +348,24 @@ int sph_dsph_gather_rk4(const float* frec, const float* brec, const int* pscene,
  * segs[nseg][2] = {first, end} partitions the sorted slots [0, g->np) into
  * runs of one scene each, one workgroup per run; any partition gives the
  * same results, runs of at most 256 slots keep the stencil records in LDS.
- * Rows of frec are in point order (g->x's); fb rows too.  SPH_T_DDSPH not
- * allowed (its gradient needs the list). */
+ * Rows of frec are in point order (g->x's).  Boundary pairs come either
+ * from the fluid -> boundary list fb (rows in point order) or, when bg is
+ * given, from bg, a grid of the boundary points over the same scene
+ * geometry (sph_grid_build of the boundary positions): then each row's
+ * boundary pairs are found in the kernel, in the order of
+ * sph_neighbor_fill for the fluid points as foreign queries against bg
+ * (the solid-boundary search of solidBoundary.py:618-642), and are walked
+ * as the packed gather walks that list.  SPH_T_DDSPH not allowed (its
+ * gradient needs the list). */
 int sph_dsph_cellwalk(const SphGrid* g, const int* segs, int nseg, const float* frec,
                       const float* brec, const SphPhysics* phys, const SphCSR* fb,
-                      unsigned terms, float* dpdt, float* acc, hipStream_t s);
+                      const SphGrid* bg, unsigned terms, float* dpdt, float* acc,
+                      hipStream_t s);
 /* sph_dsph_cellwalk followed by the RK4 stage, as sph_dsph_gather_rk4. */
 int sph_dsph_cellwalk_rk4(const SphGrid* g, const int* segs, int nseg, const float* frec,
                           const float* brec, const SphPhysics* phys, const SphCSR* fb,
-                          unsigned terms, const SphRk4Stage* rk, hipStream_t s);
+                          const SphGrid* bg, unsigned terms, const SphRk4Stage* rk,
+                          hipStream_t s);
 /* Renormalisation matrix M_i (fluid + boundary neighbours) and L_i =
  * pinv2x2(M_i); M, L are [np][2][2]. */
 int sph_dsph_normalization(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
@@ -373,6 +382,17 @@ int sph_pinv2x2(const float* M, int n, float* L, float* eig, hipStream_t s);
 int sph_mdbc_density(const SphFluid* f, const SphPhysics* phys, const float* bx,
                      const float* gx, const int* bscene, int nb, const SphCSR* gf,
                      float* rho_b, float* p_b, int* counts, hipStream_t s);
+/* sph_mdbc_density with each ghost's fluid neighbours found in the kernel
+ * from the fluid grid g (sph_grid_build of f's positions) instead of read
+ * from a list: the pairs of sph_neighbor_fill for the ghost queries gx with
+ * supports gh (radius 1), in its order, so rho_b, p_b and counts are those
+ * of sph_mdbc_density bit for bit.  Replaces the ghost search of
+ * solidBoundary.py:618-642 and computeBoundaryDensitymDBC
+ * (solidBoundary.py:561-616) in one launch. */
+int sph_mdbc_density_walk(const SphFluid* f, const SphPhysics* phys, const float* bx,
+                          const float* gx, const float* gh, const int* bscene, int nb,
+                          const SphGrid* g, float* rho_b, float* p_b, int* counts,
+                          hipStream_t s);
 /* EOS for n particles: p = EOS(rho). */
 int sph_eos(const float* rho, const int* pscene, const SphPhysics* phys, int n, float* p,
             hipStream_t s);

@@ -144,6 +144,8 @@ _SIGS = {
     "sph_pinv2x2": ([c_vp, c_int, c_vp, c_vp, c_vp], c_int),
     "sph_mdbc_density": ([_P(SphFluid), c_vp, c_vp, c_vp, c_vp, c_int, _P(SphCSR), c_vp, c_vp,
                           c_vp, c_vp], c_int),
+    "sph_mdbc_density_walk": ([_P(SphFluid), c_vp, c_vp, c_vp, c_vp, c_vp, c_int, _P(SphGrid),
+                               c_vp, c_vp, c_vp, c_vp], c_int),
     "sph_eos": ([c_vp, c_vp, c_vp, c_int, c_vp, c_vp], c_int),
     "sph_akinci_boundary_density": ([_P(SphFluid), c_vp, c_vp, c_vp, c_int, c_vp, _P(SphCSR),
                                      c_vp, c_vp], c_int),
@@ -165,10 +167,10 @@ _SIGS = {
                                       C.c_uint, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "sph_dsph_gather_rk4": ([c_vp, c_vp, c_vp, c_int, c_vp, _P(SphCSR), _P(SphCSR), c_vp,
                              C.c_uint, c_vp, _P(SphRk4Stage), c_vp], c_int),
-    "sph_dsph_cellwalk": ([_P(SphGrid), c_vp, c_int, c_vp, c_vp, c_vp, _P(SphCSR), C.c_uint, c_vp,
-                           c_vp, c_vp], c_int),
-    "sph_dsph_cellwalk_rk4": ([_P(SphGrid), c_vp, c_int, c_vp, c_vp, c_vp, _P(SphCSR), C.c_uint,
-                               _P(SphRk4Stage), c_vp], c_int),
+    "sph_dsph_cellwalk": ([_P(SphGrid), c_vp, c_int, c_vp, c_vp, c_vp, _P(SphCSR), _P(SphGrid),
+                           C.c_uint, c_vp, c_vp, c_vp], c_int),
+    "sph_dsph_cellwalk_rk4": ([_P(SphGrid), c_vp, c_int, c_vp, c_vp, c_vp, _P(SphCSR),
+                               _P(SphGrid), C.c_uint, _P(SphRk4Stage), c_vp], c_int),
     "sph_integrate_rk4": ([c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                            c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "sph_integrate_semi_implicit": ([c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,

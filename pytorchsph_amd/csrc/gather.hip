@@ -275,6 +275,22 @@ __global__ __launch_bounds__(kBS) void k_dsph_gather(SphFluid f, const SphPhysic
   if (rl.sub == 0) dsph_store(P, o, a, terms, i, dpdt, acc);
 }
 
+// A right-hand side without pair terms (gravity alone, as the DFSPH
+// prologue asks for): the gather's epilogue on empty sums, one thread per
+// row, so dp/dt and a are those of k_dsph_gather bit for bit.
+__global__ __launch_bounds__(kBS) void k_dsph_rates_only(SphFluid f,
+                                                         const SphPhysics* __restrict__ phys,
+                                                         unsigned terms, float* __restrict__ dpdt,
+                                                         float* __restrict__ acc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.np) return;
+  const SphPhysics P = phys[f.pscene[i]];
+  Own o = {};
+  o.rho = f.rho[i] * P.rho0;
+  const Acc a = {};
+  dsph_store(P, o, a, terms, i, dpdt, acc);
+}
+
 // packed records: rec[2 i] = {x, y, vx, vy}, rec[2 i + 1] = {rho, V, p, rho_hat}
 __device__ __forceinline__ void pack_fluid(float4* __restrict__ frec, int t, const SphPhysics& P,
                                            float x, float y, float vx, float vy, float rh,
@@ -542,11 +558,74 @@ __device__ __forceinline__ void walk_slow(bool slow, const SphGrid& g, int sc, i
   }
 }
 
+// Boundary pairs of the fluid point at (xq, yq) (scene sc) from the
+// boundary grid bg: the boundary points of its stencil in bg's sorted order
+// that pass the predicate -- the row of a fluid -> boundary search against
+// bg (sph_neighbor_fill with the fluid points as queries) -- walked with the
+// packed gather's lane assignment (the hit of rank k on lane k mod KL).
+// Every lane of the wavefront runs the loop (ballots and shuffles), so call
+// it converged; `valid` false: no row.
+template <int KL>
+__device__ __forceinline__ void walk_bdy(bool valid, const SphGrid& bg, int sc, float xq, float yq,
+                                         float thr, int sub, const SphPhysics& P,
+                                         const KConst& K, const Own& o, unsigned terms,
+                                         const float4* __restrict__ brec, Acc& a) {
+  Stencil3 st;
+  int ncand = 0;
+  if (valid) {
+    const SphGridDesc d = bg.scenes[bg.nscenes == 1 ? 0 : sc];
+    st.init(d, bg.cell_start + d.cell_begin, cell_coord(xq, d.qmin_x, d.hmax),
+            cell_coord(yq, d.qmin_y, d.hmax));
+    ncand = st.ncand;
+  }
+  const float2* __restrict__ bsx = reinterpret_cast<const float2*>(bg.sorted_x);
+  if constexpr (KL == 1) {
+    for (int p = 0; p < ncand; ++p) {
+      const int slot = st.slot(p);
+      const float2 xy = bsx[slot];
+      if (dist2_nofma(xq, yq, xy.x, xy.y) <= thr) {
+        const int b = bg.sorted_idx[slot];
+        const float4 r0 = brec[2 * b], r1 = brec[2 * b + 1];
+        const Nbr nb = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z};
+        dsph_pair(P, K, o, nb, 0.f, 0.f, false, terms, a);
+      }
+    }
+  } else {
+    const unsigned long long gmask = group_mask<KL>();
+    const int lane = threadIdx.x & (kWave - 1);
+    int base = 0;   // rank of the trip's first hit
+    for (int p0 = 0; __ballot(p0 < ncand) != 0; p0 += KL) {
+      const int p = p0 + sub;
+      bool hit = false;
+      int slot = 0;
+      if (p < ncand) {
+        slot = st.slot(p);
+        const float2 xy = bsx[slot];
+        hit = dist2_nofma(xq, yq, xy.x, xy.y) <= thr;
+      }
+      const unsigned long long m = __ballot(hit) & gmask;
+      const int nh = __popcll(m);
+      const int want = (sub - base) & (KL - 1);   // rank of this lane's hit in the trip
+      unsigned long long mm = m;
+      for (int i = 0; i < want && mm; ++i) mm &= mm - 1;
+      const int src = mm ? __ffsll((long long)mm) - 1 : lane;
+      const int js = __shfl(slot, src, kWave);
+      if (want < nh) {
+        const int b = bg.sorted_idx[js];
+        const float4 r0 = brec[2 * b], r1 = brec[2 * b + 1];
+        const Nbr nb = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z};
+        dsph_pair(P, K, o, nb, 0.f, 0.f, false, terms, a);
+      }
+      base += nh;
+    }
+  }
+}
+
 // segs[b] = {first, end} sorted slots of workgroup b (one scene each).
 template <unsigned T, int KL, bool FUSE>
 __global__ __launch_bounds__(kBS) void k_dsph_cellwalk(
     SphGrid g, const int2* __restrict__ segs, const float4* __restrict__ frec,
-    const float4* __restrict__ brec, const SphPhysics* __restrict__ phys, SphCSR fb,
+    const float4* __restrict__ brec, const SphPhysics* __restrict__ phys, SphCSR fb, SphGrid bg,
     unsigned rterms, float* __restrict__ dpdt, float* __restrict__ acc, SphRk4Stage rk) {
   const unsigned terms = T ? T : rterms;
   __shared__ WalkShared S;
@@ -633,7 +712,14 @@ __global__ __launch_bounds__(kBS) void k_dsph_cellwalk(
           walk_slow<KL>(slow, g, sc, spr, sq_threshold(P.h), staged ? kWalkCap : 0, sub, P, K,
                         o, terms, frec, a);
       }
-      packed_bdy_pairs<KL>(P, K, o, brec, fb, qi, sub, terms, a);
+      if (bg.cell_start) {
+        if (terms & kBdyPairTerms) {
+          const float2 q = sx[spr];
+          walk_bdy<KL>(true, bg, sc, q.x, q.y, sq_threshold(P.h), sub, P, K, o, terms, brec, a);
+        }
+      } else {
+        packed_bdy_pairs<KL>(P, K, o, brec, fb, qi, sub, terms, a);
+      }
       reduce_acc<KL>(a, terms);
       if (sub == 0) packed_row_out<FUSE>(P, o, a, terms, qi, dpdt, acc, rk);
     }
@@ -808,48 +894,45 @@ __device__ void pinv3x3f(const float A[9], float X[9]) {
   }
 }
 
-__global__ __launch_bounds__(kBS) void k_mdbc_density(SphFluid f, const SphPhysics* __restrict__ phys,
-                                                      const float* __restrict__ bx,
-                                                      const float* __restrict__ gxp,
-                                                      const int* __restrict__ bscene, int nb,
-                                                      SphCSR gf, float* __restrict__ rho_b,
-                                                      float* __restrict__ p_b,
-                                                      int* __restrict__ counts) {
-  const int bI = blockIdx.x * blockDim.x + threadIdx.x;
-  if (bI >= nb) return;
-  const SphPhysics P = phys[bscene[bI]];
-  const KConst K = kconst(P.h);
-  const float2 g = ld2(gxp, bI);
+// mDBC sums of one ghost over its fluid neighbours (solidBoundary.py:561-
+// 601), added pair by pair in the order of the ghost's row.
+struct MdbcSums {
   float num = 0.f, den = 0.f, gsx = 0.f, gsy = 0.f, vgx = 0.f, vgy = 0.f, psx = 0.f, psy = 0.f;
   float m00 = 0.f, m01 = 0.f, m10 = 0.f, m11 = 0.f;
-  const int a0 = gf.rowptr[bI], a1 = gf.rowptr[bI + 1];
-  for (int k = a0; k < a1; ++k) {
-    const int j = gf.col[k];
-    const float2 xf = ld2(f.x, j);
-    // distances of a foreign query: (ghost - fluid) / |.|, q = |.| / h_query
-    const PairGeo geo = pair_geo(g.x, g.y, xf.x, xf.y, K.inv_h);
-    const float W = wendland_k(geo.q, K);
-    const float gw = -wendland_grad_k(geo.q, K);
-    const float gWx = gw * geo.hx, gWy = gw * geo.hy;
-    const float mass = f.area[j] * P.rho0;
-    const float V = f.area[j] / f.rho[j];
-    num += mass * W;
-    den += V * W;
-    gsx += mass * gWx;
-    gsy += mass * gWy;
-    vgx += V * gWx;
-    vgy += V * gWy;
-    const float xijx = xf.x - g.x, xijy = xf.y - g.y;
-    psx += V * W * xijx;
-    psy += V * W * xijy;
-    m00 += V * xijx * gWx;
-    m01 += V * xijx * gWy;
-    m10 += V * xijy * gWx;
-    m11 += V * xijy * gWy;
-  }
-  const int cnt = a1 - a0;
+  int cnt = 0;
+};
+
+__device__ __forceinline__ void mdbc_add(MdbcSums& s, const SphPhysics& P, const KConst& K,
+                                         float2 g, float2 xf, float area, float rho_hat) {
+  // distances of a foreign query: (ghost - fluid) / |.|, q = |.| / h_query
+  const PairGeo geo = pair_geo(g.x, g.y, xf.x, xf.y, K.inv_h);
+  const float W = wendland_k(geo.q, K);
+  const float gw = -wendland_grad_k(geo.q, K);
+  const float gWx = gw * geo.hx, gWy = gw * geo.hy;
+  const float mass = area * P.rho0;
+  const float V = area / rho_hat;
+  s.num += mass * W;
+  s.den += V * W;
+  s.gsx += mass * gWx;
+  s.gsy += mass * gWy;
+  s.vgx += V * gWx;
+  s.vgy += V * gWy;
+  const float xijx = xf.x - g.x, xijy = xf.y - g.y;
+  s.psx += V * W * xijx;
+  s.psy += V * W * xijy;
+  s.m00 += V * xijx * gWx;
+  s.m01 += V * xijx * gWy;
+  s.m10 += V * xijy * gWx;
+  s.m11 += V * xijy * gWy;
+  ++s.cnt;
+}
+
+// The boundary density from the sums (solidBoundary.py:603-616): MLS for
+// more than 10 neighbours, Shepard for 1-10, 1 without; clamped below at 1.
+__device__ __forceinline__ float mdbc_rho(const MdbcSums& s, const SphPhysics& P,
+                                          const KConst& K, float2 xb, float2 g) {
   float rho = 1.f;
-  if (cnt > 10) {
+  if (s.cnt > 10) {
     // The MLS system A [rho, grad rho] = [num, sum m grad W] in units of the
     // support: unknowns z = [rho, h grad rho] and the gradient rows times h,
     //   [ den      psx / h  psy / h ] [ rho            ]   [ num   ]
@@ -866,24 +949,94 @@ __global__ __launch_bounds__(kBS) void k_mdbc_density(SphFluid f, const SphPhysi
     // which matters because the Tait EOS turns rho_b - 1 of a few ulp at
     // rest into the wall pressure.
     const float h = P.h, ih = K.inv_h;
-    const float A[9] = {den, psx * ih, psy * ih, vgx * h, m00, m01, vgy * h, m10, m11};
+    const float A[9] = {s.den, s.psx * ih, s.psy * ih, s.vgx * h, s.m00, s.m01,
+                        s.vgy * h, s.m10, s.m11};
     float X[9];
     pinv3x3f(A, X);
-    const float rs = num / den;
-    const float rhs[3] = {num - den * rs, (gsx - vgx * rs) * h, (gsy - vgy * rs) * h};
+    const float rs = s.num / s.den;
+    const float rhs[3] = {s.num - s.den * rs, (s.gsx - s.vgx * rs) * h, (s.gsy - s.vgy * rs) * h};
     float dz[3];
     for (int r = 0; r < 3; ++r)
       dz[r] = X[3 * r] * rhs[0] + X[3 * r + 1] * rhs[1] + X[3 * r + 2] * rhs[2];
-    const float2 xb = ld2(bx, bI);
     const float dx = xb.x - g.x, dy = xb.y - g.y;
     rho = (rs + dz[0] + (dx * dz[1] + dy * dz[2]) * ih) / P.rho0;
-  } else if (cnt > 0) {
-    rho = num / den / P.rho0;
+  } else if (s.cnt > 0) {
+    rho = s.num / s.den / P.rho0;
   }
-  rho = rho < 1.f ? 1.f : rho;  // solidBoundary.py:615
+  return rho < 1.f ? 1.f : rho;  // solidBoundary.py:615
+}
+
+__global__ __launch_bounds__(kBS) void k_mdbc_density(SphFluid f, const SphPhysics* __restrict__ phys,
+                                                      const float* __restrict__ bx,
+                                                      const float* __restrict__ gxp,
+                                                      const int* __restrict__ bscene, int nb,
+                                                      SphCSR gf, float* __restrict__ rho_b,
+                                                      float* __restrict__ p_b,
+                                                      int* __restrict__ counts) {
+  const int bI = blockIdx.x * blockDim.x + threadIdx.x;
+  if (bI >= nb) return;
+  const SphPhysics P = phys[bscene[bI]];
+  const KConst K = kconst(P.h);
+  const float2 g = ld2(gxp, bI);
+  MdbcSums s;
+  const int a0 = gf.rowptr[bI], a1 = gf.rowptr[bI + 1];
+  for (int k = a0; k < a1; ++k) {
+    const int j = gf.col[k];
+    mdbc_add(s, P, K, g, ld2(f.x, j), f.area[j], f.rho[j]);
+  }
+  const float rho = mdbc_rho(s, P, K, ld2(bx, bI), g);
   rho_b[bI] = rho;
   if (p_b) p_b[bI] = eos(P, rho);
-  if (counts) counts[bI] = cnt;
+  if (counts) counts[bI] = s.cnt;
+}
+
+// mDBC with the ghosts' fluid neighbours found in the kernel (no ghost ->
+// fluid list): one lane per ghost walks its stencil in the fluid grid g in
+// the fill's candidate order, with the fill's predicate (|ghost - x_j| <
+// gh), and adds each hit as k_mdbc_density adds its row, so rho_b, p_b and
+// the counts are those of sph_mdbc_density over the list of sph_neighbor_fill
+// bit for bit.  Four candidate loads in flight per lane.
+__global__ __launch_bounds__(kBS) void k_mdbc_walk(SphFluid f, const SphPhysics* __restrict__ phys,
+                                                   const float* __restrict__ bx,
+                                                   const float* __restrict__ gxp,
+                                                   const float* __restrict__ gh,
+                                                   const int* __restrict__ bscene, int nb,
+                                                   SphGrid grid, float* __restrict__ rho_b,
+                                                   float* __restrict__ p_b,
+                                                   int* __restrict__ counts) {
+  const int bI = xcd_block() * blockDim.x + threadIdx.x;
+  if (bI >= nb) return;
+  const int sc = bscene[bI];
+  const SphPhysics P = phys[sc];
+  const KConst K = kconst(P.h);
+  const float2 g = ld2(gxp, bI);
+  const SphGridDesc d = grid.scenes[grid.nscenes == 1 ? 0 : sc];
+  Stencil3 st;
+  st.init(d, grid.cell_start + d.cell_begin, cell_coord(g.x, d.qmin_x, d.hmax),
+          cell_coord(g.y, d.qmin_y, d.hmax));
+  const float thr = sq_threshold(gh[bI]);
+  const float2* __restrict__ sx = reinterpret_cast<const float2*>(grid.sorted_x);
+  MdbcSums s;
+  for (int p0 = 0; p0 < st.ncand; p0 += kBatch) {
+    float2 xf[kBatch];
+    int slot[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      slot[u] = st.slot(p0 + u);   // clamped into the candidates
+      xf[u] = sx[slot[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (p0 + u < st.ncand && dist2_nofma(g.x, g.y, xf[u].x, xf[u].y) <= thr) {
+        const int j = grid.sorted_idx[slot[u]];
+        mdbc_add(s, P, K, g, xf[u], f.area[j], f.rho[j]);
+      }
+    }
+  }
+  const float rho = mdbc_rho(s, P, K, ld2(bx, bI), g);
+  rho_b[bI] = rho;
+  if (p_b) p_b[bI] = eos(P, rho);
+  if (counts) counts[bI] = s.cnt;
 }
 
 __global__ __launch_bounds__(kBS) void k_eos(const float* __restrict__ rho,
@@ -964,6 +1117,11 @@ int dsph_gather(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, con
   int rc = check_bdy_terms(terms, b, fb, "sph_dsph_gather");
   if (rc) return rc;
   if (f->np == 0) return SPH_OK;
+  if (!(terms & (kFluidPairTerms | kBdyPairTerms))) {
+    hipLaunchKernelGGL(k_dsph_rates_only, dim3(blocks_for(f->np, kBS)), dim3(kBS), 0, s, *f,
+                       phys, terms, dpdt, acc);
+    return launch_check("sph_dsph_gather");
+  }
   with_lanes(tuning(SPH_TUNE_GATHER_LANES), [&](auto kl) {
     constexpr int KL = decltype(kl)::value;
     const int bs = tuning(SPH_TUNE_BLOCK);
@@ -1083,8 +1241,8 @@ int dsph_gather_rk4(const float* frec, const float* brec, const int* pscene, int
 template <bool FUSE>
 static void launch_cellwalk(const SphGrid& g, const int* segs, int nseg, const float* frec,
                             const float* brec, const SphPhysics* phys, const SphCSR& fb,
-                            unsigned terms, float* dpdt, float* acc, const SphRk4Stage& rk,
-                            hipStream_t s) {
+                            const SphGrid& bg, unsigned terms, float* dpdt, float* acc,
+                            const SphRk4Stage& rk, hipStream_t s) {
   const int2* sg = reinterpret_cast<const int2*>(segs);
   const float4* fr = reinterpret_cast<const float4*>(frec);
   const float4* br = reinterpret_cast<const float4*>(brec);
@@ -1092,48 +1250,59 @@ static void launch_cellwalk(const SphGrid& g, const int* segs, int nseg, const f
     constexpr int KL = decltype(kl)::value;
     if (terms == kMaskMOG)
       hipLaunchKernelGGL((k_dsph_cellwalk<kMaskMOG, KL, FUSE>), dim3(nseg), dim3(kBS), 0, s, g,
-                         sg, fr, br, phys, fb, terms, dpdt, acc, rk);
+                         sg, fr, br, phys, fb, bg, terms, dpdt, acc, rk);
     else
       hipLaunchKernelGGL((k_dsph_cellwalk<0, KL, FUSE>), dim3(nseg), dim3(kBS), 0, s, g, sg, fr,
-                         br, phys, fb, terms, dpdt, acc, rk);
+                         br, phys, fb, bg, terms, dpdt, acc, rk);
   });
+}
+
+static SphGrid grid_or_null(const SphGrid* g) {
+  SphGrid r = {};
+  if (g) r = *g;
+  return r;
 }
 
 static int check_cellwalk(const SphGrid* g, const int* segs, int nseg, const float* frec,
                           const float* brec, const SphPhysics* phys, const SphCSR* fb,
-                          unsigned terms, const std::string& who) {
+                          const SphGrid* bg, unsigned terms, const std::string& who) {
   SPH_REQUIRE(g && g->scenes && g->sorted_idx && g->sorted_x && g->cell_start && g->np >= 0 &&
                   (g->nscenes == 1 || g->pscene),
               who + ": the grid is not built");
   SPH_REQUIRE(nseg >= 0 && (nseg == 0 || segs) && frec && phys, who + ": bad arguments");
   SPH_REQUIRE(!(terms & SPH_T_DDSPH),
               who + ": the DDSPH term needs its pair list (sph_dsph_gather_packed)");
-  SPH_REQUIRE(!(terms & kBdyPairTerms) || !fb || !fb->rowptr || brec,
+  SPH_REQUIRE(!(terms & kBdyPairTerms) || ((!fb || !fb->rowptr) && !bg) || brec,
               who + ": boundary terms without boundary records");
+  SPH_REQUIRE(!bg || (bg->scenes && bg->cell_start && bg->sorted_idx && bg->sorted_x &&
+                      bg->nscenes == g->nscenes),
+              who + ": the boundary grid is not built over the fluid grid's scenes");
+  SPH_REQUIRE(!(bg && fb && fb->rowptr), who + ": boundary pairs from a grid or a list, not both");
   return SPH_OK;
 }
 
 int dsph_cellwalk(const SphGrid* g, const int* segs, int nseg, const float* frec,
-                  const float* brec, const SphPhysics* phys, const SphCSR* fb, unsigned terms,
-                  float* dpdt, float* acc, hipStream_t s) {
-  int rc = check_cellwalk(g, segs, nseg, frec, brec, phys, fb, terms, "sph_dsph_cellwalk");
+                  const float* brec, const SphPhysics* phys, const SphCSR* fb,
+                  const SphGrid* bg, unsigned terms, float* dpdt, float* acc, hipStream_t s) {
+  int rc = check_cellwalk(g, segs, nseg, frec, brec, phys, fb, bg, terms, "sph_dsph_cellwalk");
   if (rc) return rc;
   if (nseg == 0 || g->np == 0) return SPH_OK;
   const SphRk4Stage none = {};
-  launch_cellwalk<false>(*g, segs, nseg, frec, brec, phys, csr_or_null(fb), terms, dpdt, acc,
-                         none, s);
+  launch_cellwalk<false>(*g, segs, nseg, frec, brec, phys, csr_or_null(fb), grid_or_null(bg),
+                         terms, dpdt, acc, none, s);
   return launch_check("sph_dsph_cellwalk");
 }
 
 int dsph_cellwalk_rk4(const SphGrid* g, const int* segs, int nseg, const float* frec,
                       const float* brec, const SphPhysics* phys, const SphCSR* fb,
-                      unsigned terms, const SphRk4Stage* rk, hipStream_t s) {
-  int rc = check_cellwalk(g, segs, nseg, frec, brec, phys, fb, terms, "sph_dsph_cellwalk_rk4");
+                      const SphGrid* bg, unsigned terms, const SphRk4Stage* rk, hipStream_t s) {
+  int rc = check_cellwalk(g, segs, nseg, frec, brec, phys, fb, bg, terms,
+                          "sph_dsph_cellwalk_rk4");
   if (rc) return rc;
   if ((rc = check_rk4(rk, frec, terms, "sph_dsph_cellwalk_rk4"))) return rc;
   if (nseg == 0 || g->np == 0) return SPH_OK;
-  launch_cellwalk<true>(*g, segs, nseg, frec, brec, phys, csr_or_null(fb), terms, nullptr,
-                        nullptr, *rk, s);
+  launch_cellwalk<true>(*g, segs, nseg, frec, brec, phys, csr_or_null(fb), grid_or_null(bg),
+                        terms, nullptr, nullptr, *rk, s);
   return launch_check("sph_dsph_cellwalk_rk4");
 }
 
@@ -1182,6 +1351,19 @@ int mdbc_density(const SphFluid* f, const SphPhysics* phys, const float* bx, con
   hipLaunchKernelGGL(k_mdbc_density, dim3(blocks_for(nb, kBS)), dim3(kBS), 0, s, *f, phys, bx, gx,
                      bscene, nb, *gf, rho_b, p_b, counts);
   return launch_check("sph_mdbc_density");
+}
+
+int mdbc_density_walk(const SphFluid* f, const SphPhysics* phys, const float* bx,
+                      const float* gx, const float* gh, const int* bscene, int nb,
+                      const SphGrid* g, float* rho_b, float* p_b, int* counts, hipStream_t s) {
+  SPH_REQUIRE(fluid_ok(f) && phys && bx && gx && gh && bscene && nb >= 0 && rho_b,
+              "sph_mdbc_density_walk: bad arguments");
+  SPH_REQUIRE(g && g->scenes && g->cell_start && g->sorted_idx && g->sorted_x && g->np == f->np,
+              "sph_mdbc_density_walk: the fluid grid is not built");
+  if (nb == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_mdbc_walk, dim3(blocks_for(nb, kBS)), dim3(kBS), 0, s, *f, phys, bx, gx,
+                     gh, bscene, nb, *g, rho_b, p_b, counts);
+  return launch_check("sph_mdbc_density_walk");
 }
 
 int eos_all(const float* rho, const int* pscene, const SphPhysics* phys, int n, float* p,
@@ -1242,14 +1424,15 @@ int sph_dsph_gather_rk4(const float* frec, const float* brec, const int* pscene,
   return sph::dsph_gather_rk4(frec, brec, pscene, np, phys, ff, fb, gradrho, terms, order, rk, s);
 }
 int sph_dsph_cellwalk(const SphGrid* g, const int* segs, int nseg, const float* frec,
-                      const float* brec, const SphPhysics* phys, const SphCSR* fb, unsigned terms,
-                      float* dpdt, float* acc, hipStream_t s) {
-  return sph::dsph_cellwalk(g, segs, nseg, frec, brec, phys, fb, terms, dpdt, acc, s);
+                      const float* brec, const SphPhysics* phys, const SphCSR* fb,
+                      const SphGrid* bg, unsigned terms, float* dpdt, float* acc, hipStream_t s) {
+  return sph::dsph_cellwalk(g, segs, nseg, frec, brec, phys, fb, bg, terms, dpdt, acc, s);
 }
 int sph_dsph_cellwalk_rk4(const SphGrid* g, const int* segs, int nseg, const float* frec,
                           const float* brec, const SphPhysics* phys, const SphCSR* fb,
-                          unsigned terms, const SphRk4Stage* rk, hipStream_t s) {
-  return sph::dsph_cellwalk_rk4(g, segs, nseg, frec, brec, phys, fb, terms, rk, s);
+                          const SphGrid* bg, unsigned terms, const SphRk4Stage* rk,
+                          hipStream_t s) {
+  return sph::dsph_cellwalk_rk4(g, segs, nseg, frec, brec, phys, fb, bg, terms, rk, s);
 }
 int sph_dsph_normalization(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
                            const SphBoundary* b, const SphCSR* fb, const int* order, float* M,
@@ -1268,6 +1451,12 @@ int sph_mdbc_density(const SphFluid* f, const SphPhysics* phys, const float* bx,
                      const int* bscene, int nb, const SphCSR* gf, float* rho_b, float* p_b,
                      int* counts, hipStream_t s) {
   return sph::mdbc_density(f, phys, bx, gx, bscene, nb, gf, rho_b, p_b, counts, s);
+}
+int sph_mdbc_density_walk(const SphFluid* f, const SphPhysics* phys, const float* bx,
+                          const float* gx, const float* gh, const int* bscene, int nb,
+                          const SphGrid* g, float* rho_b, float* p_b, int* counts,
+                          hipStream_t s) {
+  return sph::mdbc_density_walk(f, phys, bx, gx, gh, bscene, nb, g, rho_b, p_b, counts, s);
 }
 int sph_eos(const float* rho, const int* pscene, const SphPhysics* phys, int n, float* p,
             hipStream_t s) {

@@ -222,6 +222,20 @@ class SceneBatch:
                             hmax=s.support / d, p_begin=int(pb), p_count=int(s.n))
                        for s, pb in zip(scenes, self.p_begin)]
         self.grid = NeighborGrid(grid_scenes, NP, dev)
+        # delta-SPH batches: the static boundary points sorted once into a
+        # grid of the same cells; each stage finds its fluid -> boundary
+        # pairs in it (in the cell walk, or for the list by a search with
+        # the fluid points as queries), so list and cell walk add them in
+        # the same order.  The boundary -> fluid list is built only where
+        # it is read: DFSPH's Akinci rows and delta-plus's wall rows.
+        self.bgrid = None
+        self.needs_bf = scheme == "dfsph" or self.dplus or self.shifting
+        if scheme == "dsph" and self.solid and NB:
+            bscenes = [dict(gs, p_begin=int(bb), p_count=int(s.nb))
+                       for gs, bb, s in zip(grid_scenes, self.b_begin, scenes)]
+            self.bgrid = NeighborGrid(bscenes, NB, dev)
+            if dev.type == "cuda":      # not on the meta device of footprint()
+                self.bgrid.build(self.bx, self.bscene)
         self.ff = _PairBuf(NP, int(math.ceil(pair_cap * NP)), dev)
         bcap = int(math.ceil(bdy_pair_cap * max(NB, 1)))
         self.bf = _PairBuf(NB, bcap, dev)
@@ -319,6 +333,8 @@ class SceneBatch:
 
     def scene_status(self, k: int) -> int:
         st = int(self.grid.status[k].item())
+        if self.bgrid is not None:
+            st |= int(self.bgrid.status[k].item())
         if self.scheme == "dfsph":
             st |= int(self.df["status"][k].item())
         return st
@@ -478,24 +494,47 @@ class SceneBatch:
                L.ptr(self.scan_ws), int(self.scan_ws.numel()), L.stream_handle(s))
         g.fill(q, buf.rowptr, buf.col, s)
 
-    def search(self, x, s=None, fluid_pairs: bool = True):
+    def search(self, x, s=None, fluid_pairs: bool = True, ghost_pairs: bool = True,
+               bdy_pairs: bool = True):
         """Grid + every pair list of the current positions (without the
-        fluid list when not fluid_pairs)."""
+        fluid list when not fluid_pairs, the ghost list when not
+        ghost_pairs, the boundary lists when not bdy_pairs).  Delta-SPH
+        batches build the fluid -> boundary list by a search of the fluid
+        points in the static boundary grid, so its rows follow that grid's
+        order (the order the cell walk walks them in); DFSPH batches
+        transpose the boundary -> fluid list (rows by boundary index)."""
         g = self.grid
         g.build(x, self.pscene, s)
         if fluid_pairs:
             self._pairs(self.ff, g.query_struct(x, self.h, self.pscene, self.cell_div, True), s)
         if self.NB:
-            if self.solid:
+            if self.solid and ghost_pairs:
                 self._pairs(self.gf, g.query_struct(self.bgx, self.bh, self.bscene, self.cell_div,
                                                     False), s)
-            self._pairs(self.bf, g.query_struct(self.bx, self.bh, self.bscene, self.cell_div,
-                                                False), s)
-            c = self.bf.csr().c()
-            L.call("sph_csr_transpose", C.byref(c), self.NP, L.ptr(self.fb.counts),
-                   L.ptr(self.t_slot), L.ptr(self.fb.rowptr), L.ptr(self.fb.col),
-                   L.ptr(self.scan_ws), int(self.scan_ws.numel()), L.stream_handle(s))
+            if bdy_pairs and self.needs_bf:
+                self._pairs(self.bf, g.query_struct(self.bx, self.bh, self.bscene,
+                                                    self.cell_div, False), s)
+            if bdy_pairs and self.bgrid is not None:
+                bg = self.bgrid
+                q = bg.query_struct(x, self.h, self.pscene, self.cell_div, False)
+                bg.count(q, self.fb.counts, s)
+                L.call("sph_scan_exclusive", L.ptr(self.fb.counts), self.NP, L.ptr(self.fb.rowptr),
+                       L.ptr(self.scan_ws), int(self.scan_ws.numel()), L.stream_handle(s))
+                bg.fill(q, self.fb.rowptr, self.fb.col, s)
+            elif bdy_pairs:
+                c = self.bf.csr().c()
+                L.call("sph_csr_transpose", C.byref(c), self.NP, L.ptr(self.fb.counts),
+                       L.ptr(self.t_slot), L.ptr(self.fb.rowptr), L.ptr(self.fb.col),
+                       L.ptr(self.scan_ws), int(self.scan_ws.numel()), L.stream_handle(s))
         self._sorted_valid = True
+
+    def _bgrid_arg(self):
+        """The static boundary grid as the cell walk takes it (None: no
+        boundary points)."""
+        if self.bgrid is None:
+            return None
+        self._bgrid_c = self.bgrid._grid()
+        return C.byref(self._bgrid_c)
 
     def _order(self):
         # with the state in cell order, index order is already local
@@ -521,12 +560,20 @@ class SceneBatch:
         """Search, mDBC boundary density, delta-SPH density gradient and the
         records (fluid records into frec unless it is None) of the state
         (x, v, rho).  Returns the gather's arguments."""
-        self.search(x, s, fluid_pairs=not self.cellwalk)
+        # cell-walk batches find every pair in the kernels that use it: the
+        # stage builds the fluid grid and no list
+        w = self.cellwalk
+        self.search(x, s, fluid_pairs=not w, ghost_pairs=not w, bdy_pairs=not w)
         F = self.fluid(x, v, rho)
         sh = L.stream_handle(s)
         ff = self.ff.csr().c()
         fb = self.fb.csr().c() if self.NB else None
-        if self.NB:
+        if self.NB and self.cellwalk:
+            g = self.grid._grid()
+            L.call("sph_mdbc_density_walk", C.byref(F), L.ptr(self.phys), L.ptr(self.bx),
+                   L.ptr(self.bgx), L.ptr(self.bh), L.ptr(self.bscene), self.NB, C.byref(g),
+                   L.ptr(self.brho), L.ptr(self.bp), L.ptr(self.bcount), sh)
+        elif self.NB:
             gf = self.gf.csr().c()
             L.call("sph_mdbc_density", C.byref(F), L.ptr(self.phys), L.ptr(self.bx),
                    L.ptr(self.bgx), L.ptr(self.bscene), self.NB, C.byref(gf), L.ptr(self.brho),
@@ -559,8 +606,8 @@ class SceneBatch:
         if self.cellwalk:
             g = self.grid._grid()
             L.call("sph_dsph_cellwalk", C.byref(g), L.ptr(self.segs), self.nseg,
-                   L.ptr(self.frec), L.ptr(self.brec), L.ptr(self.phys), fbp, self.terms,
-                   L.ptr(self.dpdt), L.ptr(self.acc), L.stream_handle(s))
+                   L.ptr(self.frec), L.ptr(self.brec), L.ptr(self.phys), None, self._bgrid_arg(),
+                   self.terms, L.ptr(self.dpdt), L.ptr(self.acc), L.stream_handle(s))
             return
         L.call("sph_dsph_gather_packed", L.ptr(self.frec), L.ptr(self.brec), L.ptr(self.pscene),
                self.NP, L.ptr(self.phys), C.byref(ff), fbp, grr, self.terms, self._order(),
@@ -586,8 +633,8 @@ class SceneBatch:
                 if self.cellwalk:
                     g = self.grid._grid()
                     L.call("sph_dsph_cellwalk_rk4", C.byref(g), L.ptr(self.segs), self.nseg,
-                           L.ptr(frec), L.ptr(self.brec), L.ptr(self.phys), fbp, self.terms,
-                           C.byref(rk), sh)
+                           L.ptr(frec), L.ptr(self.brec), L.ptr(self.phys), None,
+                           self._bgrid_arg(), self.terms, C.byref(rk), sh)
                     continue
                 L.call("sph_dsph_gather_rk4", L.ptr(frec), L.ptr(self.brec), L.ptr(self.pscene),
                        self.NP, L.ptr(self.phys), C.byref(ff), fbp,
@@ -772,6 +819,8 @@ class SceneBatch:
     # ------------------------------------------------------------ queries
     def status(self) -> np.ndarray:
         st = self.grid.status.cpu().numpy()
+        if self.bgrid is not None:   # its table and the fluid -> boundary list
+            st = st | self.bgrid.status.cpu().numpy()
         if self.scheme == "dfsph":
             st = st | self.df["status"].cpu().numpy()
         return st

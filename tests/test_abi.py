@@ -88,9 +88,13 @@ def test_bad_arguments_rejected_on_host():
                                                              0, None, None),
         "sph_permute_fields": lambda: lib.sph_permute_fields(None, None, 0, None, None),
         "sph_dsph_cellwalk": lambda: lib.sph_dsph_cellwalk(None, None, 0, None, None, None, None,
-                                                           0, None, None, None),
+                                                           None, 0, None, None, None),
         "sph_dsph_cellwalk_rk4": lambda: lib.sph_dsph_cellwalk_rk4(None, None, 0, None, None,
-                                                                   None, None, 0, None, None),
+                                                                   None, None, None, 0, None,
+                                                                   None),
+        "sph_mdbc_density_walk": lambda: lib.sph_mdbc_density_walk(None, None, None, None, None,
+                                                                   None, 0, None, None, None,
+                                                                   None, None),
     }
     for name, call in cases.items():
         assert call() == 1, name

@@ -27,19 +27,22 @@ def bits(t):
 
 
 def both_rhs(b, segs=None):
-    """(dpdt, acc) of the list gather and of the cell walk on b's state."""
+    """(dpdt, acc) of the list gather and of the cell walk on b's state:
+    [list gather, walk with the boundary pairs from the list, walk with the
+    boundary pairs found in the static boundary grid]."""
     sh = L.stream_handle()
-    b.search(b.x)                                     # with the fluid list
+    b.search(b.x)                                     # with every list
     ff, fbp = b._dsph_prepare(b.x, b.v, b.rho, b.frec)
     out = []
-    for walk in (False, True):
+    for mode in ("list", "walk+fb", "walk+grid"):
         dpdt = torch.full_like(b.dpdt, float("nan"))
         acc = torch.full_like(b.acc, float("nan"))
-        if walk:
+        if mode != "list":
             sg = b.segs if segs is None else segs
             g = b.grid._grid()
+            bdy = (fbp, None) if mode == "walk+fb" else (None, b._bgrid_arg())
             L.call("sph_dsph_cellwalk", C.byref(g), L.ptr(sg), int(sg.shape[0]), L.ptr(b.frec),
-                   L.ptr(b.brec), L.ptr(b.phys), fbp, b.terms, L.ptr(dpdt), L.ptr(acc), sh)
+                   L.ptr(b.brec), L.ptr(b.phys), *bdy, b.terms, L.ptr(dpdt), L.ptr(acc), sh)
         else:
             L.call("sph_dsph_gather_packed", L.ptr(b.frec), L.ptr(b.brec), L.ptr(b.pscene),
                    b.NP, L.ptr(b.phys), C.byref(ff), fbp, None, b.terms, b._order(),
@@ -50,10 +53,11 @@ def both_rhs(b, segs=None):
 
 
 def assert_bitwise(out, what):
-    (d0, a0), (d1, a1) = out
+    (d0, a0) = out[0]
     assert np.isfinite(d0.cpu().numpy()).all() and np.isfinite(a0.cpu().numpy()).all(), what
-    assert np.array_equal(bits(d0), bits(d1)), (what, "dpdt")
-    assert np.array_equal(bits(a0), bits(a1)), (what, "acc")
+    for k, (d1, a1) in enumerate(out[1:], 1):
+        assert np.array_equal(bits(d0), bits(d1)), (what, k, "dpdt")
+        assert np.array_equal(bits(a0), bits(a1)), (what, k, "acc")
 
 
 def scene_sets():
@@ -108,9 +112,10 @@ def test_cellwalk_crowded_cells_and_outside_points(cuda, lanes, kl):
     # the crowded rows have far more hits than a row's list in LDS holds,
     # and their cells far more points than a run's stencil can stage
     assert rows.max() > 900 and nnz > 900_000
-    (d0, a0), (d1, a1) = out
     # the outside points have no fluid neighbour but themselves
-    assert np.array_equal(bits(d0), bits(d1)) and np.array_equal(bits(a0), bits(a1))
+    (d0, a0) = out[0]
+    for d1, a1 in out[1:]:
+        assert np.array_equal(bits(d0), bits(d1)) and np.array_equal(bits(a0), bits(a1))
 
 
 def test_cellwalk_any_partition(cuda, lanes):
@@ -127,7 +132,7 @@ def test_cellwalk_any_partition(cuda, lanes):
     }
     for what, sg in parts.items():
         out = both_rhs(b, torch.from_numpy(np.ascontiguousarray(sg)).to(dev))
-        assert_bitwise((ref[0], out[1]), what)
+        assert_bitwise([ref[0], *out[1:]], what)
 
 
 @pytest.mark.parametrize("kl", [1, 4])
@@ -156,5 +161,41 @@ def test_cellwalk_rejects_ddsph(cuda):
     g = b.grid._grid()
     with pytest.raises(RuntimeError, match="DDSPH"):
         L.call("sph_dsph_cellwalk", C.byref(g), L.ptr(b.segs), b.nseg, L.ptr(b.frec),
-               L.ptr(b.brec), L.ptr(b.phys), None, L.T_DDSPH, L.ptr(b.dpdt), L.ptr(b.acc),
-               L.stream_handle())
+               L.ptr(b.brec), L.ptr(b.phys), None, None, L.T_DDSPH, L.ptr(b.dpdt),
+               L.ptr(b.acc), L.stream_handle())
+
+
+@pytest.mark.parametrize("kind", ["dam_break", "drops", "mix"])
+def test_mdbc_walk_equals_list(cuda, kind):
+    """sph_mdbc_density_walk (each ghost's fluid neighbours found in the
+    kernel) against sph_mdbc_density over the ghost list of
+    sph_neighbor_fill: boundary densities, pressures and neighbour counts
+    bit for bit, from rest and after the fluid has moved."""
+    b = SceneBatch(scene_sets()[kind])
+    sh = L.stream_handle()
+    for _ in range(2):
+        b.search(b.x)                             # with the ghost list
+        F = b.fluid(b.x, b.v, b.rho)
+        gf, g = b.gf.csr().c(), b.grid._grid()
+        outs = []
+        for walk in (False, True):
+            rb = torch.full_like(b.brho, float("nan"))
+            pb = torch.full_like(b.bp, float("nan"))
+            cnt = torch.full_like(b.bcount, -1)
+            if walk:
+                L.call("sph_mdbc_density_walk", C.byref(F), L.ptr(b.phys), L.ptr(b.bx),
+                       L.ptr(b.bgx), L.ptr(b.bh), L.ptr(b.bscene), b.NB, C.byref(g), L.ptr(rb),
+                       L.ptr(pb), L.ptr(cnt), sh)
+            else:
+                L.call("sph_mdbc_density", C.byref(F), L.ptr(b.phys), L.ptr(b.bx), L.ptr(b.bgx),
+                       L.ptr(b.bscene), b.NB, C.byref(gf), L.ptr(rb), L.ptr(pb), L.ptr(cnt), sh)
+            outs.append((rb[: b.NB], pb[: b.NB], cnt[: b.NB]))
+        torch.cuda.synchronize()
+        (r0, p0, c0), (r1, p1, c1) = outs
+        # the drops touch no wall: their ghosts see no fluid
+        if kind != "drops":
+            assert int((c0 > 10).sum()) > 0 and int((c0 == 0).sum()) > 0
+        assert np.array_equal(c0.cpu().numpy(), c1.cpu().numpy())
+        assert np.array_equal(bits(r0), bits(r1)) and np.array_equal(bits(p0), bits(p1))
+        for _ in range(3):
+            b.step()

@@ -64,7 +64,7 @@ def test_dsph_rhs_with_mdbc(cuda, diffusion):
 
 
 @pytest.mark.parametrize("term", ["CONT", "MOG", "PRESS", "VISC", "LAM", "CONT_B", "PRESS_B",
-                                  "VISC_B"])
+                                  "VISC_B", "GRAV"])
 def test_each_term_separately(cuda, term):
     """Every reference module's term on its own (momentum.py, pressure.py,
     diffusion.py, laminar.py, densityDiffusion.py and the solid boundary
