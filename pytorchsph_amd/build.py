@@ -33,9 +33,9 @@ COMMON = [
     "-fPIC",
     # packed fp32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    # every fp32 operation rounded on its own (no fused multiply-add
-    # formation), as the reference's separate torch operations are; a sum
-    # then has the same bits in every kernel that evaluates it
+    # no implicit fused multiply-add formation: a sum then has the same
+    # bits in every kernel that evaluates it (multiply-adds are written out
+    # with fmaf where the code wants them)
     "-ffp-contract=off",
     "-Wno-unused-result",
     f"-I{INCLUDE}",

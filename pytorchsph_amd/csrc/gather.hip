@@ -105,6 +105,17 @@ struct Own {
   float gx, gy;  // renormalised density gradient (DDSPH)
 };
 
+// Partial sums of a row.  The pair routine keeps the common factors of the
+// reference's terms out of the sums; dsph_rates applies them once per row:
+//   cont     sum (v_ji . gradW) V_j rho_j                  (continuity)
+//   mog      sum (rho_j - rho_i) F R V_j; MOG sum = -2 mog
+//   dd       sum (psi_ij . gradW) V_j                       (delta-SPH diffusion)
+//   px, py   sum (p_i + p_j) V_j gradW                      (pressure)
+//   vx, vy   sum w c_ij gradW; artificial viscosity sum = -(vx, vy)
+//   lx, ly   sum F R m_j / (rho_i + rho_j)^2 v_ji; laminar sum = 8 nu rho_i (lx, ly)
+// with gradW_ij = F r_ij (r_ij = x_i - x_j, F = -g(q) / |r_ij|),
+// R = |r|^2 / (|r|^2 + eps), w = max(v_ji . r_ij, 0) and
+// c_ij = m_j / ((|r|^2 + eps)(rho_i + rho_j)).
 struct Acc {
   float cont, mog, dd;
   float px, py, vx, vy, lx, ly;
@@ -135,66 +146,79 @@ __device__ __forceinline__ void reduce_acc(Acc& a, unsigned terms) {
 }
 
 // One pair of the delta-SPH right-hand side (fluid row i, neighbour j).
-// gradW_ij = -g(q) rhat_ij, x_ji = x_j - x_i, r2e = |x_ji|^2 + eps.
+// The reference's gradient is gradW_ij = -g(q) rhat_ij with
+// g(q) = 20 (7 / pi h^3) q (1 - q)^3 and rhat = r / |r| (kernels.py:24-29),
+// so gradW = F r with F = -(20 (7 / pi) / h^4) (1 - q)^3: no division by |r|
+// and no q factor.  F = 0 for |r|^2 < 1e-14, where the reference zeroes
+// rhat (|r| < 1e-7, neighborSearch.py:220-221).  Then
+//   x_ji . gradW = -F |r|^2,   v_ji . gradW = F (v_ji . r),
+// and the terms' scalar factors share F, R and m_j = rho_j V_j (see Acc).
+// The multiply-adds are written out (fmaf), so every kernel that inlines
+// this routine computes the same bits.
 __device__ __forceinline__ void dsph_pair(const SphPhysics& P, const KConst& K, const Own& o,
                                           const Nbr& n, float grjx, float grjy, bool fluid,
                                           unsigned terms, Acc& a) {
-  const PairGeo g = pair_geo(o.x, o.y, n.x, n.y, K.inv_h);
-  const float gw = -wendland_grad_k(g.q, K);
-  const float gWx = gw * g.hx, gWy = gw * g.hy;
-  const float xgw = -(g.rx * gWx + g.ry * gWy);     // x_ji . gradW
-  const float inv_r2e = fast_rcp(g.d2 + P.eps);
+  const float rx = o.x - n.x, ry = o.y - n.y;
+  const float d2 = fmaf(ry, ry, rx * rx);
+  const float d = __builtin_amdgcn_sqrtf(d2);
+  const float om = fmaf(-d, K.inv_h, 1.f);           // 1 - q
+  const float F = d2 >= 1e-14f ? om * om * om * K.gr : 0.f;
+  const float ie = fast_rcp(d2 + P.eps);             // 1 / (|r|^2 + eps)
+  const float R = d2 * ie;
   const float vjix = n.vx - o.vx, vjiy = n.vy - o.vy;
+  const float pv = fmaf(vjiy, ry, vjix * rx);        // v_ji . r_ij = -(v_ji . x_ji)
+  const float mj = n.vol * n.rho;
+  const float FR = F * R;
   const bool cont = fluid ? has(terms, SPH_T_CONT) : has(terms, SPH_T_CONT_B);
   const bool press = fluid ? has(terms, SPH_T_PRESS) : has(terms, SPH_T_PRESS_B);
   const bool visc = fluid ? has(terms, SPH_T_VISC) : has(terms, SPH_T_VISC_B);
   // momentum.py:31-38
-  if (cont) a.cont += (vjix * gWx + vjiy * gWy) * n.vol * n.rho;
-  // densityDiffusion.py:74-82
-  if (fluid && has(terms, SPH_T_MOG)) a.mog += 2.f * (n.rho - o.rho) * inv_r2e * xgw * n.vol;
-  // densityDiffusion.py:63-71
+  if (cont) a.cont = fmaf(F * pv, mj, a.cont);
+  // densityDiffusion.py:74-82: 2 (rho_j - rho_i) (x_ji . gradW) / (|r|^2 + eps) V_j
+  if (fluid && has(terms, SPH_T_MOG)) a.mog = fmaf(FR, (n.rho - o.rho) * n.vol, a.mog);
+  // densityDiffusion.py:63-71: psi = -2 (rho_j - rho_i) r / (|r|^2 + eps)
+  // - (<grad rho>_i + <grad rho>_j)
   if (fluid && has(terms, SPH_T_DDSPH)) {
-    const float c = 2.f * (n.rho - o.rho) * inv_r2e;
-    const float psx = -c * g.rx - (o.gx + grjx);
-    const float psy = -c * g.ry - (o.gy + grjy);
-    a.dd += (psx * gWx + psy * gWy) * n.vol;
+    const float gs = fmaf(o.gy + grjy, ry, (o.gx + grjx) * rx);
+    const float psr = fmaf(-2.f * (n.rho - o.rho), R, -gs);   // psi . r
+    a.dd = fmaf(F * psr, n.vol, a.dd);
   }
   // pressure.py:31-38
   if (press) {
-    const float c = (o.p + n.p) * n.vol;
-    a.px += c * gWx;
-    a.py += c * gWy;
+    const float c = (o.p + n.p) * n.vol * F;
+    a.px = fmaf(c, rx, a.px);
+    a.py = fmaf(c, ry, a.py);
   }
   if (visc || (fluid && has(terms, SPH_T_LAM))) {
     const float inv_rs = fast_rcp(o.rho + n.rho);
-    // diffusion.py:30-44
+    // diffusion.py:30-44: min(v_ji . x_ji, 0) = -max(v_ji . r_ij, 0)
     if (visc) {
-      float pij = -(vjix * g.rx + vjiy * g.ry);     // v_ji . x_ji
-      pij = pij > 0.f ? 0.f : pij;
-      const float c = pij * inv_r2e * n.vol * n.rho * inv_rs;
-      a.vx += c * gWx;
-      a.vy += c * gWy;
+      const float c = fmaxf(pv, 0.f) * ie * mj * inv_rs * F;
+      a.vx = fmaf(c, rx, a.vx);
+      a.vy = fmaf(c, ry, a.vy);
     }
     // laminar.py:30-46: mu_i + mu_j = 2 rho_i nu (muj indexes rhoj by i)
     if (fluid && has(terms, SPH_T_LAM)) {
-      const float mj = n.rho * n.vol;
-      const float term = 4.f * mj * (2.f * o.rho * P.nu) * xgw * inv_rs * inv_rs * inv_r2e;
-      a.lx += term * (-vjix);
-      a.ly += term * (-vjiy);
+      const float c = FR * mj * inv_rs * inv_rs;
+      a.lx = fmaf(c, vjix, a.lx);
+      a.ly = fmaf(c, vjiy, a.ly);
     }
   }
 }
 
-// dp/dt and acceleration of a row from its sums.  No FMA contraction, so
-// the values are the same whichever kernel (plain or fused with the RK4
-// combine) evaluates them.
+// dp/dt and acceleration of a row from its sums (the factors kept out of
+// the pair sums, see Acc).  Written without FMA contraction, so the values
+// are the same whichever kernel (plain or fused with the RK4 combine)
+// evaluates them.
 __device__ __forceinline__ void dsph_rates(const SphPhysics& P, const Own& o, const Acc& a,
                                            unsigned terms, float& dp, float& ax, float& ay) {
 #pragma clang fp contract(off)
-  dp = -a.cont + P.h * P.delta * P.c0 * (a.mog + a.dd);
+  dp = -a.cont + P.h * P.delta * P.c0 * (-2.f * a.mog + a.dd);
   const float inv_rho = 1.f / o.rho;
-  ax = -a.px * inv_rho + P.h * P.alpha * P.c0 * a.vx - a.lx;
-  ay = -a.py * inv_rho + P.h * P.alpha * P.c0 * a.vy - a.ly;
+  const float av = P.h * P.alpha * P.c0;
+  const float al = 8.f * P.nu * o.rho;
+  ax = -a.px * inv_rho - av * a.vx - al * a.lx;
+  ay = -a.py * inv_rho - av * a.vy - al * a.ly;
   if (has(terms, SPH_T_GRAV)) {
     ax += P.gx;
     ay += P.gy;
@@ -341,17 +365,19 @@ __device__ __forceinline__ void packed_bdy_pairs(const SphPhysics& P, const KCon
 // Output of row i from its reduced sums.  FUSE: the row's RK4 stage
 // combine (rk4_particle) and the next stage's record instead of dp/dt and
 // a (see sph_dsph_gather_rk4).
+// s0: the row's state at the step's start (rk.x0, rk.v0, rk.r0 of row i),
+// when FUSE.
 template <bool FUSE>
 __device__ __forceinline__ void packed_row_out(const SphPhysics& P, const Own& o, const Acc& a,
                                                unsigned terms, int i, float* __restrict__ dpdt,
-                                               float* __restrict__ acc, const SphRk4Stage& rk) {
+                                               float* __restrict__ acc, const SphRk4Stage& rk,
+                                               const Rk4State& s0) {
   if (!FUSE) {
     dsph_store(P, o, a, terms, i, dpdt, acc);
     return;
   }
   float dp, ax, ay;
   dsph_rates(P, o, a, terms, dp, ax, ay);
-  const Rk4State s0 = {rk.x0[2 * i], rk.x0[2 * i + 1], rk.v0[2 * i], rk.v0[2 * i + 1], rk.r0[i]};
   float2 asum;
   const Rk4State n = rk4_particle(rk.stage, P, ax, ay, o.vx, o.vy, dp, true,
                                   rk.K + 5 * (size_t)i, s0, &asum);
@@ -431,7 +457,11 @@ __global__ __launch_bounds__(kBS) void k_dsph_gather_packed(
   }
   packed_bdy_pairs<KL>(P, K, o, brec, fb, i, rl.sub, terms, a);
   reduce_acc<KL>(a, terms);
-  if (rl.sub == 0) packed_row_out<FUSE>(P, o, a, terms, i, dpdt, acc, rk);
+  if (rl.sub == 0) {
+    Rk4State s0 = {};
+    if (FUSE) s0 = {rk.x0[2 * i], rk.x0[2 * i + 1], rk.v0[2 * i], rk.v0[2 * i + 1], rk.r0[i]};
+    packed_row_out<FUSE>(P, o, a, terms, i, dpdt, acc, rk, s0);
+  }
 }
 
 // ------------------------------------------- cell-walk delta-SPH gather
@@ -479,7 +509,7 @@ struct WalkShared {
 template <int KL>
 __device__ __forceinline__ void walk_list(const SphPhysics& P, const KConst& K, const Own& o,
                                           unsigned terms, const unsigned short* lst, int n,
-                                          int sub, const int* ridx,
+                                          int sub, const WalkShared& S,
                                           const float4* __restrict__ frec, Acc& a) {
   for (int kb = sub; kb < n; kb += KL * kPackedBatch) {
     bool ok[kPackedBatch];
@@ -489,7 +519,7 @@ __device__ __forceinline__ void walk_list(const SphPhysics& P, const KConst& K, 
       const int k = kb + u * KL;
       ok[u] = k < n;
       if (ok[u]) {
-        const int j = ridx[lst[k]];
+        const int j = S.ridx[lst[k]];
         r0[u] = frec[2 * j];
         r1[u] = frec[2 * j + 1];
       } else {
@@ -558,26 +588,19 @@ __device__ __forceinline__ void walk_slow(bool slow, const SphGrid& g, int sc, i
   }
 }
 
-// Boundary pairs of the fluid point at (xq, yq) (scene sc) from the
-// boundary grid bg: the boundary points of its stencil in bg's sorted order
-// that pass the predicate -- the row of a fluid -> boundary search against
-// bg (sph_neighbor_fill with the fluid points as queries) -- walked with the
-// packed gather's lane assignment (the hit of rank k on lane k mod KL).
-// Every lane of the wavefront runs the loop (ballots and shuffles), so call
-// it converged; `valid` false: no row.
+// Boundary pairs of the fluid point at (xq, yq) from the boundary grid bg:
+// the boundary points of its stencil st in bg (ncand candidates, 0: no
+// row) in bg's sorted order that pass the predicate -- the row of a fluid
+// -> boundary search against bg (sph_neighbor_fill with the fluid points as
+// queries) -- walked with the packed gather's lane assignment (the hit of
+// rank k on lane k mod KL).  Every lane of the wavefront runs the loop
+// (ballots and shuffles), so call it converged.
 template <int KL>
-__device__ __forceinline__ void walk_bdy(bool valid, const SphGrid& bg, int sc, float xq, float yq,
-                                         float thr, int sub, const SphPhysics& P,
-                                         const KConst& K, const Own& o, unsigned terms,
-                                         const float4* __restrict__ brec, Acc& a) {
-  Stencil3 st;
-  int ncand = 0;
-  if (valid) {
-    const SphGridDesc d = bg.scenes[bg.nscenes == 1 ? 0 : sc];
-    st.init(d, bg.cell_start + d.cell_begin, cell_coord(xq, d.qmin_x, d.hmax),
-            cell_coord(yq, d.qmin_y, d.hmax));
-    ncand = st.ncand;
-  }
+__device__ __forceinline__ void walk_bdy(const Stencil3& st, int ncand, const SphGrid& bg,
+                                         float xq, float yq, float thr, int sub,
+                                         const SphPhysics& P, const KConst& K, const Own& o,
+                                         unsigned terms, const float4* __restrict__ brec,
+                                         Acc& a) {
   const float2* __restrict__ bsx = reinterpret_cast<const float2*>(bg.sorted_x);
   if constexpr (KL == 1) {
     for (int p = 0; p < ncand; ++p) {
@@ -621,9 +644,20 @@ __device__ __forceinline__ void walk_bdy(bool valid, const SphGrid& bg, int sc, 
   }
 }
 
+// The stencil of the point at (xq, yq) in the boundary grid's scene bd;
+// (fix, fiy) is its cell in the fluid grid, taken when bd has the fluid
+// grid's origin and cell size (`same`).
+__device__ __forceinline__ void bdy_stencil(const SphGrid& bg, const SphGridDesc& bd, bool same,
+                                            float xq, float yq, int fix, int fiy, Stencil3& st) {
+  st.init(bd, bg.cell_start + bd.cell_begin, same ? fix : cell_coord(xq, bd.qmin_x, bd.hmax),
+          same ? fiy : cell_coord(yq, bd.qmin_y, bd.hmax));
+}
+
 // segs[b] = {first, end} sorted slots of workgroup b (one scene each).
+// Five wavefronts per SIMD is what the LDS of the stage allows; the
+// register budget is held to it (96 VGPRs), since 100 registers left four.
 template <unsigned T, int KL, bool FUSE>
-__global__ __launch_bounds__(kBS) void k_dsph_cellwalk(
+__global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5))) void k_dsph_cellwalk(
     SphGrid g, const int2* __restrict__ segs, const float4* __restrict__ frec,
     const float4* __restrict__ brec, const SphPhysics* __restrict__ phys, SphCSR fb, SphGrid bg,
     unsigned rterms, float* __restrict__ dpdt, float* __restrict__ acc, SphRk4Stage rk) {
@@ -638,39 +672,62 @@ __global__ __launch_bounds__(kBS) void k_dsph_cellwalk(
     const int sp = c0 + t;
     const bool valid = sp < s1;
     Stencil3 st;
+    SphGridDesc d = {};
     float xq = 0.f, yq = 0.f, thr = 0.f;
+    int qi = 0, sc = 0, ix = 0, iy = 0;
     if (valid) {
-      const int qi = g.sorted_idx[sp];
-      const int sc = g.nscenes == 1 ? 0 : g.pscene[qi];
-      const SphGridDesc d = g.scenes[sc];
+      qi = g.sorted_idx[sp];
+      sc = g.nscenes == 1 ? 0 : g.pscene[qi];
+      d = g.scenes[sc];
       const float2 q = sx[sp];
       xq = q.x;
       yq = q.y;
-      st.init(d, g.cell_start + d.cell_begin, cell_coord(xq, d.qmin_x, d.hmax),
-              cell_coord(yq, d.qmin_y, d.hmax));
+      ix = cell_coord(xq, d.qmin_x, d.hmax);
+      iy = cell_coord(yq, d.qmin_y, d.hmax);
+      st.init(d, g.cell_start + d.cell_begin, ix, iy);
       thr = sq_threshold(phys[sc].h);
     }
-    // ---- stage: the chunk's three stencil runs, positions and indices
+    // ---- stage: the chunk's three stencil runs, positions and indices,
+    // each run copied by its own loop (no run select per element)
     const ChunkRuns cr = chunk_runs(st, valid, S.wlo, S.whi);
     const int tot = cr.tot;
     const bool staged = tot <= kWalkRecs;
     if (staged) {
-      for (int k = t; k < tot; k += kBS) {
-        const int p = cr.slot(k);
-        S.pos[k] = sx[p];
-        S.ridx[k] = g.sorted_idx[p];
+      const int rlo[3] = {cr.lo0, cr.lo1, cr.lo2};
+      const int rbeg[3] = {0, cr.off1, cr.off2};
+      const int rend[3] = {cr.off1, cr.off2, tot};
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const int shift = rlo[r] - rbeg[r];
+        for (int k = rbeg[r] + t; k < rend[r]; k += kBS) {
+          S.pos[k] = sx[k + shift];
+          S.ridx[k] = g.sorted_idx[k + shift];
+        }
       }
     }
     __syncthreads();
-    // ---- test: row t's hits, in candidate order
+    // ---- test: row t's hits, in candidate order; whole batches of
+    // kWalkBatch LDS loads need no bound, a run's last partial batch is
+    // masked
     if (valid && staged) {
       unsigned short* lst = S.list[t];
       int cnt = 0;
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
         const int b = cr.first(r, st), e = b + (st.rb[r] - st.ra[r]);
-        // kWalkBatch LDS loads in flight per trip, no branch per candidate
-        for (int k = b; k < e; k += kWalkBatch) {
+        int k = b;
+        for (; k + kWalkBatch <= e; k += kWalkBatch) {
+          float2 xy[kWalkBatch];
+#pragma unroll
+          for (int u = 0; u < kWalkBatch; ++u) xy[u] = S.pos[k + u];
+#pragma unroll
+          for (int u = 0; u < kWalkBatch; ++u) {
+            const float d2 = dist2_nofma(xq, yq, xy[u].x, xy[u].y);
+            lst[cnt < kWalkCap ? cnt : kWalkCap] = (unsigned short)(k + u);
+            cnt += d2 <= thr ? 1 : 0;
+          }
+        }
+        if (k < e) {
           float2 xy[kWalkBatch];
 #pragma unroll
           for (int u = 0; u < kWalkBatch; ++u) xy[u] = S.pos[k + u];
@@ -686,15 +743,26 @@ __global__ __launch_bounds__(kBS) void k_dsph_cellwalk(
       S.cnt[t] = (unsigned short)(cnt > kWalkCap ? kWalkCap + 1 : cnt);
     }
     __syncthreads();
-    // ---- gather: KL lanes per row
+    // ---- gather: KL lanes per row.  With one lane per row, row rr is the
+    // thread's own row of the test, whose index, scene, cell and threshold
+    // are still in registers.
     const int nrow = s1 - c0 < kWalkRows ? s1 - c0 : kWalkRows;
     const int sub = t % KL;
     for (int rr = t / KL; rr < nrow; rr += kBS / KL) {
       const int spr = c0 + rr;
-      const int qi = g.sorted_idx[spr];
-      const int sc = g.nscenes == 1 ? 0 : g.pscene[qi];
+      if (KL > 1) {
+        qi = g.sorted_idx[spr];
+        sc = g.nscenes == 1 ? 0 : g.pscene[qi];
+        d = g.scenes[sc];
+        const float2 q = sx[spr];
+        xq = q.x;
+        yq = q.y;
+        ix = cell_coord(xq, d.qmin_x, d.hmax);
+        iy = cell_coord(yq, d.qmin_y, d.hmax);
+      }
       const SphPhysics P = phys[sc];
       const KConst K = kconst(P.h);
+      if (KL > 1) thr = sq_threshold(P.h);
       const float4 a0 = frec[2 * qi], a1 = frec[2 * qi + 1];
       Own o;
       o.x = a0.x; o.y = a0.y; o.vx = a0.z; o.vy = a0.w;
@@ -705,23 +773,30 @@ __global__ __launch_bounds__(kBS) void k_dsph_cellwalk(
       Acc a = {};
       if (terms & kFluidPairTerms) {
         const int n = staged ? S.cnt[rr] : 0;
-        walk_list<KL>(P, K, o, terms, S.list[rr], n < kWalkCap ? n : kWalkCap, sub, S.ridx,
-                      frec, a);
+        walk_list<KL>(P, K, o, terms, S.list[rr], n < kWalkCap ? n : kWalkCap, sub, S, frec, a);
         const bool slow = !staged || n > kWalkCap;
         if (__ballot(slow) != 0)
-          walk_slow<KL>(slow, g, sc, spr, sq_threshold(P.h), staged ? kWalkCap : 0, sub, P, K,
-                        o, terms, frec, a);
+          walk_slow<KL>(slow, g, sc, spr, thr, staged ? kWalkCap : 0, sub, P, K, o, terms, frec,
+                        a);
       }
       if (bg.cell_start) {
         if (terms & kBdyPairTerms) {
-          const float2 q = sx[spr];
-          walk_bdy<KL>(true, bg, sc, q.x, q.y, sq_threshold(P.h), sub, P, K, o, terms, brec, a);
+          const SphGridDesc bd = bg.scenes[bg.nscenes == 1 ? 0 : sc];
+          const bool same = bd.qmin_x == d.qmin_x && bd.qmin_y == d.qmin_y && bd.hmax == d.hmax;
+          Stencil3 bst;
+          bdy_stencil(bg, bd, same, xq, yq, ix, iy, bst);
+          walk_bdy<KL>(bst, bst.ncand, bg, xq, yq, thr, sub, P, K, o, terms, brec, a);
         }
       } else {
         packed_bdy_pairs<KL>(P, K, o, brec, fb, qi, sub, terms, a);
       }
       reduce_acc<KL>(a, terms);
-      if (sub == 0) packed_row_out<FUSE>(P, o, a, terms, qi, dpdt, acc, rk);
+      if (sub == 0) {
+        Rk4State s0 = {};
+        if (FUSE) s0 = {rk.x0[2 * qi], rk.x0[2 * qi + 1], rk.v0[2 * qi], rk.v0[2 * qi + 1],
+                        rk.r0[qi]};
+        packed_row_out<FUSE>(P, o, a, terms, qi, dpdt, acc, rk, s0);
+      }
     }
     __syncthreads();
   }

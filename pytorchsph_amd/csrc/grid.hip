@@ -18,8 +18,8 @@ constexpr int kBS = 256;
 // keeps its state that way) put runs of equal keys in neighbouring lanes:
 // each run takes its slots with one atomic, issued by the run's first lane,
 // and its lanes take consecutive slots.  (Which points fill which slots of
-// a cell still depends on the order the atomics land; k_grid_cell_sort
-// makes it deterministic.)
+// a cell still depends on the order the atomics land; k_grid_rank makes it
+// deterministic.)
 // keys_in == nullptr: the key is computed from the position; else it is
 // keys_in[p] (sph_grid_build_keyed).
 __global__ __launch_bounds__(kBS) void k_grid_keys(SphGrid g, const int* __restrict__ keys_in) {
@@ -79,18 +79,17 @@ __global__ __launch_bounds__(kBS) void k_grid_keys(SphGrid g, const int* __restr
   if (valid) g.slot[p] = base + (lane - start);
 }
 
+// Places each point at its atomic slot of its cell: the pair {point, global
+// cell} goes into the sorted position, staged in sorted_x's memory (8 bytes
+// per slot) until k_grid_rank and k_grid_place have ordered each cell.
 __global__ __launch_bounds__(kBS) void k_grid_scatter(SphGrid g) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= g.np) return;
   const int s = g.pscene[p];
-  const int pos = g.cell_start[g.scenes[s].cell_begin + g.key[p]] + g.slot[p];
-  g.sorted_idx[pos] = p;
+  const int kg = g.scenes[s].cell_begin + g.key[p];
+  const int pos = g.cell_start[kg] + g.slot[p];
+  reinterpret_cast<int2*>(g.sorted_x)[pos] = make_int2(p, kg);
 }
-
-// One thread per cell: order the cell's points by index (atomics gave them
-// in arbitrary order) and copy their positions into sorted order.
-constexpr int kRegSort = 16;   // cells up to this size are sorted in registers
-constexpr int kCandBatch = 4;  // candidates each lane loads at once in a search pass
 
 // Whether global cell c is its scene's slot of outside points (the scene's
 // last slot): stencils never visit it, so its order does not matter.
@@ -104,56 +103,39 @@ __device__ bool outside_slot(const SphGrid& g, int c) {
   return (int64_t)c == (int64_t)d.cell_begin + (int64_t)d.cells_x * d.cells_y;
 }
 
-// Insertion network over N registers (unused entries hold INT_MAX).
-template <int N>
-__device__ __forceinline__ void sort_cell_regs(int* __restrict__ idx, int a, int b) {
-  int v[N];
-#pragma unroll
-  for (int u = 0; u < N; ++u) v[u] = a + u < b ? idx[a + u] : 0x7fffffff;
-#pragma unroll
-  for (int i = 1; i < N; ++i) {
-#pragma unroll
-    for (int j = i; j > 0; --j) {
-      const int lo = min(v[j - 1], v[j]), hi = max(v[j - 1], v[j]);
-      v[j - 1] = lo;
-      v[j] = hi;
-    }
+// An outside slot holding more points than this keeps the atomic order
+// (ranking its k points costs k^2 loads, and no stencil reads it).
+constexpr int kRankOutsideMax = 64;
+
+// One thread per sorted position: the rank of its point among the points of
+// its cell by index (the order the atomics left them in is arbitrary), as
+// the point's final slot.  Reads the staged pairs of k_grid_scatter; points
+// of the engine's cell-ordered state make these reads nearly contiguous.
+__global__ __launch_bounds__(kBS) void k_grid_rank(SphGrid g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.np) return;
+  const int2* __restrict__ tmp = reinterpret_cast<const int2*>(g.sorted_x);
+  const int2 e = tmp[i];
+  const int a = g.cell_start[e.y], b = g.cell_start[e.y + 1];
+  int rank = 0;
+  if (b - a > kRankOutsideMax && outside_slot(g, e.y)) {
+    rank = i - a;
+  } else {
+    for (int q = a; q < b; ++q) rank += tmp[q].x < e.x ? 1 : 0;
   }
-#pragma unroll
-  for (int u = 0; u < N; ++u)
-    if (a + u < b) idx[a + u] = v[u];
+  g.slot[e.x] = a + rank;
 }
 
-__global__ __launch_bounds__(kBS) void k_grid_cell_sort(SphGrid g) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.ncells) return;
-  const int a = g.cell_start[c];
-  const int b = g.cell_start[c + 1];
-  const float2* x = reinterpret_cast<const float2*>(g.x);
-  float2* sx = reinterpret_cast<float2*>(g.sorted_x);
-  int* idx = g.sorted_idx;
-  // most cells of a tank hold no point or one: nothing to order
-  if (b - a <= 1) {
-    if (b > a) sx[a] = x[idx[a]];
-    return;
-  }
-  if (b - a <= 8) {
-    sort_cell_regs<8>(idx, a, b);
-  } else if (b - a <= kRegSort) {
-    sort_cell_regs<kRegSort>(idx, a, b);
-  } else if (!outside_slot(g, c)) {
-    for (int i = a + 1; i < b; ++i) {
-      const int v = idx[i];
-      int j = i - 1;
-      while (j >= a && idx[j] > v) {
-        idx[j + 1] = idx[j];
-        --j;
-      }
-      idx[j + 1] = v;
-    }
-  }
-  for (int i = a; i < b; ++i) sx[i] = x[idx[i]];
+// One thread per point: the point and its position into their sorted slot.
+__global__ __launch_bounds__(kBS) void k_grid_place(SphGrid g) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  const int pos = g.slot[p];
+  g.sorted_idx[pos] = p;
+  reinterpret_cast<float2*>(g.sorted_x)[pos] = reinterpret_cast<const float2*>(g.x)[p];
 }
+
+constexpr int kCandBatch = 4;  // candidates each lane loads at once in a search pass
 
 // Count (FILL = false) or emit (FILL = true) the pairs of every query.
 // Stencil rows are walked bottom to top and each row is one contiguous range
@@ -438,9 +420,12 @@ static int grid_build_impl(const SphGrid* gp, const int* keys, void* ws, size_t 
     hipLaunchKernelGGL(k_grid_keys, dim3(blocks_for(g.np, kBS)), dim3(kBS), 0, s, g, keys);
   if ((rc = launch_check("sph_grid_build(keys)"))) return rc;
   if ((rc = scan_exclusive(g.cell_count, g.ncells, g.cell_start, ws, ws_bytes, s))) return rc;
-  if (g.np > 0)
-    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks_for(g.np, kBS)), dim3(kBS), 0, s, g);
-  hipLaunchKernelGGL(k_grid_cell_sort, dim3(blocks_for(g.ncells, kBS)), dim3(kBS), 0, s, g);
+  if (g.np > 0) {
+    const dim3 grid(blocks_for(g.np, kBS));
+    hipLaunchKernelGGL(k_grid_scatter, grid, dim3(kBS), 0, s, g);
+    hipLaunchKernelGGL(k_grid_rank, grid, dim3(kBS), 0, s, g);
+    hipLaunchKernelGGL(k_grid_place, grid, dim3(kBS), 0, s, g);
+  }
   return launch_check("sph_grid_build");
 }
 

@@ -256,6 +256,7 @@ struct KConst {
   float w0;     // 7 / (pi h^2)
   float g0;     // 20 * 7 / (pi h^3)
   float s0;     // 30 / (pi h^3)
+  float gr;     // -20 * 7 / (pi h^4): grad W = gr (1 - q)^3 (x_i - x_j)
 };
 __device__ __forceinline__ KConst kconst(float h) {
   KConst k;
@@ -263,6 +264,7 @@ __device__ __forceinline__ KConst kconst(float h) {
   k.w0 = kC7pi * k.inv_h * k.inv_h;
   k.g0 = 20.f * kC7pi * k.inv_h * k.inv_h * k.inv_h;
   k.s0 = kC30pi * k.inv_h * k.inv_h * k.inv_h;
+  k.gr = -k.g0 * k.inv_h;
   return k;
 }
 __device__ __forceinline__ float wendland_k(float q, const KConst& k) {
