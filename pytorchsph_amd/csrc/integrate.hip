@@ -70,33 +70,60 @@ __global__ __launch_bounds__(kBS) void k_permute(const float* __restrict__ in,
   out[t] = in[src * width + c];
 }
 
-// Row k of tmp = every field of element perm[k], fields one after the other.
-__global__ __launch_bounds__(kBS) void k_fields_gather(SphFields f, const int* __restrict__ perm,
-                                                       int n, int rw, float* __restrict__ tmp) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const int64_t src = perm[k];
-  float* __restrict__ row = tmp + (int64_t)k * rw;
-  int o = 0;
-  for (int q = 0; q < f.nfields; ++q) {
-    const int w = f.width[q];
-    const float* __restrict__ in = f.ptr[q] + src * w;
-    for (int c = 0; c < w; ++c) row[o + c] = in[c];
-    o += w;
+// Field q of element perm[k] into tmp's block of field q (fields stored one
+// after the other, each [n][width]): one thread per element, each field
+// moved with one load and one store of its width, so a wavefront writes
+// each field's rows contiguously.
+__device__ __forceinline__ void move_words(const float* __restrict__ in, float* __restrict__ out,
+                                           int w) {
+  // a vector access where both ends are aligned to it (a field block of
+  // tmp starts wherever the widths before it put it)
+  const uintptr_t al = reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out);
+  if (w == 2 && (al & 7) == 0) {
+    *reinterpret_cast<float2*>(out) = *reinterpret_cast<const float2*>(in);
+  } else if (w == 4 && (al & 15) == 0) {
+    *reinterpret_cast<float4*>(out) = *reinterpret_cast<const float4*>(in);
+  } else {
+    for (int c = 0; c < w; ++c) out[c] = in[c];
   }
 }
 
-__global__ __launch_bounds__(kBS) void k_fields_scatter(SphFields f, int n, int rw,
-                                                        const float* __restrict__ tmp) {
+__global__ __launch_bounds__(kBS) void k_fields_gather(SphFields f, const int* __restrict__ perm,
+                                                       int n, float* __restrict__ tmp) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const float* __restrict__ row = tmp + (int64_t)k * rw;
-  int o = 0;
+  const int64_t src = perm[k];
+  int64_t o = 0;   // words of tmp before field q
   for (int q = 0; q < f.nfields; ++q) {
     const int w = f.width[q];
-    float* __restrict__ out = f.ptr[q] + (int64_t)k * w;
-    for (int c = 0; c < w; ++c) out[c] = row[o + c];
-    o += w;
+    move_words(f.ptr[q] + src * w, tmp + o + (int64_t)k * w, w);
+    o += (int64_t)n * w;
+  }
+}
+
+// tmp's field blocks back into the fields: a flat copy of every field's
+// n * width words, four at a time (the host checks that every field and
+// block is 16-byte aligned; otherwise one word at a time, vec = false).
+template <bool VEC>
+__global__ __launch_bounds__(kBS) void k_fields_copy(SphFields f, int n,
+                                                     const float* __restrict__ tmp) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t o = 0;
+  for (int q = 0; q < f.nfields; ++q) {
+    const int64_t words = (int64_t)n * f.width[q];
+    const int64_t units = VEC ? (words + 3) / 4 : words;
+    if (t < units) {
+      float* __restrict__ out = f.ptr[q];
+      const float* __restrict__ in = tmp + o;
+      if (VEC && 4 * t + 4 <= words) {
+        reinterpret_cast<float4*>(out)[t] = reinterpret_cast<const float4*>(in)[t];
+      } else if (VEC) {
+        for (int64_t c = 4 * t; c < words; ++c) out[c] = in[c];
+      } else {
+        out[t] = in[t];
+      }
+    }
+    o += words;
   }
 }
 
@@ -113,9 +140,23 @@ int permute_fields(const SphFields* f, const int* perm, int n, float* tmp, hipSt
     rw += f->width[q];
   }
   if (n == 0 || rw == 0) return SPH_OK;
-  hipLaunchKernelGGL(k_fields_gather, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, *f, perm, n, rw,
+  hipLaunchKernelGGL(k_fields_gather, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, *f, perm, n,
                      tmp);
-  hipLaunchKernelGGL(k_fields_scatter, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, *f, n, rw, tmp);
+  // every field and every block of tmp 16-byte aligned: four words per copy
+  bool vec = (reinterpret_cast<uintptr_t>(tmp) & 15) == 0;
+  int64_t o = 0, most = 0;
+  for (int q = 0; q < f->nfields; ++q) {
+    const int64_t words = (int64_t)n * f->width[q];
+    vec = vec && (reinterpret_cast<uintptr_t>(f->ptr[q]) & 15) == 0 && (o & 3) == 0;
+    o += words;
+    most = words > most ? words : most;
+  }
+  if (vec)
+    hipLaunchKernelGGL(k_fields_copy<true>, dim3(blocks_for((most + 3) / 4, kBS)), dim3(kBS), 0, s,
+                       *f, n, tmp);
+  else
+    hipLaunchKernelGGL(k_fields_copy<false>, dim3(blocks_for(most, kBS)), dim3(kBS), 0, s, *f, n,
+                       tmp);
   return launch_check("sph_permute_fields");
 }
 

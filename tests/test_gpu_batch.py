@@ -112,15 +112,17 @@ def test_fused_rk4_step_matches_separate_kernels(cuda, reorder):
                                                     float(d.max()), float(np.abs(b[key]).max()))
 
 
-def test_permute_fields_matches_numpy(cuda):
+@pytest.mark.parametrize("n", [100_003, 100_000])
+def test_permute_fields_matches_numpy(cuda, n):
     """sph_permute_fields moves every field by the same permutation, int64
-    ids bit for bit, with fields of widths 1, 2 and 4."""
+    ids bit for bit, with fields of widths 1, 2 and 4: with an odd count the
+    field blocks of the temporary sit at unaligned offsets (word moves), with
+    a multiple of four every block is aligned (vector moves)."""
     import ctypes as C
 
     from pytorchsph_amd import _lib as L
 
     rng = np.random.default_rng(5)
-    n = 100_003
     perm = rng.permutation(n).astype(np.int32)
     a = rng.standard_normal((n, 2)).astype(np.float32)
     b = rng.standard_normal(n).astype(np.float32)
