@@ -496,6 +496,18 @@ int sph_dfsph_control(const SphPhysics* phys, SphDfsph* d, hipStream_t s);
 /* Final pressure acceleration, v* += dt a*, acc += a*, p = p2. */
 int sph_dfsph_finish(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
                      const SphCSR* fb, SphDfsph* d, float* acc, hipStream_t s);
+/* sph_dfsph_finish and the velocity-diffusion / laminar gather that follows
+ * it in a DFSPH step (vterms of SPH_T_VISC and SPH_T_LAM: diffusion.py:30-44,
+ * laminar.py:30-46; sph_dsph_gather_packed with vterms | SPH_T_ACCUM over
+ * ff) in one walk of the fluid list: acc = (acc + a*) + those terms, frec
+ * the packed records of the step's state (sph_dsph_pack, V = A over the
+ * fluid-only density).  With equal lanes per row for the solve and the
+ * gathers, bit for bit the two calls.  No counterpart in the reference,
+ * whose timestep evaluates the two modules one after the other
+ * (dfsph.py:263-311). */
+int sph_dfsph_finish_visc(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                          const SphCSR* fb, SphDfsph* d, const float* frec, unsigned vterms,
+                          float* acc, hipStream_t s);
 
 /* ------------------------------------------------------------- integrators */
 /* RK4 stage s in 1..4 (simulationBase.py:679-705): accumulates

@@ -160,6 +160,8 @@ _SIGS = {
     "sph_dfsph_control": ([c_vp, _P(SphDfsph), c_vp], c_int),
     "sph_dfsph_finish": ([_P(SphFluid), c_vp, _P(SphCSR), _P(SphCSR), _P(SphDfsph), c_vp, c_vp],
                          c_int),
+    "sph_dfsph_finish_visc": ([_P(SphFluid), c_vp, _P(SphCSR), _P(SphCSR), _P(SphDfsph), c_vp,
+                               C.c_uint, c_vp, c_vp], c_int),
     "sph_dsph_pack": ([_P(SphFluid), c_vp, _P(SphBoundary), c_vp, c_vp, c_vp], c_int),
     "sph_dsph_gather_packed": ([c_vp, c_vp, c_vp, c_int, c_vp, _P(SphCSR), _P(SphCSR), c_vp,
                                 C.c_uint, c_vp, c_vp, c_vp, c_vp], c_int),

@@ -18,6 +18,7 @@
 // rhat_ij = (x_i - x_j) / |x_i - x_j|.  Boundary neighbours enter fluid rows
 // like fluid neighbours with their own area (V_b), actual area
 // (V_b / rho_b), predicted velocity (0) and pressure acceleration (0).
+#include "dsph_pair.h"
 #include "sph_common.h"
 
 namespace sph {
@@ -441,7 +442,9 @@ __device__ __forceinline__ double pressure_fluid_row(const SphFluid& f, const Sp
 }
 
 template <int KL>
-__global__ __launch_bounds__(kBS) void k_pressure(SphFluid f, const SphPhysics* __restrict__ phys,
+// Eight wavefronts per SIMD: held to 64 VGPRs (a few spill), it runs ~7%
+// faster than at 73 VGPRs and six wavefronts.
+__global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8))) void k_pressure(SphFluid f, const SphPhysics* __restrict__ phys,
                                                   SphCSR ff, SphBoundary b, SphCSR fb, SphCSR bf,
                                                   SphDfsph d) {
   const int nact = d.act_n[2];
@@ -583,6 +586,71 @@ __global__ __launch_bounds__(kBS) void k_finish(SphFluid f, const SphPhysics* __
   }
 }
 
+// k_finish and the velocity-diffusion and laminar gather of the step
+// (diffusion.py:30-44, laminar.py:30-46, the terms sph_dsph_gather_packed
+// adds with SPH_T_VISC | SPH_T_LAM | SPH_T_ACCUM) over one walk of the
+// fluid list: per pair the pressure record xrec[j] and the packed record
+// frec[j] of the step's state (V = A over the fluid-only density).  acc
+// becomes (acc + a*) + the diffusion terms, summed in the order and on the
+// lanes of the two separate kernels, so with the same lanes per row the
+// result is theirs bit for bit.
+template <int KL>
+__global__ __launch_bounds__(kBS) void k_finish_visc(SphFluid f,
+                                                     const SphPhysics* __restrict__ phys,
+                                                     SphCSR ff, SphCSR fb, SphDfsph d,
+                                                     const float4* __restrict__ frec,
+                                                     unsigned vterms, float* __restrict__ acc) {
+  const int i = item0<KL>();
+  const int sub = threadIdx.x % KL;
+  if (i >= f.np) return;
+  const SphPhysics P = phys[f.pscene[i]];
+  const KConst K = kconst(P.h);
+  const float4 ri = ld4(d.xrec, i);
+  const float4 a0 = frec[2 * i], a1 = frec[2 * i + 1];
+  Own o;
+  o.x = a0.x; o.y = a0.y; o.vx = a0.z; o.vy = a0.w;
+  o.rho = a1.x;
+  o.p = a1.z;
+  o.gx = 0.f;
+  o.gy = 0.f;
+  float ax = 0.f, ay = 0.f;
+  Acc va = {};
+  struct Rec3 {
+    float4 x, r0, r1;
+  };
+  pair_pipeline_with<KL>(
+      ff.rowptr[i], ff.rowptr[i + 1], sub, [&](int k) { return ff.col[k]; },
+      [&](int j) { return Rec3{ld4(d.xrec, j), frec[2 * j], frec[2 * j + 1]}; },
+      [&](const Rec3& r) {
+        const float2 G = spiky_G(ri.x, ri.y, r.x.x, r.x.y, K);
+        const float c = -r.x.w * (ri.z + r.x.z);
+        ax += c * G.x;
+        ay += c * G.y;
+        const Nbr nb = {r.r0.x, r.r0.y, r.r0.z, r.r0.w, r.r1.x, r.r1.y, r.r1.z};
+        dsph_pair(P, K, o, nb, 0.f, 0.f, true, vterms, va);
+      });
+  if (fb.rowptr)
+    pair_pipeline<KL>(fb.rowptr[i], fb.rowptr[i + 1], sub, d.bxrec,
+                      [&](int k) { return fb.col[k]; }, [&](const float4& rj) {
+                        const float2 G = spiky_G(ri.x, ri.y, rj.x, rj.y, K);
+                        const float c = -rj.w * (ri.z + rj.z);
+                        ax += c * G.x;
+                        ay += c * G.y;
+                      });
+  const float2 a = make_float2(group_sum<KL>(ax), group_sum<KL>(ay));
+  reduce_acc<KL>(va, vterms);
+  if (sub != 0) return;
+  st2(d.apred, i, a);
+  d.p[i] = d.p2[i];
+  const float2 vp = ld2(d.vpred, i);
+  st2(d.vpred, i, make_float2(vp.x + P.dt * a.x, vp.y + P.dt * a.y));
+  float dp, vx, vy;
+  dsph_rates(P, o, va, vterms, dp, vx, vy);
+  const float2 old = ld2(acc, i);
+  const float2 w = make_float2(old.x + a.x, old.y + a.y);
+  st2(acc, i, make_float2(w.x + vx, w.y + vy));
+}
+
 }  // namespace
 
 static bool dfsph_ok(const SphDfsph* d) {
@@ -704,6 +772,24 @@ int dfsph_finish(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff, co
   return launch_check("sph_dfsph_finish");
 }
 
+int dfsph_finish_visc(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                      const SphCSR* fb, SphDfsph* d, const float* frec, unsigned vterms,
+                      float* acc, hipStream_t s) {
+  SPH_REQUIRE(f && f->pscene && phys && ff && ff->rowptr && dfsph_ok(d) && frec && acc,
+              "sph_dfsph_finish_visc: bad arguments");
+  SPH_REQUIRE(!d->divergence, "sph_dfsph_finish_visc: not for the divergence solve");
+  SPH_REQUIRE((vterms & ~(unsigned)(SPH_T_VISC | SPH_T_LAM)) == 0,
+              "sph_dfsph_finish_visc: only SPH_T_VISC and SPH_T_LAM");
+  if (f->np == 0) return SPH_OK;
+  with_lanes(tuning(SPH_TUNE_SOLVE_LANES), [&](auto kl) {
+    constexpr int KL = decltype(kl)::value;
+    hipLaunchKernelGGL((k_finish_visc<KL>), dim3(blocks_for((int64_t)f->np * KL, kBS)), dim3(kBS),
+                       0, s, *f, phys, *ff, csr_or_null(fb), *d,
+                       reinterpret_cast<const float4*>(frec), vterms, acc);
+  });
+  return launch_check("sph_dfsph_finish_visc");
+}
+
 }  // namespace sph
 
 extern "C" {
@@ -736,5 +822,10 @@ int sph_dfsph_control(const SphPhysics* phys, SphDfsph* d, hipStream_t s) {
 int sph_dfsph_finish(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
                      const SphCSR* fb, SphDfsph* d, float* acc, hipStream_t s) {
   return sph::dfsph_finish(f, phys, ff, fb, d, acc, s);
+}
+int sph_dfsph_finish_visc(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
+                          const SphCSR* fb, SphDfsph* d, const float* frec, unsigned vterms,
+                          float* acc, hipStream_t s) {
+  return sph::dfsph_finish_visc(f, phys, ff, fb, d, frec, vterms, acc, s);
 }
 }

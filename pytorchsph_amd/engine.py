@@ -712,14 +712,14 @@ class SceneBatch:
         fb = self.fb.csr().c() if self.NB else None
         F = self.fluid(self.x, self.v, self.rho)
         D = self._dfsph_struct()
-        L.call("sph_dfsph_finish", C.byref(F), L.ptr(self.phys), C.byref(ff),
-               C.byref(fb) if fb is not None else None, C.byref(D), L.ptr(self.acc), sh)
-        # velocity diffusion + laminar with V = A / fluid-only density (density.py:53)
+        # the final pressure acceleration, then velocity diffusion + laminar
+        # with V = A / fluid-only density (density.py:53), in one walk of the
+        # fluid list
         Fv = self.fluid(self.x, self.v, self.rho, self.vol)
         L.call("sph_dsph_pack", C.byref(Fv), L.ptr(self.phys), None, L.ptr(self.frec), None, sh)
-        L.call("sph_dsph_gather_packed", L.ptr(self.frec), None, L.ptr(self.pscene), self.NP,
-               L.ptr(self.phys), C.byref(ff), None, None, L.T_VISC | L.T_LAM | L.T_ACCUM,
-               self._order(), None, L.ptr(self.acc), sh)
+        L.call("sph_dfsph_finish_visc", C.byref(F), L.ptr(self.phys), C.byref(ff),
+               C.byref(fb) if fb is not None else None, C.byref(D), L.ptr(self.frec),
+               L.T_VISC | L.T_LAM, L.ptr(self.acc), sh)
         L.call("sph_integrate_semi_implicit", L.ptr(self.pscene), L.ptr(self.phys), self.NP,
                L.ptr(self.x), L.ptr(self.v), None, L.ptr(self.v), L.ptr(self.acc), None,
                L.ptr(self.x), L.ptr(self.v), None, sh)

@@ -143,3 +143,53 @@ def test_permute_fields_matches_numpy(cuda, n):
     assert np.array_equal(tb.cpu().numpy(), b[perm])
     assert np.array_equal(tid.cpu().numpy(), ids[perm])
     assert np.array_equal(tc.cpu().numpy(), c[perm])
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4])
+def test_finish_visc_matches_finish_then_gather(cuda, lanes):
+    """sph_dfsph_finish_visc (the final pressure acceleration and the
+    velocity-diffusion / laminar gather over one walk of the fluid list)
+    gives acc, v* and a* of sph_dfsph_finish followed by
+    sph_dsph_gather_packed(SPH_T_VISC | SPH_T_LAM | SPH_T_ACCUM), bit for bit
+    when the solve and the gathers take the same lanes per row."""
+    import ctypes as C
+
+    from pytorchsph_amd import _lib as L
+
+    old = L.set_tuning(solve=lanes, gather=lanes)
+    try:
+        b = SceneBatch(mix("dfsph"))
+        b.step()                         # a flowed state with pressures
+        b._dfsph_pre()
+        b._dfsph_loop()
+        torch.cuda.synchronize()
+        keep = {k: b.df[k].clone() for k in ("vpred", "apred", "p")}
+        acc0 = b.acc.clone()
+        ff = b.ff.csr().c()
+        fb = b.fb.csr().c() if b.NB else None
+        fbp = C.byref(fb) if fb is not None else None
+        F = b.fluid(b.x, b.v, b.rho)
+        Fv = b.fluid(b.x, b.v, b.rho, b.vol)
+        D = b._dfsph_struct()
+        sh = L.stream_handle()
+        L.call("sph_dsph_pack", C.byref(Fv), L.ptr(b.phys), None, L.ptr(b.frec), None, sh)
+        L.call("sph_dfsph_finish", C.byref(F), L.ptr(b.phys), C.byref(ff), fbp, C.byref(D),
+               L.ptr(b.acc), sh)
+        L.call("sph_dsph_gather_packed", L.ptr(b.frec), None, L.ptr(b.pscene), b.NP,
+               L.ptr(b.phys), C.byref(ff), None, None, L.T_VISC | L.T_LAM | L.T_ACCUM, None,
+               None, L.ptr(b.acc), sh)
+        torch.cuda.synchronize()
+        split = {k: b.df[k].clone() for k in keep}
+        split_acc = b.acc.clone()
+        for k, v in keep.items():
+            b.df[k].copy_(v)
+        b.acc.copy_(acc0)
+        L.call("sph_dfsph_finish_visc", C.byref(F), L.ptr(b.phys), C.byref(ff), fbp,
+               C.byref(D), L.ptr(b.frec), L.T_VISC | L.T_LAM, L.ptr(b.acc), sh)
+        torch.cuda.synchronize()
+        assert torch.equal(b.acc, split_acc)
+        assert not torch.equal(b.acc, acc0)
+        for k in keep:
+            assert torch.equal(b.df[k], split[k]), k
+    finally:
+        L.set_tuning(**old)
