@@ -357,6 +357,7 @@ constexpr int kWalkRows = kBS;                  // rows per chunk
 constexpr int kWalkRecs = 3 * kWalkRows * 3 / 2 + 256;
 constexpr int kWalkCap = 27;                    // listed hits per row
 constexpr int kWalkBatch = 4;                   // candidates tested per trip
+constexpr int kBdyBatch = 2;                    // boundary candidates per trip
 
 struct WalkShared {
   // sorted positions of the staged runs; the test loads up to
@@ -467,14 +468,35 @@ __device__ __forceinline__ void walk_bdy(const Stencil3& st, int ncand, const Sp
                                          Acc& a) {
   const float2* __restrict__ bsx = reinterpret_cast<const float2*>(bg.sorted_x);
   if constexpr (KL == 1) {
-    for (int p = 0; p < ncand; ++p) {
-      const int slot = st.slot(p);
-      const float2 xy = bsx[slot];
-      if (dist2_nofma(xq, yq, xy.x, xy.y) <= thr) {
-        const int b = bg.sorted_idx[slot];
-        const float4 r0 = brec[2 * b], r1 = brec[2 * b + 1];
-        const Nbr nb = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z};
-        dsph_pair(P, K, o, nb, 0.f, 0.f, false, terms, a);
+    // kBdyBatch candidates per trip: their positions and indices loaded
+    // together, then the records of the trip's hits together, and the hits
+    // added in candidate order.  (One candidate per trip made a row by a
+    // wall a chain of ~40 dependent loads that held its whole wavefront.)
+    for (int p0 = 0; p0 < ncand; p0 += kBdyBatch) {
+      float2 xy[kBdyBatch];
+      int bi[kBdyBatch];
+#pragma unroll
+      for (int u = 0; u < kBdyBatch; ++u) {
+        const int slot = st.slot(p0 + u);   // clamped into the candidates
+        xy[u] = bsx[slot];
+        bi[u] = bg.sorted_idx[slot];
+      }
+      bool hit[kBdyBatch];
+      float4 r0[kBdyBatch], r1[kBdyBatch];
+#pragma unroll
+      for (int u = 0; u < kBdyBatch; ++u) {
+        hit[u] = p0 + u < ncand && dist2_nofma(xq, yq, xy[u].x, xy[u].y) <= thr;
+        if (hit[u]) {
+          r0[u] = brec[2 * bi[u]];
+          r1[u] = brec[2 * bi[u] + 1];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kBdyBatch; ++u) {
+        if (hit[u]) {
+          const Nbr nb = {r0[u].x, r0[u].y, r0[u].z, r0[u].w, r1[u].x, r1[u].y, r1[u].z};
+          dsph_pair(P, K, o, nb, 0.f, 0.f, false, terms, a);
+        }
       }
     }
   } else {

@@ -129,30 +129,33 @@ struct ChunkRuns {
   }
 };
 
-// Every thread of the block calls this with its row's stencil (`valid`
-// false: no row); wlo / whi are LDS scratch of 3 ints per wavefront.  Two
-// wavefront reductions per run and one barrier.
-__device__ __forceinline__ ChunkRuns chunk_runs(const Stencil3& st, bool valid, int (*wlo)[3],
-                                                int (*whi)[3]) {
-  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-  if (valid) {
+// Every thread of the block calls this with its row's stencils in NS grids
+// (valid[s] false: no row in grid s); wlo / whi are LDS scratch of 3 NS
+// ints per wavefront.  Two wavefront reductions per run and one barrier
+// for all NS grids; c[s] are the chunk's runs in grid s.
+template <int NS>
+__device__ __forceinline__ void chunk_runs_n(const Stencil3 (&st)[NS], const bool (&valid)[NS],
+                                             int (*wlo)[3 * NS], int (*whi)[3 * NS],
+                                             ChunkRuns (&c)[NS]) {
+  int lo[3 * NS], hi[3 * NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      if (st.ra[r] < st.rb[r]) {
-        lo[r] = st.ra[r];
-        hi[r] = st.rb[r];
-      }
+      const bool on = valid[s] && st[s].ra[r] < st[s].rb[r];
+      lo[3 * s + r] = on ? st[s].ra[r] : INT_MAX;
+      hi[3 * s + r] = on ? st[s].rb[r] : INT_MIN;
     }
   }
 #pragma unroll
-  for (int r = 0; r < 3; ++r) {
+  for (int r = 0; r < 3 * NS; ++r) {
     lo[r] = __ockl_wfred_min_i32(lo[r]);
     hi[r] = __ockl_wfred_max_i32(hi[r]);
   }
   const int w = threadIdx.x / kWave;
   if ((threadIdx.x & (kWave - 1)) == 0) {
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
+    for (int r = 0; r < 3 * NS; ++r) {
       wlo[w][r] = lo[r];
       whi[w][r] = hi[r];
     }
@@ -161,20 +164,33 @@ __device__ __forceinline__ ChunkRuns chunk_runs(const Stencil3& st, bool valid, 
   const int nw = blockDim.x / kWave;
   for (int v = 0; v < nw; ++v) {
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
+    for (int r = 0; r < 3 * NS; ++r) {
       lo[r] = wlo[v][r] < lo[r] ? wlo[v][r] : lo[r];
       hi[r] = whi[v][r] > hi[r] ? whi[v][r] : hi[r];
     }
   }
-  ChunkRuns c;
-  const bool e0 = hi[0] > lo[0], e1 = hi[1] > lo[1], e2 = hi[2] > lo[2];
-  c.lo0 = e0 ? lo[0] : 0;
-  c.lo1 = e1 ? lo[1] : 0;
-  c.lo2 = e2 ? lo[2] : 0;
-  c.off1 = e0 ? hi[0] - c.lo0 : 0;
-  c.off2 = c.off1 + (e1 ? hi[1] - c.lo1 : 0);
-  c.tot = c.off2 + (e2 ? hi[2] - c.lo2 : 0);
-  return c;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int* l = lo + 3 * s;
+    const int* h = hi + 3 * s;
+    const bool e0 = h[0] > l[0], e1 = h[1] > l[1], e2 = h[2] > l[2];
+    c[s].lo0 = e0 ? l[0] : 0;
+    c[s].lo1 = e1 ? l[1] : 0;
+    c[s].lo2 = e2 ? l[2] : 0;
+    c[s].off1 = e0 ? h[0] - c[s].lo0 : 0;
+    c[s].off2 = c[s].off1 + (e1 ? h[1] - c[s].lo1 : 0);
+    c[s].tot = c[s].off2 + (e2 ? h[2] - c[s].lo2 : 0);
+  }
+}
+
+// chunk_runs_n for one grid.
+__device__ __forceinline__ ChunkRuns chunk_runs(const Stencil3& st, bool valid, int (*wlo)[3],
+                                                int (*whi)[3]) {
+  const Stencil3 sts[1] = {st};
+  const bool v[1] = {valid};
+  ChunkRuns c[1];
+  chunk_runs_n<1>(sts, v, wlo, whi, c);
+  return c[0];
 }
 
 // Squared distance fl(fl(dx*dx) + fl(dy*dy)) with no FMA contraction.
