@@ -345,7 +345,7 @@ def plain_paths(steps: int = 3, warmup: int = 1, which=None):
         ps = torch.zeros(n, dtype=torch.int32, device=dev)
         counts = torch.empty(n, dtype=torch.int32, device=dev)
         rowptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
-        g.build(xt, ps)
+        g.build(xt, ps, ordered=False)
         q = g.query_struct(xt, ht, ps, 1, True)
         g.count(q, counts)
         scan(counts, n, rowptr)
@@ -353,7 +353,7 @@ def plain_paths(steps: int = 3, warmup: int = 1, which=None):
         col = torch.empty(nnz, dtype=torch.int32, device=dev)
 
         def one_search():
-            g.build(xt, ps)
+            g.build(xt, ps, ordered=False)
             g.count(q, counts)
             scan(counts, n, rowptr)
             g.fill(q, rowptr, col)

@@ -242,6 +242,18 @@ int sph_grid_build(const SphGrid* g, void* ws, size_t ws_bytes, hipStream_t s);
  * neighSearch.cpp:172-177, takes no qMin). */
 int sph_grid_build_keyed(const SphGrid* g, const int* keys, void* ws, size_t ws_bytes,
                          hipStream_t s);
+/* sph_grid_build (keys NULL) or sph_grid_build_keyed (keys given) for
+ * points in any order: the same sorted_idx, sorted_x, cell_start, key and
+ * status (every cell, its outside slot included, in index order), by a
+ * stable least-significant-digit radix sort of (global cell, index) instead
+ * of per-cell atomics and scatters, which serialise and write one word per
+ * cache line when the points are not in cell order (a point cloud, the
+ * reference's own particle order).  slot is not used.  ws holds
+ * sph_grid_sort_workspace_bytes(np, ncells) bytes.  Replaces sortPositions
+ * (neighborSearch.py:30-55) like sph_grid_build. */
+size_t sph_grid_sort_workspace_bytes(int np, int ncells);
+int sph_grid_build_sorted(const SphGrid* g, const int* keys, void* ws, size_t ws_bytes,
+                          hipStream_t s);
 /* The squared-distance threshold the search tests each support with:
  * out[i] = the largest float T with fl(sqrt(T)) < h[i], so that for every
  * float s >= 0, fl(sqrt(s)) < h[i] (the reference's predicate with a

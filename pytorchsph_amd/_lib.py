@@ -121,6 +121,8 @@ _SIGS = {
     "sph_scan_exclusive": ([c_vp, c_int, c_vp, c_vp, C.c_size_t, c_vp], c_int),
     "sph_scan_exclusive_noalloc": ([c_vp, c_int, c_vp, c_vp], c_int),
     "sph_grid_build_keyed": ([_P(SphGrid), c_vp, c_vp, C.c_size_t, c_vp], c_int),
+    "sph_grid_sort_workspace_bytes": ([c_int, c_int], C.c_size_t),
+    "sph_grid_build_sorted": ([_P(SphGrid), c_vp, c_vp, C.c_size_t, c_vp], c_int),
     "sph_pair_thresholds": ([c_vp, c_int, c_vp, c_vp], c_int),
     "sph_fill_i32": ([c_vp, c_int, c_int, c_vp], c_int),
     "sph_fill_f32": ([c_vp, c_int, C.c_float, c_vp], c_int),

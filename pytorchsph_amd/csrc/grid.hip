@@ -396,6 +396,180 @@ __global__ __launch_bounds__(kBS) void k_pair_thresholds(const float* __restrict
   if (i < n) out[i] = sq_threshold(h[i]);
 }
 
+// ------------------------------------------------ radix-sort grid build
+// For points in arbitrary order (the atomics of k_grid_keys serialise on
+// random cells, and the scatters of k_grid_scatter / k_grid_place write one
+// word per random line): a least-significant-digit radix sort of the pairs
+// (global cell, point index), 8 bits per pass, stable, so the points come
+// out in (scene, key, index) order -- the order sph_grid_build gives them.
+// Each pass: a per-tile digit histogram (LDS atomics), an exclusive scan of
+// the digit-major histograms (every digit's tiles in order), then per tile a
+// stable rank of every item among the tile's items of its digit, a reorder
+// of the tile in LDS by digit, and coalesced writes of each digit's run.
+constexpr int kRsBits = 8;
+constexpr int kRsDigits = 1 << kRsBits;
+constexpr int kRsItems = 16;                      // items per thread
+constexpr int kRsTile = kBS * kRsItems;          // items per tile
+constexpr int kRsWaves = kBS / kWave;
+
+// Global cell of every point (k_grid_keys' cell and outside rules, without
+// the slot atomics); the local key into g.key.
+__global__ __launch_bounds__(kBS) void k_rs_keys(SphGrid g, const int* __restrict__ keys_in,
+                                                 int* __restrict__ kout) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  const int s = g.pscene[p];
+  const SphGridDesc d = g.scenes[s];
+  const float x = g.x[2 * p], y = g.x[2 * p + 1];
+  int ix = 0, iy = 0;
+  int64_t key;
+  if (keys_in) {
+    key = keys_in[p];
+  } else {
+    ix = cell_coord(x, d.qmin_x, d.hmax);
+    iy = cell_coord(y, d.qmin_y, d.hmax);
+    key = (int64_t)ix + (int64_t)d.cells_x * (int64_t)iy;
+  }
+  const int64_t ncell = (int64_t)d.cells_x * d.cells_y;
+  int k;
+  if (isfinite(x) && isfinite(y) && key >= 0 && key < ncell) {
+    k = (int)key;
+    if (!keys_in && (ix < 0 || ix >= d.cells_x || iy < 0 || iy >= d.cells_y))
+      atomicOr(&g.status[s], SPH_ST_OUTSIDE);
+  } else {
+    k = (int)ncell;
+    atomicOr(&g.status[s], SPH_ST_OUTSIDE);
+  }
+  g.key[p] = k;
+  kout[p] = d.cell_begin + k;
+}
+
+// Digit histogram of each tile: hist[digit * ntiles + tile].
+__global__ __launch_bounds__(kBS) void k_rs_hist(const int* __restrict__ keys, int n, int shift,
+                                                 int* __restrict__ hist) {
+  __shared__ int cnt[kRsDigits];
+  const int t = threadIdx.x;
+  cnt[t] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * kRsTile;
+#pragma unroll 4
+  for (int u = 0; u < kRsItems; ++u) {
+    const int i = base + u * kBS + t;
+    if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & (kRsDigits - 1)], 1);
+  }
+  __syncthreads();
+  hist[t * gridDim.x + blockIdx.x] = cnt[t];
+}
+
+// Lanes of this wavefront whose `d` equals this lane's, among `act`.
+__device__ __forceinline__ unsigned long long match_digit(int d, unsigned long long act) {
+  unsigned long long m = act;
+#pragma unroll
+  for (int b = 0; b < kRsBits; ++b) {
+    const unsigned long long on = __ballot((d >> b) & 1);
+    m &= ((d >> b) & 1) ? on : ~on;
+  }
+  return m;
+}
+
+// One pass: items of tile b move to off[digit * ntiles + b] + (their rank
+// among the tile's items of that digit, in tile order).  Wavefront w ranks
+// items [w * kRsTile / kRsWaves, (w + 1) * kRsTile / kRsWaves) of the tile,
+// 64 consecutive ones per round, with a per-wavefront running count per
+// digit.  vals_in == nullptr: the value is the item's index (first pass).
+// LAST: the values are point indices, written to g.sorted_idx with their
+// positions to g.sorted_x.
+template <bool LAST>
+__global__ __launch_bounds__(kBS) void k_rs_scatter(const int* __restrict__ keys_in,
+                                                    const int* __restrict__ vals_in, int n,
+                                                    int shift, const int* __restrict__ off,
+                                                    int* __restrict__ keys_out,
+                                                    int* __restrict__ vals_out, SphGrid g) {
+  __shared__ int s_key[kRsTile];
+  __shared__ int s_val[kRsTile];
+  __shared__ int s_cnt[kRsWaves][kRsDigits];   // per-wavefront counts, then prefixes
+  __shared__ int s_start[kRsDigits];           // tile-local first slot of each digit
+  __shared__ int s_goff[kRsDigits];            // global first slot of the tile's digit run
+  __shared__ int s_scan[kRsWaves + 1];
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  for (int k = t; k < kRsWaves * kRsDigits; k += kBS) (&s_cnt[0][0])[k] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * kRsTile + w * (kRsTile / kRsWaves);
+  int key[kRsItems], val[kRsItems], rank[kRsItems];
+#pragma unroll
+  for (int u = 0; u < kRsItems; ++u) {
+    const int i = base + u * kWave + lane;
+    const bool ok = i < n;
+    key[u] = ok ? keys_in[i] : 0;
+    val[u] = ok ? (vals_in ? vals_in[i] : i) : 0;
+  }
+  const unsigned long long lower = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int u = 0; u < kRsItems; ++u) {
+    const bool ok = base + u * kWave + lane < n;
+    const int d = (key[u] >> shift) & (kRsDigits - 1);
+    const unsigned long long act = __ballot(ok);
+    const unsigned long long peers = match_digit(d, act);
+    const int before = s_cnt[w][d];
+    rank[u] = before + __popcll(peers & lower);
+    // the group's last lane carries the count forward
+    if (ok && (peers >> lane) == 1ull) s_cnt[w][d] = before + __popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // digit t: prefixes over the wavefronts, the tile's total, its start in
+  // the tile and in the pass's output
+  int tot = 0;
+#pragma unroll
+  for (int v = 0; v < kRsWaves; ++v) {
+    const int c = s_cnt[v][t];
+    s_cnt[v][t] = tot;
+    tot += c;
+  }
+  int all;
+  const int start = block_exclusive_scan(tot, s_scan, &all);
+  s_start[t] = start;
+  s_goff[t] = off[t * gridDim.x + blockIdx.x];
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kRsItems; ++u) {
+    if (base + u * kWave + lane < n) {
+      const int d = (key[u] >> shift) & (kRsDigits - 1);
+      const int lp = s_start[d] + s_cnt[w][d] + rank[u];
+      s_key[lp] = key[u];
+      s_val[lp] = val[u];
+    }
+  }
+  __syncthreads();
+  const int tn = n - blockIdx.x * kRsTile < kRsTile ? n - blockIdx.x * kRsTile : kRsTile;
+  for (int j = t; j < tn; j += kBS) {
+    const int k = s_key[j], v = s_val[j];
+    const int d = (k >> shift) & (kRsDigits - 1);
+    const int gp = s_goff[d] + (j - s_start[d]);
+    keys_out[gp] = k;
+    if (LAST) {
+      g.sorted_idx[gp] = v;
+      reinterpret_cast<float2*>(g.sorted_x)[gp] = reinterpret_cast<const float2*>(g.x)[v];
+    } else {
+      vals_out[gp] = v;
+    }
+  }
+}
+
+// cell_count[c] = the length of cell c's run in the sorted keys (the
+// caller zeroed cell_count), one thread per sorted slot, the run's first
+// slot writing.
+__global__ __launch_bounds__(kBS) void k_rs_runs(const int* __restrict__ skeys, int n,
+                                                 int* __restrict__ cell_count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = skeys[i];
+  if (i > 0 && skeys[i - 1] == k) return;
+  int e = i + 1;
+  while (e < n && skeys[e] == k) ++e;
+  cell_count[k] = e - i;
+}
+
 }  // namespace
 
 int pair_thresholds(const float* h, int n, float* out, hipStream_t s) {
@@ -427,6 +601,78 @@ static int grid_build_impl(const SphGrid* gp, const int* keys, void* ws, size_t 
     hipLaunchKernelGGL(k_grid_place, grid, dim3(kBS), 0, s, g);
   }
   return launch_check("sph_grid_build");
+}
+
+// Workspace of the radix-sort build: two key and two value buffers of np
+// words, the digit histograms and their offsets (kRsDigits per tile), and
+// the scan workspace of the larger of those and the cell table.
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+static int rs_tiles(int np) { return (np + kRsTile - 1) / kRsTile; }
+size_t grid_sort_ws_bytes(int np, int ncells) {
+  if (np < 0 || ncells < 0) return 0;
+  const int64_t hist = (int64_t)kRsDigits * rs_tiles(np);
+  const int64_t scan_n = hist > ncells ? hist : ncells;
+  return 4 * align256(4 * (size_t)(np > 0 ? np : 1)) + 2 * align256(4 * (size_t)(hist + 1)) +
+         align256(scan_ws_bytes((int)scan_n));
+}
+
+int grid_build_sorted(const SphGrid* gp, const int* keys, void* ws, size_t ws_bytes,
+                      hipStream_t s) {
+  SPH_REQUIRE(gp, "sph_grid_build_sorted: null grid");
+  const SphGrid& g = *gp;
+  SPH_REQUIRE(g.np >= 0 && g.ncells > 0 && g.nscenes > 0, "sph_grid_build_sorted: bad sizes");
+  SPH_REQUIRE(g.scenes && g.cell_count && g.cell_start && g.status &&
+                  (g.np == 0 || (g.pscene && g.x && g.key && g.sorted_idx && g.sorted_x)),
+              "sph_grid_build_sorted: null buffer");
+  const size_t need = grid_sort_ws_bytes(g.np, g.ncells);
+  if (!ws || ws_bytes < need) {
+    set_error("sph_grid_build_sorted: workspace smaller than sph_grid_sort_workspace_bytes");
+    return SPH_E_SPACE;
+  }
+  const int np = g.np;
+  const int tiles = rs_tiles(np);
+  const int64_t hist_n = (int64_t)kRsDigits * tiles;
+  char* w = static_cast<char*>(ws);
+  int* ka = reinterpret_cast<int*>(w);   w += align256(4 * (size_t)(np > 0 ? np : 1));
+  int* va = reinterpret_cast<int*>(w);   w += align256(4 * (size_t)(np > 0 ? np : 1));
+  int* kb = reinterpret_cast<int*>(w);   w += align256(4 * (size_t)(np > 0 ? np : 1));
+  int* vb = reinterpret_cast<int*>(w);   w += align256(4 * (size_t)(np > 0 ? np : 1));
+  int* hist = reinterpret_cast<int*>(w); w += align256(4 * (size_t)(hist_n + 1));
+  int* off = reinterpret_cast<int*>(w);  w += align256(4 * (size_t)(hist_n + 1));
+  void* sws = w;
+  const size_t sws_bytes = need - (size_t)(w - static_cast<char*>(ws));
+  int rc;
+  if ((rc = fill_i32(g.cell_count, g.ncells, 0, s))) return rc;
+  if (np > 0) {
+    hipLaunchKernelGGL(k_rs_keys, dim3(blocks_for(np, kBS)), dim3(kBS), 0, s, g, keys, ka);
+    if ((rc = launch_check("sph_grid_build_sorted(keys)"))) return rc;
+    // digits of the largest global cell
+    const int bits = 32 - __builtin_clz((unsigned)g.ncells);
+    const int passes = (bits + kRsBits - 1) / kRsBits;
+    const int* kin = ka;
+    const int* vin = nullptr;
+    for (int ps = 0; ps < passes; ++ps) {
+      const int shift = ps * kRsBits;
+      const bool last = ps == passes - 1;
+      int* kout = (kin == ka) ? kb : ka;
+      int* vout = (kin == ka) ? vb : va;
+      hipLaunchKernelGGL(k_rs_hist, dim3(tiles), dim3(kBS), 0, s, kin, np, shift, hist);
+      if ((rc = scan_exclusive(hist, (int)hist_n, off, sws, sws_bytes, s))) return rc;
+      if (last)
+        hipLaunchKernelGGL(k_rs_scatter<true>, dim3(tiles), dim3(kBS), 0, s, kin, vin, np, shift,
+                           off, kout, vout, g);
+      else
+        hipLaunchKernelGGL(k_rs_scatter<false>, dim3(tiles), dim3(kBS), 0, s, kin, vin, np,
+                           shift, off, kout, vout, g);
+      if ((rc = launch_check("sph_grid_build_sorted(pass)"))) return rc;
+      kin = kout;
+      vin = vout;
+    }
+    hipLaunchKernelGGL(k_rs_runs, dim3(blocks_for(np, kBS)), dim3(kBS), 0, s, kin, np,
+                       g.cell_count);
+  }
+  if ((rc = scan_exclusive(g.cell_count, g.ncells, g.cell_start, sws, sws_bytes, s))) return rc;
+  return launch_check("sph_grid_build_sorted");
 }
 
 int grid_build(const SphGrid* gp, void* ws, size_t ws_bytes, hipStream_t s) {
@@ -541,6 +787,13 @@ int sph_grid_build(const SphGrid* g, void* ws, size_t ws_bytes, hipStream_t s) {
 int sph_grid_build_keyed(const SphGrid* g, const int* keys, void* ws, size_t ws_bytes,
                          hipStream_t s) {
   return sph::grid_build_keyed(g, keys, ws, ws_bytes, s);
+}
+size_t sph_grid_sort_workspace_bytes(int np, int ncells) {
+  return sph::grid_sort_ws_bytes(np, ncells);
+}
+int sph_grid_build_sorted(const SphGrid* g, const int* keys, void* ws, size_t ws_bytes,
+                          hipStream_t s) {
+  return sph::grid_build_sorted(g, keys, ws, ws_bytes, s);
 }
 int sph_pair_thresholds(const float* h, int n, float* out, hipStream_t s) {
   return sph::pair_thresholds(h, n, out, s);

@@ -67,7 +67,8 @@ class neighborSearchModule(Module):
         n = x.shape[0]
         hmax = float(h.max())                  # the reference reads float(hMax) too
         g = self._grid_for(n, hmax)
-        g.build(x, self._pscene)
+        # the reference's particle order: not cell order
+        g.build(x, self._pscene, ordered=False)
         self.csr = g.search(x, h, self._pscene, 1, self_sorted=True, sort_rows=True)
         rows, cols, rhat, q = pair_geometry(self.csr, x, x, None, self.support, self.threshold)
         fluidNeighbors = torch.stack([rows, cols], dim=0)

@@ -88,14 +88,32 @@ class NeighborGrid:
             self.ncells, L.ptr(self.cell_count), L.ptr(self.cell_start), L.ptr(self.key),
             L.ptr(self.slot), L.ptr(self.sorted_idx), L.ptr(self.sorted_x), L.ptr(self.status))
 
-    def build(self, x: torch.Tensor, pscene: torch.Tensor, stream=None) -> None:
-        """Sorts the points into the cell table (positions x [np, 2] fp32)."""
+    def build(self, x: torch.Tensor, pscene: torch.Tensor, stream=None,
+              ordered: bool = True) -> None:
+        """Sorts the points into the cell table (positions x [np, 2] fp32).
+        ordered: the points are (nearly) in cell order, as the engine keeps
+        its state, and the per-cell atomics are cheap (sph_grid_build);
+        False: any order, by the radix sort (sph_grid_build_sorted).  The
+        result is the same."""
         assert x.dtype == torch.float32 and x.is_contiguous() and x.shape == (self.np, 2)
         assert pscene.dtype == torch.int32 and pscene.numel() == self.np
         self.x = x
         self.pscene = pscene
         g = self._grid()
-        L.call("sph_grid_build", C.byref(g), L.ptr(self.ws), self.ws_bytes, L.stream_handle(stream))
+        if ordered:
+            L.call("sph_grid_build", C.byref(g), L.ptr(self.ws), self.ws_bytes,
+                   L.stream_handle(stream))
+            return
+        ws = self._sort_ws()
+        L.call("sph_grid_build_sorted", C.byref(g), None, L.ptr(ws), int(ws.numel()),
+               L.stream_handle(stream))
+
+    def _sort_ws(self) -> torch.Tensor:
+        """Workspace of the radix-sort build (allocated on first use)."""
+        if getattr(self, "_sws", None) is None:
+            n = int(L.load().sph_grid_sort_workspace_bytes(self.np, self.ncells))
+            self._sws = torch.empty(n, dtype=torch.uint8, device=self.device)
+        return self._sws
 
     def query_struct(self, xq, hq, qscene, radius=1, self_sorted=False, qcell=None) -> L.SphQuery:
         return L.SphQuery(L.ptr(xq), L.ptr(hq), L.ptr(qscene), int(xq.shape[0]), int(radius),

@@ -27,14 +27,15 @@ def assert_same_pairs(rp_gpu, col_gpu, orc_csr, sort=True):
     assert np.array_equal(c, orc_csr.col.astype(np.int64)), "columns differ"
 
 
-def gpu_search(x, h, vmin, vmax, hmax, queries=None, radius=1):
-    """Self search (queries None) or foreign queries (xq, hq) on one scene."""
+def gpu_search(x, h, vmin, vmax, hmax, queries=None, radius=1, ordered=True):
+    """Self search (queries None) or foreign queries (xq, hq) on one scene;
+    ordered False: the grid by the radix-sort build."""
     n = len(x)
     dev = torch.device("cuda")
     g = NeighborGrid([dict(vmin=vmin, vmax=vmax, hmax=hmax, p_begin=0, p_count=n)], n, dev)
     xt = torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(dev)
     ps = torch.zeros(n, dtype=torch.int32, device=dev)
-    g.build(xt, ps)
+    g.build(xt, ps, ordered=ordered)
     if queries is None:
         ht = torch.from_numpy(np.ascontiguousarray(h, np.float32)).to(dev)
         csr = g.search(xt, ht, ps, radius, self_sorted=True)

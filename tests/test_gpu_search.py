@@ -244,11 +244,12 @@ def test_point_past_a_side_edge_is_flagged_and_keeps_reference_pairs(cuda):
 @pytest.mark.slow
 def test_c2_four_million_points(cuda):
     """Neighbour search alone at 4M points (half uniform, half 8 Gaussian
-    clusters, ~20 mean neighbours), bit-exact against the oracle."""
+    clusters, ~20 mean neighbours), bit-exact against the oracle, with the
+    grid of the radix-sort build (the benchmark's build for a point cloud)."""
     x = S.point_cloud(4_000_000, seed=1)
     hmax = S.support_for_mean_neighbors(x, 20.0)
     h = np.full(len(x), hmax, np.float32)
-    rp, c, st, _ = gpu_search(x, h, (0, 0), (1, 1), hmax)
+    rp, c, st, _ = gpu_search(x, h, (0, 0), (1, 1), hmax, ordered=False)
     assert st == 0
     orc = oracle_self(x, h, (0, 0), (1, 1), hmax)
     assert_same_pairs(rp, c, orc)
@@ -360,3 +361,83 @@ def test_device_pair_thresholds_are_exact(cuda):
     L.call("sph_pair_thresholds", L.ptr(ht), len(h), L.ptr(out), L.stream_handle())
     torch.cuda.synchronize()
     assert np.array_equal(out.cpu().numpy().view(np.uint32), exact_threshold(h).view(np.uint32))
+
+
+def _grid_arrays(g):
+    torch.cuda.synchronize()
+    return {k: getattr(g, k).cpu().numpy().copy()
+            for k in ("sorted_idx", "sorted_x", "cell_start", "key", "status")}
+
+
+@pytest.mark.parametrize("case", ["random", "clustered", "cell_order", "multi_scene",
+                                  "outside_and_nan", "tiny", "keyed"])
+def test_sorted_build_equals_atomic_build(cuda, case):
+    """sph_grid_build_sorted (radix sort) gives sph_grid_build's grid bit for
+    bit: the same sorted order (scene, key, index), positions, cell table,
+    keys and status bits, on point clouds in random order, in cell order,
+    over several scenes, with points outside the table and non-finite
+    positions (fewer than the atomic build's unranked outside-slot limit),
+    from one point up, and with the cell keys given."""
+    rng = np.random.default_rng(11)
+    dev = torch.device("cuda")
+    keys = None
+    if case in ("random", "clustered", "cell_order", "tiny", "keyed"):
+        n = {"random": 300_000, "clustered": 300_000, "cell_order": 200_000, "tiny": 5,
+             "keyed": 50_000}[case]
+        x = (S.point_cloud(n, seed=3) if case == "clustered"
+             else rng.uniform(0, 1, (n, 2))).astype(np.float32)
+        hmax = np.float32(np.sqrt(20 / (np.pi * n)))
+        scenes = [dict(vmin=(0, 0), vmax=(1, 1), hmax=hmax, p_begin=0, p_count=n)]
+        P = np.zeros(n, np.int32)
+        if case == "cell_order":
+            qx, qy, cx, _, _ = grid_params((0, 0), (1, 1), hmax)
+            k = (np.ceil((x[:, 0] - qx) / hmax) + cx * np.ceil((x[:, 1] - qy) / hmax))
+            x = x[np.argsort(k, kind="stable")]
+        if case == "keyed":
+            qx, qy, cx, cy, _ = grid_params((0, 0), (1, 1), hmax)
+            keys = (np.ceil((x[:, 0] - qx) / hmax) + cx * np.ceil((x[:, 1] - qy) / hmax))
+            keys = keys.astype(np.int32)
+            keys[::1000] = -5                    # out of range: the outside slot
+    else:
+        specs = [((0, 0), (1, 1), 0.03, 3000), ((-2, -1), (2, 1), 0.05, 5000),
+                 ((10, 10), (10.5, 12), 0.02, 4000)]
+        xs, scenes, off = [], [], 0
+        for vmin, vmax, h, m in specs:
+            xs.append(rng.uniform(vmin, vmax, (m, 2)).astype(np.float32))
+            scenes.append(dict(vmin=vmin, vmax=vmax, hmax=np.float32(h), p_begin=off,
+                               p_count=m))
+            off += m
+        x = np.concatenate(xs)
+        P = np.concatenate([np.full(m, s, np.int32) for s, (_, _, _, m) in enumerate(specs)])
+        if case == "outside_and_nan":
+            x[5] = (50.0, 50.0)
+            x[3001] = (-30.0, 0.0)
+            x[3002] = (np.nan, 0.5)
+            x[9000] = (10.25, np.inf)
+            x[9001] = (10.9, 11.0)               # past the side edge: aliased key
+    n = len(x)
+    xt = torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+    pt = torch.from_numpy(P).to(dev)
+    out = []
+    for ordered in (True, False):
+        g = NeighborGrid(scenes, n, dev)
+        g.x, g.pscene = xt, pt
+        c = g._grid()
+        if keys is None:
+            g.build(xt, pt, ordered=ordered)
+        else:
+            kt = torch.from_numpy(keys).to(dev)
+            if ordered:
+                L.call("sph_grid_build_keyed", __import__("ctypes").byref(c), L.ptr(kt),
+                       L.ptr(g.ws), g.ws_bytes, L.stream_handle())
+            else:
+                ws = g._sort_ws()
+                L.call("sph_grid_build_sorted", __import__("ctypes").byref(c), L.ptr(kt),
+                       L.ptr(ws), int(ws.numel()), L.stream_handle())
+        out.append(_grid_arrays(g))
+    a, b = out
+    for k in a:
+        if k == "sorted_x":
+            assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), (case, k)
+        else:
+            assert np.array_equal(a[k], b[k]), (case, k)
