@@ -546,9 +546,10 @@ int sph_permute_f32(const float* in, const int* perm, int n, int width, float* o
 
 /* Up to SPH_PERMUTE_MAX_FIELDS arrays of 32-bit words permuted in place
  * together: field[k] = old field[perm[k]] for every field, through `tmp`
- * (n * sum of widths words).  Two launches whatever the number of fields:
- * one gathers every field of element perm[k] into tmp (each field a block
- * of its own there), one copies the blocks back.  The engine's per-step
+ * (n * sum of widths + 4 * nfields words).  Two launches whatever the
+ * number of fields: one gathers every field of element perm[k] into tmp
+ * (each field a block of its own there, starting on a 16-byte boundary),
+ * one copies the blocks back.  The engine's per-step
  * move of the particle state into cell order; no counterpart in the
  * reference, which does not reorder. */
 #define SPH_PERMUTE_MAX_FIELDS 8

@@ -71,9 +71,15 @@ __global__ __launch_bounds__(kBS) void k_permute(const float* __restrict__ in,
 }
 
 // Field q of element perm[k] into tmp's block of field q (fields stored one
-// after the other, each [n][width]): one thread per element, each field
-// moved with one load and one store of its width, so a wavefront writes
-// each field's rows contiguously.
+// after the other, each [n][width] from a 16-byte boundary): one thread per
+// element, each field moved with one load and one store of its width, so a
+// wavefront writes each field's rows contiguously.
+// Words of tmp a field of width w takes: n * w rounded up to four, so
+// every field's block starts on a 16-byte boundary.
+__host__ __device__ __forceinline__ int64_t block_words(int n, int w) {
+  return ((int64_t)n * w + 3) & ~(int64_t)3;
+}
+
 __device__ __forceinline__ void move_words(const float* __restrict__ in, float* __restrict__ out,
                                            int w) {
   // a vector access where both ends are aligned to it (a field block of
@@ -97,13 +103,13 @@ __global__ __launch_bounds__(kBS) void k_fields_gather(SphFields f, const int* _
   for (int q = 0; q < f.nfields; ++q) {
     const int w = f.width[q];
     move_words(f.ptr[q] + src * w, tmp + o + (int64_t)k * w, w);
-    o += (int64_t)n * w;
+    o += block_words(n, w);
   }
 }
 
 // tmp's field blocks back into the fields: a flat copy of every field's
-// n * width words, four at a time (the host checks that every field and
-// block is 16-byte aligned; otherwise one word at a time, vec = false).
+// n * width words, four at a time (the host checks that tmp and every field
+// are 16-byte aligned; otherwise one word at a time, vec = false).
 template <bool VEC>
 __global__ __launch_bounds__(kBS) void k_fields_copy(SphFields f, int n,
                                                      const float* __restrict__ tmp) {
@@ -123,7 +129,7 @@ __global__ __launch_bounds__(kBS) void k_fields_copy(SphFields f, int n,
         out[t] = in[t];
       }
     }
-    o += words;
+    o += block_words(n, f.width[q]);
   }
 }
 
@@ -142,13 +148,13 @@ int permute_fields(const SphFields* f, const int* perm, int n, float* tmp, hipSt
   if (n == 0 || rw == 0) return SPH_OK;
   hipLaunchKernelGGL(k_fields_gather, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, *f, perm, n,
                      tmp);
-  // every field and every block of tmp 16-byte aligned: four words per copy
+  // tmp and every field 16-byte aligned (the blocks of tmp then are): four
+  // words per copy
   bool vec = (reinterpret_cast<uintptr_t>(tmp) & 15) == 0;
-  int64_t o = 0, most = 0;
+  int64_t most = 0;
   for (int q = 0; q < f->nfields; ++q) {
     const int64_t words = (int64_t)n * f->width[q];
-    vec = vec && (reinterpret_cast<uintptr_t>(f->ptr[q]) & 15) == 0 && (o & 3) == 0;
-    o += words;
+    vec = vec && (reinterpret_cast<uintptr_t>(f->ptr[q]) & 15) == 0;
     most = words > most ? words : most;
   }
   if (vec)

@@ -167,7 +167,8 @@ class SceneBatch:
                                                  self.n)).to(dev)
         self.uid = torch.arange(NP, dtype=torch.int64, device=dev)
         self.carry = carry
-        self.tmp = torch.empty(max(7 * NP, 1), dtype=f32, device=dev)   # x, v, rho, uid rows
+        # x, v, rho, uid blocks of sph_permute_fields (n * 7 words + 4 per field)
+        self.tmp = torch.empty(7 * NP + 16, dtype=f32, device=dev)
         # ---- boundary state (static)
         self.bx = cat([s.bx for s in scenes]).float().contiguous()
         self.bgx = cat([s.bgx for s in scenes]).float().contiguous() if self.solid else None

@@ -115,9 +115,8 @@ def test_fused_rk4_step_matches_separate_kernels(cuda, reorder):
 @pytest.mark.parametrize("n", [100_003, 100_000])
 def test_permute_fields_matches_numpy(cuda, n):
     """sph_permute_fields moves every field by the same permutation, int64
-    ids bit for bit, with fields of widths 1, 2 and 4: with an odd count the
-    field blocks of the temporary sit at unaligned offsets (word moves), with
-    a multiple of four every block is aligned (vector moves)."""
+    ids bit for bit, with fields of widths 1, 2 and 4, for an odd count and
+    a multiple of four (the temporary's field blocks padded to 16 bytes)."""
     import ctypes as C
 
     from pytorchsph_amd import _lib as L
@@ -130,7 +129,7 @@ def test_permute_fields_matches_numpy(cuda, n):
     c = rng.standard_normal((n, 4)).astype(np.float32)
     dev = torch.device("cuda")
     ta, tb, tid, tc = (torch.from_numpy(t.copy()).to(dev) for t in (a, b, ids, c))
-    tmp = torch.empty(n * 9, dtype=torch.float32, device=dev)
+    tmp = torch.empty(n * 9 + 16, dtype=torch.float32, device=dev)
     fl = L.SphFields()
     for q, (t, w) in enumerate(((ta, 2), (tb, 1), (tid, 2), (tc, 4))):
         fl.ptr[q] = L.ptr(t)
