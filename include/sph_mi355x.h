@@ -330,8 +330,10 @@ typedef struct SphRk4Stage {
   const float* area;    /* [np] particle areas (for frec_next's volumes)    */
   float* K;             /* [np][5] running sums, as sph_integrate_rk4's kacc */
   float* x_out;         /* [np][2] next stage's state; the step's result at */
-  float* v_out;         /*         stage 4 (may alias x0 / v0 / r0 there)   */
-  float* rho_out;       /* [np]                                             */
+  float* v_out;         /*         stage 4 (may alias x0 / v0 / r0 there);  */
+  float* rho_out;       /* [np]    v_out may be NULL at stages 1-3 when
+                           frec_next is given (the next stage reads its
+                           velocities from those records)                   */
   float* acc_out;       /* stage 4: summed acceleration (as acc_out of
                            sph_integrate_rk4), or NULL                      */
   float* frec_next;     /* stages 1-3: packed records of the next stage's

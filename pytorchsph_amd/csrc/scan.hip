@@ -97,19 +97,27 @@ __global__ __launch_bounds__(kScanBS) void k_scan_reduce(const int* __restrict__
   }
 }
 
+// Exclusive scan of the nb tile sums in place (one workgroup): thread t owns
+// the contiguous segment [t seg, (t + 1) seg), sums it, takes its offset from
+// one workgroup scan of those sums and rewrites the segment -- one scan
+// whatever nb (a loop of workgroup scans over 1024 sums at a time cost a
+// few microseconds of barriers per round).
 __global__ __launch_bounds__(1024) void k_scan_partials(int* __restrict__ partial, int nb,
                                                         int* __restrict__ out, int n) {
   __shared__ int lds[1024 / kWave + 1];
-  int carry = 0;
-  for (int base = 0; base < nb; base += blockDim.x) {
-    int i = base + threadIdx.x;
-    int v = i < nb ? partial[i] : 0;
-    int tot;
-    int ex = block_exclusive_scan(v, lds, &tot);
-    if (i < nb) partial[i] = ex + carry;
-    carry += tot;
+  const int seg = (nb + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int a = threadIdx.x * seg;
+  const int b = a + seg < nb ? a + seg : nb;
+  int sum = 0;
+  for (int i = a; i < b; ++i) sum += partial[i];
+  int tot;
+  int run = block_exclusive_scan(sum, lds, &tot);
+  for (int i = a; i < b; ++i) {
+    const int v = partial[i];
+    partial[i] = run;
+    run += v;
   }
-  if (threadIdx.x == 0) out[n] = carry;
+  if (threadIdx.x == 0) out[n] = tot;
 }
 
 template <bool VEC>

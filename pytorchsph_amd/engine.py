@@ -627,8 +627,11 @@ class SceneBatch:
             for k, (cur, out) in enumerate(stages, start=1):
                 frec, nxt = recs[(k - 1) % 2], recs[k % 2]
                 ff, fbp = self._dsph_prepare(*cur, frec if k == 1 else None, s)
+                # stages 1-3 need no velocity array: the next stage reads the
+                # velocities from the records
                 rk = L.SphRk4Stage(k, L.ptr(self.x), L.ptr(self.v), L.ptr(self.rho),
-                                   L.ptr(self.area), L.ptr(self.K), L.ptr(out[0]), L.ptr(out[1]),
+                                   L.ptr(self.area), L.ptr(self.K), L.ptr(out[0]),
+                                   L.ptr(out[1]) if k == 4 else None,
                                    L.ptr(out[2]), L.ptr(self.acc) if k == 4 else None,
                                    L.ptr(nxt) if k < 4 else None)
                 if self.cellwalk:
