@@ -246,7 +246,7 @@ __device__ __forceinline__ void packed_row_out(const SphPhysics& P, const Own& o
   const Rk4State n = rk4_particle(rk.stage, P, ax, ay, o.vx, o.vy, dp, true,
                                   rk.K + 5 * (size_t)i, s0, &asum);
   st2(rk.x_out, i, make_float2(n.x, n.y));
-  st2(rk.v_out, i, make_float2(n.vx, n.vy));
+  if (rk.v_out) st2(rk.v_out, i, make_float2(n.vx, n.vy));
   rk.rho_out[i] = n.r;
   if (rk.stage == 4) {
     if (rk.acc_out) st2(rk.acc_out, i, asum);
@@ -1173,7 +1173,8 @@ static int check_rk4(const SphRk4Stage* rk, const float* frec, unsigned terms,
                      const std::string& who) {
   SPH_REQUIRE(rk, who + ": bad arguments");
   SPH_REQUIRE(rk->stage >= 1 && rk->stage <= 4, who + ": stage must be 1..4");
-  SPH_REQUIRE(rk->x0 && rk->v0 && rk->r0 && rk->K && rk->x_out && rk->v_out && rk->rho_out,
+  SPH_REQUIRE(rk->x0 && rk->v0 && rk->r0 && rk->K && rk->x_out && rk->rho_out &&
+                  (rk->v_out || (rk->stage < 4 && rk->frec_next)),
               who + ": missing state arrays");
   SPH_REQUIRE(!rk->frec_next || (rk->area && rk->frec_next != frec),
               who + ": next records need the areas and a buffer of their own");
