@@ -226,16 +226,31 @@ __device__ __forceinline__ void packed_bdy_pairs(const SphPhysics& P, const KCon
   }
 }
 
+// What the RK4 stage combine of row i reads besides the row's sums and the
+// running sums K: its state at the step's start and its area (stages 1-3,
+// for the next stage's record).  Loaded apart from the combine so that a
+// kernel can issue these loads long before it needs them.  (K is read in
+// the combine itself: held as well, its five registers spill in the cell
+// walk.)
+struct Rk4In {
+  Rk4State s0;
+  float area;
+};
+__device__ __forceinline__ Rk4In load_rk4_in(const SphRk4Stage& rk, int i) {
+  Rk4In r;
+  r.s0 = {rk.x0[2 * i], rk.x0[2 * i + 1], rk.v0[2 * i], rk.v0[2 * i + 1], rk.r0[i]};
+  r.area = rk.stage < 4 && rk.frec_next ? rk.area[i] : 1.f;
+  return r;
+}
+
 // Output of row i from its reduced sums.  FUSE: the row's RK4 stage
 // combine (rk4_particle) and the next stage's record instead of dp/dt and
-// a (see sph_dsph_gather_rk4).
-// s0: the row's state at the step's start (rk.x0, rk.v0, rk.r0 of row i),
-// when FUSE.
+// a (see sph_dsph_gather_rk4), with `in` the row's load_rk4_in.
 template <bool FUSE>
 __device__ __forceinline__ void packed_row_out(const SphPhysics& P, const Own& o, const Acc& a,
                                                unsigned terms, int i, float* __restrict__ dpdt,
                                                float* __restrict__ acc, const SphRk4Stage& rk,
-                                               const Rk4State& s0) {
+                                               const Rk4In& in) {
   if (!FUSE) {
     dsph_store(P, o, a, terms, i, dpdt, acc);
     return;
@@ -244,7 +259,7 @@ __device__ __forceinline__ void packed_row_out(const SphPhysics& P, const Own& o
   dsph_rates(P, o, a, terms, dp, ax, ay);
   float2 asum;
   const Rk4State n = rk4_particle(rk.stage, P, ax, ay, o.vx, o.vy, dp, true,
-                                  rk.K + 5 * (size_t)i, s0, &asum);
+                                  rk.K + 5 * (size_t)i, in.s0, &asum);
   st2(rk.x_out, i, make_float2(n.x, n.y));
   if (rk.v_out) st2(rk.v_out, i, make_float2(n.vx, n.vy));
   rk.rho_out[i] = n.r;
@@ -252,7 +267,7 @@ __device__ __forceinline__ void packed_row_out(const SphPhysics& P, const Own& o
     if (rk.acc_out) st2(rk.acc_out, i, asum);
   } else if (rk.frec_next) {
     pack_fluid(reinterpret_cast<float4*>(rk.frec_next), i, P, n.x, n.y, n.vx, n.vy, n.r,
-               rk.area[i] / n.r);
+               in.area / n.r);
   }
 }
 
@@ -322,9 +337,9 @@ __global__ __launch_bounds__(kBS) void k_dsph_gather_packed(
   packed_bdy_pairs<KL>(P, K, o, brec, fb, i, rl.sub, terms, a);
   reduce_acc<KL>(a, terms);
   if (rl.sub == 0) {
-    Rk4State s0 = {};
-    if (FUSE) s0 = {rk.x0[2 * i], rk.x0[2 * i + 1], rk.v0[2 * i], rk.v0[2 * i + 1], rk.r0[i]};
-    packed_row_out<FUSE>(P, o, a, terms, i, dpdt, acc, rk, s0);
+    Rk4In in = {};
+    if (FUSE) in = load_rk4_in(rk, i);
+    packed_row_out<FUSE>(P, o, a, terms, i, dpdt, acc, rk, in);
   }
 }
 
@@ -358,6 +373,11 @@ constexpr int kWalkRecs = 3 * kWalkRows * 3 / 2 + 256;
 constexpr int kWalkCap = 27;                    // listed hits per row
 constexpr int kWalkBatch = 4;                   // candidates tested per trip
 constexpr int kBdyBatch = 2;                    // boundary candidates per trip
+// The stage loads its slots in rounds of kStageRound per thread: a round
+// issues all its loads before it writes any to LDS, so the stage waits on
+// memory once per round instead of once per slot (six slots per round
+// spill registers at the kernel's 96).
+constexpr int kStageRound = 3;
 
 struct WalkShared {
   // sorted positions of the staged runs; the test loads up to
@@ -561,6 +581,8 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5))) void k
     SphGridDesc d = {};
     float xq = 0.f, yq = 0.f, thr = 0.f;
     int qi = 0, sc = 0, ix = 0, iy = 0;
+    float4 a0 = {}, a1 = {};
+    Rk4In rin = {};
     if (valid) {
       qi = g.sorted_idx[sp];
       sc = g.nscenes == 1 ? 0 : g.pscene[qi];
@@ -572,22 +594,40 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5))) void k
       iy = cell_coord(yq, d.qmin_y, d.hmax);
       st.init(d, g.cell_start + d.cell_begin, ix, iy);
       thr = sq_threshold(phys[sc].h);
+      // one lane per row: the row's own record and RK4 inputs are loaded
+      // here, their wait hidden behind the stage's
+      if (KL == 1) {
+        a0 = frec[2 * qi];
+        a1 = frec[2 * qi + 1];
+        if (FUSE) rin = load_rk4_in(rk, qi);
+      }
     }
-    // ---- stage: the chunk's three stencil runs, positions and indices,
-    // each run copied by its own loop (no run select per element)
+    // ---- stage: the chunk's three stencil runs, positions and indices
     const ChunkRuns cr = chunk_runs(st, valid, S.wlo, S.whi);
     const int tot = cr.tot;
     const bool staged = tot <= kWalkRecs;
     if (staged) {
-      const int rlo[3] = {cr.lo0, cr.lo1, cr.lo2};
-      const int rbeg[3] = {0, cr.off1, cr.off2};
-      const int rend[3] = {cr.off1, cr.off2, tot};
+      // a wavefront runs a round when any of its slots is staged; slots
+      // past the end re-read the last one, so no divergent branch surrounds
+      // the loads and their values stay in registers
+      const int w0 = __builtin_amdgcn_readfirstlane(t);
+      for (int u0 = 0; w0 + u0 * kBS < tot; u0 += kStageRound) {
+        float2 pv[kStageRound];
+        int iv[kStageRound];
 #pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        const int shift = rlo[r] - rbeg[r];
-        for (int k = rbeg[r] + t; k < rend[r]; k += kBS) {
-          S.pos[k] = sx[k + shift];
-          S.ridx[k] = g.sorted_idx[k + shift];
+        for (int u = 0; u < kStageRound; ++u) {
+          const int k = t + (u0 + u) * kBS;
+          const int sl = cr.slot(k < tot ? k : tot - 1);
+          pv[u] = sx[sl];
+          iv[u] = g.sorted_idx[sl];
+        }
+#pragma unroll
+        for (int u = 0; u < kStageRound; ++u) {
+          const int k = t + (u0 + u) * kBS;
+          if (k < tot) {
+            S.pos[k] = pv[u];
+            S.ridx[k] = iv[u];
+          }
         }
       }
     }
@@ -628,7 +668,8 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5))) void k
       }
       S.cnt[t] = (unsigned short)(cnt > kWalkCap ? kWalkCap + 1 : cnt);
     }
-    __syncthreads();
+    // with one lane per row a row's list is its own thread's: no wait
+    if (KL > 1) __syncthreads();
     // ---- gather: KL lanes per row.  With one lane per row, row rr is the
     // thread's own row of the test, whose index, scene, cell and threshold
     // are still in registers.
@@ -649,7 +690,10 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5))) void k
       const SphPhysics P = phys[sc];
       const KConst K = kconst(P.h);
       if (KL > 1) thr = sq_threshold(P.h);
-      const float4 a0 = frec[2 * qi], a1 = frec[2 * qi + 1];
+      if (KL > 1) {
+        a0 = frec[2 * qi];
+        a1 = frec[2 * qi + 1];
+      }
       Own o;
       o.x = a0.x; o.y = a0.y; o.vx = a0.z; o.vy = a0.w;
       o.rho = a1.x;
@@ -678,13 +722,12 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5))) void k
       }
       reduce_acc<KL>(a, terms);
       if (sub == 0) {
-        Rk4State s0 = {};
-        if (FUSE) s0 = {rk.x0[2 * qi], rk.x0[2 * qi + 1], rk.v0[2 * qi], rk.v0[2 * qi + 1],
-                        rk.r0[qi]};
-        packed_row_out<FUSE>(P, o, a, terms, qi, dpdt, acc, rk, s0);
+        if (FUSE && KL > 1) rin = load_rk4_in(rk, qi);
+        packed_row_out<FUSE>(P, o, a, terms, qi, dpdt, acc, rk, rin);
       }
     }
-    __syncthreads();
+    // the next chunk restages the LDS arrays
+    if (c0 + kWalkRows < s1) __syncthreads();
   }
 }
 

@@ -174,12 +174,13 @@ __device__ __forceinline__ void chunk_runs_n(const Stencil3 (&st)[NS], const boo
     const int* l = lo + 3 * s;
     const int* h = hi + 3 * s;
     const bool e0 = h[0] > l[0], e1 = h[1] > l[1], e2 = h[2] > l[2];
-    c[s].lo0 = e0 ? l[0] : 0;
-    c[s].lo1 = e1 ? l[1] : 0;
-    c[s].lo2 = e2 ? l[2] : 0;
-    c[s].off1 = e0 ? h[0] - c[s].lo0 : 0;
-    c[s].off2 = c[s].off1 + (e1 ? h[1] - c[s].lo1 : 0);
-    c[s].tot = c[s].off2 + (e2 ? h[2] - c[s].lo2 : 0);
+    // the same in every lane: held in scalar registers
+    c[s].lo0 = __builtin_amdgcn_readfirstlane(e0 ? l[0] : 0);
+    c[s].lo1 = __builtin_amdgcn_readfirstlane(e1 ? l[1] : 0);
+    c[s].lo2 = __builtin_amdgcn_readfirstlane(e2 ? l[2] : 0);
+    c[s].off1 = __builtin_amdgcn_readfirstlane(e0 ? h[0] - c[s].lo0 : 0);
+    c[s].off2 = __builtin_amdgcn_readfirstlane(c[s].off1 + (e1 ? h[1] - c[s].lo1 : 0));
+    c[s].tot = __builtin_amdgcn_readfirstlane(c[s].off2 + (e2 ? h[2] - c[s].lo2 : 0));
   }
 }
 
