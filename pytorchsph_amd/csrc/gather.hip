@@ -390,31 +390,25 @@ struct WalkShared {
 };
 
 // Listed fluid pairs of a row: entry k (rank k) on lane k mod KL, in
-// batches of kPackedBatch as the packed gather takes them.
+// batches of kPackedBatch.  Every slot of a batch loads (slots past the row
+// re-read its last entry), so no divergent branch surrounds the loads.
 template <int KL>
 __device__ __forceinline__ void walk_list(const SphPhysics& P, const KConst& K, const Own& o,
                                           unsigned terms, const unsigned short* lst, int n,
                                           int sub, const WalkShared& S,
                                           const float4* __restrict__ frec, Acc& a) {
   for (int kb = sub; kb < n; kb += KL * kPackedBatch) {
-    bool ok[kPackedBatch];
     float4 r0[kPackedBatch], r1[kPackedBatch];
 #pragma unroll
     for (int u = 0; u < kPackedBatch; ++u) {
       const int k = kb + u * KL;
-      ok[u] = k < n;
-      if (ok[u]) {
-        const int j = S.ridx[lst[k]];
-        r0[u] = frec[2 * j];
-        r1[u] = frec[2 * j + 1];
-      } else {
-        r0[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-        r1[u] = r0[u];
-      }
+      const int j = S.ridx[lst[k < n ? k : n - 1]];
+      r0[u] = frec[2 * j];
+      r1[u] = frec[2 * j + 1];
     }
 #pragma unroll
     for (int u = 0; u < kPackedBatch; ++u) {
-      if (ok[u]) {
+      if (kb + u * KL < n) {
         const Nbr nb = {r0[u].x, r0[u].y, r0[u].z, r0[u].w, r1[u].x, r1[u].y, r1[u].z};
         dsph_pair(P, K, o, nb, 0.f, 0.f, true, terms, a);
       }
@@ -559,41 +553,49 @@ __device__ __forceinline__ void bdy_stencil(const SphGrid& bg, const SphGridDesc
           same ? fiy : cell_coord(yq, bd.qmin_y, bd.hmax));
 }
 
-// segs[b] = {first, end} sorted slots of workgroup b (one scene each).
-// Five wavefronts per SIMD is what the LDS of the stage allows; the
-// register budget is held to it (96 VGPRs), since 100 registers left four.
-template <unsigned T, int KL, bool FUSE>
-__global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5))) void k_dsph_cellwalk(
-    SphGrid g, const int2* __restrict__ segs, const float4* __restrict__ frec,
-    const float4* __restrict__ brec, const SphPhysics* __restrict__ phys, SphCSR fb, SphGrid bg,
-    unsigned rterms, float* __restrict__ dpdt, float* __restrict__ acc, SphRk4Stage rk) {
-  const unsigned terms = T ? T : rterms;
-  __shared__ WalkShared S;
-  const int2 seg = segs[xcd_block()];
-  const int s0 = seg.x > 0 ? seg.x : 0, s1 = seg.y < g.np ? seg.y : g.np;
+// The chunks of one workgroup's segment [s0, s1) of sorted slots.  UNI:
+// every row of the segment is in scene usc, whose grid, constants and
+// threshold are then the same in every lane: loaded once, into scalar
+// registers, which leaves the vector registers to the pair loop.
+template <unsigned T, int KL, bool FUSE, bool UNI>
+__device__ __forceinline__ void cellwalk_segment(WalkShared& S, int s0, int s1, int usc,
+                                                 const SphGrid& g, const float4* __restrict__ frec,
+                                                 const float4* __restrict__ brec,
+                                                 const SphPhysics* __restrict__ phys,
+                                                 const SphCSR& fb, const SphGrid& bg,
+                                                 unsigned terms, float* __restrict__ dpdt,
+                                                 float* __restrict__ acc, const SphRk4Stage& rk) {
   const int t = threadIdx.x;
   const float2* __restrict__ sx = reinterpret_cast<const float2*>(g.sorted_x);
+  SphGridDesc ud = {};
+  float uthr = 0.f;
+  if (UNI) {
+    ud = g.scenes[usc];
+    uthr = sq_threshold(phys[usc].h);
+  }
   for (int c0 = s0; c0 < s1; c0 += kWalkRows) {
     // ---- the stencil of row t (one lane per row)
     const int sp = c0 + t;
     const bool valid = sp < s1;
     Stencil3 st;
-    SphGridDesc d = {};
-    float xq = 0.f, yq = 0.f, thr = 0.f;
-    int qi = 0, sc = 0, ix = 0, iy = 0;
+    SphGridDesc d = ud;
+    float xq = 0.f, yq = 0.f, thr = uthr;
+    int qi = 0, sc = UNI ? usc : 0, ix = 0, iy = 0;
     float4 a0 = {}, a1 = {};
     Rk4In rin = {};
     if (valid) {
       qi = g.sorted_idx[sp];
-      sc = g.nscenes == 1 ? 0 : g.pscene[qi];
-      d = g.scenes[sc];
+      if (!UNI) {
+        sc = g.nscenes == 1 ? 0 : g.pscene[qi];
+        d = g.scenes[sc];
+        thr = sq_threshold(phys[sc].h);
+      }
       const float2 q = sx[sp];
       xq = q.x;
       yq = q.y;
       ix = cell_coord(xq, d.qmin_x, d.hmax);
       iy = cell_coord(yq, d.qmin_y, d.hmax);
       st.init(d, g.cell_start + d.cell_begin, ix, iy);
-      thr = sq_threshold(phys[sc].h);
       // one lane per row: the row's own record and RK4 inputs are loaded
       // here, their wait hidden behind the stage's
       if (KL == 1) {
@@ -679,8 +681,10 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5))) void k
       const int spr = c0 + rr;
       if (KL > 1) {
         qi = g.sorted_idx[spr];
-        sc = g.nscenes == 1 ? 0 : g.pscene[qi];
-        d = g.scenes[sc];
+        if (!UNI) {
+          sc = g.nscenes == 1 ? 0 : g.pscene[qi];
+          d = g.scenes[sc];
+        }
         const float2 q = sx[spr];
         xq = q.x;
         yq = q.y;
@@ -689,7 +693,7 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5))) void k
       }
       const SphPhysics P = phys[sc];
       const KConst K = kconst(P.h);
-      if (KL > 1) thr = sq_threshold(P.h);
+      if (KL > 1 && !UNI) thr = sq_threshold(P.h);
       if (KL > 1) {
         a0 = frec[2 * qi];
         a1 = frec[2 * qi + 1];
@@ -729,6 +733,36 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5))) void k
     // the next chunk restages the LDS arrays
     if (c0 + kWalkRows < s1) __syncthreads();
   }
+}
+
+// segs[b] = {first, end} sorted slots of workgroup b.  Five wavefronts per
+// SIMD is what the LDS of the stage allows; the register budget is held to
+// it (96 VGPRs), since 100 registers left four.
+template <unsigned T, int KL, bool FUSE>
+__global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5))) void k_dsph_cellwalk(
+    SphGrid g, const int2* __restrict__ segs, const float4* __restrict__ frec,
+    const float4* __restrict__ brec, const SphPhysics* __restrict__ phys, SphCSR fb, SphGrid bg,
+    unsigned rterms, float* __restrict__ dpdt, float* __restrict__ acc, SphRk4Stage rk) {
+  const unsigned terms = T ? T : rterms;
+  __shared__ WalkShared S;
+  const int2 seg = segs[xcd_block()];
+  const int s0 = seg.x > 0 ? seg.x : 0, s1 = seg.y < g.np ? seg.y : g.np;
+  // A segment inside one scene (scenes are contiguous in sorted order, so
+  // its first and last rows tell) takes the uniform form.
+  int usc = -1;
+  if (g.nscenes == 1) {
+    usc = 0;
+  } else if (s0 < s1) {
+    const int a = g.pscene[g.sorted_idx[s0]], z = g.pscene[g.sorted_idx[s1 - 1]];
+    usc = a == z ? a : -1;
+  }
+  usc = __builtin_amdgcn_readfirstlane(usc);
+  if (usc >= 0)
+    cellwalk_segment<T, KL, FUSE, true>(S, s0, s1, usc, g, frec, brec, phys, fb, bg, terms, dpdt,
+                                        acc, rk);
+  else
+    cellwalk_segment<T, KL, FUSE, false>(S, s0, s1, -1, g, frec, brec, phys, fb, bg, terms, dpdt,
+                                         acc, rk);
 }
 
 // ------------------------------------------ renormalised density gradient

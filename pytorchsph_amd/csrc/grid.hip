@@ -251,6 +251,7 @@ __global__ __launch_bounds__(kBS) void k_neighbors(SphGrid g, SphQuery q,
 // the L2 in more pieces and took 15-60% longer.)
 constexpr int kSWalkRecs = 3 * kBS * 3 / 2 + 256;   // staged stencil points per chunk
 constexpr int kSWalkBatch = 4;                      // candidates tested per trip
+constexpr int kSWalkRound = 6;                      // staged slots per thread per round
 
 __global__ __launch_bounds__(kBS) void k_walk_count(SphGrid g, SphQuery q,
                                                     int* __restrict__ counts) {
@@ -287,7 +288,24 @@ __global__ __launch_bounds__(kBS) void k_walk_count(SphGrid g, SphQuery q,
   const ChunkRuns cr = chunk_runs(st, valid, s_wlo, s_whi);
   const bool staged = cr.tot <= kSWalkRecs;
   if (staged) {
-    for (int k = t; k < cr.tot; k += kBS) s_pos[k] = sx[cr.slot(k)];
+    // kSWalkRound slots per thread in flight before their LDS writes (one
+    // wait per round, not per slot); a wavefront runs a round when any of
+    // its slots is staged, slots past the end re-reading the last one
+    const int w0 = __builtin_amdgcn_readfirstlane(t);
+    const int tot = cr.tot;
+    for (int u0 = 0; w0 + u0 * kBS < tot; u0 += kSWalkRound) {
+      float2 pv[kSWalkRound];
+#pragma unroll
+      for (int u = 0; u < kSWalkRound; ++u) {
+        const int k = t + (u0 + u) * kBS;
+        pv[u] = sx[cr.slot(k < tot ? k : tot - 1)];
+      }
+#pragma unroll
+      for (int u = 0; u < kSWalkRound; ++u) {
+        const int k = t + (u0 + u) * kBS;
+        if (k < tot) s_pos[k] = pv[u];
+      }
+    }
   }
   __syncthreads();
   if (!valid) return;
