@@ -407,6 +407,18 @@ int sph_mdbc_density_walk(const SphFluid* f, const SphPhysics* phys, const float
                           const float* gx, const float* gh, const int* bscene, int nb,
                           const SphGrid* g, float* rho_b, float* p_b, int* counts,
                           hipStream_t s);
+/* sph_mdbc_density_walk that also writes the boundary records brec [nb][8]
+ * of sph_dsph_pack from the densities and pressures it computes ({x, y, vx,
+ * vy, rho_b rho0, V / rho_b, p_b, rho_b}; bv [nb][2] may be NULL for walls
+ * at rest, bvol [nb] holds the volumes V): the same values with no second
+ * pass over the boundary.  Replaces, with sph_mdbc_density_walk, the
+ * boundary density and pressure of computeBoundaryDensitymDBC
+ * (solidBoundary.py:561-616) that the stage's boundary terms read. */
+int sph_mdbc_density_walk_records(const SphFluid* f, const SphPhysics* phys, const float* bx,
+                                  const float* bv, const float* bvol, const float* gx,
+                                  const float* gh, const int* bscene, int nb, const SphGrid* g,
+                                  float* rho_b, float* p_b, int* counts, float* brec,
+                                  hipStream_t s);
 /* EOS for n particles: p = EOS(rho). */
 int sph_eos(const float* rho, const int* pscene, const SphPhysics* phys, int n, float* p,
             hipStream_t s);

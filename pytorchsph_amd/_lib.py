@@ -148,6 +148,8 @@ _SIGS = {
                           c_vp, c_vp], c_int),
     "sph_mdbc_density_walk": ([_P(SphFluid), c_vp, c_vp, c_vp, c_vp, c_vp, c_int, _P(SphGrid),
                                c_vp, c_vp, c_vp, c_vp], c_int),
+    "sph_mdbc_density_walk_records": ([_P(SphFluid), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                       c_int, _P(SphGrid), c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "sph_eos": ([c_vp, c_vp, c_vp, c_int, c_vp, c_vp], c_int),
     "sph_akinci_boundary_density": ([_P(SphFluid), c_vp, c_vp, c_vp, c_int, c_vp, _P(SphCSR),
                                      c_vp, c_vp], c_int),

@@ -1033,7 +1033,9 @@ __global__ __launch_bounds__(kBS) void k_mdbc_density(SphFluid f, const SphPhysi
 // the fill's candidate order, with the fill's predicate (|ghost - x_j| <
 // gh), and adds each hit as k_mdbc_density adds its row, so rho_b, p_b and
 // the counts are those of sph_mdbc_density over the list of sph_neighbor_fill
-// bit for bit.  Four candidate loads in flight per lane.
+// bit for bit.  Four candidate loads in flight per lane.  brec: the ghost's
+// boundary record as k_dsph_pack writes it from rho_b and p_b (velocity bv,
+// zero when NULL; volume bvol), so the stage needs no separate pack.
 __global__ __launch_bounds__(kBS) void k_mdbc_walk(SphFluid f, const SphPhysics* __restrict__ phys,
                                                    const float* __restrict__ bx,
                                                    const float* __restrict__ gxp,
@@ -1041,7 +1043,10 @@ __global__ __launch_bounds__(kBS) void k_mdbc_walk(SphFluid f, const SphPhysics*
                                                    const int* __restrict__ bscene, int nb,
                                                    SphGrid grid, float* __restrict__ rho_b,
                                                    float* __restrict__ p_b,
-                                                   int* __restrict__ counts) {
+                                                   int* __restrict__ counts,
+                                                   const float* __restrict__ bv,
+                                                   const float* __restrict__ bvol,
+                                                   float4* __restrict__ brec) {
   const int bI = xcd_block() * blockDim.x + threadIdx.x;
   if (bI >= nb) return;
   const int sc = bscene[bI];
@@ -1071,10 +1076,17 @@ __global__ __launch_bounds__(kBS) void k_mdbc_walk(SphFluid f, const SphPhysics*
       }
     }
   }
-  const float rho = mdbc_rho(s, P, K, ld2(bx, bI), g);
+  const float2 xb = ld2(bx, bI);
+  const float rho = mdbc_rho(s, P, K, xb, g);
+  const float p = eos(P, rho);
   rho_b[bI] = rho;
-  if (p_b) p_b[bI] = eos(P, rho);
+  if (p_b) p_b[bI] = p;
   if (counts) counts[bI] = s.cnt;
+  if (brec) {
+    const float2 v = bv ? ld2(bv, bI) : make_float2(0.f, 0.f);
+    brec[2 * bI] = make_float4(xb.x, xb.y, v.x, v.y);
+    brec[2 * bI + 1] = make_float4(rho * P.rho0, bvol[bI] / rho, p, rho);
+  }
 }
 
 __global__ __launch_bounds__(kBS) void k_eos(const float* __restrict__ rho,
@@ -1401,8 +1413,25 @@ int mdbc_density_walk(const SphFluid* f, const SphPhysics* phys, const float* bx
               "sph_mdbc_density_walk: the fluid grid is not built");
   if (nb == 0) return SPH_OK;
   hipLaunchKernelGGL(k_mdbc_walk, dim3(blocks_for(nb, kBS)), dim3(kBS), 0, s, *f, phys, bx, gx,
-                     gh, bscene, nb, *g, rho_b, p_b, counts);
+                     gh, bscene, nb, *g, rho_b, p_b, counts, (const float*)nullptr,
+                     (const float*)nullptr, (float4*)nullptr);
   return launch_check("sph_mdbc_density_walk");
+}
+
+int mdbc_density_walk_records(const SphFluid* f, const SphPhysics* phys, const float* bx,
+                              const float* bv, const float* bvol, const float* gx,
+                              const float* gh, const int* bscene, int nb, const SphGrid* g,
+                              float* rho_b, float* p_b, int* counts, float* brec, hipStream_t s) {
+  SPH_REQUIRE(fluid_ok(f) && phys && bx && bvol && gx && gh && bscene && nb >= 0 && rho_b &&
+                  brec,
+              "sph_mdbc_density_walk_records: bad arguments");
+  SPH_REQUIRE(g && g->scenes && g->cell_start && g->sorted_idx && g->sorted_x && g->np == f->np,
+              "sph_mdbc_density_walk_records: the fluid grid is not built");
+  if (nb == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_mdbc_walk, dim3(blocks_for(nb, kBS)), dim3(kBS), 0, s, *f, phys, bx, gx,
+                     gh, bscene, nb, *g, rho_b, p_b, counts, bv, bvol,
+                     reinterpret_cast<float4*>(brec));
+  return launch_check("sph_mdbc_density_walk_records");
 }
 
 int eos_all(const float* rho, const int* pscene, const SphPhysics* phys, int n, float* p,
@@ -1496,6 +1525,14 @@ int sph_mdbc_density_walk(const SphFluid* f, const SphPhysics* phys, const float
                           const SphGrid* g, float* rho_b, float* p_b, int* counts,
                           hipStream_t s) {
   return sph::mdbc_density_walk(f, phys, bx, gx, gh, bscene, nb, g, rho_b, p_b, counts, s);
+}
+int sph_mdbc_density_walk_records(const SphFluid* f, const SphPhysics* phys, const float* bx,
+                                  const float* bv, const float* bvol, const float* gx,
+                                  const float* gh, const int* bscene, int nb, const SphGrid* g,
+                                  float* rho_b, float* p_b, int* counts, float* brec,
+                                  hipStream_t s) {
+  return sph::mdbc_density_walk_records(f, phys, bx, bv, bvol, gx, gh, bscene, nb, g, rho_b, p_b,
+                                        counts, brec, s);
 }
 int sph_eos(const float* rho, const int* pscene, const SphPhysics* phys, int n, float* p,
             hipStream_t s) {

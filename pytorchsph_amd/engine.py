@@ -570,10 +570,12 @@ class SceneBatch:
         ff = self.ff.csr().c()
         fb = self.fb.csr().c() if self.NB else None
         if self.NB and self.cellwalk:
+            # the walk writes the boundary records too (walls at rest)
             g = self.grid._grid()
-            L.call("sph_mdbc_density_walk", C.byref(F), L.ptr(self.phys), L.ptr(self.bx),
-                   L.ptr(self.bgx), L.ptr(self.bh), L.ptr(self.bscene), self.NB, C.byref(g),
-                   L.ptr(self.brho), L.ptr(self.bp), L.ptr(self.bcount), sh)
+            L.call("sph_mdbc_density_walk_records", C.byref(F), L.ptr(self.phys), L.ptr(self.bx),
+                   None, L.ptr(self.bvol), L.ptr(self.bgx), L.ptr(self.bh), L.ptr(self.bscene),
+                   self.NB, C.byref(g), L.ptr(self.brho), L.ptr(self.bp), L.ptr(self.bcount),
+                   L.ptr(self.brec), sh)
         elif self.NB:
             gf = self.gf.csr().c()
             L.call("sph_mdbc_density", C.byref(F), L.ptr(self.phys), L.ptr(self.bx),
@@ -588,7 +590,9 @@ class SceneBatch:
                    fbp, self._order(), None, L.ptr(self.Lmat), None, sh)
             L.call("sph_dsph_density_gradient", C.byref(F), L.ptr(self.phys), C.byref(ff),
                    C.byref(B), fbp, L.ptr(self.Lmat), self._order(), L.ptr(self.gradrho), sh)
-        L.call("sph_dsph_pack", C.byref(F), L.ptr(self.phys), C.byref(B),
+        # boundary records unless the mDBC walk wrote them
+        packed_b = bool(self.NB and self.cellwalk)
+        L.call("sph_dsph_pack", C.byref(F), L.ptr(self.phys), None if packed_b else C.byref(B),
                L.ptr(frec) if frec is not None else None, L.ptr(self.brec), sh)
         return ff, fbp
 

@@ -95,6 +95,8 @@ def test_bad_arguments_rejected_on_host():
         "sph_mdbc_density_walk": lambda: lib.sph_mdbc_density_walk(None, None, None, None, None,
                                                                    None, 0, None, None, None,
                                                                    None, None),
+        "sph_mdbc_density_walk_records": lambda: lib.sph_mdbc_density_walk_records(
+            None, None, None, None, None, None, None, None, 0, None, None, None, None, None, None),
     }
     for name, call in cases.items():
         assert call() == 1, name

@@ -165,6 +165,46 @@ def test_cellwalk_rejects_ddsph(cuda):
                L.ptr(b.acc), L.stream_handle())
 
 
+@pytest.mark.parametrize("kind", ["dam_break", "mix"])
+def test_mdbc_walk_records_equal_pack(cuda, kind):
+    """sph_mdbc_density_walk_records: the boundary densities, pressures and
+    counts of sph_mdbc_density_walk, and the boundary records that
+    sph_dsph_pack writes from them, bit for bit (walls at rest and, with a
+    velocity array, moving)."""
+    b = SceneBatch(scene_sets()[kind])
+    sh = L.stream_handle()
+    bv = torch.randn((max(b.NB, 1), 2), dtype=torch.float32, device="cuda")
+    for _ in range(2):
+        b.search(b.x, fluid_pairs=False, ghost_pairs=False, bdy_pairs=False)
+        F = b.fluid(b.x, b.v, b.rho)
+        g = b.grid._grid()
+        for vel in (None, bv):
+            rb, pb = torch.full_like(b.brho, float("nan")), torch.full_like(b.bp, float("nan"))
+            cnt = torch.full_like(b.bcount, -1)
+            L.call("sph_mdbc_density_walk", C.byref(F), L.ptr(b.phys), L.ptr(b.bx),
+                   L.ptr(b.bgx), L.ptr(b.bh), L.ptr(b.bscene), b.NB, C.byref(g), L.ptr(rb),
+                   L.ptr(pb), L.ptr(cnt), sh)
+            Bd = L.SphBoundary(L.ptr(b.bx), L.ptr(vel) if vel is not None else None, L.ptr(rb),
+                               L.ptr(b.bvol), L.ptr(pb), L.ptr(b.brest), L.ptr(b.bscene), b.NB)
+            ref = torch.full_like(b.brec, float("nan"))
+            L.call("sph_dsph_pack", C.byref(F), L.ptr(b.phys), C.byref(Bd), None, L.ptr(ref), sh)
+            rb2, pb2 = torch.full_like(rb, float("nan")), torch.full_like(pb, float("nan"))
+            cnt2 = torch.full_like(cnt, -1)
+            rec = torch.full_like(b.brec, float("nan"))
+            L.call("sph_mdbc_density_walk_records", C.byref(F), L.ptr(b.phys), L.ptr(b.bx),
+                   L.ptr(vel) if vel is not None else None, L.ptr(b.bvol), L.ptr(b.bgx),
+                   L.ptr(b.bh), L.ptr(b.bscene), b.NB, C.byref(g), L.ptr(rb2), L.ptr(pb2),
+                   L.ptr(cnt2), L.ptr(rec), sh)
+            torch.cuda.synchronize()
+            n = b.NB
+            assert np.array_equal(bits(rb[:n]), bits(rb2[:n]))
+            assert np.array_equal(bits(pb[:n]), bits(pb2[:n]))
+            assert np.array_equal(cnt[:n].cpu().numpy(), cnt2[:n].cpu().numpy())
+            assert np.array_equal(bits(ref[:n]), bits(rec[:n]))
+        for _ in range(3):
+            b.step()
+
+
 @pytest.mark.parametrize("kind", ["dam_break", "drops", "mix"])
 def test_mdbc_walk_equals_list(cuda, kind):
     """sph_mdbc_density_walk (each ghost's fluid neighbours found in the
