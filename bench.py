@@ -76,18 +76,20 @@ def _events_avg(launch, stream, reps=20, warm=3):
 
 
 def pressure_roofline(b, reps=20):
-    """sph_dfsph_pressure (k_pressure), the DFSPH pressure update, with every
-    DFSPH scene active (after one sph_dfsph_accel, whose records it reads),
-    and its algorithmic HBM bytes:
+    """The DFSPH pressure update with every DFSPH scene active (after one
+    accel, whose records it reads): sph_dfsph_pressure_sl (k_pressure_sl)
+    on a stencil-list batch, else sph_dfsph_pressure (k_pressure), and its
+    algorithmic HBM bytes:
       per fluid row      76 B: fluid and boundary row pointers (4 + 4), own
                          xrec and arec records (16 + 16), src, p2, alpha,
                          rho, rest (20), p, p2, resid written (12), xrec.z
-                         written (4)
+                         written (4); with stencil lists no fluid row
+                         pointer but the row's runs and masks (40)
       per boundary row   56 B: row pointer (4), bxrec (16), b_aact (4),
                          b_src, b_p, b_alpha, rho_b, rest_b (20), b_p,
                          b_resid written (8), bxrec.z written (4)
-      per pair           4 B column index (fluid-fluid, fluid-boundary,
-                         boundary-fluid lists)
+      per pair           4 B column index (fluid-fluid with a CSR list,
+                         fluid-boundary, boundary-fluid lists)
     Neighbour records are counted once (as their owner's row), so re-reads
     of a neighbour are cache traffic, not algorithmic bytes."""
     import ctypes as C
@@ -98,7 +100,8 @@ def pressure_roofline(b, reps=20):
 
     b._dfsph_pre()              # pair lists, density, alpha/source; all scenes active
     torch.cuda.synchronize()
-    nnz_ff = int(b.ff.rowptr[b.NP].item())
+    sl = b.stencil_list() if b.stencil_lists else None
+    nnz_ff = 0 if sl is not None else int(b.ff.rowptr[b.NP].item())
     nnz_fb = int(b.fb.rowptr[b.NP].item()) if b.NB else 0
     nnz_bf = int(b.bf.rowptr[b.NB].item()) if b.NB else 0
     F = b.fluid(b.x, b.v, b.rho)
@@ -110,16 +113,26 @@ def pressure_roofline(b, reps=20):
     stream = torch.cuda.current_stream()
     sh = L.stream_handle(stream)
 
-    L.call("sph_dfsph_accel", C.byref(F), L.ptr(b.phys), C.byref(ff),
-           C.byref(fb) if fb is not None else None, C.byref(D), sh)
+    fbp = C.byref(fb) if fb is not None else None
+    if sl is not None:
+        L.call("sph_dfsph_accel_sl", C.byref(F), L.ptr(b.phys), C.byref(sl), fbp, C.byref(D), sh)
+    else:
+        L.call("sph_dfsph_accel", C.byref(F), L.ptr(b.phys), C.byref(ff), fbp, C.byref(D), sh)
 
     def launch():
-        L.call("sph_dfsph_pressure", C.byref(F), L.ptr(b.phys), C.byref(ff), C.byref(B),
-               C.byref(fb) if fb is not None else None, C.byref(bf), C.byref(D), sh)
+        if sl is not None:
+            L.call("sph_dfsph_pressure_sl", C.byref(F), L.ptr(b.phys), C.byref(sl), C.byref(B),
+                   fbp, C.byref(bf), C.byref(D), sh)
+        else:
+            L.call("sph_dfsph_pressure", C.byref(F), L.ptr(b.phys), C.byref(ff), C.byref(B),
+                   fbp, C.byref(bf), C.byref(D), sh)
 
     sec = _events_avg(launch, stream, reps)
-    nbytes = 76 * b.NP + 56 * b.NB + 4 * (nnz_ff + nnz_fb + nnz_bf)
-    return dict(kernel="k_pressure (sph_dfsph_pressure)", seconds=sec, bytes=nbytes,
+    row = 76 + (36 if sl is not None else 0)
+    nbytes = row * b.NP + 56 * b.NB + 4 * (nnz_ff + nnz_fb + nnz_bf)
+    name = ("k_pressure_sl (sph_dfsph_pressure_sl)" if sl is not None
+            else "k_pressure (sph_dfsph_pressure)")
+    return dict(kernel=name, seconds=sec, bytes=nbytes,
                 rows=b.NP + b.NB, pairs=nnz_ff + nnz_fb + nnz_bf,
                 achieved_gbs=nbytes / sec / 1e9)
 

@@ -268,6 +268,29 @@ int sph_neighbor_count(const SphGrid* g, const SphQuery* q, int* counts, hipStre
  * the truncated list stays inside col. */
 int sph_neighbor_fill(const SphGrid* g, const SphQuery* q, int* rowptr, int* col,
                       int64_t cap, hipStream_t s);
+/* Stencil lists: the pairs of a grid's own points (self-sorted queries,
+ * radius 1, supports h[point]) as bitmasks over each row's stencil instead
+ * of a CSR column list.  Row r is sorted slot r; its three stencil rows
+ * dy = q - 1 (q = 0, 1, 2) are the candidate runs of sorted slots
+ * [runs[4r + q], runs[4r + q] + len_q), len_q = byte q of runs[4r + 3]
+ * (at most 255), and bit k of masks[3r + q] is set when candidate
+ * runs[4r + q] + k is a pair (the search predicate of sph_neighbor_fill).
+ * The set bits, run by run, are the row of sph_neighbor_fill in its order.
+ * A run of more than 64 candidates keeps its first 64 and flags its scene
+ * SPH_ST_OVERFLOW.  12 + 4 + 24 bytes per row and no count / scan pass:
+ * a neighbour list the DFSPH gathers walk with the chunk's stencil records
+ * staged in LDS (sph_*_sl).  Replaces the fluid-fluid list of
+ * buildNeighborList (neighSearch.cpp:172-259) for those gathers. */
+typedef struct SphStencilList {
+  const int* runs;                   /* [n][4]                             */
+  const unsigned long long* masks;   /* [n][3]                             */
+  int n;                             /* rows: the grid's np                */
+} SphStencilList;
+int sph_stencil_list_build(const SphGrid* g, const float* h, int* runs,
+                           unsigned long long* masks, hipStream_t s);
+/* a[i] = i for i < n: the grid's sorted_idx once its points have been
+ * permuted into their sorted order (sph_permute_fields by sorted_idx). */
+int sph_iota(int* a, int n, hipStream_t s);
 /* Sorts the columns of every row ascending (lexicographic (i, j) order). */
 int sph_csr_sort_rows(const SphCSR* a, int* col_mut, hipStream_t s);
 /* Transposes an nrows x ncols_t CSR.  t_counts is [ncols_t] scratch,
@@ -534,6 +557,37 @@ int sph_dfsph_finish(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff
 int sph_dfsph_finish_visc(const SphFluid* f, const SphPhysics* phys, const SphCSR* ff,
                           const SphCSR* fb, SphDfsph* d, const float* frec, unsigned vterms,
                           float* acc, hipStream_t s);
+
+/* The DFSPH gathers over a stencil list sl of the fluid (sph_stencil_list_build
+ * of the fluid grid) instead of the fluid-fluid CSR list ff: the same
+ * arguments otherwise and the same results as the CSR entry points with one
+ * lane per row (SPH_TUNE_SOLVE_LANES / SPH_TUNE_GATHER_LANES = 1), bit for
+ * bit.  The fluid state must be stored in the grid's sorted order (point
+ * index = sorted slot; sph_permute_fields by sorted_idx after the build,
+ * then sph_iota on sorted_idx).  A workgroup stages the records of its rows'
+ * stencil runs in LDS and walks the mask bits from there.  Boundary pairs
+ * keep their CSR lists (fb, bf).  Each replaces its CSR form's reference
+ * function (computeDensity density.py:26-53, dfsph.py:25-100,
+ * 164-190, 263-311, akinciBoundary.py:166-389). */
+int sph_fluid_density_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                         const SphBoundary* b, const SphCSR* fb, float* rho_out, float* vol_out,
+                         hipStream_t s);
+int sph_dfsph_alpha_source_sl(const SphFluid* f, const SphPhysics* phys,
+                              const SphStencilList* sl, const SphBoundary* b, const SphCSR* fb,
+                              const SphCSR* bf, SphDfsph* d, hipStream_t s);
+int sph_dfsph_accel_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                       const SphCSR* fb, SphDfsph* d, hipStream_t s);
+int sph_dfsph_pressure_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                          const SphBoundary* b, const SphCSR* fb, const SphCSR* bf, SphDfsph* d,
+                          hipStream_t s);
+int sph_dfsph_iterate_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                         const SphBoundary* b, const SphCSR* fb, const SphCSR* bf, SphDfsph* d,
+                         hipStream_t s);
+int sph_dfsph_finish_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                        const SphCSR* fb, SphDfsph* d, float* acc, hipStream_t s);
+int sph_dfsph_finish_visc_sl(const SphFluid* f, const SphPhysics* phys,
+                             const SphStencilList* sl, const SphCSR* fb, SphDfsph* d,
+                             const float* frec, unsigned vterms, float* acc, hipStream_t s);
 
 /* ------------------------------------------------------------- integrators */
 /* RK4 stage s in 1..4 (simulationBase.py:679-705): accumulates

@@ -68,6 +68,10 @@ class SphCSR(C.Structure):
     _fields_ = [("rowptr", c_vp), ("col", c_vp), ("nrows", c_int)]
 
 
+class SphStencilList(C.Structure):
+    _fields_ = [("runs", c_vp), ("masks", c_vp), ("n", c_int)]
+
+
 DFSPH_FIELDS = ("vpred", "aact", "alpha", "src", "p", "p2", "apred", "resid", "xrec", "arec",
                 "prec", "wvec", "b_aact", "b_alpha", "b_src", "b_p", "b_resid", "bxrec", "err_acc", "err_hist",
                 "iters", "active", "any_active", "status", "nfluid", "nbdy", "fbegin", "bbegin",
@@ -132,6 +136,22 @@ _SIGS = {
     "sph_neighbor_count": ([_P(SphGrid), _P(SphQuery), c_vp, c_vp], c_int),
     "sph_neighbor_fill": ([_P(SphGrid), _P(SphQuery), c_vp, c_vp, C.c_int64, c_vp], c_int),
     "sph_csr_sort_rows": ([_P(SphCSR), c_vp, c_vp], c_int),
+    "sph_stencil_list_build": ([_P(SphGrid), c_vp, c_vp, c_vp, c_vp], c_int),
+    "sph_iota": ([c_vp, c_int, c_vp], c_int),
+    "sph_fluid_density_sl": ([_P(SphFluid), c_vp, _P(SphStencilList), _P(SphBoundary),
+                              _P(SphCSR), c_vp, c_vp, c_vp], c_int),
+    "sph_dfsph_alpha_source_sl": ([_P(SphFluid), c_vp, _P(SphStencilList), _P(SphBoundary),
+                                   _P(SphCSR), _P(SphCSR), _P(SphDfsph), c_vp], c_int),
+    "sph_dfsph_accel_sl": ([_P(SphFluid), c_vp, _P(SphStencilList), _P(SphCSR), _P(SphDfsph),
+                            c_vp], c_int),
+    "sph_dfsph_pressure_sl": ([_P(SphFluid), c_vp, _P(SphStencilList), _P(SphBoundary),
+                               _P(SphCSR), _P(SphCSR), _P(SphDfsph), c_vp], c_int),
+    "sph_dfsph_iterate_sl": ([_P(SphFluid), c_vp, _P(SphStencilList), _P(SphBoundary),
+                              _P(SphCSR), _P(SphCSR), _P(SphDfsph), c_vp], c_int),
+    "sph_dfsph_finish_sl": ([_P(SphFluid), c_vp, _P(SphStencilList), _P(SphCSR), _P(SphDfsph),
+                             c_vp, c_vp], c_int),
+    "sph_dfsph_finish_visc_sl": ([_P(SphFluid), c_vp, _P(SphStencilList), _P(SphCSR),
+                                  _P(SphDfsph), c_vp, C.c_uint, c_vp, c_vp], c_int),
     "sph_csr_transpose": ([_P(SphCSR), c_int, c_vp, c_vp, c_vp, c_vp, c_vp, C.c_size_t, c_vp], c_int),
     "sph_pair_geometry": ([_P(SphCSR), c_vp, c_vp, c_vp, C.c_float, C.c_float, c_vp, c_vp, c_vp,
                            c_vp, c_vp], c_int),

@@ -37,6 +37,8 @@ def _time(fn, reps: int) -> float:
 
 
 def _search_fn(b):
+    if getattr(b, "stencil_lists", False):
+        return lambda: b._dfsph_search()
     return lambda: b.search(b.x)
 
 
@@ -47,8 +49,14 @@ def _gather_fn(b):
     ff = b.ff.csr().c()
     fb = b.fb.csr().c() if b.NB else None
     B = b.boundary()
+    sl = b.stencil_list() if b.stencil_lists else None
 
     def run():
+        if sl is not None:
+            L.call("sph_fluid_density_sl", C.byref(F), L.ptr(b.phys), C.byref(sl), C.byref(B),
+                   C.byref(fb) if fb is not None else None, L.ptr(b.rho), L.ptr(b.vol),
+                   L.stream_handle())
+            return
         L.call("sph_fluid_density", C.byref(F), L.ptr(b.phys), C.byref(ff), C.byref(B),
                C.byref(fb) if fb is not None else None, b._order(), L.ptr(b.rho), L.ptr(b.vol),
                L.stream_handle())
@@ -64,12 +72,20 @@ def _solve_fn(b):
     B = b.boundary()
     sh = L.stream_handle()
 
+    sl = b.stencil_list() if b.stencil_lists else None
+
     def run():
         D = b._dfsph_struct()     # grid size follows the current knobs
-        L.call("sph_dfsph_accel", C.byref(F), L.ptr(b.phys), C.byref(ff),
-               C.byref(fb) if fb is not None else None, C.byref(D), sh)
+        fbp = C.byref(fb) if fb is not None else None
+        if sl is not None:
+            L.call("sph_dfsph_accel_sl", C.byref(F), L.ptr(b.phys), C.byref(sl), fbp,
+                   C.byref(D), sh)
+            L.call("sph_dfsph_pressure_sl", C.byref(F), L.ptr(b.phys), C.byref(sl), C.byref(B),
+                   fbp, C.byref(bf), C.byref(D), sh)
+            return
+        L.call("sph_dfsph_accel", C.byref(F), L.ptr(b.phys), C.byref(ff), fbp, C.byref(D), sh)
         L.call("sph_dfsph_pressure", C.byref(F), L.ptr(b.phys), C.byref(ff), C.byref(B),
-               C.byref(fb) if fb is not None else None, C.byref(bf), C.byref(D), sh)
+               fbp, C.byref(bf), C.byref(D), sh)
     return run
 
 

@@ -111,6 +111,141 @@ __device__ __forceinline__ int advance_rank(const int* __restrict__ pre, int nac
   return r;
 }
 
+// Row epilogues shared by the CSR and stencil-list forms of the gathers
+// below, so that both compute the same bits from the same sums.
+
+// alpha_i, the source s_i and W_i of fluid row i from its sums (dfsph.py:58-100).
+__device__ __forceinline__ void alpha_store(const SphFluid& f, const SphPhysics& P,
+                                            const SphDfsph& d, int i, float k1x, float k1y,
+                                            float k2, float src) {
+  const float ai = d.aact[i];
+  const float fac = -P.dt * P.dt * ai;
+  const float mass = f.area[i] * f.rest[i];
+  float al = fac / mass * (k1x * k1x + k1y * k1y) + fac * k2;
+  al = al < -1.f ? -1.f : (al > -1e-7f ? -1e-7f : al);  // dfsph.py:82
+  d.alpha[i] = al;
+  // divergenceSolve: the source is the divergence term only (dfsph.py:161)
+  d.src[i] = d.divergence ? src : 1.f - f.rho[i] + src;
+  // W_i = dt^2 k1_i: the part of the pressure-update sum that does not
+  // change during the solve
+  st2(d.wvec, i, make_float2(P.dt * P.dt * k1x, P.dt * P.dt * k1y));
+}
+
+// alpha and source of boundary row k0 over its fluid neighbours
+// (akinciBoundary.py:317-352); lane `sub` of a group of KL.
+template <int KL>
+__device__ __forceinline__ void alpha_boundary_row(const SphPhysics* __restrict__ phys,
+                                                   const SphBoundary& b, const SphCSR& bf,
+                                                   const SphDfsph& d, int k0, int sub) {
+  const SphPhysics P = phys[b.bscene[k0]];
+  const KConst K = kconst(P.h);
+  const float2 xb = ld2(b.x, k0);
+  float s2 = 0.f, src = 0.f;
+  pair_pipeline_with<KL>(
+      bf.rowptr[k0], bf.rowptr[k0 + 1], sub, [&](int k) { return bf.col[k]; },
+      [&](int j) { return Rec2{ld4(d.xrec, j), ld4(d.prec, j)}; }, [&](const Rec2& r) {
+        const float2 G = spiky_G(xb.x, xb.y, r.a.x, r.a.y, K);
+        const float aa = r.b.z;
+        s2 += aa * aa / r.a.w * (G.x * G.x + G.y * G.y);
+        src += aa * ((0.f - r.b.x) * G.x + (0.f - r.b.y) * G.y);
+      });
+  s2 = group_sum<KL>(s2);
+  src = group_sum<KL>(src);
+  if (sub != 0) return;
+  d.b_alpha[k0] = -P.dt * P.dt * d.b_aact[k0] * s2;   // unclamped (akinciBoundary.py:340)
+  d.b_src[k0] = 1.f - b.rho[k0] - P.dt * src;
+}
+
+// The relaxed update of fluid row i from its fluid sum ks and the clamped
+// residual for the scene sum (dfsph.py:44-57,164-184):
+//   r_i = a*_i . W_i - ks - s_i.
+__device__ __forceinline__ double pressure_row_update(const SphFluid& f, const SphDfsph& d,
+                                                      float omega, float tau, int s, int i,
+                                                      float ks) {
+  const float2 ai = ld2(d.arec, 2 * i);
+  const float2 wi = ld2(d.wvec, i);
+  const float rr = (ai.x * wi.x + ai.y * wi.y) - ks - d.src[i];
+  const float pold = d.p2[i];
+  d.p[i] = pold;
+  float pn = pold - omega * rr / d.alpha[i];
+  // densitySolve clamps at 0 (clamp(min = 0) keeps NaN); divergenceSolve
+  // does not (dfsph.py:177)
+  if (!d.divergence) pn = pn < 0.f ? 0.f : pn;
+  if (isnan(pn) || isinf(pn)) atomicOr(&d.status[s], SPH_ST_NAN);
+  d.p2[i] = pn;
+  d.resid[i] = rr;
+  const float rho = f.rho[i] * f.rest[i];
+  d.xrec[4 * i + 2] = pn / (rho * rho);
+  return (double)(rr < -tau ? -tau : rr);
+}
+
+// The pressure update of boundary item tb of the active list
+// (akinciBoundary.py:280-315): its scene into *sc and, on the group's first
+// lane, *valid and the clamped residual.
+template <int KL>
+__device__ __forceinline__ double pressure_boundary_row(const SphBoundary& b, const SphCSR& bf,
+                                                        const SphDfsph& d,
+                                                        const SphPhysics* __restrict__ phys,
+                                                        const int* preb, int nact, int r0b,
+                                                        int tb, int sub, int* sc, bool* valid) {
+  const int r = advance_rank(preb, nact, r0b, tb);
+  const int s = d.act_scene[r];
+  *sc = s;
+  const int k0 = d.bbegin[s] + (tb - preb[r]);
+  const SphPhysics P = phys[s];
+  const KConst K = kconst(P.h);
+  const float2 xb = ld2(d.bxrec, 2 * k0);
+  float ks = 0.f;
+  // sum_f w_f a*_f . (-G_bf) with {x_f, y_f, w_f a*_f} from prec
+  pair_pipeline<KL>(bf.rowptr[k0], bf.rowptr[k0 + 1], sub, d.prec,
+                    [&](int k) { return bf.col[k]; }, [&](const float4& pf) {
+                      const float2 G = spiky_G(xb.x, xb.y, pf.x, pf.y, K);
+                      ks -= pf.z * G.x + pf.w * G.y;
+                    });
+  ks = group_sum<KL>(ks);
+  if (sub != 0) return 0.0;
+  const float rr = ks - d.b_src[k0];
+  float pn = d.b_p[k0] - P.omega * rr / d.b_alpha[k0];
+  pn = pn < 0.f ? 0.f : pn;
+  d.b_p[k0] = pn;
+  d.b_resid[k0] = rr;
+  const float rb = b.rho[k0] * b.rest[k0];
+  d.bxrec[4 * k0 + 2] = pn / (rb * rb);
+  *valid = true;
+  return (double)(rr < -P.tau ? -P.tau : rr);
+}
+
+// The end of the solve for fluid row i with its final pressure
+// acceleration a: a*, p, v* and (acc != NULL) acc += a.
+__device__ __forceinline__ void finish_store(const SphPhysics& P, const SphDfsph& d, int i,
+                                             float2 a, float* __restrict__ acc) {
+  st2(d.apred, i, a);
+  d.p[i] = d.p2[i];
+  const float2 vp = ld2(d.vpred, i);
+  st2(d.vpred, i, make_float2(vp.x + P.dt * a.x, vp.y + P.dt * a.y));
+  if (acc) {
+    const float2 o = ld2(acc, i);
+    st2(acc, i, make_float2(o.x + a.x, o.y + a.y));
+  }
+}
+
+// finish_store with the diffusion terms: acc = (acc + a) + their
+// acceleration.
+__device__ __forceinline__ void finish_visc_store(const SphPhysics& P, const Own& o,
+                                                  const SphDfsph& d, const Acc& va,
+                                                  unsigned vterms, int i, float2 a,
+                                                  float* __restrict__ acc) {
+  st2(d.apred, i, a);
+  d.p[i] = d.p2[i];
+  const float2 vp = ld2(d.vpred, i);
+  st2(d.vpred, i, make_float2(vp.x + P.dt * a.x, vp.y + P.dt * a.y));
+  float dp, vx, vy;
+  dsph_rates(P, o, va, vterms, dp, vx, vy);
+  const float2 old = ld2(acc, i);
+  const float2 w = make_float2(old.x + a.x, old.y + a.y);
+  st2(acc, i, make_float2(w.x + vx, w.y + vy));
+}
+
 __global__ __launch_bounds__(kBS) void k_prepare(SphFluid f, const float* __restrict__ acc,
                                                  const SphPhysics* __restrict__ phys, SphBoundary b,
                                                  SphDfsph d) {
@@ -228,36 +363,9 @@ __global__ __launch_bounds__(kBS) void k_alpha_source(SphFluid f, const SphPhysi
     k2 = group_sum<KL>(k2);
     src = group_sum<KL>(src);
     if (sub != 0) return;
-    const float ai = d.aact[i];
-    const float fac = -P.dt * P.dt * ai;
-    const float mass = f.area[i] * f.rest[i];
-    float al = fac / mass * (k1x * k1x + k1y * k1y) + fac * k2;
-    al = al < -1.f ? -1.f : (al > -1e-7f ? -1e-7f : al);  // dfsph.py:82
-    d.alpha[i] = al;
-    // divergenceSolve: the source is the divergence term only (dfsph.py:161)
-    d.src[i] = d.divergence ? src : 1.f - f.rho[i] + src;
-    // W_i = dt^2 k1_i: the part of the pressure-update sum that does not
-    // change during the solve
-    st2(d.wvec, i, make_float2(P.dt * P.dt * k1x, P.dt * P.dt * k1y));
+    alpha_store(f, P, d, i, k1x, k1y, k2, src);
   } else if (t < f.np + b.nb) {
-    const int k0 = t - f.np;
-    const SphPhysics P = phys[b.bscene[k0]];
-    const KConst K = kconst(P.h);
-    const float2 xb = ld2(b.x, k0);
-    float s2 = 0.f, src = 0.f;
-    pair_pipeline_with<KL>(
-        bf.rowptr[k0], bf.rowptr[k0 + 1], sub, [&](int k) { return bf.col[k]; },
-        [&](int j) { return Rec2{ld4(d.xrec, j), ld4(d.prec, j)}; }, [&](const Rec2& r) {
-          const float2 G = spiky_G(xb.x, xb.y, r.a.x, r.a.y, K);
-          const float aa = r.b.z;
-          s2 += aa * aa / r.a.w * (G.x * G.x + G.y * G.y);
-          src += aa * ((0.f - r.b.x) * G.x + (0.f - r.b.y) * G.y);
-        });
-    s2 = group_sum<KL>(s2);
-    src = group_sum<KL>(src);
-    if (sub != 0) return;
-    d.b_alpha[k0] = -P.dt * P.dt * d.b_aact[k0] * s2;   // unclamped (akinciBoundary.py:340)
-    d.b_src[k0] = 1.f - b.rho[k0] - P.dt * src;
+    alpha_boundary_row<KL>(phys, b, bf, d, t - f.np, sub);
   }
 }
 
@@ -423,22 +531,8 @@ __device__ __forceinline__ double pressure_fluid_row(const SphFluid& f, const Sp
   ks = group_sum<KL>(ks);
   *wrote = false;
   if (sub != 0) return 0.0;
-  const float2 ai = ld2(d.arec, 2 * i);
-  const float2 wi = ld2(d.wvec, i);
-  const float rr = (ai.x * wi.x + ai.y * wi.y) - ks - d.src[i];
-  const float pold = d.p2[i];
-  d.p[i] = pold;
-  float pn = pold - omega * rr / d.alpha[i];
-  // densitySolve clamps at 0 (clamp(min = 0) keeps NaN); divergenceSolve
-  // does not (dfsph.py:177)
-  if (!d.divergence) pn = pn < 0.f ? 0.f : pn;
-  if (isnan(pn) || isinf(pn)) atomicOr(&d.status[s], SPH_ST_NAN);
-  d.p2[i] = pn;
-  d.resid[i] = rr;
-  const float rho = f.rho[i] * f.rest[i];
-  d.xrec[4 * i + 2] = pn / (rho * rho);
   *wrote = true;
-  return (double)(rr < -tau ? -tau : rr);
+  return pressure_row_update(f, d, omega, tau, s, i, ks);
 }
 
 template <int KL>
@@ -501,32 +595,8 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8))) void k
                        : pressure_fluid_row<KL, false>(f, ff, d, K, P.fluid_omega, P.tau, s, i, k0,
                                                        k1, sub, w, wcols, &valid);
     } else if (t < nitems) {
-      const int tb = t - nf;
-      const int r = advance_rank(preb, nact, r0b, tb);
-      s = d.act_scene[r];
-      const int k0 = d.bbegin[s] + (tb - preb[r]);
-      const SphPhysics P = phys[s];
-      const KConst K = kconst(P.h);
-      const float2 xb = ld2(d.bxrec, 2 * k0);
-      float ks = 0.f;
-      // sum_f w_f a*_f . (-G_bf) with {x_f, y_f, w_f a*_f} from prec
-      pair_pipeline<KL>(bf.rowptr[k0], bf.rowptr[k0 + 1], sub, d.prec,
-                        [&](int k) { return bf.col[k]; }, [&](const float4& pf) {
-                          const float2 G = spiky_G(xb.x, xb.y, pf.x, pf.y, K);
-                          ks -= pf.z * G.x + pf.w * G.y;
-                        });
-      ks = group_sum<KL>(ks);
-      if (sub == 0) {
-        const float rr = ks - d.b_src[k0];
-        float pn = d.b_p[k0] - P.omega * rr / d.b_alpha[k0];
-        pn = pn < 0.f ? 0.f : pn;
-        d.b_p[k0] = pn;
-        d.b_resid[k0] = rr;
-        const float rb = b.rho[k0] * b.rest[k0];
-        d.bxrec[4 * k0 + 2] = pn / (rb * rb);
-        contrib = (double)(rr < -P.tau ? -P.tau : rr);
-        valid = true;
-      }
+      contrib = pressure_boundary_row<KL>(b, bf, d, phys, preb, nact, r0b, t - nf, sub, &s,
+                                          &valid);
     }
     scene_sum(contrib, s, valid, acc_slot, s_wsum, s_wscene);
   }
@@ -576,14 +646,7 @@ __global__ __launch_bounds__(kBS) void k_finish(SphFluid f, const SphPhysics* __
   const WaveCols none = {0, 0, false};
   const float2 a = pressure_accel<KL>(ff, fb, d, K, i, sub, none, nullptr);
   if (sub != 0) return;
-  st2(d.apred, i, a);
-  d.p[i] = d.p2[i];
-  const float2 vp = ld2(d.vpred, i);
-  st2(d.vpred, i, make_float2(vp.x + P.dt * a.x, vp.y + P.dt * a.y));
-  if (acc) {
-    const float2 o = ld2(acc, i);
-    st2(acc, i, make_float2(o.x + a.x, o.y + a.y));
-  }
+  finish_store(P, d, i, a, acc);
 }
 
 // k_finish and the velocity-diffusion and laminar gather of the step
@@ -640,15 +703,423 @@ __global__ __launch_bounds__(kBS) void k_finish_visc(SphFluid f,
   const float2 a = make_float2(group_sum<KL>(ax), group_sum<KL>(ay));
   reduce_acc<KL>(va, vterms);
   if (sub != 0) return;
-  st2(d.apred, i, a);
-  d.p[i] = d.p2[i];
-  const float2 vp = ld2(d.vpred, i);
-  st2(d.vpred, i, make_float2(vp.x + P.dt * a.x, vp.y + P.dt * a.y));
-  float dp, vx, vy;
-  dsph_rates(P, o, va, vterms, dp, vx, vy);
-  const float2 old = ld2(acc, i);
-  const float2 w = make_float2(old.x + a.x, old.y + a.y);
-  st2(acc, i, make_float2(w.x + vx, w.y + vy));
+  finish_visc_store(P, o, d, va, vterms, i, a, acc);
+}
+
+// ------------------------------------------------ stencil-list gathers
+// The DFSPH gathers over stencil lists (SphStencilList) of a state held in
+// its grid's sorted order (point index = sorted slot).  A workgroup takes
+// the fluid rows of a pass (kBS consecutive rows, or the pass's rows of the
+// active list), stages the records of their three stencil runs in LDS with
+// coalesced loads and one wait, and walks each row's mask bits from there,
+// one lane per row: the pairs of the CSR row in its order, so every sum is
+// the CSR gather's with one lane per row bit for bit.  A pass whose runs do
+// not fit in LDS (rows of two scenes, crowded cells) reads the records from
+// global memory.  The boundary pairs keep their CSR lists.
+template <int NQ>
+struct SlRecs {
+  float4 a[NQ];
+};
+template <int NQ>
+struct SlCap {
+  static constexpr int v = NQ == 1 ? 3 * kBS * 3 / 2 + 256 : 1024;   // staged points
+};
+template <int NQ>
+struct SlShared {
+  float4 rec[NQ][SlCap<NQ>::v];
+  int wlo[kBS / kWave][3], whi[kBS / kWave][3];
+};
+constexpr int kSlBatch = 4;   // pairs a lane loads before it adds them
+
+// Row r of a stencil list: its runs (as a Stencil3's ra / rb) and masks.
+struct SlRow {
+  Stencil3 st;
+  unsigned long long m[3];
+};
+__device__ __forceinline__ SlRow sl_row(const SphStencilList& sl, int r) {
+  SlRow w;
+  const int4 R = reinterpret_cast<const int4*>(sl.runs)[r];
+  w.st.ra[0] = R.x;
+  w.st.ra[1] = R.y;
+  w.st.ra[2] = R.z;
+  w.st.rb[0] = R.x + (R.w & 255);
+  w.st.rb[1] = R.y + ((R.w >> 8) & 255);
+  w.st.rb[2] = R.z + ((R.w >> 16) & 255);
+  w.st.n0 = w.st.n01 = w.st.ncand = 0;
+  w.m[0] = sl.masks[3 * (size_t)r];
+  w.m[1] = sl.masks[3 * (size_t)r + 1];
+  w.m[2] = sl.masks[3 * (size_t)r + 2];
+  return w;
+}
+__device__ __forceinline__ SlRow sl_none() {
+  SlRow w = {};
+  return w;
+}
+
+// The chunk's runs and whether they are staged (workgroup-uniform).  Every
+// thread of the workgroup calls this (two barriers); load(j) gives the
+// records of point j.
+template <int NQ, typename Load>
+__device__ __forceinline__ bool sl_stage(SlShared<NQ>& S, const Stencil3& st, bool valid,
+                                         Load load, ChunkRuns& cr) {
+  cr = chunk_runs(st, valid, S.wlo, S.whi);
+  const int tot = cr.tot;
+  const bool staged = tot <= SlCap<NQ>::v;
+  if (staged) {
+    // slots per round: all loads of a round before its LDS writes
+    constexpr int R = NQ == 1 ? 3 : 2;
+    const int t = threadIdx.x;
+    const int w0 = __builtin_amdgcn_readfirstlane(t);
+    for (int u0 = 0; w0 + u0 * kBS < tot; u0 += R) {
+      SlRecs<NQ> v[R];
+#pragma unroll
+      for (int u = 0; u < R; ++u) {
+        const int k = t + (u0 + u) * kBS;
+        v[u] = load(cr.slot(k < tot ? k : tot - 1));
+      }
+#pragma unroll
+      for (int u = 0; u < R; ++u) {
+        const int k = t + (u0 + u) * kBS;
+        if (k < tot) {
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) S.rec[q][k] = v[u].a[q];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  return staged;
+}
+
+// f(records) for each pair of row w in order: from LDS (STAGED) or global
+// memory, kSlBatch loads in flight.
+template <bool STAGED, int NQ, typename Load, typename F>
+__device__ __forceinline__ void sl_walk_t(const SlShared<NQ>& S, const ChunkRuns& cr,
+                                          const SlRow& w, Load load, F f) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    unsigned long long mm = w.m[r];
+    const int b = STAGED ? cr.first(r, w.st) : w.st.ra[r];
+    while (mm) {
+      int k[kSlBatch];
+      bool ok[kSlBatch];
+#pragma unroll
+      for (int u = 0; u < kSlBatch; ++u) {
+        ok[u] = mm != 0ull;
+        k[u] = ok[u] ? (int)__builtin_ctzll(mm) : 0;
+        mm &= mm - 1ull;
+      }
+      SlRecs<NQ> v[kSlBatch];
+#pragma unroll
+      for (int u = 0; u < kSlBatch; ++u) {
+        if (STAGED) {
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) v[u].a[q] = S.rec[q][b + k[u]];
+        } else {
+          v[u] = load(b + k[u]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kSlBatch; ++u)
+        if (ok[u]) f(v[u]);
+    }
+  }
+}
+template <int NQ, typename Load, typename F>
+__device__ __forceinline__ void sl_walk(const SlShared<NQ>& S, const ChunkRuns& cr, bool staged,
+                                        const SlRow& w, Load load, F f) {
+  if (staged)
+    sl_walk_t<true, NQ>(S, cr, w, load, f);
+  else
+    sl_walk_t<false, NQ>(S, cr, w, load, f);
+}
+
+// k_fluid_density (gather.hip) over a stencil list, one lane per row.
+__global__ __launch_bounds__(kBS) void k_fluid_density_sl(SphFluid f,
+                                                          const SphPhysics* __restrict__ phys,
+                                                          SphStencilList sl, SphBoundary b,
+                                                          SphCSR fb, float* __restrict__ rho_out,
+                                                          float* __restrict__ vol_out) {
+  __shared__ SlShared<1> S;
+  const int i = xcd_block() * kBS + threadIdx.x;
+  const bool valid = i < f.np;
+  const SlRow w = valid ? sl_row(sl, i) : sl_none();
+  auto load = [&](int j) {
+    SlRecs<1> r;
+    const float2 x = ld2(f.x, j);
+    r.a[0] = make_float4(x.x, x.y, f.area[j], 0.f);
+    return r;
+  };
+  ChunkRuns cr;
+  const bool staged = sl_stage<1>(S, w.st, valid, load, cr);
+  if (!valid) return;
+  const KConst K = kconst(phys[f.pscene[i]].h);
+  const float2 xi = ld2(f.x, i);
+  float sum = 0.f;
+  sl_walk<1>(S, cr, staged, w, load, [&](const SlRecs<1>& r) {
+    const PairGeo g = pair_geo(xi.x, xi.y, r.a[0].x, r.a[0].y, K.inv_h);
+    sum += r.a[0].z * wendland_k(g.q, K);
+  });
+  if (vol_out) vol_out[i] = f.area[i] / sum;
+  if (fb.rowptr) {
+    float bs = 0.f;
+    const int e1 = fb.rowptr[i + 1];
+    for (int k = fb.rowptr[i]; k < e1; ++k) {
+      const int j = fb.col[k];
+      const float2 xb = ld2(b.x, j);
+      const PairGeo g = pair_geo(xb.x, xb.y, xi.x, xi.y, K.inv_h);
+      bs += wendland_k(g.q, K) * b.vol[j];
+    }
+    sum += bs;
+  }
+  rho_out[i] = sum;
+}
+
+// k_alpha_source's fluid rows over a stencil list; boundary rows (threads
+// past the fluid) as in k_alpha_source with one lane per row.
+__global__ __launch_bounds__(kBS) void k_alpha_source_sl(SphFluid f,
+                                                         const SphPhysics* __restrict__ phys,
+                                                         SphStencilList sl, SphBoundary b,
+                                                         SphCSR fb, SphCSR bf, SphDfsph d) {
+  __shared__ SlShared<2> S;
+  const int t = xcd_block() * kBS + threadIdx.x;
+  const bool fluid = t < f.np;
+  const SlRow w = fluid ? sl_row(sl, t) : sl_none();
+  auto load = [&](int j) {
+    SlRecs<2> r;
+    r.a[0] = ld4(d.xrec, j);
+    r.a[1] = ld4(d.prec, j);
+    return r;
+  };
+  ChunkRuns cr;
+  const bool staged = sl_stage<2>(S, w.st, fluid, load, cr);
+  if (fluid) {
+    const int i = t;
+    const SphPhysics P = phys[f.pscene[i]];
+    const KConst K = kconst(P.h);
+    const float2 xi = ld2(f.x, i), vi = ld2(d.vpred, i);
+    float k1x = 0.f, k1y = 0.f, k2 = 0.f, src = 0.f;
+    sl_walk<2>(S, cr, staged, w, load, [&](const SlRecs<2>& rr) {
+      const Rec2 r = {rr.a[0], rr.a[1]};
+      const float2 G = spiky_G(xi.x, xi.y, r.a.x, r.a.y, K);
+      const float aa = r.b.z;
+      k1x += aa * G.x;
+      k1y += aa * G.y;
+      k2 += aa * aa / r.a.w * (G.x * G.x + G.y * G.y);
+      src += -P.dt * aa * ((vi.x - r.b.x) * G.x + (vi.y - r.b.y) * G.y);
+    });
+    if (fb.rowptr) {
+      pair_pipeline_with<1>(
+          fb.rowptr[i], fb.rowptr[i + 1], 0, [&](int k) { return fb.col[k]; },
+          [&](int j) { return Rec2{ld4(d.bxrec, j), make_float4(d.b_aact[j], 0.f, 0.f, 0.f)}; },
+          [&](const Rec2& r) {
+            const float2 G = spiky_G(xi.x, xi.y, r.a.x, r.a.y, K);
+            const float aa = r.b.x;
+            k1x += aa * G.x;
+            k1y += aa * G.y;
+            k2 += aa * aa / r.a.w * (G.x * G.x + G.y * G.y);
+            src += -P.dt * aa * (vi.x * G.x + vi.y * G.y);
+          });
+    }
+    alpha_store(f, P, d, i, k1x, k1y, k2, src);
+  } else if (t < f.np + b.nb) {
+    alpha_boundary_row<1>(phys, b, bf, d, t - f.np, 0);
+  }
+}
+
+// pressure_accel_t's sum over a stencil list (fluid pairs from the walk,
+// boundary pairs from fb), one lane per row.
+template <typename Walk>
+__device__ __forceinline__ float2 pressure_accel_sl(const SphCSR& fb, const SphDfsph& d,
+                                                    const KConst& K, int i, Walk walk) {
+  const float4 ri = ld4(d.xrec, i);
+  float ax = 0.f, ay = 0.f;
+  auto add = [&](const float4& rj) {
+    const float2 G = spiky_G(ri.x, ri.y, rj.x, rj.y, K);
+    const float c = -rj.w * (ri.z + rj.z);
+    ax += c * G.x;
+    ay += c * G.y;
+  };
+  walk([&](const SlRecs<1>& r) { add(r.a[0]); });
+  if (fb.rowptr)
+    pair_pipeline<1>(fb.rowptr[i], fb.rowptr[i + 1], 0, d.bxrec, [&](int k) { return fb.col[k]; },
+                     add);
+  return make_float2(ax, ay);
+}
+
+// k_accel over a stencil list.
+__global__ __launch_bounds__(kBS) void k_accel_sl(SphFluid f, const SphPhysics* __restrict__ phys,
+                                                  SphStencilList sl, SphCSR fb, SphDfsph d) {
+  __shared__ SlShared<1> S;
+  const int nact = d.act_n[2];
+  const int nitems = d.act_n[0];
+  auto load = [&](int j) {
+    SlRecs<1> r;
+    r.a[0] = ld4(d.xrec, j);
+    return r;
+  };
+  for (int base = xcd_block() * kBS; base < nitems; base += gridDim.x * kBS) {
+    const int t = base + threadIdx.x;
+    const int r0 = find_rank(d.act_pre, nact, base);
+    const bool uni = base + kBS <= d.act_pre[r0 + 1];
+    const bool valid = t < nitems;
+    int s = 0, i = 0;
+    if (uni) {
+      s = d.act_scene[r0];
+      i = d.fbegin[s] + (t - d.act_pre[r0]);
+    } else if (valid) {
+      const int r = advance_rank(d.act_pre, nact, r0, t);
+      s = d.act_scene[r];
+      i = d.fbegin[s] + (t - d.act_pre[r]);
+    }
+    const SlRow w = valid ? sl_row(sl, i) : sl_none();
+    ChunkRuns cr;
+    const bool staged = sl_stage<1>(S, w.st, valid, load, cr);
+    if (!valid) continue;
+    const SphPhysics P = phys[uni ? d.act_scene[r0] : s];
+    const KConst K = kconst(P.h);
+    const float2 a = pressure_accel_sl(fb, d, K, i, [&](auto f2) {
+      sl_walk<1>(S, cr, staged, w, load, f2);
+    });
+    store_accel(d, i, a, P.dt * P.dt);
+  }
+}
+
+// k_pressure over a stencil list: the fluid rows' sums from the walk, the
+// boundary rows as in k_pressure with one lane per row.
+__global__ __launch_bounds__(kBS) void k_pressure_sl(SphFluid f, const SphPhysics* __restrict__ phys,
+                                                     SphStencilList sl, SphBoundary b, SphCSR fb,
+                                                     SphCSR bf, SphDfsph d) {
+  __shared__ SlShared<1> S;
+  __shared__ double s_wsum[kBS / kWave];
+  __shared__ int s_wscene[kBS / kWave];
+  const int nact = d.act_n[2];
+  const int nf = d.act_n[0];
+  const int nitems = nf + d.act_n[1];
+  const int stride = gridDim.x * kBS;
+  const int* preb = d.act_pre + d.nscenes + 1;
+  double* acc_slot =
+      d.err_acc + (size_t)(blockIdx.x % SPH_DFSPH_ERR_SLOTS) * SPH_DFSPH_ERR_STRIDE(d.nscenes);
+  auto load = [&](int j) {
+    SlRecs<1> r;
+    r.a[0] = ld4(d.prec, j);
+    return r;
+  };
+  for (int base = xcd_block() * kBS; base < nitems; base += stride) {
+    const int t = base + threadIdx.x;
+    double contrib = 0.0;
+    int s = -1;
+    bool valid = false;
+    const int r0f = base < nf ? find_rank(d.act_pre, nact, base) : 0;
+    const int r0b = base + kBS > nf ? find_rank(preb, nact, base > nf ? base - nf : 0) : 0;
+    const bool uni = base + kBS <= nf && base + kBS <= d.act_pre[r0f + 1];
+    int i = 0;
+    if (uni) {
+      s = d.act_scene[r0f];
+      i = d.fbegin[s] + (t - d.act_pre[r0f]);
+    } else if (t < nf) {
+      const int r = advance_rank(d.act_pre, nact, r0f, t);
+      s = d.act_scene[r];
+      i = d.fbegin[s] + (t - d.act_pre[r]);
+    }
+    const SlRow w = t < nf ? sl_row(sl, i) : sl_none();
+    ChunkRuns cr;
+    const bool staged = sl_stage<1>(S, w.st, t < nf, load, cr);
+    if (t < nf) {
+      const SphPhysics P = phys[uni ? d.act_scene[r0f] : s];
+      const KConst K = kconst(P.h);
+      const float2 xi = ld2(d.xrec, 2 * i);
+      float ks = 0.f;
+      sl_walk<1>(S, cr, staged, w, load, [&](const SlRecs<1>& r) {
+        const float4 pj = r.a[0];
+        const float2 G = spiky_G(xi.x, xi.y, pj.x, pj.y, K);
+        ks += pj.z * G.x + pj.w * G.y;
+      });
+      contrib = pressure_row_update(f, d, P.fluid_omega, P.tau, s, i, ks);
+      valid = true;
+    } else if (t < nitems) {
+      contrib = pressure_boundary_row<1>(b, bf, d, phys, preb, nact, r0b, t - nf, 0, &s, &valid);
+    }
+    scene_sum(contrib, s, valid, acc_slot, s_wsum, s_wscene);
+  }
+}
+
+// k_finish over a stencil list.
+__global__ __launch_bounds__(kBS) void k_finish_sl(SphFluid f, const SphPhysics* __restrict__ phys,
+                                                   SphStencilList sl, SphCSR fb, SphDfsph d,
+                                                   float* __restrict__ acc) {
+  __shared__ SlShared<1> S;
+  const int i = xcd_block() * kBS + threadIdx.x;
+  const bool valid = i < f.np;
+  const SlRow w = valid ? sl_row(sl, i) : sl_none();
+  auto load = [&](int j) {
+    SlRecs<1> r;
+    r.a[0] = ld4(d.xrec, j);
+    return r;
+  };
+  ChunkRuns cr;
+  const bool staged = sl_stage<1>(S, w.st, valid, load, cr);
+  if (!valid) return;
+  const SphPhysics P = phys[f.pscene[i]];
+  const KConst K = kconst(P.h);
+  const float2 a = pressure_accel_sl(fb, d, K, i, [&](auto f2) {
+    sl_walk<1>(S, cr, staged, w, load, f2);
+  });
+  finish_store(P, d, i, a, acc);
+}
+
+// k_finish_visc over a stencil list: per neighbour its pressure record and
+// {vx, vy, rho, V} of its packed record (the position is the pressure
+// record's, the same values; the pressure is not read: SPH_T_VISC and
+// SPH_T_LAM only).
+__global__ __launch_bounds__(kBS) void k_finish_visc_sl(SphFluid f,
+                                                        const SphPhysics* __restrict__ phys,
+                                                        SphStencilList sl, SphCSR fb, SphDfsph d,
+                                                        const float4* __restrict__ frec,
+                                                        unsigned vterms, float* __restrict__ acc) {
+  __shared__ SlShared<2> S;
+  const int i = xcd_block() * kBS + threadIdx.x;
+  const bool valid = i < f.np;
+  const SlRow w = valid ? sl_row(sl, i) : sl_none();
+  auto load = [&](int j) {
+    SlRecs<2> r;
+    r.a[0] = ld4(d.xrec, j);
+    const float4 p0 = frec[2 * j], p1 = frec[2 * j + 1];
+    r.a[1] = make_float4(p0.z, p0.w, p1.x, p1.y);
+    return r;
+  };
+  ChunkRuns cr;
+  const bool staged = sl_stage<2>(S, w.st, valid, load, cr);
+  if (!valid) return;
+  const SphPhysics P = phys[f.pscene[i]];
+  const KConst K = kconst(P.h);
+  const float4 ri = ld4(d.xrec, i);
+  const float4 a0 = frec[2 * i], a1 = frec[2 * i + 1];
+  Own o;
+  o.x = a0.x; o.y = a0.y; o.vx = a0.z; o.vy = a0.w;
+  o.rho = a1.x;
+  o.p = a1.z;
+  o.gx = 0.f;
+  o.gy = 0.f;
+  float ax = 0.f, ay = 0.f;
+  Acc va = {};
+  sl_walk<2>(S, cr, staged, w, load, [&](const SlRecs<2>& r) {
+    const float2 G = spiky_G(ri.x, ri.y, r.a[0].x, r.a[0].y, K);
+    const float c = -r.a[0].w * (ri.z + r.a[0].z);
+    ax += c * G.x;
+    ay += c * G.y;
+    const Nbr nb = {r.a[0].x, r.a[0].y, r.a[1].x, r.a[1].y, r.a[1].z, r.a[1].w, 0.f};
+    dsph_pair(P, K, o, nb, 0.f, 0.f, true, vterms, va);
+  });
+  if (fb.rowptr)
+    pair_pipeline<1>(fb.rowptr[i], fb.rowptr[i + 1], 0, d.bxrec, [&](int k) { return fb.col[k]; },
+                     [&](const float4& rj) {
+                       const float2 G = spiky_G(ri.x, ri.y, rj.x, rj.y, K);
+                       const float c = -rj.w * (ri.z + rj.z);
+                       ax += c * G.x;
+                       ay += c * G.y;
+                     });
+  finish_visc_store(P, o, d, va, vterms, i, make_float2(ax, ay), acc);
 }
 
 }  // namespace
@@ -790,6 +1261,104 @@ int dfsph_finish_visc(const SphFluid* f, const SphPhysics* phys, const SphCSR* f
   return launch_check("sph_dfsph_finish_visc");
 }
 
+// ---------------------------------------------- stencil-list entry points
+static bool sl_ok(const SphStencilList* sl, int np) {
+  return sl && sl->n == np && (np == 0 || (sl->runs && sl->masks));
+}
+
+int fluid_density_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                     const SphBoundary* b, const SphCSR* fb, float* rho_out, float* vol_out,
+                     hipStream_t s) {
+  SPH_REQUIRE(f && f->x && f->area && f->pscene && f->np >= 0 && phys && rho_out &&
+                  sl_ok(sl, f->np),
+              "sph_fluid_density_sl: bad arguments");
+  SPH_REQUIRE(!fb || !fb->rowptr || (b && b->x && b->vol),
+              "sph_fluid_density_sl: boundary list without boundary");
+  if (f->np == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_fluid_density_sl, dim3(blocks_for(f->np, kBS)), dim3(kBS), 0, s, *f, phys,
+                     *sl, bdy_or_null(b), csr_or_null(fb), rho_out, vol_out);
+  return launch_check("sph_fluid_density_sl");
+}
+
+int dfsph_alpha_source_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                          const SphBoundary* b, const SphCSR* fb, const SphCSR* bf, SphDfsph* d,
+                          hipStream_t s) {
+  SPH_REQUIRE(f && f->x && f->area && f->rest && f->rho && f->pscene && phys && dfsph_ok(d) &&
+                  sl_ok(sl, f->np),
+              "sph_dfsph_alpha_source_sl: bad arguments");
+  const SphBoundary bb = bdy_or_null(b);
+  SPH_REQUIRE(bdy_ok(bb, d) && (bb.nb == 0 || (bf && bf->rowptr)),
+              "sph_dfsph_alpha_source_sl: boundary needs its pair list");
+  SPH_REQUIRE(!d->divergence || (bb.nb == 0 && (!fb || !fb->rowptr)),
+              "sph_dfsph_alpha_source_sl: the divergence solve takes no boundary");
+  const int n = f->np + bb.nb;
+  if (n == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_alpha_source_sl, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, *f, phys, *sl,
+                     bb, csr_or_null(fb), csr_or_null(bf), *d);
+  return launch_check("sph_dfsph_alpha_source_sl");
+}
+
+int dfsph_accel_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                   const SphCSR* fb, SphDfsph* d, hipStream_t s) {
+  SPH_REQUIRE(f && phys && dfsph_ok(d) && sl_ok(sl, f->np), "sph_dfsph_accel_sl: bad arguments");
+  SPH_REQUIRE(!fb || !fb->rowptr || d->bxrec, "sph_dfsph_accel_sl: boundary list without records");
+  SPH_REQUIRE(!d->divergence || !fb || !fb->rowptr,
+              "sph_dfsph_accel_sl: the divergence solve takes no boundary");
+  hipLaunchKernelGGL(k_accel_sl, dim3(d->grid_blocks), dim3(kBS), 0, s, *f, phys, *sl,
+                     csr_or_null(fb), *d);
+  return launch_check("sph_dfsph_accel_sl");
+}
+
+int dfsph_pressure_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                      const SphBoundary* b, const SphCSR* fb, const SphCSR* bf, SphDfsph* d,
+                      hipStream_t s) {
+  SPH_REQUIRE(f && f->rho && f->rest && phys && dfsph_ok(d) && sl_ok(sl, f->np),
+              "sph_dfsph_pressure_sl: bad arguments");
+  const SphBoundary bb = bdy_or_null(b);
+  SPH_REQUIRE(bdy_ok(bb, d) && (bb.nb == 0 || (bf && bf->rowptr)),
+              "sph_dfsph_pressure_sl: bad boundary");
+  SPH_REQUIRE(!d->divergence || (bb.nb == 0 && (!fb || !fb->rowptr)),
+              "sph_dfsph_pressure_sl: the divergence solve takes no boundary");
+  hipLaunchKernelGGL(k_pressure_sl, dim3(d->grid_blocks), dim3(kBS), 0, s, *f, phys, *sl, bb,
+                     csr_or_null(fb), csr_or_null(bf), *d);
+  return launch_check("sph_dfsph_pressure_sl");
+}
+
+int dfsph_iterate_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                     const SphBoundary* b, const SphCSR* fb, const SphCSR* bf, SphDfsph* d,
+                     hipStream_t s) {
+  int rc;
+  if ((rc = dfsph_accel_sl(f, phys, sl, fb, d, s))) return rc;
+  if ((rc = dfsph_pressure_sl(f, phys, sl, b, fb, bf, d, s))) return rc;
+  return dfsph_control(phys, d, s);
+}
+
+int dfsph_finish_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                    const SphCSR* fb, SphDfsph* d, float* acc, hipStream_t s) {
+  SPH_REQUIRE(f && f->pscene && phys && dfsph_ok(d) && sl_ok(sl, f->np),
+              "sph_dfsph_finish_sl: bad arguments");
+  SPH_REQUIRE(!d->divergence || !fb || !fb->rowptr,
+              "sph_dfsph_finish_sl: the divergence solve takes no boundary");
+  if (f->np == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_finish_sl, dim3(blocks_for(f->np, kBS)), dim3(kBS), 0, s, *f, phys, *sl,
+                     csr_or_null(fb), *d, acc);
+  return launch_check("sph_dfsph_finish_sl");
+}
+
+int dfsph_finish_visc_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                         const SphCSR* fb, SphDfsph* d, const float* frec, unsigned vterms,
+                         float* acc, hipStream_t s) {
+  SPH_REQUIRE(f && f->pscene && phys && dfsph_ok(d) && frec && acc && sl_ok(sl, f->np),
+              "sph_dfsph_finish_visc_sl: bad arguments");
+  SPH_REQUIRE(!d->divergence, "sph_dfsph_finish_visc_sl: not for the divergence solve");
+  SPH_REQUIRE((vterms & ~(unsigned)(SPH_T_VISC | SPH_T_LAM)) == 0,
+              "sph_dfsph_finish_visc_sl: only SPH_T_VISC and SPH_T_LAM");
+  if (f->np == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_finish_visc_sl, dim3(blocks_for(f->np, kBS)), dim3(kBS), 0, s, *f, phys,
+                     *sl, csr_or_null(fb), *d, reinterpret_cast<const float4*>(frec), vterms, acc);
+  return launch_check("sph_dfsph_finish_visc_sl");
+}
+
 }  // namespace sph
 
 extern "C" {
@@ -827,5 +1396,38 @@ int sph_dfsph_finish_visc(const SphFluid* f, const SphPhysics* phys, const SphCS
                           const SphCSR* fb, SphDfsph* d, const float* frec, unsigned vterms,
                           float* acc, hipStream_t s) {
   return sph::dfsph_finish_visc(f, phys, ff, fb, d, frec, vterms, acc, s);
+}
+int sph_fluid_density_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                         const SphBoundary* b, const SphCSR* fb, float* rho_out, float* vol_out,
+                         hipStream_t s) {
+  return sph::fluid_density_sl(f, phys, sl, b, fb, rho_out, vol_out, s);
+}
+int sph_dfsph_alpha_source_sl(const SphFluid* f, const SphPhysics* phys,
+                              const SphStencilList* sl, const SphBoundary* b, const SphCSR* fb,
+                              const SphCSR* bf, SphDfsph* d, hipStream_t s) {
+  return sph::dfsph_alpha_source_sl(f, phys, sl, b, fb, bf, d, s);
+}
+int sph_dfsph_accel_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                       const SphCSR* fb, SphDfsph* d, hipStream_t s) {
+  return sph::dfsph_accel_sl(f, phys, sl, fb, d, s);
+}
+int sph_dfsph_pressure_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                          const SphBoundary* b, const SphCSR* fb, const SphCSR* bf, SphDfsph* d,
+                          hipStream_t s) {
+  return sph::dfsph_pressure_sl(f, phys, sl, b, fb, bf, d, s);
+}
+int sph_dfsph_iterate_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                         const SphBoundary* b, const SphCSR* fb, const SphCSR* bf, SphDfsph* d,
+                         hipStream_t s) {
+  return sph::dfsph_iterate_sl(f, phys, sl, b, fb, bf, d, s);
+}
+int sph_dfsph_finish_sl(const SphFluid* f, const SphPhysics* phys, const SphStencilList* sl,
+                        const SphCSR* fb, SphDfsph* d, float* acc, hipStream_t s) {
+  return sph::dfsph_finish_sl(f, phys, sl, fb, d, acc, s);
+}
+int sph_dfsph_finish_visc_sl(const SphFluid* f, const SphPhysics* phys,
+                             const SphStencilList* sl, const SphCSR* fb, SphDfsph* d,
+                             const float* frec, unsigned vterms, float* acc, hipStream_t s) {
+  return sph::dfsph_finish_visc_sl(f, phys, sl, fb, d, frec, vterms, acc, s);
 }
 }

@@ -334,6 +334,93 @@ __global__ __launch_bounds__(kBS) void k_walk_count(SphGrid g, SphQuery q,
   counts[qi] = cnt;
 }
 
+// Stencil lists (sph_stencil_list_build): k_walk_count's chunks, stage and
+// test, with each hit set as a bit of its run's mask instead of counted.
+// Rows are sorted slots, so a row's candidates are its stencil runs as
+// they are and its bits follow the fill's candidate order.
+constexpr int kSlMaxRun = 64;   // candidates a run's mask holds
+
+__global__ __launch_bounds__(kBS) void k_stencil_list(SphGrid g, const float* __restrict__ h,
+                                                      int4* __restrict__ runs,
+                                                      unsigned long long* __restrict__ masks) {
+  __shared__ float2 s_pos[kSWalkRecs + kSWalkBatch];   // padded: a trip reads past a run's end
+  __shared__ int s_wlo[kBS / kWave][3], s_whi[kBS / kWave][3];
+  const int t = threadIdx.x;
+  const int sp = xcd_block() * kBS + t;
+  const bool valid = sp < g.np;
+  const float2* __restrict__ sx = reinterpret_cast<const float2*>(g.sorted_x);
+  int sc = 0;
+  float xq = 0.f, yq = 0.f, thr = 0.f;
+  Stencil3 st;
+  if (valid) {
+    const int qi = g.sorted_idx[sp];
+    sc = g.nscenes == 1 ? 0 : g.pscene[qi];
+    const SphGridDesc d = g.scenes[sc];
+    const float2 xy = sx[sp];
+    xq = xy.x;
+    yq = xy.y;
+    st.init(d, g.cell_start + d.cell_begin, cell_coord(xq, d.qmin_x, d.hmax),
+            cell_coord(yq, d.qmin_y, d.hmax));
+    thr = sq_threshold(h[qi]);
+  }
+  const ChunkRuns cr = chunk_runs(st, valid, s_wlo, s_whi);
+  const int tot = cr.tot;
+  const bool staged = tot <= kSWalkRecs;
+  if (staged) {
+    const int w0 = __builtin_amdgcn_readfirstlane(t);
+    for (int u0 = 0; w0 + u0 * kBS < tot; u0 += kSWalkRound) {
+      float2 pv[kSWalkRound];
+#pragma unroll
+      for (int u = 0; u < kSWalkRound; ++u) {
+        const int k = t + (u0 + u) * kBS;
+        pv[u] = sx[cr.slot(k < tot ? k : tot - 1)];
+      }
+#pragma unroll
+      for (int u = 0; u < kSWalkRound; ++u) {
+        const int k = t + (u0 + u) * kBS;
+        if (k < tot) s_pos[k] = pv[u];
+      }
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  unsigned long long m[3] = {0ull, 0ull, 0ull};
+  int len[3];
+  bool over = false;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    len[r] = st.rb[r] - st.ra[r];
+    over |= len[r] > kSlMaxRun;
+    const int n = len[r] < kSlMaxRun ? len[r] : kSlMaxRun;
+    const int b = cr.first(r, st);
+    for (int k = 0; k < n; k += kSWalkBatch) {
+      float2 xy[kSWalkBatch];
+#pragma unroll
+      for (int u = 0; u < kSWalkBatch; ++u) {
+        const int c = k + u < n ? k + u : n - 1;
+        xy[u] = staged ? s_pos[b + c] : sx[st.ra[r] + c];
+      }
+#pragma unroll
+      for (int u = 0; u < kSWalkBatch; ++u) {
+        const bool hit = (k + u < n) && dist2_nofma(xq, yq, xy[u].x, xy[u].y) <= thr;
+        m[r] |= hit ? (1ull << (k + u)) : 0ull;
+      }
+    }
+  }
+  if (over) atomicOr(&g.status[sc], SPH_ST_OVERFLOW);
+  const int l0 = len[0] < 255 ? len[0] : 255, l1 = len[1] < 255 ? len[1] : 255,
+            l2 = len[2] < 255 ? len[2] : 255;
+  runs[sp] = make_int4(st.ra[0], st.ra[1], st.ra[2], l0 | (l1 << 8) | (l2 << 16));
+  masks[3 * (size_t)sp] = m[0];
+  masks[3 * (size_t)sp + 1] = m[1];
+  masks[3 * (size_t)sp + 2] = m[2];
+}
+
+__global__ __launch_bounds__(kBS) void k_iota(int* __restrict__ a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
+}
+
 __global__ __launch_bounds__(kBS) void k_clamp_rowptr(int* __restrict__ rowptr, int n, int64_t cap) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -755,6 +842,24 @@ int neighbor_fill(const SphGrid* g, const SphQuery* q, int* rowptr, int* col, in
   return launch_check("sph_neighbor_fill");
 }
 
+int stencil_list_build(const SphGrid* g, const float* h, int* runs, unsigned long long* masks,
+                       hipStream_t s) {
+  SPH_REQUIRE(g && g->scenes && g->cell_start && g->sorted_idx && g->sorted_x && g->status &&
+                  (g->nscenes == 1 || g->pscene) && g->np >= 0 && (g->np == 0 || (h && runs && masks)),
+              "sph_stencil_list_build: bad arguments (the grid must be built)");
+  if (g->np == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_stencil_list, dim3(blocks_for(g->np, kBS)), dim3(kBS), 0, s, *g, h,
+                     reinterpret_cast<int4*>(runs), masks);
+  return launch_check("sph_stencil_list_build");
+}
+
+int iota(int* a, int n, hipStream_t s) {
+  SPH_REQUIRE(n >= 0 && (n == 0 || a), "sph_iota: bad arguments");
+  if (n == 0) return SPH_OK;
+  hipLaunchKernelGGL(k_iota, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, a, n);
+  return launch_check("sph_iota");
+}
+
 int csr_sort_rows(const SphCSR* a, int* col, hipStream_t s) {
   SPH_REQUIRE(a && a->rowptr && a->nrows >= 0, "sph_csr_sort_rows: bad csr");
   if (a->nrows == 0) return SPH_OK;
@@ -826,6 +931,11 @@ int sph_neighbor_fill(const SphGrid* g, const SphQuery* q, int* rowptr, int* col
 int sph_csr_sort_rows(const SphCSR* a, int* col_mut, hipStream_t s) {
   return sph::csr_sort_rows(a, col_mut, s);
 }
+int sph_stencil_list_build(const SphGrid* g, const float* h, int* runs,
+                           unsigned long long* masks, hipStream_t s) {
+  return sph::stencil_list_build(g, h, runs, masks, s);
+}
+int sph_iota(int* a, int n, hipStream_t s) { return sph::iota(a, n, s); }
 int sph_csr_transpose(const SphCSR* a, int ncols_t, int* t_counts, int* t_slot, int* t_rowptr,
                       int* t_col, void* ws, size_t ws_bytes, hipStream_t s) {
   return sph::csr_transpose(a, ncols_t, t_counts, t_slot, t_rowptr, t_col, ws, ws_bytes, s);

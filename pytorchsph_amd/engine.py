@@ -95,7 +95,7 @@ class SceneBatch:
     def __init__(self, scenes: list[Scene], device="cuda", pair_cap: float = 40.0,
                  bdy_pair_cap: float = 40.0, dfsph_chunk: int = 8, reorder: bool = True,
                  carry: list | None = None, fuse_rk4: bool | None = None,
-                 cellwalk: bool = True, cell_div: int = 1):
+                 cellwalk: bool = True, cell_div: int = 1, stencil_lists: bool = True):
         """fuse_rk4: delta-SPH stages run the RK4 combine and the next
         stage's records inside the gather (sph_dsph_gather_rk4 or
         sph_dsph_cellwalk_rk4) instead of as separate kernels; the results
@@ -115,6 +115,14 @@ class SceneBatch:
         of radius cell_div (1 or 2).  The pair sets are the same; the order
         inside a row, and so the rounding of the sums, follows the stencil.
         The cell walk needs cell_div 1.
+
+        stencil_lists: DFSPH steps permute the state into the order of the
+        step's own grid and keep each particle's fluid neighbours as
+        bitmasks over its stencil (sph_stencil_list_build) instead of a CSR
+        list; the density, alpha / source, solve and finish gathers stage
+        their rows' stencil records in LDS (the sph_*_sl entry points, one
+        lane per row).  The results are those of the CSR gathers with one
+        lane per row, bit for bit.  Needs reorder and cell_div 1.
 
         carry[k], when given, continues scene k from another batch: a dict
         from `export` with device tensors of its positions, velocities and
@@ -208,6 +216,7 @@ class SceneBatch:
         self.cellwalk = (cellwalk and cell_div == 1 and scheme == "dsph"
                          and not (self.dplus or self.shifting) and not self.terms & L.T_DDSPH)
         self.fuse_rk4 = self.cellwalk if fuse_rk4 is None else bool(fuse_rk4)
+        self.stencil_lists = (stencil_lists and scheme == "dfsph" and reorder and cell_div == 1)
         if self.cellwalk:
             # the cell walk's pair predicate takes the support from the
             # physics records; it must be the support the grid was sized for
@@ -243,6 +252,9 @@ class SceneBatch:
         self.gf = _PairBuf(NB, bcap, dev) if self.solid else None
         self.fb = _PairBuf(NP, bcap, dev)
         self.t_slot = torch.zeros(bcap, dtype=torch.int32, device=dev)
+        if self.stencil_lists:
+            self.sl_runs = torch.zeros((max(NP, 1), 4), dtype=torch.int32, device=dev)
+            self.sl_masks = torch.zeros((max(NP, 1), 3), dtype=torch.int64, device=dev)
         segs = walk_segments(self.p_begin, self.n)
         self.nseg = len(segs)
         self.segs = torch.from_numpy(segs).to(dev) if self.cellwalk and self.nseg else None
@@ -496,16 +508,18 @@ class SceneBatch:
         g.fill(q, buf.rowptr, buf.col, s)
 
     def search(self, x, s=None, fluid_pairs: bool = True, ghost_pairs: bool = True,
-               bdy_pairs: bool = True):
+               bdy_pairs: bool = True, build: bool = True):
         """Grid + every pair list of the current positions (without the
         fluid list when not fluid_pairs, the ghost list when not
         ghost_pairs, the boundary lists when not bdy_pairs).  Delta-SPH
         batches build the fluid -> boundary list by a search of the fluid
         points in the static boundary grid, so its rows follow that grid's
         order (the order the cell walk walks them in); DFSPH batches
-        transpose the boundary -> fluid list (rows by boundary index)."""
+        transpose the boundary -> fluid list (rows by boundary index).
+        build=False: the grid is already built (and its points sorted)."""
         g = self.grid
-        g.build(x, self.pscene, s)
+        if build:
+            g.build(x, self.pscene, s)
         if fluid_pairs:
             self._pairs(self.ff, g.query_struct(x, self.h, self.pscene, self.cell_div, True), s)
         if self.NB:
@@ -659,17 +673,45 @@ class SceneBatch:
             self._dplus_shift(s)
 
     # -------------------------------------------------------------- DFSPH
+    def stencil_list(self) -> L.SphStencilList:
+        return L.SphStencilList(L.ptr(self.sl_runs), L.ptr(self.sl_masks), self.NP)
+
+    def _dfsph_search(self, s=None):
+        """The step's grid and pair lists.  Stencil-list batches build the
+        grid first, permute the state into its order (so a point's index is
+        its sorted slot) and then list each point's fluid neighbours as
+        stencil masks; the others permute into the previous search's order
+        and build the CSR fluid list."""
+        if not self.stencil_lists:
+            self.reorder_state(s)
+            self.search(self.x, s)
+            return
+        sh = L.stream_handle(s)
+        g = self.grid
+        g.build(self.x, self.pscene, s)
+        self._sorted_valid = True
+        self.reorder_state(s)
+        L.call("sph_iota", L.ptr(g.sorted_idx), self.NP, sh)
+        gc = g._grid()
+        L.call("sph_stencil_list_build", C.byref(gc), L.ptr(self.h), L.ptr(self.sl_runs),
+               L.ptr(self.sl_masks), sh)
+        self.search(self.x, s, fluid_pairs=False, build=False)
+
     def _dfsph_pre(self, s=None):
         sh = L.stream_handle(s)
-        self.reorder_state(s)
-        self.search(self.x, s)
+        self._dfsph_search(s)
         F = self.fluid(self.x, self.v, self.rho)
         ff = self.ff.csr().c()
         fb = self.fb.csr().c() if self.NB else None
         B = self.boundary()
-        L.call("sph_fluid_density", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B),
-               C.byref(fb) if fb is not None else None, self._order(), L.ptr(self.rho),
-               L.ptr(self.vol), sh)
+        fbp = C.byref(fb) if fb is not None else None
+        if self.stencil_lists:
+            sl = self.stencil_list()
+            L.call("sph_fluid_density_sl", C.byref(F), L.ptr(self.phys), C.byref(sl), C.byref(B),
+                   fbp, L.ptr(self.rho), L.ptr(self.vol), sh)
+        else:
+            L.call("sph_fluid_density", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B),
+                   fbp, self._order(), L.ptr(self.rho), L.ptr(self.vol), sh)
         if self.NB:
             bf = self.bf.csr().c()
             L.call("sph_akinci_boundary_density", C.byref(F), L.ptr(self.phys), L.ptr(self.bx),
@@ -682,8 +724,12 @@ class SceneBatch:
         L.call("sph_dfsph_prepare", C.byref(F), L.ptr(self.acc), L.ptr(self.phys), C.byref(B),
                C.byref(D), sh)
         bfc = self.bf.csr().c()
-        L.call("sph_dfsph_alpha_source", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B),
-               C.byref(fb) if fb is not None else None, C.byref(bfc), C.byref(D), sh)
+        if self.stencil_lists:
+            L.call("sph_dfsph_alpha_source_sl", C.byref(F), L.ptr(self.phys), C.byref(sl),
+                   C.byref(B), fbp, C.byref(bfc), C.byref(D), sh)
+        else:
+            L.call("sph_dfsph_alpha_source", C.byref(F), L.ptr(self.phys), C.byref(ff),
+                   C.byref(B), fbp, C.byref(bfc), C.byref(D), sh)
 
     def _divergence_solve(self, F, ff, sh):
         """divergenceSolver (dfsph.py:175-179, 236-260): fluid-only solve on
@@ -694,6 +740,16 @@ class SceneBatch:
         D = self._dfsph_struct(divergence=True)
         pd = L.ptr(self.phys_div)
         L.call("sph_dfsph_prepare", C.byref(F), L.ptr(self.acc), pd, None, C.byref(D), sh)
+        if self.stencil_lists:
+            sl = self.stencil_list()
+            L.call("sph_dfsph_alpha_source_sl", C.byref(F), pd, C.byref(sl), None, None, None,
+                   C.byref(D), sh)
+            for _ in range(self.div_chunk):
+                L.call("sph_dfsph_iterate_sl", C.byref(F), pd, C.byref(sl), None, None, None,
+                       C.byref(D), sh)
+            L.call("sph_dfsph_finish_sl", C.byref(F), pd, C.byref(sl), None, C.byref(D),
+                   L.ptr(self.acc), sh)
+            return
         L.call("sph_dfsph_alpha_source", C.byref(F), pd, C.byref(ff), None, None, None,
                C.byref(D), sh)
         for _ in range(self.div_chunk):
@@ -710,9 +766,16 @@ class SceneBatch:
         bfc = self.bf.csr().c()
         B = self.boundary()
         D = self._dfsph_struct()
+        fbp = C.byref(fb) if fb is not None else None
+        if self.stencil_lists:
+            sl = self.stencil_list()
+            for _ in range(k):
+                L.call("sph_dfsph_iterate_sl", C.byref(F), L.ptr(self.phys), C.byref(sl),
+                       C.byref(B), fbp, C.byref(bfc), C.byref(D), sh)
+            return
         for _ in range(k):
             L.call("sph_dfsph_iterate", C.byref(F), L.ptr(self.phys), C.byref(ff), C.byref(B),
-                   C.byref(fb) if fb is not None else None, C.byref(bfc), C.byref(D), sh)
+                   fbp, C.byref(bfc), C.byref(D), sh)
 
     def _dfsph_post(self, s=None):
         sh = L.stream_handle(s)
@@ -725,9 +788,14 @@ class SceneBatch:
         # fluid list
         Fv = self.fluid(self.x, self.v, self.rho, self.vol)
         L.call("sph_dsph_pack", C.byref(Fv), L.ptr(self.phys), None, L.ptr(self.frec), None, sh)
-        L.call("sph_dfsph_finish_visc", C.byref(F), L.ptr(self.phys), C.byref(ff),
-               C.byref(fb) if fb is not None else None, C.byref(D), L.ptr(self.frec),
-               L.T_VISC | L.T_LAM, L.ptr(self.acc), sh)
+        fbp = C.byref(fb) if fb is not None else None
+        if self.stencil_lists:
+            sl = self.stencil_list()
+            L.call("sph_dfsph_finish_visc_sl", C.byref(F), L.ptr(self.phys), C.byref(sl), fbp,
+                   C.byref(D), L.ptr(self.frec), L.T_VISC | L.T_LAM, L.ptr(self.acc), sh)
+        else:
+            L.call("sph_dfsph_finish_visc", C.byref(F), L.ptr(self.phys), C.byref(ff), fbp,
+                   C.byref(D), L.ptr(self.frec), L.T_VISC | L.T_LAM, L.ptr(self.acc), sh)
         L.call("sph_integrate_semi_implicit", L.ptr(self.pscene), L.ptr(self.phys), self.NP,
                L.ptr(self.x), L.ptr(self.v), None, L.ptr(self.v), L.ptr(self.acc), None,
                L.ptr(self.x), L.ptr(self.v), None, sh)
@@ -838,7 +906,8 @@ class SceneBatch:
         if (st & L.ST_OVERFLOW).any():
             raise RuntimeError("pair-list capacity exceeded; rebuild the batch with a larger cap")
         # cell-walk steps build no fluid list
-        nnz = None if self.cellwalk else int(self.ff.rowptr[self.NP].item())
+        nnz = (None if self.cellwalk or self.stencil_lists
+               else int(self.ff.rowptr[self.NP].item()))
         return st, nnz
 
     def scene_state(self, k: int, extra: bool = False) -> dict:

@@ -51,6 +51,15 @@ def gpu_search(x, h, vmin, vmax, hmax, queries=None, radius=1, ordered=True):
     return rp, c, st, g
 
 
+def in_scene_order(b, t, k=0):
+    """Fluid array t of scene k of SceneBatch b in the scene's original
+    particle order (the batch keeps its state in cell order)."""
+    a, e = int(b.p_begin[k]), int(b.p_begin[k] + b.n[k])
+    inv = np.empty(e - a, dtype=np.int64)
+    inv[b.uid[a:e].cpu().numpy() - a] = np.arange(e - a)
+    return t[a:e].detach().cpu().numpy()[inv]
+
+
 def rel_err(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)

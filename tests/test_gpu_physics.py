@@ -16,7 +16,7 @@ from oracle import oracle as O
 from pytorchsph_amd import _lib as L
 from pytorchsph_amd import scenes as S
 from pytorchsph_amd.engine import SceneBatch
-from tests.gpu_helpers import assert_same_pairs, csr_numpy, rel_err
+from tests.gpu_helpers import assert_same_pairs, csr_numpy, in_scene_order, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -101,13 +101,17 @@ def test_fluid_and_akinci_density(cuda):
     torch.cuda.synchronize()
     so = O.SceneOracle(sc)
     ff, bf, _, fb = so.pairs(sc.x)
-    assert_same_pairs(*csr_numpy(b.bf.rowptr, b.bf.col, sc.nb), bf)
+    # the batch holds its fluid in cell order: columns and fluid arrays go
+    # back to the scene's order through the particle ids
+    rp, col = csr_numpy(b.bf.rowptr, b.bf.col, sc.nb)
+    assert_same_pairs(rp, b.uid.cpu().numpy()[col], bf)
     f = dict(x=sc.x, v=sc.v, rho=np.ones(sc.n), area=so.area, rest=so.rest)
     bd = dict(x=sc.bx, vol=sc.bvol, rest=so.brest)
     rho, vol = O.fluid_density(f, sc.phys, ff, bd, fb)
-    assert rel_err(b.rho.cpu().numpy(), rho) < 1e-5
-    assert np.max(np.abs(b.rho.cpu().numpy() / rho - 1)) < 1e-5
-    assert np.max(np.abs(b.vol.cpu().numpy() / vol - 1)) < 1e-5
+    rho_g, vol_g = in_scene_order(b, b.rho), in_scene_order(b, b.vol)
+    assert rel_err(rho_g, rho) < 1e-5
+    assert np.max(np.abs(rho_g / rho - 1)) < 1e-5
+    assert np.max(np.abs(vol_g / vol - 1)) < 1e-5
     brho = O.akinci_bdensity(f, sc.phys, bd, sc.bterm, bf)
     assert np.max(np.abs(b.brho[: sc.nb].cpu().numpy() / brho - 1)) < 1e-5
 
@@ -124,21 +128,21 @@ def test_dfsph_alpha_source_and_fixed_iterations(cuda):
     torch.cuda.synchronize()
     so = O.SceneOracle(sc)
     ff, bf, _, fb = so.pairs(sc.x)
-    rho = b.rho.cpu().numpy().astype(np.float64)
+    rho = in_scene_order(b, b.rho).astype(np.float64)
     brho = b.brho[: sc.nb].cpu().numpy().astype(np.float64)
     f = dict(x=sc.x, v=sc.v, rho=rho, area=so.area, rest=so.rest)
     bd = dict(x=sc.bx, vol=sc.bvol, rest=so.brest, rho=brho)
     acc = np.tile([0.0, -S.G], (sc.n, 1))
     out = O.dfsph(f, sc.phys, ff, bd, fb, bf, acc, fixed_iters=5)
     d = b.df
-    assert rel_err(d["alpha"].cpu().numpy(), out["alpha"]) < 1e-4
-    assert rel_err(d["src"].cpu().numpy(), out["src"]) < 1e-4
+    assert rel_err(in_scene_order(b, d["alpha"]), out["alpha"]) < 1e-4
+    assert rel_err(in_scene_order(b, d["src"]), out["src"]) < 1e-4
     assert rel_err(d["b_alpha"][: sc.nb].cpu().numpy(), out["b_alpha"]) < 1e-4
     assert rel_err(d["b_src"][: sc.nb].cpu().numpy(), out["b_src"]) < 1e-4
     assert int(d["iters"][0].item()) == 5
     hist = d["err_hist"][0, :5].cpu().numpy()
     assert np.allclose(hist, out["err_hist"][:5], rtol=1e-3, atol=1e-7)
-    assert rel_err(d["p2"].cpu().numpy(), out["p2"]) < 1e-3
+    assert rel_err(in_scene_order(b, d["p2"]), out["p2"]) < 1e-3
 
 
 def test_dfsph_converged_step(cuda):

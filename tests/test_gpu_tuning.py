@@ -70,8 +70,10 @@ def test_dfsph_lanes(cuda, kl):
     sc.phys["dt"] = 0.04 * float(sc.support)
     rng = np.random.default_rng(4)
     sc.v = rng.normal(0, 0.5, sc.v.shape).astype(np.float32)
+    # the CSR gathers, which take the lanes-per-row knobs (the stencil-list
+    # gathers run one lane per row)
     with tuned(solve=kl, gather=kl):
-        b = SceneBatch([sc])
+        b = SceneBatch([sc], stencil_lists=False)
         b._dfsph_pre()
         b._dfsph_iters(5)
         torch.cuda.synchronize()
