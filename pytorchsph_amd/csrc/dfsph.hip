@@ -156,26 +156,40 @@ __device__ __forceinline__ void alpha_boundary_row(const SphPhysics* __restrict_
   d.b_src[k0] = 1.f - b.rho[k0] - P.dt * src;
 }
 
+// What the pressure update of fluid row i reads besides its fluid sum:
+// loaded apart so that a kernel can issue the loads before the sum.
+struct PressIn {
+  float2 ai, wi;         // a*_i, W_i
+  float src, p2, alpha;  // s_i, the current iterate, alpha_i
+  float rho;             // rho_hat rho0
+};
+__device__ __forceinline__ PressIn press_in(const SphFluid& f, const SphDfsph& d, int i) {
+  PressIn r;
+  r.ai = ld2(d.arec, 2 * i);
+  r.wi = ld2(d.wvec, i);
+  r.src = d.src[i];
+  r.p2 = d.p2[i];
+  r.alpha = d.alpha[i];
+  r.rho = f.rho[i] * f.rest[i];
+  return r;
+}
+
 // The relaxed update of fluid row i from its fluid sum ks and the clamped
 // residual for the scene sum (dfsph.py:44-57,164-184):
 //   r_i = a*_i . W_i - ks - s_i.
-__device__ __forceinline__ double pressure_row_update(const SphFluid& f, const SphDfsph& d,
+__device__ __forceinline__ double pressure_row_update(const SphDfsph& d, const PressIn& in,
                                                       float omega, float tau, int s, int i,
                                                       float ks) {
-  const float2 ai = ld2(d.arec, 2 * i);
-  const float2 wi = ld2(d.wvec, i);
-  const float rr = (ai.x * wi.x + ai.y * wi.y) - ks - d.src[i];
-  const float pold = d.p2[i];
-  d.p[i] = pold;
-  float pn = pold - omega * rr / d.alpha[i];
+  const float rr = (in.ai.x * in.wi.x + in.ai.y * in.wi.y) - ks - in.src;
+  d.p[i] = in.p2;
+  float pn = in.p2 - omega * rr / in.alpha;
   // densitySolve clamps at 0 (clamp(min = 0) keeps NaN); divergenceSolve
   // does not (dfsph.py:177)
   if (!d.divergence) pn = pn < 0.f ? 0.f : pn;
   if (isnan(pn) || isinf(pn)) atomicOr(&d.status[s], SPH_ST_NAN);
   d.p2[i] = pn;
   d.resid[i] = rr;
-  const float rho = f.rho[i] * f.rest[i];
-  d.xrec[4 * i + 2] = pn / (rho * rho);
+  d.xrec[4 * i + 2] = pn / (in.rho * in.rho);
   return (double)(rr < -tau ? -tau : rr);
 }
 
@@ -532,7 +546,7 @@ __device__ __forceinline__ double pressure_fluid_row(const SphFluid& f, const Sp
   *wrote = false;
   if (sub != 0) return 0.0;
   *wrote = true;
-  return pressure_row_update(f, d, omega, tau, s, i, ks);
+  return pressure_row_update(d, press_in(f, d, i), omega, tau, s, i, ks);
 }
 
 template <int KL>
@@ -931,8 +945,8 @@ __global__ __launch_bounds__(kBS) void k_alpha_source_sl(SphFluid f,
 // boundary pairs from fb), one lane per row.
 template <typename Walk>
 __device__ __forceinline__ float2 pressure_accel_sl(const SphCSR& fb, const SphDfsph& d,
-                                                    const KConst& K, int i, Walk walk) {
-  const float4 ri = ld4(d.xrec, i);
+                                                    const KConst& K, int i, const float4& ri,
+                                                    Walk walk) {
   float ax = 0.f, ay = 0.f;
   auto add = [&](const float4& rj) {
     const float2 G = spiky_G(ri.x, ri.y, rj.x, rj.y, K);
@@ -947,6 +961,41 @@ __device__ __forceinline__ float2 pressure_accel_sl(const SphCSR& fb, const SphD
   return make_float2(ax, ay);
 }
 
+// Fluid row i of active item t of the pass of kBS items from `base` (every
+// item of which is a fluid item when the pass is full), its scene s, and
+// whether the pass lies in one scene su (that scene's data then uniform).
+// With every scene active, the items are the fluid rows in order: no
+// search of the active list.
+struct PassRow {
+  int i, s, su;
+  bool uni;
+};
+__device__ __forceinline__ PassRow pass_row(const SphFluid& f, const SphDfsph& d, bool all,
+                                            int nact, int nfl, int base, int t) {
+  PassRow w = {0, 0, 0, false};
+  const bool valid = t < nfl;
+  if (all) {
+    w.su = __builtin_amdgcn_readfirstlane(f.pscene[base]);
+    w.uni = base + kBS <= nfl &&
+            __builtin_amdgcn_readfirstlane(f.pscene[base + kBS - 1]) == w.su;
+    w.i = t;
+    w.s = valid ? (w.uni ? w.su : f.pscene[t]) : 0;
+    return w;
+  }
+  const int r0 = find_rank(d.act_pre, nact, base);
+  w.uni = base + kBS <= nfl && base + kBS <= d.act_pre[r0 + 1];
+  w.su = d.act_scene[r0];
+  if (w.uni) {
+    w.s = w.su;
+    w.i = d.fbegin[w.s] + (t - d.act_pre[r0]);
+  } else if (valid) {
+    const int r = advance_rank(d.act_pre, nact, r0, t);
+    w.s = d.act_scene[r];
+    w.i = d.fbegin[w.s] + (t - d.act_pre[r]);
+  }
+  return w;
+}
+
 // k_accel over a stencil list.
 __global__ __launch_bounds__(kBS) void k_accel_sl(SphFluid f, const SphPhysics* __restrict__ phys,
                                                   SphStencilList sl, SphCSR fb, SphDfsph d) {
@@ -958,30 +1007,28 @@ __global__ __launch_bounds__(kBS) void k_accel_sl(SphFluid f, const SphPhysics* 
     r.a[0] = ld4(d.xrec, j);
     return r;
   };
+  const bool all = nact == d.nscenes;
   for (int base = xcd_block() * kBS; base < nitems; base += gridDim.x * kBS) {
     const int t = base + threadIdx.x;
-    const int r0 = find_rank(d.act_pre, nact, base);
-    const bool uni = base + kBS <= d.act_pre[r0 + 1];
     const bool valid = t < nitems;
-    int s = 0, i = 0;
-    if (uni) {
-      s = d.act_scene[r0];
-      i = d.fbegin[s] + (t - d.act_pre[r0]);
-    } else if (valid) {
-      const int r = advance_rank(d.act_pre, nact, r0, t);
-      s = d.act_scene[r];
-      i = d.fbegin[s] + (t - d.act_pre[r]);
-    }
+    const PassRow pr = pass_row(f, d, all, nact, nitems, base, t);
+    const int i = pr.i;
+    // the row's list, record and area, loaded under the stage's wait
     const SlRow w = valid ? sl_row(sl, i) : sl_none();
+    const float4 ri = valid ? ld4(d.xrec, i) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float aact = valid ? d.aact[i] : 0.f;
     ChunkRuns cr;
     const bool staged = sl_stage<1>(S, w.st, valid, load, cr);
     if (!valid) continue;
-    const SphPhysics P = phys[uni ? d.act_scene[r0] : s];
+    const SphPhysics P = phys[pr.uni ? pr.su : pr.s];
     const KConst K = kconst(P.h);
-    const float2 a = pressure_accel_sl(fb, d, K, i, [&](auto f2) {
+    const float2 a = pressure_accel_sl(fb, d, K, i, ri, [&](auto f2) {
       sl_walk<1>(S, cr, staged, w, load, f2);
     });
-    store_accel(d, i, a, P.dt * P.dt);
+    // store_accel with the row's area and position loaded above
+    const float wv = P.dt * P.dt * aact;
+    st4(d.arec, i, make_float4(a.x, a.y, wv, 0.f));
+    st4(d.prec, i, make_float4(ri.x, ri.y, wv * a.x, wv * a.y));
   }
 }
 
@@ -1005,37 +1052,38 @@ __global__ __launch_bounds__(kBS) void k_pressure_sl(SphFluid f, const SphPhysic
     r.a[0] = ld4(d.prec, j);
     return r;
   };
+  const bool all = nact == d.nscenes;
   for (int base = xcd_block() * kBS; base < nitems; base += stride) {
     const int t = base + threadIdx.x;
     double contrib = 0.0;
     int s = -1;
     bool valid = false;
-    const int r0f = base < nf ? find_rank(d.act_pre, nact, base) : 0;
     const int r0b = base + kBS > nf ? find_rank(preb, nact, base > nf ? base - nf : 0) : 0;
-    const bool uni = base + kBS <= nf && base + kBS <= d.act_pre[r0f + 1];
-    int i = 0;
-    if (uni) {
-      s = d.act_scene[r0f];
-      i = d.fbegin[s] + (t - d.act_pre[r0f]);
-    } else if (t < nf) {
-      const int r = advance_rank(d.act_pre, nact, r0f, t);
-      s = d.act_scene[r];
-      i = d.fbegin[s] + (t - d.act_pre[r]);
+    PassRow pr = {0, 0, 0, false};
+    if (base < nf) pr = pass_row(f, d, all, nact, nf, base, t);
+    const int i = pr.i;
+    const bool fl = t < nf;
+    if (fl) s = pr.s;
+    // the row's list, position and update inputs, loaded under the stage's wait
+    const SlRow w = fl ? sl_row(sl, i) : sl_none();
+    float2 xi = make_float2(0.f, 0.f);
+    PressIn pin = {};
+    if (fl) {
+      xi = ld2(d.xrec, 2 * i);
+      pin = press_in(f, d, i);
     }
-    const SlRow w = t < nf ? sl_row(sl, i) : sl_none();
     ChunkRuns cr;
-    const bool staged = sl_stage<1>(S, w.st, t < nf, load, cr);
-    if (t < nf) {
-      const SphPhysics P = phys[uni ? d.act_scene[r0f] : s];
+    const bool staged = sl_stage<1>(S, w.st, fl, load, cr);
+    if (fl) {
+      const SphPhysics P = phys[pr.uni ? pr.su : s];
       const KConst K = kconst(P.h);
-      const float2 xi = ld2(d.xrec, 2 * i);
       float ks = 0.f;
       sl_walk<1>(S, cr, staged, w, load, [&](const SlRecs<1>& r) {
         const float4 pj = r.a[0];
         const float2 G = spiky_G(xi.x, xi.y, pj.x, pj.y, K);
         ks += pj.z * G.x + pj.w * G.y;
       });
-      contrib = pressure_row_update(f, d, P.fluid_omega, P.tau, s, i, ks);
+      contrib = pressure_row_update(d, pin, P.fluid_omega, P.tau, s, i, ks);
       valid = true;
     } else if (t < nitems) {
       contrib = pressure_boundary_row<1>(b, bf, d, phys, preb, nact, r0b, t - nf, 0, &s, &valid);
@@ -1062,7 +1110,7 @@ __global__ __launch_bounds__(kBS) void k_finish_sl(SphFluid f, const SphPhysics*
   if (!valid) return;
   const SphPhysics P = phys[f.pscene[i]];
   const KConst K = kconst(P.h);
-  const float2 a = pressure_accel_sl(fb, d, K, i, [&](auto f2) {
+  const float2 a = pressure_accel_sl(fb, d, K, i, ld4(d.xrec, i), [&](auto f2) {
     sl_walk<1>(S, cr, staged, w, load, f2);
   });
   finish_store(P, d, i, a, acc);

@@ -13,6 +13,7 @@
 //    two 16-byte loads from one cache line, with V = A / rho_hat and the EOS
 //    evaluated once per particle instead of once per pair.
 #include <climits>
+#include <type_traits>
 
 #include "dsph_pair.h"
 #include "sph_common.h"
@@ -735,6 +736,197 @@ __device__ __forceinline__ void cellwalk_segment(WalkShared& S, int s0, int s1, 
   }
 }
 
+// ---- one lane per row: the neighbours' records staged in LDS
+// With one lane per row the test and the pair sums of a row run in the same
+// thread, so a row's hits stay in registers as bitmasks over its three
+// stencil runs (as in a stencil list) and the stage copies, besides the
+// runs' positions, the fields of their packed records the pair routine
+// reads: {vx, vy, rho, V} and p.  The pair loop then reads LDS only, each
+// record once per chunk from memory instead of once per pair.  Positions
+// come from the grid (sorted_x), the same values as the records' own.  A
+// run longer than a mask, or a chunk whose runs do not fit, takes the
+// global-memory walk (walk_slow) for all its fluid pairs.
+constexpr int kWalk1Cap = 1088;        // staged stencil points per chunk
+constexpr int kWalk1MaxRun = 64;       // candidates a run's mask holds
+constexpr int kWalk1Round = 2;         // staged slots per thread in flight
+
+struct Walk1Shared {
+  float2 pos[kWalk1Cap + kWalkBatch];  // the test reads past a run's end
+  float4 rec[kWalk1Cap];               // {vx, vy, rho, V}
+  float p[kWalk1Cap];
+  int wlo[kBS / kWave][3], whi[kBS / kWave][3];
+};
+
+template <unsigned T, bool FUSE, bool UNI>
+__device__ __forceinline__ void cellwalk_segment1(Walk1Shared& S, int s0, int s1, int usc,
+                                                  const SphGrid& g,
+                                                  const float4* __restrict__ frec,
+                                                  const float4* __restrict__ brec,
+                                                  const SphPhysics* __restrict__ phys,
+                                                  const SphCSR& fb, const SphGrid& bg,
+                                                  unsigned terms, float* __restrict__ dpdt,
+                                                  float* __restrict__ acc,
+                                                  const SphRk4Stage& rk) {
+  const int t = threadIdx.x;
+  const float2* __restrict__ sx = reinterpret_cast<const float2*>(g.sorted_x);
+  SphGridDesc ud = {};
+  float uthr = 0.f;
+  if (UNI) {
+    ud = g.scenes[usc];
+    uthr = sq_threshold(phys[usc].h);
+  }
+  for (int c0 = s0; c0 < s1; c0 += kWalkRows) {
+    // ---- the stencil of row t, its own record and RK4 inputs
+    const int sp = c0 + t;
+    const bool valid = sp < s1;
+    Stencil3 st;
+    SphGridDesc d = ud;
+    float xq = 0.f, yq = 0.f, thr = uthr;
+    int qi = 0, sc = UNI ? usc : 0, ix = 0, iy = 0;
+    float4 a0 = {}, a1 = {};
+    Rk4In rin = {};
+    if (valid) {
+      qi = g.sorted_idx[sp];
+      if (!UNI) {
+        sc = g.nscenes == 1 ? 0 : g.pscene[qi];
+        d = g.scenes[sc];
+        thr = sq_threshold(phys[sc].h);
+      }
+      const float2 q = sx[sp];
+      xq = q.x;
+      yq = q.y;
+      ix = cell_coord(xq, d.qmin_x, d.hmax);
+      iy = cell_coord(yq, d.qmin_y, d.hmax);
+      st.init(d, g.cell_start + d.cell_begin, ix, iy);
+      a0 = frec[2 * qi];
+      a1 = frec[2 * qi + 1];
+      if (FUSE) rin = load_rk4_in(rk, qi);
+    }
+    // ---- stage: the runs' positions and the fields of their records
+    const ChunkRuns cr = chunk_runs(st, valid, S.wlo, S.whi);
+    const int tot = cr.tot;
+    const bool staged = tot <= kWalk1Cap;
+    if (staged) {
+      const int w0 = __builtin_amdgcn_readfirstlane(t);
+      for (int u0 = 0; w0 + u0 * kBS < tot; u0 += kWalk1Round) {
+        float2 pv[kWalk1Round];
+        int jv[kWalk1Round];
+#pragma unroll
+        for (int u = 0; u < kWalk1Round; ++u) {
+          const int k = t + (u0 + u) * kBS;
+          const int sl = cr.slot(k < tot ? k : tot - 1);
+          pv[u] = sx[sl];
+          jv[u] = g.sorted_idx[sl];
+        }
+        float4 r0[kWalk1Round], r1[kWalk1Round];
+#pragma unroll
+        for (int u = 0; u < kWalk1Round; ++u) {
+          r0[u] = frec[2 * jv[u]];
+          r1[u] = frec[2 * jv[u] + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kWalk1Round; ++u) {
+          const int k = t + (u0 + u) * kBS;
+          if (k < tot) {
+            S.pos[k] = pv[u];
+            S.rec[k] = make_float4(r0[u].z, r0[u].w, r1[u].x, r1[u].y);
+            S.p[k] = r1[u].z;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // ---- test: the row's hits as bits of its runs' masks
+    unsigned long long m[3] = {0ull, 0ull, 0ull};
+    bool slow = !staged;
+    if (valid && staged) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const int len = st.rb[r] - st.ra[r];
+        slow |= len > kWalk1MaxRun;
+        const int n = len < kWalk1MaxRun ? len : kWalk1MaxRun;
+        const int b = cr.first(r, st);
+        for (int k = 0; k < n; k += kWalkBatch) {
+          float2 xy[kWalkBatch];
+#pragma unroll
+          for (int u = 0; u < kWalkBatch; ++u) xy[u] = S.pos[b + k + u];
+#pragma unroll
+          for (int u = 0; u < kWalkBatch; ++u) {
+            const bool hit =
+                (k + u < n) && dist2_nofma(xq, yq, xy[u].x, xy[u].y) <= thr;
+            m[r] |= hit ? (1ull << (k + u)) : 0ull;
+          }
+        }
+      }
+    }
+    // ---- gather: the row's own pairs
+    if (valid) {
+      const SphPhysics P = phys[sc];
+      const KConst K = kconst(P.h);
+      Own o;
+      o.x = a0.x; o.y = a0.y; o.vx = a0.z; o.vy = a0.w;
+      o.rho = a1.x;
+      o.p = a1.z;
+      o.gx = 0.f;
+      o.gy = 0.f;
+      Acc a = {};
+      if (terms & kFluidPairTerms) {
+        if (!slow) {
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            unsigned long long mm = m[r];
+            const int b = cr.first(r, st);
+            while (mm) {
+              int k[kPackedBatch];
+              bool ok[kPackedBatch];
+#pragma unroll
+              for (int u = 0; u < kPackedBatch; ++u) {
+                ok[u] = mm != 0ull;
+                k[u] = b + (ok[u] ? (int)__builtin_ctzll(mm) : 0);
+                mm &= mm - 1ull;
+              }
+              float2 xy[kPackedBatch];
+              float4 rc[kPackedBatch];
+              float pj[kPackedBatch];
+#pragma unroll
+              for (int u = 0; u < kPackedBatch; ++u) {
+                xy[u] = S.pos[k[u]];
+                rc[u] = S.rec[k[u]];
+                pj[u] = S.p[k[u]];
+              }
+#pragma unroll
+              for (int u = 0; u < kPackedBatch; ++u) {
+                if (ok[u]) {
+                  const Nbr nb = {xy[u].x, xy[u].y, rc[u].x, rc[u].y, rc[u].z, rc[u].w, pj[u]};
+                  dsph_pair(P, K, o, nb, 0.f, 0.f, true, terms, a);
+                }
+              }
+            }
+          }
+        }
+        // a row with a long run, or every row of an unstaged chunk: all its
+        // fluid pairs from memory
+        if (__ballot(slow) != 0)
+          walk_slow<1>(slow, g, sc, sp, thr, 0, 0, P, K, o, terms, frec, a);
+      }
+      if (bg.cell_start) {
+        if (terms & kBdyPairTerms) {
+          const SphGridDesc bd = bg.scenes[bg.nscenes == 1 ? 0 : sc];
+          const bool same = bd.qmin_x == d.qmin_x && bd.qmin_y == d.qmin_y && bd.hmax == d.hmax;
+          Stencil3 bst;
+          bdy_stencil(bg, bd, same, xq, yq, ix, iy, bst);
+          walk_bdy<1>(bst, bst.ncand, bg, xq, yq, thr, 0, P, K, o, terms, brec, a);
+        }
+      } else {
+        packed_bdy_pairs<1>(P, K, o, brec, fb, qi, 0, terms, a);
+      }
+      packed_row_out<FUSE>(P, o, a, terms, qi, dpdt, acc, rk, rin);
+    }
+    // the next chunk restages the LDS arrays
+    if (c0 + kWalkRows < s1) __syncthreads();
+  }
+}
+
 // segs[b] = {first, end} sorted slots of workgroup b.  Five wavefronts per
 // SIMD is what the LDS of the stage allows; the register budget is held to
 // it (96 VGPRs), since 100 registers left four.
@@ -744,7 +936,7 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5))) void k
     const float4* __restrict__ brec, const SphPhysics* __restrict__ phys, SphCSR fb, SphGrid bg,
     unsigned rterms, float* __restrict__ dpdt, float* __restrict__ acc, SphRk4Stage rk) {
   const unsigned terms = T ? T : rterms;
-  __shared__ WalkShared S;
+  __shared__ typename std::conditional<KL == 1, Walk1Shared, WalkShared>::type S;
   const int2 seg = segs[xcd_block()];
   const int s0 = seg.x > 0 ? seg.x : 0, s1 = seg.y < g.np ? seg.y : g.np;
   // A segment inside one scene (scenes are contiguous in sorted order, so
@@ -757,12 +949,21 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5))) void k
     usc = a == z ? a : -1;
   }
   usc = __builtin_amdgcn_readfirstlane(usc);
-  if (usc >= 0)
-    cellwalk_segment<T, KL, FUSE, true>(S, s0, s1, usc, g, frec, brec, phys, fb, bg, terms, dpdt,
+  if constexpr (KL == 1) {
+    if (usc >= 0)
+      cellwalk_segment1<T, FUSE, true>(S, s0, s1, usc, g, frec, brec, phys, fb, bg, terms, dpdt,
+                                       acc, rk);
+    else
+      cellwalk_segment1<T, FUSE, false>(S, s0, s1, -1, g, frec, brec, phys, fb, bg, terms, dpdt,
                                         acc, rk);
-  else
-    cellwalk_segment<T, KL, FUSE, false>(S, s0, s1, -1, g, frec, brec, phys, fb, bg, terms, dpdt,
-                                         acc, rk);
+  } else {
+    if (usc >= 0)
+      cellwalk_segment<T, KL, FUSE, true>(S, s0, s1, usc, g, frec, brec, phys, fb, bg, terms,
+                                          dpdt, acc, rk);
+    else
+      cellwalk_segment<T, KL, FUSE, false>(S, s0, s1, -1, g, frec, brec, phys, fb, bg, terms,
+                                           dpdt, acc, rk);
+  }
 }
 
 // ------------------------------------------ renormalised density gradient
