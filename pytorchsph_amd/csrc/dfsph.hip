@@ -115,17 +115,23 @@ __device__ __forceinline__ int advance_rank(const int* __restrict__ pre, int nac
 // below, so that both compute the same bits from the same sums.
 
 // alpha_i, the source s_i and W_i of fluid row i from its sums (dfsph.py:58-100).
-__device__ __forceinline__ void alpha_store(const SphFluid& f, const SphPhysics& P,
-                                            const SphDfsph& d, int i, float k1x, float k1y,
-                                            float k2, float src) {
-  const float ai = d.aact[i];
-  const float fac = -P.dt * P.dt * ai;
-  const float mass = f.area[i] * f.rest[i];
-  float al = fac / mass * (k1x * k1x + k1y * k1y) + fac * k2;
+// (The row inputs of these epilogues are loaded apart, *_in, so that a
+// kernel can issue those loads before its pair sums.)
+struct AlphaIn {
+  float aact, mass, rho;
+};
+__device__ __forceinline__ AlphaIn alpha_in(const SphFluid& f, const SphDfsph& d, int i) {
+  return AlphaIn{d.aact[i], f.area[i] * f.rest[i], f.rho[i]};
+}
+__device__ __forceinline__ void alpha_store(const SphPhysics& P, const SphDfsph& d, int i,
+                                            const AlphaIn& in, float k1x, float k1y, float k2,
+                                            float src) {
+  const float fac = -P.dt * P.dt * in.aact;
+  float al = fac / in.mass * (k1x * k1x + k1y * k1y) + fac * k2;
   al = al < -1.f ? -1.f : (al > -1e-7f ? -1e-7f : al);  // dfsph.py:82
   d.alpha[i] = al;
   // divergenceSolve: the source is the divergence term only (dfsph.py:161)
-  d.src[i] = d.divergence ? src : 1.f - f.rho[i] + src;
+  d.src[i] = d.divergence ? src : 1.f - in.rho + src;
   // W_i = dt^2 k1_i: the part of the pressure-update sum that does not
   // change during the solve
   st2(d.wvec, i, make_float2(P.dt * P.dt * k1x, P.dt * P.dt * k1y));
@@ -231,32 +237,35 @@ __device__ __forceinline__ double pressure_boundary_row(const SphBoundary& b, co
 
 // The end of the solve for fluid row i with its final pressure
 // acceleration a: a*, p, v* and (acc != NULL) acc += a.
+struct FinishIn {
+  float2 vp, acc;   // v*_i, acc_i (0 without acc)
+  float p2;
+};
+__device__ __forceinline__ FinishIn finish_in(const SphDfsph& d, int i,
+                                              const float* __restrict__ acc) {
+  return FinishIn{ld2(d.vpred, i), acc ? ld2(acc, i) : make_float2(0.f, 0.f), d.p2[i]};
+}
 __device__ __forceinline__ void finish_store(const SphPhysics& P, const SphDfsph& d, int i,
-                                             float2 a, float* __restrict__ acc) {
+                                             const FinishIn& in, float2 a,
+                                             float* __restrict__ acc) {
   st2(d.apred, i, a);
-  d.p[i] = d.p2[i];
-  const float2 vp = ld2(d.vpred, i);
-  st2(d.vpred, i, make_float2(vp.x + P.dt * a.x, vp.y + P.dt * a.y));
-  if (acc) {
-    const float2 o = ld2(acc, i);
-    st2(acc, i, make_float2(o.x + a.x, o.y + a.y));
-  }
+  d.p[i] = in.p2;
+  st2(d.vpred, i, make_float2(in.vp.x + P.dt * a.x, in.vp.y + P.dt * a.y));
+  if (acc) st2(acc, i, make_float2(in.acc.x + a.x, in.acc.y + a.y));
 }
 
 // finish_store with the diffusion terms: acc = (acc + a) + their
 // acceleration.
 __device__ __forceinline__ void finish_visc_store(const SphPhysics& P, const Own& o,
                                                   const SphDfsph& d, const Acc& va,
-                                                  unsigned vterms, int i, float2 a,
-                                                  float* __restrict__ acc) {
+                                                  unsigned vterms, int i, const FinishIn& in,
+                                                  float2 a, float* __restrict__ acc) {
   st2(d.apred, i, a);
-  d.p[i] = d.p2[i];
-  const float2 vp = ld2(d.vpred, i);
-  st2(d.vpred, i, make_float2(vp.x + P.dt * a.x, vp.y + P.dt * a.y));
+  d.p[i] = in.p2;
+  st2(d.vpred, i, make_float2(in.vp.x + P.dt * a.x, in.vp.y + P.dt * a.y));
   float dp, vx, vy;
   dsph_rates(P, o, va, vterms, dp, vx, vy);
-  const float2 old = ld2(acc, i);
-  const float2 w = make_float2(old.x + a.x, old.y + a.y);
+  const float2 w = make_float2(in.acc.x + a.x, in.acc.y + a.y);
   st2(acc, i, make_float2(w.x + vx, w.y + vy));
 }
 
@@ -377,7 +386,7 @@ __global__ __launch_bounds__(kBS) void k_alpha_source(SphFluid f, const SphPhysi
     k2 = group_sum<KL>(k2);
     src = group_sum<KL>(src);
     if (sub != 0) return;
-    alpha_store(f, P, d, i, k1x, k1y, k2, src);
+    alpha_store(P, d, i, alpha_in(f, d, i), k1x, k1y, k2, src);
   } else if (t < f.np + b.nb) {
     alpha_boundary_row<KL>(phys, b, bf, d, t - f.np, sub);
   }
@@ -660,7 +669,7 @@ __global__ __launch_bounds__(kBS) void k_finish(SphFluid f, const SphPhysics* __
   const WaveCols none = {0, 0, false};
   const float2 a = pressure_accel<KL>(ff, fb, d, K, i, sub, none, nullptr);
   if (sub != 0) return;
-  finish_store(P, d, i, a, acc);
+  finish_store(P, d, i, finish_in(d, i, acc), a, acc);
 }
 
 // k_finish and the velocity-diffusion and laminar gather of the step
@@ -717,7 +726,7 @@ __global__ __launch_bounds__(kBS) void k_finish_visc(SphFluid f,
   const float2 a = make_float2(group_sum<KL>(ax), group_sum<KL>(ay));
   reduce_acc<KL>(va, vterms);
   if (sub != 0) return;
-  finish_visc_store(P, o, d, va, vterms, i, a, acc);
+  finish_visc_store(P, o, d, va, vterms, i, finish_in(d, i, acc), a, acc);
 }
 
 // ------------------------------------------------ stencil-list gathers
@@ -864,17 +873,24 @@ __global__ __launch_bounds__(kBS) void k_fluid_density_sl(SphFluid f,
     r.a[0] = make_float4(x.x, x.y, f.area[j], 0.f);
     return r;
   };
+  // the row's own data, loaded under the stage's wait
+  float2 xi = make_float2(0.f, 0.f);
+  float hi = 0.f, ai = 0.f;
+  if (valid) {
+    xi = ld2(f.x, i);
+    hi = phys[f.pscene[i]].h;
+    ai = f.area[i];
+  }
   ChunkRuns cr;
   const bool staged = sl_stage<1>(S, w.st, valid, load, cr);
   if (!valid) return;
-  const KConst K = kconst(phys[f.pscene[i]].h);
-  const float2 xi = ld2(f.x, i);
+  const KConst K = kconst(hi);
   float sum = 0.f;
   sl_walk<1>(S, cr, staged, w, load, [&](const SlRecs<1>& r) {
     const PairGeo g = pair_geo(xi.x, xi.y, r.a[0].x, r.a[0].y, K.inv_h);
     sum += r.a[0].z * wendland_k(g.q, K);
   });
-  if (vol_out) vol_out[i] = f.area[i] / sum;
+  if (vol_out) vol_out[i] = ai / sum;
   if (fb.rowptr) {
     float bs = 0.f;
     const int e1 = fb.rowptr[i + 1];
@@ -905,13 +921,22 @@ __global__ __launch_bounds__(kBS) void k_alpha_source_sl(SphFluid f,
     r.a[1] = ld4(d.prec, j);
     return r;
   };
+  // the row's own data, loaded under the stage's wait
+  float2 xi = make_float2(0.f, 0.f), vi = xi;
+  int sc = 0;
+  AlphaIn ain = {};
+  if (fluid) {
+    xi = ld2(f.x, t);
+    vi = ld2(d.vpred, t);
+    sc = f.pscene[t];
+    ain = alpha_in(f, d, t);
+  }
   ChunkRuns cr;
   const bool staged = sl_stage<2>(S, w.st, fluid, load, cr);
   if (fluid) {
     const int i = t;
-    const SphPhysics P = phys[f.pscene[i]];
+    const SphPhysics P = phys[sc];
     const KConst K = kconst(P.h);
-    const float2 xi = ld2(f.x, i), vi = ld2(d.vpred, i);
     float k1x = 0.f, k1y = 0.f, k2 = 0.f, src = 0.f;
     sl_walk<2>(S, cr, staged, w, load, [&](const SlRecs<2>& rr) {
       const Rec2 r = {rr.a[0], rr.a[1]};
@@ -935,7 +960,7 @@ __global__ __launch_bounds__(kBS) void k_alpha_source_sl(SphFluid f,
             src += -P.dt * aa * (vi.x * G.x + vi.y * G.y);
           });
     }
-    alpha_store(f, P, d, i, k1x, k1y, k2, src);
+    alpha_store(P, d, i, ain, k1x, k1y, k2, src);
   } else if (t < f.np + b.nb) {
     alpha_boundary_row<1>(phys, b, bf, d, t - f.np, 0);
   }
@@ -1105,15 +1130,24 @@ __global__ __launch_bounds__(kBS) void k_finish_sl(SphFluid f, const SphPhysics*
     r.a[0] = ld4(d.xrec, j);
     return r;
   };
+  // the row's own data, loaded under the stage's wait
+  float4 ri = make_float4(0.f, 0.f, 0.f, 0.f);
+  int sc = 0;
+  FinishIn fin = {};
+  if (valid) {
+    ri = ld4(d.xrec, i);
+    sc = f.pscene[i];
+    fin = finish_in(d, i, acc);
+  }
   ChunkRuns cr;
   const bool staged = sl_stage<1>(S, w.st, valid, load, cr);
   if (!valid) return;
-  const SphPhysics P = phys[f.pscene[i]];
+  const SphPhysics P = phys[sc];
   const KConst K = kconst(P.h);
-  const float2 a = pressure_accel_sl(fb, d, K, i, ld4(d.xrec, i), [&](auto f2) {
+  const float2 a = pressure_accel_sl(fb, d, K, i, ri, [&](auto f2) {
     sl_walk<1>(S, cr, staged, w, load, f2);
   });
-  finish_store(P, d, i, a, acc);
+  finish_store(P, d, i, fin, a, acc);
 }
 
 // k_finish_visc over a stencil list: per neighbour its pressure record and
@@ -1136,13 +1170,22 @@ __global__ __launch_bounds__(kBS) void k_finish_visc_sl(SphFluid f,
     r.a[1] = make_float4(p0.z, p0.w, p1.x, p1.y);
     return r;
   };
+  // the row's own data, loaded under the stage's wait
+  float4 ri = make_float4(0.f, 0.f, 0.f, 0.f), a0 = ri, a1 = ri;
+  int sc = 0;
+  FinishIn fin = {};
+  if (valid) {
+    ri = ld4(d.xrec, i);
+    a0 = frec[2 * i];
+    a1 = frec[2 * i + 1];
+    sc = f.pscene[i];
+    fin = finish_in(d, i, acc);
+  }
   ChunkRuns cr;
   const bool staged = sl_stage<2>(S, w.st, valid, load, cr);
   if (!valid) return;
-  const SphPhysics P = phys[f.pscene[i]];
+  const SphPhysics P = phys[sc];
   const KConst K = kconst(P.h);
-  const float4 ri = ld4(d.xrec, i);
-  const float4 a0 = frec[2 * i], a1 = frec[2 * i + 1];
   Own o;
   o.x = a0.x; o.y = a0.y; o.vx = a0.z; o.vy = a0.w;
   o.rho = a1.x;
@@ -1167,7 +1210,7 @@ __global__ __launch_bounds__(kBS) void k_finish_visc_sl(SphFluid f,
                        ax += c * G.x;
                        ay += c * G.y;
                      });
-  finish_visc_store(P, o, d, va, vterms, i, make_float2(ax, ay), acc);
+  finish_visc_store(P, o, d, va, vterms, i, fin, make_float2(ax, ay), acc);
 }
 
 }  // namespace
