@@ -273,7 +273,7 @@ int sph_neighbor_fill(const SphGrid* g, const SphQuery* q, int* rowptr, int* col
  * of a CSR column list.  Row r is sorted slot r; its three stencil rows
  * dy = q - 1 (q = 0, 1, 2) are the candidate runs of sorted slots
  * [runs[4r + q], runs[4r + q] + len_q), len_q = byte q of runs[4r + 3]
- * (at most 255), and bit k of masks[3r + q] is set when candidate
+ * (at most 255), and bit k of masks[q n + r] is set when candidate
  * runs[4r + q] + k is a pair (the search predicate of sph_neighbor_fill).
  * The set bits, run by run, are the row of sph_neighbor_fill in its order.
  * A run of more than 64 candidates keeps its first 64 and flags its scene
@@ -283,7 +283,7 @@ int sph_neighbor_fill(const SphGrid* g, const SphQuery* q, int* rowptr, int* col
  * buildNeighborList (neighSearch.cpp:172-259) for those gathers. */
 typedef struct SphStencilList {
   const int* runs;                   /* [n][4]                             */
-  const unsigned long long* masks;   /* [n][3]                             */
+  const unsigned long long* masks;   /* [3][n]                             */
   int n;                             /* rows: the grid's np                */
 } SphStencilList;
 int sph_stencil_list_build(const SphGrid* g, const float* h, int* runs,

@@ -769,9 +769,9 @@ __device__ __forceinline__ SlRow sl_row(const SphStencilList& sl, int r) {
   w.st.rb[1] = R.y + ((R.w >> 8) & 255);
   w.st.rb[2] = R.z + ((R.w >> 16) & 255);
   w.st.n0 = w.st.n01 = w.st.ncand = 0;
-  w.m[0] = sl.masks[3 * (size_t)r];
-  w.m[1] = sl.masks[3 * (size_t)r + 1];
-  w.m[2] = sl.masks[3 * (size_t)r + 2];
+  w.m[0] = sl.masks[r];
+  w.m[1] = sl.masks[(size_t)sl.n + r];
+  w.m[2] = sl.masks[2 * (size_t)sl.n + r];
   return w;
 }
 __device__ __forceinline__ SlRow sl_none() {

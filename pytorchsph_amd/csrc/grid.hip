@@ -392,18 +392,24 @@ __global__ __launch_bounds__(kBS) void k_stencil_list(SphGrid g, const float* __
     len[r] = st.rb[r] - st.ra[r];
     over |= len[r] > kSlMaxRun;
     const int n = len[r] < kSlMaxRun ? len[r] : kSlMaxRun;
-    const int b = cr.first(r, st);
-    for (int k = 0; k < n; k += kSWalkBatch) {
-      float2 xy[kSWalkBatch];
+    // LDS or global memory for the whole run (a choice per candidate
+    // between the two would compile to flat loads)
+    if (staged) {
+      const int b = cr.first(r, st);
+      for (int k = 0; k < n; k += kSWalkBatch) {
+        float2 xy[kSWalkBatch];
 #pragma unroll
-      for (int u = 0; u < kSWalkBatch; ++u) {
-        const int c = k + u < n ? k + u : n - 1;
-        xy[u] = staged ? s_pos[b + c] : sx[st.ra[r] + c];
+        for (int u = 0; u < kSWalkBatch; ++u) xy[u] = s_pos[b + k + u];   // padded
+        unsigned nib = 0;
+#pragma unroll
+        for (int u = 0; u < kSWalkBatch; ++u)
+          nib |= ((k + u < n) && dist2_nofma(xq, yq, xy[u].x, xy[u].y) <= thr) ? 1u << u : 0u;
+        m[r] |= (unsigned long long)nib << k;
       }
-#pragma unroll
-      for (int u = 0; u < kSWalkBatch; ++u) {
-        const bool hit = (k + u < n) && dist2_nofma(xq, yq, xy[u].x, xy[u].y) <= thr;
-        m[r] |= hit ? (1ull << (k + u)) : 0ull;
+    } else {
+      for (int k = 0; k < n; ++k) {
+        const float2 xy = sx[st.ra[r] + k];
+        m[r] |= dist2_nofma(xq, yq, xy.x, xy.y) <= thr ? 1ull << k : 0ull;
       }
     }
   }
@@ -411,9 +417,10 @@ __global__ __launch_bounds__(kBS) void k_stencil_list(SphGrid g, const float* __
   const int l0 = len[0] < 255 ? len[0] : 255, l1 = len[1] < 255 ? len[1] : 255,
             l2 = len[2] < 255 ? len[2] : 255;
   runs[sp] = make_int4(st.ra[0], st.ra[1], st.ra[2], l0 | (l1 << 8) | (l2 << 16));
-  masks[3 * (size_t)sp] = m[0];
-  masks[3 * (size_t)sp + 1] = m[1];
-  masks[3 * (size_t)sp + 2] = m[2];
+  const size_t n = (size_t)g.np;
+  masks[sp] = m[0];
+  masks[n + sp] = m[1];
+  masks[2 * n + sp] = m[2];
 }
 
 __global__ __launch_bounds__(kBS) void k_iota(int* __restrict__ a, int n) {

@@ -254,7 +254,7 @@ class SceneBatch:
         self.t_slot = torch.zeros(bcap, dtype=torch.int32, device=dev)
         if self.stencil_lists:
             self.sl_runs = torch.zeros((max(NP, 1), 4), dtype=torch.int32, device=dev)
-            self.sl_masks = torch.zeros((max(NP, 1), 3), dtype=torch.int64, device=dev)
+            self.sl_masks = torch.zeros((3, max(NP, 1)), dtype=torch.int64, device=dev)
         segs = walk_segments(self.p_begin, self.n)
         self.nseg = len(segs)
         self.segs = torch.from_numpy(segs).to(dev) if self.cellwalk and self.nseg else None

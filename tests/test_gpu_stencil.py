@@ -52,14 +52,14 @@ def check_against_fill(x, h, pscene, scenes):
     g.build(xt, ps)
     csr = g.search(xt, ht, ps, 1, self_sorted=True)
     runs = torch.full((n, 4), -7, dtype=torch.int32, device=dev)
-    masks = torch.zeros((n, 3), dtype=torch.int64, device=dev)
+    masks = torch.zeros((3, n), dtype=torch.int64, device=dev)
     gc = g._grid()
     L.call("sph_stencil_list_build", C.byref(gc), L.ptr(ht), L.ptr(runs), L.ptr(masks),
            L.stream_handle())
     torch.cuda.synchronize()
     rp, col = csr_numpy(csr.rowptr, csr.col, csr.nrows)
     sidx = g.sorted_idx[:n].cpu().numpy().astype(np.int64)
-    slots, ok = decode(runs.cpu().numpy(), masks.cpu().numpy())
+    slots, ok = decode(runs.cpu().numpy(), masks.cpu().numpy().T.copy())
     over = (np.stack([(runs[:, 3].cpu().numpy() >> (8 * q)) & 255 for q in range(3)], 1) > 64)
     for r in range(n):
         p = sidx[r]
