@@ -747,7 +747,7 @@ __device__ __forceinline__ void cellwalk_segment(WalkShared& S, int s0, int s1, 
 // run longer than a mask, or a chunk whose runs do not fit, takes the
 // global-memory walk (walk_slow) for all its fluid pairs.
 constexpr int kWalk1Cap = 1088;        // staged stencil points per chunk
-constexpr int kWalk1MaxRun = 64;       // candidates a run's mask holds
+constexpr int kWalk1MaxRun = 32;       // candidates a run's mask holds
 constexpr int kWalk1Round = 2;         // staged slots per thread in flight
 
 struct Walk1Shared {
@@ -837,7 +837,7 @@ __device__ __forceinline__ void cellwalk_segment1(Walk1Shared& S, int s0, int s1
     }
     __syncthreads();
     // ---- test: the row's hits as bits of its runs' masks
-    unsigned long long m[3] = {0ull, 0ull, 0ull};
+    unsigned m[3] = {0u, 0u, 0u};
     bool slow = !staged;
     if (valid && staged) {
 #pragma unroll
@@ -850,12 +850,11 @@ __device__ __forceinline__ void cellwalk_segment1(Walk1Shared& S, int s0, int s1
           float2 xy[kWalkBatch];
 #pragma unroll
           for (int u = 0; u < kWalkBatch; ++u) xy[u] = S.pos[b + k + u];
+          unsigned nib = 0;   // the batch's hits, then set four at a time
 #pragma unroll
-          for (int u = 0; u < kWalkBatch; ++u) {
-            const bool hit =
-                (k + u < n) && dist2_nofma(xq, yq, xy[u].x, xy[u].y) <= thr;
-            m[r] |= hit ? (1ull << (k + u)) : 0ull;
-          }
+          for (int u = 0; u < kWalkBatch; ++u)
+            nib |= ((k + u < n) && dist2_nofma(xq, yq, xy[u].x, xy[u].y) <= thr) ? 1u << u : 0u;
+          m[r] |= nib << k;
         }
       }
     }
@@ -874,16 +873,16 @@ __device__ __forceinline__ void cellwalk_segment1(Walk1Shared& S, int s0, int s1
         if (!slow) {
 #pragma unroll
           for (int r = 0; r < 3; ++r) {
-            unsigned long long mm = m[r];
+            unsigned mm = m[r];
             const int b = cr.first(r, st);
             while (mm) {
               int k[kPackedBatch];
               bool ok[kPackedBatch];
 #pragma unroll
               for (int u = 0; u < kPackedBatch; ++u) {
-                ok[u] = mm != 0ull;
-                k[u] = b + (ok[u] ? (int)__builtin_ctzll(mm) : 0);
-                mm &= mm - 1ull;
+                ok[u] = mm != 0u;
+                k[u] = b + (ok[u] ? (int)__builtin_ctz(mm) : 0);
+                mm &= mm - 1u;
               }
               float2 xy[kPackedBatch];
               float4 rc[kPackedBatch];
