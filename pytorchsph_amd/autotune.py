@@ -101,14 +101,19 @@ def tune(parts, reps: int = 5, log=None) -> dict:
     for b in dfsph:
         b._dfsph_pre()
     torch.cuda.synchronize()
+    # a knob is timed only on the parts whose kernels take it: the delta-SPH
+    # right-hand side of a cell-walk batch runs on the walk knob, the list
+    # gathers on the gather knob and the CSR solve on the solve knob; the
+    # stencil-list DFSPH gathers run one lane per row (they keep the grid
+    # cap, timed below)
+    csr_dfsph = [b for b in dfsph if not getattr(b, "stencil_lists", False)]
     plans = {
         "search": [_search_fn(b) for b in parts],
         "fill": [_search_fn(b) for b in parts],
-        # the delta-SPH right-hand side of a cell-walk batch runs on the walk
-        # knob, every other gather on the gather knob
-        "gather": [_gather_fn(b) for b in parts if not getattr(b, "cellwalk", False)],
+        "gather": [_gather_fn(b) for b in dsph if not b.cellwalk]
+                  + [_gather_fn(b) for b in csr_dfsph],
         "walk": [_gather_fn(b) for b in dsph if b.cellwalk],
-        "solve": [_solve_fn(b) for b in dfsph],
+        "solve": [_solve_fn(b) for b in csr_dfsph],
     }
     # the workgroup size of the list gathers and the per-query search, timed
     # on both once their lanes are chosen
@@ -130,7 +135,7 @@ def tune(parts, reps: int = 5, log=None) -> dict:
             log(f"autotune {knob}: " + ", ".join(f"{k}:{t * 1e6:.0f}us" for k, t in res.items())
                 + f" -> {best}")
     if dfsph:
-        fns = plans["solve"]
+        fns = [_solve_fn(b) for b in dfsph]
         res = {}
         for cap in GRID_CAPS:
             L.DFSPH_GRID_CAP = cap
