@@ -738,16 +738,16 @@ __device__ __forceinline__ void cellwalk_segment(WalkShared& S, int s0, int s1, 
 
 // ---- one lane per row: the neighbours' records staged in LDS
 // With one lane per row the test and the pair sums of a row run in the same
-// thread, so a row's hits stay in registers as bitmasks over its three
-// stencil runs (as in a stencil list) and the stage copies, besides the
-// runs' positions, the fields of their packed records the pair routine
-// reads: {vx, vy, rho, V} and p.  The pair loop then reads LDS only, each
-// record once per chunk from memory instead of once per pair.  Positions
-// come from the grid (sorted_x), the same values as the records' own.  A
-// run longer than a mask, or a chunk whose runs do not fit, takes the
-// global-memory walk (walk_slow) for all its fluid pairs.
+// thread, so a row's hits stay in registers, as one 64-bit mask over its
+// three stencil runs' candidates laid end to end, and the stage copies,
+// besides the runs' positions, the fields of their packed records the pair
+// routine reads: {vx, vy, rho, V} and p.  The pair loop then reads LDS
+// only, each record once per chunk from memory instead of once per pair.
+// Positions come from the grid (sorted_x), the same values as the records'
+// own.  A row with more than 64 candidates, or a chunk whose runs do not
+// fit, takes the global-memory walk (walk_slow) for all its fluid pairs.
 constexpr int kWalk1Cap = 1088;        // staged stencil points per chunk
-constexpr int kWalk1MaxRun = 32;       // candidates a run's mask holds
+constexpr int kWalk1MaxCand = 64;      // candidates of a row's three runs a mask holds
 constexpr int kWalk1Round = 2;         // staged slots per thread in flight
 
 struct Walk1Shared {
@@ -836,16 +836,18 @@ __device__ __forceinline__ void cellwalk_segment1(Walk1Shared& S, int s0, int s1
       }
     }
     __syncthreads();
-    // ---- test: the row's hits as bits of its runs' masks
-    unsigned m[3] = {0u, 0u, 0u};
-    bool slow = !staged;
-    if (valid && staged) {
+    // ---- test: the row's hits as bits of one mask over its three runs'
+    // candidates, run after run: bit t1 starts the second run, t2 the third
+    unsigned long long m = 0ull;
+    const int len0 = st.rb[0] - st.ra[0], len1 = st.rb[1] - st.ra[1];
+    const int t1 = len0, t2 = len0 + len1;
+    bool slow = !staged || t2 + (st.rb[2] - st.ra[2]) > kWalk1MaxCand;
+    if (valid && !slow) {
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
-        const int len = st.rb[r] - st.ra[r];
-        slow |= len > kWalk1MaxRun;
-        const int n = len < kWalk1MaxRun ? len : kWalk1MaxRun;
+        const int n = st.rb[r] - st.ra[r];
         const int b = cr.first(r, st);
+        const int off = r == 0 ? 0 : (r == 1 ? t1 : t2);
         for (int k = 0; k < n; k += kWalkBatch) {
           float2 xy[kWalkBatch];
 #pragma unroll
@@ -854,7 +856,7 @@ __device__ __forceinline__ void cellwalk_segment1(Walk1Shared& S, int s0, int s1
 #pragma unroll
           for (int u = 0; u < kWalkBatch; ++u)
             nib |= ((k + u < n) && dist2_nofma(xq, yq, xy[u].x, xy[u].y) <= thr) ? 1u << u : 0u;
-          m[r] |= nib << k;
+          m |= (unsigned long long)nib << (off + k);
         }
       }
     }
@@ -871,18 +873,21 @@ __device__ __forceinline__ void cellwalk_segment1(Walk1Shared& S, int s0, int s1
       Acc a = {};
       if (terms & kFluidPairTerms) {
         if (!slow) {
-#pragma unroll
-          for (int r = 0; r < 3; ++r) {
-            unsigned mm = m[r];
-            const int b = cr.first(r, st);
+          // one stream over the three runs (the runs' pairs in order), so
+          // a wavefront pads to its longest row, not to its longest run
+          // three times over
+          const int b0 = cr.first(0, st), b1 = cr.first(1, st) - t1, b2 = cr.first(2, st) - t2;
+          unsigned long long mm = m;
+          {
             while (mm) {
               int k[kPackedBatch];
               bool ok[kPackedBatch];
 #pragma unroll
               for (int u = 0; u < kPackedBatch; ++u) {
-                ok[u] = mm != 0u;
-                k[u] = b + (ok[u] ? (int)__builtin_ctz(mm) : 0);
-                mm &= mm - 1u;
+                ok[u] = mm != 0ull;
+                const int c = ok[u] ? (int)__builtin_ctzll(mm) : 0;
+                k[u] = ok[u] ? c + (c < t1 ? b0 : (c < t2 ? b1 : b2)) : 0;
+                mm &= mm - 1ull;
               }
               float2 xy[kPackedBatch];
               float4 rc[kPackedBatch];
