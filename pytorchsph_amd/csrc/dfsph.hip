@@ -815,21 +815,51 @@ __device__ __forceinline__ bool sl_stage(SlShared<NQ>& S, const Stencil3& st, bo
 }
 
 // f(records) for each pair of row w in order: from LDS (STAGED) or global
-// memory, kSlBatch loads in flight.
+// memory, kSlBatch loads in flight.  When a row's three masks hold 64
+// candidates or fewer they are laid end to end in one mask and walked as
+// one stream (bit t1 starts the second run, t2 the third), so a wavefront
+// pads its loop to its longest row rather than to its longest run three
+// times; a row with more walks its runs one after the other, in passes 1
+// and 2 of the same loop.
+__device__ __forceinline__ unsigned long long shl64(unsigned long long x, int s) {
+  return s < 64 ? x << s : 0ull;
+}
 template <bool STAGED, int NQ, typename Load, typename F>
 __device__ __forceinline__ void sl_walk_t(const SlShared<NQ>& S, const ChunkRuns& cr,
                                           const SlRow& w, Load load, F f) {
+  int n[3], a[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
-    unsigned long long mm = w.m[r];
-    const int b = STAGED ? cr.first(r, w.st) : w.st.ra[r];
+    const int len = w.st.rb[r] - w.st.ra[r];
+    n[r] = len < 64 ? len : 64;
+    a[r] = STAGED ? cr.first(r, w.st) : w.st.ra[r];
+  }
+  const bool one = n[0] + n[1] + n[2] <= 64;
+#pragma unroll 1
+  for (int pass = 0; pass < 3; ++pass) {
+    unsigned long long mm;
+    int t1, t2, b0, b1, b2;
+    if (pass == 0) {
+      mm = one ? (w.m[0] | shl64(w.m[1], n[0]) | shl64(w.m[2], n[0] + n[1])) : w.m[0];
+      t1 = one ? n[0] : 64;
+      t2 = one ? n[0] + n[1] : 64;
+      b0 = a[0];
+      b1 = a[1] - n[0];
+      b2 = a[2] - n[0] - n[1];
+    } else {
+      mm = one ? 0ull : (pass == 1 ? w.m[1] : w.m[2]);
+      t1 = t2 = 64;
+      b0 = b1 = b2 = pass == 1 ? a[1] : a[2];
+    }
     while (mm) {
       int k[kSlBatch];
       bool ok[kSlBatch];
 #pragma unroll
       for (int u = 0; u < kSlBatch; ++u) {
         ok[u] = mm != 0ull;
-        k[u] = ok[u] ? (int)__builtin_ctzll(mm) : 0;
+        const int c = ok[u] ? (int)__builtin_ctzll(mm) : 0;
+        // a slot past the row's last pair reloads its first (in bounds)
+        k[u] = ok[u] ? c + (c < t1 ? b0 : (c < t2 ? b1 : b2)) : k[0];
         mm &= mm - 1ull;
       }
       SlRecs<NQ> v[kSlBatch];
@@ -837,9 +867,9 @@ __device__ __forceinline__ void sl_walk_t(const SlShared<NQ>& S, const ChunkRuns
       for (int u = 0; u < kSlBatch; ++u) {
         if (STAGED) {
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) v[u].a[q] = S.rec[q][b + k[u]];
+          for (int q = 0; q < NQ; ++q) v[u].a[q] = S.rec[q][k[u]];
         } else {
-          v[u] = load(b + k[u]);
+          v[u] = load(k[u]);
         }
       }
 #pragma unroll
