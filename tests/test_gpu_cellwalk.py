@@ -89,6 +89,28 @@ def test_cellwalk_equals_list_gather(cuda, lanes, kind, kl):
         b.step()
 
 
+def compress_patch(sc, frac=0.75, n=400):
+    """Squeezes the n points nearest the block's centre toward it by frac
+    (density x 1 / frac^2): their rows hold more than 64 candidates while
+    the rows around them do not."""
+    c = sc.x.mean(0)
+    near = np.argsort(((sc.x - c) ** 2).sum(1))[:n]
+    sc.x[near] = (c + (sc.x[near] - c) * frac).astype(np.float32)
+    return sc
+
+
+@pytest.mark.parametrize("kl", [1, 4])
+def test_cellwalk_dense_patch(cuda, lanes, kl):
+    """Rows with more candidates than the one-lane walk's mask holds, in
+    chunks whose stencil still fits in LDS, beside rows that fit."""
+    L.set_tuning(gather=kl, walk=kl)
+    b = SceneBatch([compress_patch(S.dam_break(dx=0.01, scheme="dsph", seed=4))])
+    assert_bitwise(both_rhs(b), ("dense", kl))
+    rows = np.diff(b.ff.rowptr[: b.NP + 1].cpu().numpy())
+    assert rows.max() > 32 and np.median(rows) < 25
+    assert not b.status()[0] & L.ST_OVERFLOW
+
+
 def crowded_scene():
     """C1's dam break with 1,000 points squeezed into half a cell (each of
     them has ~1,000 neighbours) and three points outside the cell table."""

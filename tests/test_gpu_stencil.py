@@ -155,3 +155,26 @@ def test_stencil_list_steps_equal_csr_steps(cuda, kind, graphs):
             for key in ("x", "v", "rho", "acc", "p"):
                 assert np.isfinite(a[key]).all(), (k, key)
                 assert np.array_equal(a[key].view(np.int32), c[key].view(np.int32)), (k, key)
+
+
+def test_stencil_list_dense_patch_equals_csr(cuda):
+    """A patch squeezed to 1.8x the density: its rows' three runs hold more
+    than 64 candidates together (walked run after run) but at most 64 each
+    (no overflow), beside rows walked as one stream."""
+    from test_gpu_cellwalk import compress_patch
+    sc = compress_patch(S.dam_break(dx=0.02, scheme="dfsph", seed=1), n=300)
+    with tuned(gather=1, solve=1):
+        sl = SceneBatch([sc], stencil_lists=True)
+        cs = SceneBatch([sc], stencil_lists=False)
+        sl.step()
+        cs.step()
+        torch.cuda.synchronize()
+        runs = sl.sl_runs.cpu().numpy()
+        lens = np.stack([(runs[:, 3] >> (8 * q)) & 255 for q in range(3)], 1)
+        assert lens.sum(1).max() > 64 and lens.max() <= 64
+        assert (np.asarray(sl.status()) == np.asarray(cs.status())).all()
+        assert not sl.status()[0] & L.ST_OVERFLOW
+        a, c = sl.scene_state(0, extra=True), cs.scene_state(0, extra=True)
+        for key in ("x", "v", "rho", "acc", "p"):
+            assert np.isfinite(a[key]).all(), key
+            assert np.array_equal(a[key].view(np.int32), c[key].view(np.int32)), key
