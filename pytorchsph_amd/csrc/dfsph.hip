@@ -752,7 +752,7 @@ struct SlShared {
   float4 rec[NQ][SlCap<NQ>::v];
   int wlo[kBS / kWave][3], whi[kBS / kWave][3];
 };
-constexpr int kSlBatch = 4;   // pairs a lane loads before it adds them
+constexpr int kSlBatch = 3;   // pairs a lane loads before it adds them
 
 // Row r of a stencil list: its runs (as a Stencil3's ra / rb) and masks.
 struct SlRow {
