@@ -1089,7 +1089,9 @@ __global__ __launch_bounds__(kBS) void k_accel_sl(SphFluid f, const SphPhysics* 
 
 // k_pressure over a stencil list: the fluid rows' sums from the walk, the
 // boundary rows as in k_pressure with one lane per row.
-__global__ __launch_bounds__(kBS) void k_pressure_sl(SphFluid f, const SphPhysics* __restrict__ phys,
+// Held to 71 VGPRs, seven wavefronts per SIMD (what its LDS allows): at 88
+// registers it ran five (C3 70 -> 65 us per launch).
+__global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k_pressure_sl(SphFluid f, const SphPhysics* __restrict__ phys,
                                                      SphStencilList sl, SphBoundary b, SphCSR fb,
                                                      SphCSR bf, SphDfsph d) {
   __shared__ SlShared<1> S;
