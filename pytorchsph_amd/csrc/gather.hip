@@ -839,9 +839,10 @@ __device__ __forceinline__ void cellwalk_segment1(Walk1Shared& S, int s0, int s1
     // ---- test: the row's hits as bits of one mask over its three runs'
     // candidates, run after run: bit t1 starts the second run, t2 the third
     unsigned long long m = 0ull;
-    const int len0 = st.rb[0] - st.ra[0], len1 = st.rb[1] - st.ra[1];
+    // (a lane past the segment has no stencil: nothing of it is read)
+    const int len0 = valid ? st.rb[0] - st.ra[0] : 0, len1 = valid ? st.rb[1] - st.ra[1] : 0;
     const int t1 = len0, t2 = len0 + len1;
-    bool slow = !staged || t2 + (st.rb[2] - st.ra[2]) > kWalk1MaxCand;
+    bool slow = !staged || (valid && t2 + (st.rb[2] - st.ra[2]) > kWalk1MaxCand);
     if (valid && !slow) {
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
