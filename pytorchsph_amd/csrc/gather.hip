@@ -748,7 +748,7 @@ __device__ __forceinline__ void cellwalk_segment(WalkShared& S, int s0, int s1, 
 // fit, takes the global-memory walk (walk_slow) for all its fluid pairs.
 constexpr int kWalk1Cap = 1088;        // staged stencil points per chunk
 constexpr int kWalk1MaxCand = 64;      // candidates of a row's three runs a mask holds
-constexpr int kWalk1Round = 2;         // staged slots per thread in flight
+constexpr int kWalk1Round = 4;         // staged slots per thread in flight
 
 struct Walk1Shared {
   float2 pos[kWalk1Cap + kWalkBatch];  // the test reads past a run's end
